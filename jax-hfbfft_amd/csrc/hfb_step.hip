@@ -1,0 +1,274 @@
+// Damped gradient step with the FFT-diagonal preconditioner
+// (static.py:93-146 e0dmp_gt_zero, levels.py:397-437 laplace).
+//
+// The 3-D transform of the residual is five pencil passes over each spin
+// field: z (residual r formed from ps1 and lagrange or h_exp*psi, forward),
+// y (forward), x (forward, divide by the preconditioner, inverse), y
+// (inverse), z (inverse, psi <- psi - x0dmp * P r).  A scratch array of
+// the spinor block's size carries the transform between passes.
+#include <math.h>
+
+#include "hfb_ctx.h"
+
+using namespace hfb;
+
+namespace {
+
+enum StepMode { SM_IN = 0, SM_FWD = 1, SM_DIV = 2, SM_INV = 3, SM_OUT = 4 };
+
+struct StepArgs {
+  int nx, ny, nz;
+  const void* ps1;       // w h psi - wuv v_pair psi
+  const void* lag;       // Lagrange field
+  void* psi;             // in/out
+  void* work;            // transform scratch, same shape as psi
+  const double* hsum;    // (S,2): (sum Re psi* h psi, sum Re psi* ps1)
+  double wxyz;
+  int use_lag;           // (iteration > 1) && !tbcs  (static.py:99-104)
+  const double* pw;      // max(weight, 0.1) per state
+  const double* pshift;  // 0.5*max(weightuv, 0.1)*max(v_pair) per state
+  double e0, h2ma, x0;
+  double kfx, kfy, kfz;  // 2 pi / (d n) per axis
+  double* fpart;         // (2S fields, ntile, 2) partials of |r|^2, |lag - h_exp psi|^2
+};
+
+template <typename T, int N, int AXIS, int MODE, int P>
+__global__ void __launch_bounds__(8 * P) step_pass(StepArgs a, const cx<T>* __restrict__ twg) {
+  constexpr int NT = 8 * P;
+  __shared__ cx<T> tw[N];
+  __shared__ cx<T> b0[N * P], b1[N * P];
+  __shared__ double red[(NT + 63) / 64 * 2];
+  const int tile = blockIdx.x;
+  const int f = blockIdx.y;  // field = 2*state + spin
+  const int s = f >> 1;
+  const int64_t nvox = (int64_t)a.nx * a.ny * a.nz;
+  const int64_t fo = (int64_t)f * nvox;
+  const Tile t = make_tile<AXIS>(tile, a.nx, a.ny, a.nz, P);
+  cx<T>* psi = (cx<T>*)a.psi;
+  cx<T>* R = (cx<T>*)a.work;
+  load_twiddles(tw, twg, N);
+  double acc[2] = {0.0, 0.0};
+  for (int e = threadIdx.x; e < N * P; e += blockDim.x) {
+    int k, p;
+    elem_of<AXIS, N, P>(e, k, p);
+    const int64_t off = fo + t.base + k * t.sk + p * t.sp;
+    cx<T> v;
+    if constexpr (MODE == SM_IN) {
+      const T hexp = T(a.wxyz * a.hsum[2 * s + 1]);
+      const cx<T> p1 = ((const cx<T>*)a.ps1)[off];
+      const cx<T> ps = psi[off];
+      const cx<T> lg = ((const cx<T>*)a.lag)[off];
+      const cx<T> hp = cscale(ps, hexp);
+      v = a.use_lag ? csub(p1, lg) : csub(p1, hp);
+      const cx<T> e2 = csub(lg, hp);
+      acc[0] += double(v.x) * double(v.x) + double(v.y) * double(v.y);
+      acc[1] += double(e2.x) * double(e2.x) + double(e2.y) * double(e2.y);
+    } else {
+      v = R[off];
+    }
+    b0[k * P + p] = v;
+  }
+  if constexpr (MODE == SM_IN) {
+    if (a.fpart) {
+      block_sum<2>(acc, red);
+      if (threadIdx.x == 0) {
+        a.fpart[((int64_t)f * gridDim.x + tile) * 2 + 0] = acc[0];
+        a.fpart[((int64_t)f * gridDim.x + tile) * 2 + 1] = acc[1];
+      }
+    }
+  }
+  __syncthreads();
+  cx<T>* r;
+  if constexpr (MODE == SM_INV || MODE == SM_OUT)
+    r = fft_lds<T, N, P, +1>(b0, b1, tw, P);
+  else
+    r = fft_lds<T, N, P, -1>(b0, b1, tw, P);
+  if constexpr (MODE == SM_DIV) {
+    const double ntot = double(nvox);
+    for (int e = threadIdx.x; e < N * P; e += blockDim.x) {
+      const int k = e / P, p = e % P;
+      int x, y, z;
+      coords_of<AXIS>(tile, k, p, a.nx, a.ny, a.nz, P, x, y, z);
+      const int mx = x <= a.nx / 2 ? x : a.nx - x;
+      const int my = y <= a.ny / 2 ? y : a.ny - y;
+      const int mz = z <= a.nz / 2 ? z : a.nz - z;
+      const double k2 = (mx * a.kfx) * (mx * a.kfx) + (my * a.kfy) * (my * a.kfy) + (mz * a.kfz) * (mz * a.kfz);
+      const double den = (a.pw[s] * (a.e0 + a.h2ma * k2) + a.pshift[s]) * ntot;
+      r[e] = cscale(r[e], T(1.0 / den));
+    }
+    __syncthreads();
+    r = fft_lds<T, N, P, +1>(r, r == b0 ? b1 : b0, tw, P);
+  }
+  for (int e = threadIdx.x; e < N * P; e += blockDim.x) {
+    int k, p;
+    elem_of<AXIS, N, P>(e, k, p);
+    const int64_t off = fo + t.base + k * t.sk + p * t.sp;
+    if constexpr (MODE == SM_OUT) {
+      const cx<T> ps = psi[off];
+      const cx<T> pr = r[k * P + p];
+      psi[off] = csub(ps, cscale(pr, T(a.x0)));
+    } else {
+      R[off] = r[k * P + p];
+    }
+  }
+}
+
+template <typename T>
+int step_P(const hfb_ctx* c) {
+  if (c->nx % 16 == 0 && c->ny % 16 == 0 && c->nz % 16 == 0) return 16;
+  return 8;
+}
+
+template <typename T, int AXIS, int MODE, int P>
+int launch_step(const hfb_ctx* c, const StepArgs& a, int nfields, hipStream_t st) {
+  const int n = axis_len(c, AXIS);
+  const int nt = tiles_per_field<AXIS>(c->nx, c->ny, c->nz, P);
+  HFB_DISPATCH_N(n, {
+    hipLaunchKernelGGL((step_pass<T, N_, AXIS, MODE, P>), dim3(nt, nfields), dim3(8 * P), 0, st, a,
+                       twiddles<T>(c, N_));
+  })
+  return launch_status();
+}
+
+template <typename T, int P>
+int step_all(const hfb_ctx* c, const StepArgs& a, int ns, hipStream_t st) {
+  int rc = launch_step<T, 2, SM_IN, P>(c, a, 2 * ns, st);
+  if (rc == HFB_OK) rc = launch_step<T, 1, SM_FWD, P>(c, a, 2 * ns, st);
+  if (rc == HFB_OK) rc = launch_step<T, 0, SM_DIV, P>(c, a, 2 * ns, st);
+  if (rc == HFB_OK) rc = launch_step<T, 1, SM_INV, P>(c, a, 2 * ns, st);
+  if (rc == HFB_OK) rc = launch_step<T, 2, SM_OUT, P>(c, a, 2 * ns, st);
+  return rc;
+}
+
+// Per-state weights of the gradient step (static.py:152-170, 125-136):
+// hpsi weights (w, wuv) and the preconditioner's (max(w,.1), shift).
+__global__ void grstep_weights_kernel(int ns, int npsi0, int tbcs, const double* __restrict__ wocc,
+                                      const double* __restrict__ wstates, const double* __restrict__ wguv,
+                                      const double* __restrict__ pairwg, const double* __restrict__ vpair,
+                                      int64_t nvox, double* __restrict__ w, double* __restrict__ wuv,
+                                      double* __restrict__ pw, double* __restrict__ pshift) {
+  __shared__ double vmax_s[2];
+  __shared__ double red[16];
+  for (int iq = 0; iq < 2; ++iq) {
+    double m = -INFINITY;
+    if (!tbcs)
+      for (int64_t i = threadIdx.x; i < nvox; i += blockDim.x) m = fmax(m, vpair[iq * nvox + i]);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+    if (lane == 0) red[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double mm = -INFINITY;
+      for (int j = 0; j < (int)(blockDim.x + 63) / 64; ++j) mm = fmax(mm, red[j]);
+      vmax_s[iq] = mm;
+    }
+    __syncthreads();
+  }
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    const int iq = s >= npsi0 ? 1 : 0;
+    if (tbcs) {
+      w[s] = 1.0;
+      wuv[s] = 0.0;
+      pw[s] = 1.0;
+      pshift[s] = 0.0;  // laplace(..., 1.0, 0.0, ps1, 0.0, e0dmp)
+    } else {
+      w[s] = wstates[s] * wocc[s];
+      wuv[s] = wstates[s] * wguv[s] * pairwg[s];
+      pw[s] = fmax(wocc[s], 0.1);
+      pshift[s] = 0.5 * fmax(wguv[s] * pairwg[s], 0.1) * vmax_s[iq];
+    }
+  }
+}
+
+// Bookkeeping after the step (static.py:173, 211, 247-257).
+__global__ void grstep_finish_kernel(int ns, const double* __restrict__ hsum, double wxyz,
+                                     double* __restrict__ sp_energy, const double* __restrict__ wocc,
+                                     const double* __restrict__ wstates, const double* __restrict__ fsum,
+                                     double* __restrict__ efl1, double* __restrict__ efl2, double* __restrict__ acc) {
+  __shared__ double red[16 * 2];
+  double v[2] = {0.0, 0.0};
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    const double spe_old = sp_energy[s];
+    const double spe_new = wxyz * hsum[2 * s];
+    const double denerg = (spe_old - spe_new) / fabs(spe_new);
+    if (fsum) {
+      efl1[s] = sqrt(fsum[2 * s]);
+      efl2[s] = sqrt(fsum[2 * s + 1]);
+    }
+    v[1] += wocc[s] * wstates[s] * denerg;
+    sp_energy[s] = spe_new;
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) v[0] += efl1[s];
+  block_sum<2>(v, red);
+  if (threadIdx.x == 0) {
+    acc[0] += v[0];
+    acc[1] += v[1];
+  }
+}
+
+}  // namespace
+
+extern "C" int hfb_step_ntiles(hfb_ctx* c, int prec) {
+  if (!c) return HFB_E_NOCTX;
+  const int P = prec == HFB_F32 ? step_P<float>(c) : step_P<double>(c);
+  return tiles_per_field<2>(c->nx, c->ny, c->nz, P);
+}
+
+extern "C" int hfb_grstep_weights(int nstates, int npsi0, int tbcs, const double* wocc, const double* wstates,
+                                  const double* wguv, const double* pairwg, const double* v_pair, long long nvox,
+                                  double* w, double* wuv, double* pw, double* pshift, void* stream) {
+  if (nstates <= 0 || !wocc || !wstates || !wguv || !pairwg || !v_pair || !w || !wuv || !pw || !pshift)
+    return HFB_E_ARG;
+  hipLaunchKernelGGL(grstep_weights_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, nstates, npsi0, tbcs,
+                     wocc, wstates, wguv, pairwg, v_pair, (int64_t)nvox, w, wuv, pw, pshift);
+  return launch_status();
+}
+
+extern "C" int hfb_precond_step(hfb_ctx* c, int prec, void* psi, const void* ps1, const void* lagrange,
+                                int nstates, const double* hsum, double wxyz, int use_lagrange, const double* pw,
+                                const double* pshift, double e0dmp, double h2ma, double x0dmp, double* fpart,
+                                void* work, void* stream) {
+  if (!c) return HFB_E_NOCTX;
+  if (!psi || !ps1 || !lagrange || !hsum || !pw || !pshift || !work || nstates <= 0) return HFB_E_ARG;
+  StepArgs a{};
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.nz = c->nz;
+  a.ps1 = ps1;
+  a.lag = lagrange;
+  a.psi = psi;
+  a.work = work;
+  a.hsum = hsum;
+  a.wxyz = wxyz;
+  a.use_lag = use_lagrange;
+  a.pw = pw;
+  a.pshift = pshift;
+  a.e0 = e0dmp;
+  a.h2ma = h2ma;
+  a.x0 = x0dmp;
+  a.kfx = 2.0 * M_PI / (c->dx * c->nx);
+  a.kfy = 2.0 * M_PI / (c->dy * c->ny);
+  a.kfz = 2.0 * M_PI / (c->dz * c->nz);
+  a.fpart = fpart;
+  hipStream_t st = (hipStream_t)stream;
+  if (prec == HFB_F64) {
+    if (step_P<double>(c) == 16) return step_all<double, 16>(c, a, nstates, st);
+    return step_all<double, 8>(c, a, nstates, st);
+  }
+  if (prec == HFB_F32) {
+    if (step_P<float>(c) == 16) return step_all<float, 16>(c, a, nstates, st);
+    return step_all<float, 8>(c, a, nstates, st);
+  }
+  return HFB_E_ARG;
+}
+
+extern "C" int hfb_grstep_finish(int nstates, const double* hsum, double wxyz, double* sp_energy,
+                                 const double* wocc, const double* wstates, const double* fsum, double* sp_efluct1,
+                                 double* sp_efluct2, double* acc, void* stream) {
+  if (nstates <= 0 || !hsum || !sp_energy || !wocc || !wstates || !sp_efluct1 || !sp_efluct2 || !acc)
+    return HFB_E_ARG;
+  hipLaunchKernelGGL(grstep_finish_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, nstates, hsum, wxyz,
+                     sp_energy, wocc, wstates, fsum, sp_efluct1, sp_efluct2, acc);
+  return launch_status();
+}
