@@ -37,6 +37,30 @@ enum { HFB_F64 = 0, HFB_F32 = 1 };
 
 typedef struct hfb_ctx hfb_ctx;
 
+/* Skyrme couplings and switches passed by value to the potential and
+ * energy kernels; field meanings as in forces.py:7-58 (the b- and
+ * C-couplings of forces.py:95-122), with h2m already c.m.-scaled as
+ * init_static leaves it (static.py:87-88). */
+typedef struct hfb_force {
+  double b0, b0p, b1, b1p, b2, b2p, b3, b3p, b4, b4p, power, slate;
+  double h2m[2];
+  double Crho0, Crho1, Crho0D, Crho1D, Cdrho0, Cdrho1, Ctau0, Ctau1, CdJ0, CdJ1;
+  double v0neut, v0prot, rho0pr;
+  double e2, mass_number;
+  int ipair, ex, zpe, tcoul;
+} hfb_force;
+
+/* Slots of the device energy vector (double[HFB_EN_LEN]), energies.py:6-43. */
+enum {
+  HFB_EN_EHFT = 0, HFB_EN_EHF0, HFB_EN_EHF1, HFB_EN_EHF2, HFB_EN_EHF3, HFB_EN_EHFLS, HFB_EN_EHFLSODD,
+  HFB_EN_EHFC, HFB_EN_ECORC, HFB_EN_EHFINT, HFB_EN_E3CORR, HFB_EN_EZPE, HFB_EN_CRHO0, HFB_EN_CRHO1,
+  HFB_EN_CDRHO0, HFB_EN_CDRHO1, HFB_EN_CTAU0, HFB_EN_CTAU1, HFB_EN_CDJ0, HFB_EN_CDJ1, HFB_EN_CJ0,
+  HFB_EN_CJ1, HFB_EN_EHF, HFB_EN_TKE, HFB_EN_ORB /* 3 */, HFB_EN_SPIN = HFB_EN_ORB + 3,
+  HFB_EN_JTOT = HFB_EN_SPIN + 3, HFB_EN_EFL1 = HFB_EN_JTOT + 3, HFB_EN_EFL2, HFB_EN_EFL1Q /* 2 */,
+  HFB_EN_EFL2Q = HFB_EN_EFL1Q + 2, HFB_EN_ECORRP = HFB_EN_EFL2Q + 2, HFB_EN_SUMFLU, HFB_EN_DELESUM,
+  HFB_EN_EPAIR /* 2 */, HFB_EN_LEN = 48
+};
+
 const char* hfb_strerror(int code);
 
 /* Grid + tables.  Replaces the device-side state of init_grids

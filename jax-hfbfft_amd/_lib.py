@@ -31,6 +31,38 @@ _SIGS = {
     "hfb_reduce_partials": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_void_p, c_void_p]),
     "hfb_poisson": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
     "hfb_selftest_mfma_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "hfb_step_ntiles": (c_int, [c_void_p, c_int]),
+    "hfb_grstep_weights": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   ctypes.c_longlong, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "hfb_precond_step": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_double, c_int,
+                                 c_void_p, c_void_p, c_double, c_double, c_double, c_void_p, c_void_p, c_void_p]),
+    "hfb_grstep_finish": (c_int, [c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p]),
+    "hfb_gram_workspace": (c_int, [c_int, ctypes.c_longlong]),
+    "hfb_gram": (c_int, [c_int, c_void_p, POINTER(c_void_p), c_int, c_int, ctypes.c_longlong, c_double, c_void_p,
+                         c_void_p, c_void_p]),
+    "hfb_rotate": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, ctypes.c_longlong,
+                           c_void_p]),
+    "hfb_diag_small": (c_int, [c_int, c_int, c_int, POINTER(c_void_p), c_int, c_int, c_int, c_int, c_void_p,
+                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_void_p),
+                               c_void_p, c_void_p]),
+    "hfb_density_sizes": (c_int, [c_void_p, c_int, POINTER(ctypes.c_longlong), POINTER(ctypes.c_longlong)]),
+    "hfb_density": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "hfb_density_finish": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_int, c_double, c_void_p]),
+    "hfb_sp_finish": (c_int, [c_void_p, c_void_p, c_int, c_int, c_double, c_double, c_void_p, c_void_p, c_void_p,
+                              c_void_p, c_void_p]),
+    "hfb_density_derivs": (c_int, [c_void_p] + [c_void_p] * 9 + [c_void_p]),
+    "hfb_meanfield_derivs": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "hfb_skyrme_nblocks": (c_int, [c_void_p]),
+    "hfb_skyrme": (c_int, [c_void_p, c_void_p] + [c_void_p] * 11 + [c_void_p] * 6 + [c_void_p] * 6 +
+                   [c_void_p, c_void_p, c_void_p]),
+    "hfb_integ_energy": (c_int, [c_void_p, c_void_p] + [c_void_p] * 8 + [c_void_p, c_void_p, c_void_p]),
+    "hfb_sum_energy": (c_int, [c_int] + [c_void_p] * 7 + [c_void_p]),
+    "hfb_bcs": (c_int, [c_int, c_int, c_double, c_double, POINTER(c_double), POINTER(c_double), c_double, c_double] +
+                [c_void_p] * 13 + [c_void_p]),
 }
 
 EXPORTED = tuple(_SIGS)

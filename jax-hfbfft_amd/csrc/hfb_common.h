@@ -85,14 +85,14 @@ __device__ __forceinline__ void dft_small(cx<T>* u) {
 // pencils.  tw[m] = exp(-2 pi i m / N).  Returns the buffer holding the
 // result (src or dst of the first call).
 template <typename T, int N, int Q, int SIGN, int PASS, int SPAN>
-__device__ __forceinline__ cx<T>* fft_pass(cx<T>* src, cx<T>* dst, const cx<T>* tw, int ldq) {
+__device__ __forceinline__ cx<T>* fft_pass(cx<T>* src, cx<T>* dst, const cx<T>* tw, int ldq, int tid, int nthr) {
   if constexpr (PASS >= Plan<N>::np) {
     return src;
   } else {
     constexpr int R = Plan<N>::r[PASS];
     constexpr int M = N / R;
     constexpr int TWS = N / (SPAN * R);
-    for (int w = threadIdx.x; w < Q * M; w += blockDim.x) {
+    for (int w = tid; w < Q * M; w += nthr) {
       const int q = w % Q;
       const int i = w / Q;
       const int k = i % SPAN;
@@ -111,15 +111,23 @@ __device__ __forceinline__ cx<T>* fft_pass(cx<T>* src, cx<T>* dst, const cx<T>* 
       for (int t = 0; t < R; ++t) dst[(j + t * SPAN) * ldq + q] = u[t];
     }
     __syncthreads();
-    return fft_pass<T, N, Q, SIGN, PASS + 1, SPAN * R>(dst, src, tw, ldq);
+    return fft_pass<T, N, Q, SIGN, PASS + 1, SPAN * R>(dst, src, tw, ldq, tid, nthr);
   }
 }
 
 // Forward (SIGN=-1, unnormalised) or inverse (SIGN=+1, unnormalised) DFT of
-// Q pencils of length N held in LDS.  Callers must __syncthreads() before.
+// Q pencils of length N held in LDS.  Callers must __syncthreads() before;
+// every thread of the block must call (the passes synchronise the block).
+// (tid, nthr) select the threads that do the work, so several independent
+// tiles can be transformed by sub-groups of one block.
+template <typename T, int N, int Q, int SIGN>
+__device__ __forceinline__ cx<T>* fft_lds(cx<T>* a, cx<T>* b, const cx<T>* tw, int ldq, int tid, int nthr) {
+  return fft_pass<T, N, Q, SIGN, 0, 1>(a, b, tw, ldq, tid, nthr);
+}
+
 template <typename T, int N, int Q, int SIGN>
 __device__ __forceinline__ cx<T>* fft_lds(cx<T>* a, cx<T>* b, const cx<T>* tw, int ldq) {
-  return fft_pass<T, N, Q, SIGN, 0, 1>(a, b, tw, ldq);
+  return fft_pass<T, N, Q, SIGN, 0, 1>(a, b, tw, ldq, (int)threadIdx.x, (int)blockDim.x);
 }
 
 template <typename T>

@@ -1,0 +1,478 @@
+// Mean-field potentials from the densities (meanfield.py:163-391) and the
+// energy-functional integrals (energies.py:90-371), FP64.
+//
+// The reference's derivatives of real fields are dense n x n Fourier
+// matrices applied along one axis (der1 = sder, der2 = sder2; grids.py:36-61,
+// einsum forms 'ij,jkl->ikl', 'jl,ilk->ijk', 'kl,ijl->ijk').  They are
+// applied here as axis stencils: each output point sums n products along the
+// axis, with neighbouring threads on consecutive z so the reads coalesce.
+// This work is O(n^4) per field, independent of the number of states.
+#include <math.h>
+
+#include "hfb_ctx.h"
+
+using namespace hfb;
+
+namespace {
+
+// One derived field = sum of up to three axis derivatives of source fields.
+struct DerivJob {
+  const double* src[3];
+  int axis[3];
+  int order[3];  // 1: der1, 2: der2
+  double coef[3];
+  int nterm;
+  double* dst;
+};
+
+constexpr int MAXJOB = 24;
+struct DerivJobs {
+  DerivJob j[MAXJOB];
+  int njob;
+};
+
+__global__ void __launch_bounds__(256) deriv_kernel(DerivJobs jobs, int nx, int ny, int nz, const double* __restrict__ d1x,
+                                                    const double* __restrict__ d1y, const double* __restrict__ d1z,
+                                                    const double* __restrict__ d2x, const double* __restrict__ d2y,
+                                                    const double* __restrict__ d2z) {
+  const DerivJob& jb = jobs.j[blockIdx.y];
+  const int64_t nvox = (int64_t)nx * ny * nz;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nvox; e += (int64_t)gridDim.x * blockDim.x) {
+    const int x = e / ((int64_t)ny * nz);
+    const int y = (e / nz) % ny;
+    const int z = e % nz;
+    double acc = 0.0;
+    for (int t = 0; t < jb.nterm; ++t) {
+      const double* s = jb.src[t];
+      const int ax = jb.axis[t];
+      const double* D;
+      int n, i;
+      int64_t base, stride;
+      if (ax == 0) {
+        D = jb.order[t] == 1 ? d1x : d2x;
+        n = nx;
+        i = x;
+        base = (int64_t)y * nz + z;
+        stride = (int64_t)ny * nz;
+      } else if (ax == 1) {
+        D = jb.order[t] == 1 ? d1y : d2y;
+        n = ny;
+        i = y;
+        base = (int64_t)x * ny * nz + z;
+        stride = nz;
+      } else {
+        D = jb.order[t] == 1 ? d1z : d2z;
+        n = nz;
+        i = z;
+        base = ((int64_t)x * ny + y) * nz;
+        stride = 1;
+      }
+      double v = 0.0;
+      for (int l = 0; l < n; ++l) v += D[i * n + l] * s[base + l * stride];
+      acc += jb.coef[t] * v;
+    }
+    jb.dst[e] = acc;
+  }
+}
+
+struct SkyArgs {
+  hfb_force f;
+  int64_t nvox;
+  const double *rho, *tau, *chi, *cur, *sdens;  // densities
+  const double *divj, *lap, *grad, *curls, *curlj;  // derived, [2] / [2][3]
+  const double* wcoul;
+  double *upot, *bmass, *wlspot, *aq, *spot, *vpair;  // FP64 out
+  float *upot32, *bmass32, *wlspot32, *aq32, *spot32, *vpair32;  // optional FP32 copies
+  double* epart;  // per-block partial of sum rho_tot * rearr (DDDI)
+};
+
+__global__ void __launch_bounds__(256) skyrme_combine_kernel(SkyArgs a) {
+  const hfb_force& f = a.f;
+  const int64_t n = a.nvox;
+  __shared__ double red[4];
+  double ep[1] = {0.0};
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    const double rn = a.rho[v], rp = a.rho[n + v];
+    const double rt = rn + rp;
+    const double rtp = pow(rt, f.power);
+    double rearr = 0.0;
+    if (f.ipair == 6) {
+      const double cn = a.chi[v], cp = a.chi[n + v];
+      rearr = (f.v0neut / f.rho0pr) * cn * cn + (f.v0prot / f.rho0pr) * cp * cp;
+      ep[0] += rt * rearr;
+    }
+    for (int iq = 0; iq < 2; ++iq) {
+      const int ic = 1 - iq;
+      const double rq = a.rho[iq * n + v], rc = a.rho[ic * n + v];
+      // meanfield.py:181-190: three-body term
+      double u = rtp * ((f.b3 * (f.power + 2.0) / 3.0 - 2.0 * f.b3p / 3.0) * rq + f.b3 * (f.power + 2.0) / 3.0 * rc -
+                        (f.b3p * f.power / 3.0) * (rn * rn + rp * rp) / (rt + 1.0e-25));
+      // meanfield.py:202-206: spin-orbit divergence
+      u += -(f.b4 + f.b4p) * a.divj[iq * n + v] - f.b4 * a.divj[ic * n + v];
+      // meanfield.py:211-221: Coulomb and Slater exchange for protons
+      if (iq == 1 && f.tcoul) {
+        u += a.wcoul[v];
+        if (f.ex != 0) u += -f.slate * pow(rp, 1.0 / 3.0);
+      }
+      // meanfield.py:234-241
+      u += (f.b0 - f.b0p) * rq + f.b0 * rc + (f.b1 - f.b1p) * a.tau[iq * n + v] + f.b1 * a.tau[ic * n + v] -
+           (f.b2 - f.b2p) * a.lap[iq * n + v] - f.b2 * a.lap[ic * n + v];
+      if (f.ipair == 6) u += rearr;  // meanfield.py:352-360
+      const double b = f.h2m[iq] + (f.b1 - f.b1p) * rq + f.b1 * rc;  // meanfield.py:253-256
+      const double v0 = iq == 1 ? f.v0prot : f.v0neut;
+      const double vp = f.ipair == 6 ? v0 * a.chi[iq * n + v] * (1.0 - rt / f.rho0pr) : v0 * a.chi[iq * n + v];
+      a.upot[iq * n + v] = u;
+      a.bmass[iq * n + v] = b;
+      a.vpair[iq * n + v] = vp;
+      if (a.upot32) {
+        a.upot32[iq * n + v] = float(u);
+        a.bmass32[iq * n + v] = float(b);
+        a.vpair32[iq * n + v] = float(vp);
+      }
+      for (int c = 0; c < 3; ++c) {
+        const int64_t iqc = (int64_t)(iq * 3 + c) * n + v, icc = (int64_t)(ic * 3 + c) * n + v;
+        const double w = (f.b4 + f.b4p) * a.grad[iqc] + f.b4 * a.grad[icc];                      // 270-274
+        const double aq = -2.0 * (f.b1 - f.b1p) * a.cur[iqc] - 2.0 * f.b1 * a.cur[icc] -
+                          (f.b4 + f.b4p) * a.curls[iqc] - f.b4 * a.curls[icc];                      // 294-299
+        const double sp = -(f.b4 + f.b4p) * a.curlj[iqc] - f.b4 * a.curlj[icc];                   // 320-324
+        a.wlspot[iqc] = w;
+        a.aq[iqc] = aq;
+        a.spot[iqc] = sp;
+        if (a.upot32) {
+          a.wlspot32[iqc] = float(w);
+          a.aq32[iqc] = float(aq);
+          a.spot32[iqc] = float(sp);
+        }
+      }
+    }
+  }
+  if (a.epart) {
+    block_sum<1>(ep, red);
+    if (threadIdx.x == 0) a.epart[blockIdx.x] = ep[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Energy integrals (energies.py:90-306): per-block partial sums.
+// ---------------------------------------------------------------------------
+constexpr int NES = 20;
+enum {
+  S_EHF0, S_CRHO0, S_CRHO1, S_EHF3, S_EHF2, S_CDRHO0, S_CDRHO1, S_CTAU0, S_CTAU1, S_EHF1,
+  S_EHFLS, S_CDJ0, S_CDJ1, S_LSODD, S_EHFC, S_ECORC, S_EHFT, S_ECORRP_UNUSED, S_PAD0, S_PAD1
+};
+
+struct EnArgs {
+  hfb_force f;
+  int64_t nvox;
+  const double *rho, *tau, *sdens;
+  const double *divj, *lap, *curlj;
+  const double* wcoul;
+  double* part;  // [block][NES]
+};
+
+__global__ void __launch_bounds__(256) energy_kernel(EnArgs a) {
+  const hfb_force& f = a.f;
+  const int64_t n = a.nvox;
+  __shared__ double red[4 * NES];
+  double s[NES];
+#pragma unroll
+  for (int j = 0; j < NES; ++j) s[j] = 0.0;
+  const double sc = f.ex != 0 ? -0.75 * f.slate : 0.0;
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    const double rn = a.rho[v], rp = a.rho[n + v];
+    const double rt = rn + rp, r1 = -rn + rp;
+    const double ln = a.lap[v], lp = a.lap[n + v];
+    const double lt = ln + lp, l1 = -ln + lp;
+    const double tn = a.tau[v], tp = a.tau[n + v];
+    const double t0 = tn + tp, t1 = -tn + tp;
+    const double rpow = pow(rt, f.power);
+    s[S_EHF0] += (f.b0 * rt * rt - f.b0p * (rp * rp + rn * rn)) / 2.0;
+    s[S_CRHO0] += (f.Crho0 + f.Crho0D * rpow) * rt * rt;
+    s[S_CRHO1] += (f.Crho1 + f.Crho1D * rpow) * r1 * r1;
+    s[S_EHF3] += rpow * (f.b3 * rt * rt - f.b3p * (rp * rp + rn * rn)) / 3.0;
+    s[S_EHF2] += (-f.b2 * rt * lt + f.b2p * (rp * lp + rn * ln)) / 2.0;
+    s[S_CDRHO0] += f.Cdrho0 * rt * lt;
+    s[S_CDRHO1] += f.Cdrho1 * r1 * l1;
+    s[S_CTAU0] += f.Ctau0 * t0 * rt;
+    s[S_CTAU1] += f.Ctau1 * t1 * r1;
+    s[S_EHF1] += f.b1 * rt * t0 - f.b1p * (rn * tn + rp * tp);
+    const double dn = a.divj[v], dp = a.divj[n + v];
+    s[S_EHFLS] += -f.b4 * rt * (dn + dp) - f.b4p * (rp * dp + rn * dn);
+    s[S_CDJ0] += f.CdJ0 * rt * (dn + dp);
+    s[S_CDJ1] += f.CdJ1 * r1 * (-dn + dp);
+    double odd = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      const double c0 = a.curlj[(0 * 3 + c) * n + v], c1 = a.curlj[(1 * 3 + c) * n + v];
+      const double s0 = a.sdens[(0 * 3 + c) * n + v], s1 = a.sdens[(1 * 3 + c) * n + v];
+      odd += -f.b4 * (s0 * (c0 + c1) + s1 * (c0 + c1)) - f.b4p * (s0 * c0 + s1 * c1);
+    }
+    s[S_LSODD] += odd;
+    if (f.tcoul) {
+      const double r43 = pow(rp, 4.0 / 3.0);
+      s[S_EHFC] += 0.5 * rp * a.wcoul[v] + sc * r43;
+      s[S_ECORC] += sc / 3.0 * r43;
+    }
+    s[S_EHFT] += f.h2m[0] * tn + f.h2m[1] * tp;
+  }
+  block_sum<NES>(s, red);
+  if (threadIdx.x == 0)
+    for (int j = 0; j < NES; ++j) a.part[(int64_t)blockIdx.x * NES + j] = s[j];
+}
+
+__global__ void energy_finish_kernel(EnArgs a, int nblk, double wxyz, const double* __restrict__ cur, int nx,
+                                     int ny, int nz, double* __restrict__ E) {
+  __shared__ double red[4 * NES];
+  double s[NES];
+  for (int j = 0; j < NES; ++j) s[j] = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+    for (int j = 0; j < NES; ++j) s[j] += a.part[(int64_t)b * NES + j];
+  block_sum<NES>(s, red);
+  if (threadIdx.x != 0) return;
+  const hfb_force& f = a.f;
+  // energies.py:157-167 index current[:,:,:,0,0] .. current[:,:,:,2,1]: the
+  // (y, z) = (0..2, 0..1) corner of every (iq, component, x) -- kept as written.
+  const int64_t nvox = (int64_t)nx * ny * nz;
+  double cj0 = 0.0, cj1 = 0.0;
+  for (int iq = 0; iq < 2; ++iq)
+    for (int c = 0; c < 3; ++c)
+      for (int x = 0; x < nx; ++x) {
+        const double* p = cur + (int64_t)(iq * 3 + c) * nvox + (int64_t)x * ny * nz;
+        for (int y = 0; y < 3; ++y) {
+          const double a0 = p[(int64_t)y * nz + 0], a1 = p[(int64_t)y * nz + 1];
+          cj0 += (a0 + a1) * (a0 + a1);
+          cj1 += (-a0 + a1) * (-a0 + a1);
+        }
+      }
+  const double ehf0 = wxyz * s[S_EHF0], ehf1 = wxyz * s[S_EHF1], ehf2 = wxyz * s[S_EHF2], ehf3 = wxyz * s[S_EHF3];
+  const double ehflsodd = wxyz * s[S_LSODD];
+  const double ehfls = wxyz * s[S_EHFLS] + ehflsodd;
+  const double ehfc = f.tcoul ? wxyz * s[S_EHFC] : 0.0;
+  const double ecorc = f.tcoul ? wxyz * s[S_ECORC] : 0.0;
+  const double ehft = wxyz * s[S_EHFT];
+  const double ezpe = f.zpe == 1 ? 17.3 / pow(f.mass_number, 0.2) : 0.0;
+  const double epair = E[HFB_EN_EPAIR] + E[HFB_EN_EPAIR + 1];
+  E[HFB_EN_EHFT] = ehft;
+  E[HFB_EN_EHF0] = ehf0;
+  E[HFB_EN_EHF1] = ehf1;
+  E[HFB_EN_EHF2] = ehf2;
+  E[HFB_EN_EHF3] = ehf3;
+  E[HFB_EN_EHFLS] = ehfls;
+  E[HFB_EN_EHFLSODD] = ehflsodd;
+  E[HFB_EN_EHFC] = ehfc;
+  E[HFB_EN_ECORC] = ecorc;
+  E[HFB_EN_E3CORR] = -f.power * ehf3 / 2.0;
+  E[HFB_EN_EZPE] = ezpe;
+  E[HFB_EN_CRHO0] = wxyz * s[S_CRHO0];
+  E[HFB_EN_CRHO1] = wxyz * s[S_CRHO1];
+  E[HFB_EN_CDRHO0] = wxyz * s[S_CDRHO0];
+  E[HFB_EN_CDRHO1] = wxyz * s[S_CDRHO1];
+  E[HFB_EN_CTAU0] = wxyz * s[S_CTAU0];
+  E[HFB_EN_CTAU1] = wxyz * s[S_CTAU1];
+  E[HFB_EN_CDJ0] = wxyz * s[S_CDJ0];
+  E[HFB_EN_CDJ1] = wxyz * s[S_CDJ1];
+  E[HFB_EN_CJ0] = -wxyz * f.Ctau0 * cj0;
+  E[HFB_EN_CJ1] = -wxyz * f.Ctau1 * cj1;
+  E[HFB_EN_EHFINT] = ehft + ehf0 + ehf1 + ehf2 + ehf3 + ehfls + ehfc - epair - ezpe;
+}
+
+__global__ void ecorrp_finish_kernel(const double* part, int nblk, double wxyz, int ipair, double* E) {
+  __shared__ double red[4];
+  double s[1] = {0.0};
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s[0] += part[b];
+  block_sum<1>(s, red);
+  if (threadIdx.x == 0) E[HFB_EN_ECORRP] = ipair == 6 ? -s[0] * wxyz / 2.0 : 0.0;
+}
+
+// sum_energy (energies.py:309-371)
+__global__ void sum_energy_kernel(int ns, const double* wocc, const double* wstates, const double* kin,
+                                  const double* spe, const double* orb, const double* spin, double* E) {
+  __shared__ double red[4 * 8];
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    const double w = wocc[s] * wstates[s];
+    v[0] += w * (kin[s] + spe[s]);
+    v[1] += w * kin[s];
+    for (int c = 0; c < 3; ++c) {
+      v[2 + c] += w * orb[3 * s + c];
+      v[5 + c] += w * spin[3 * s + c];
+    }
+  }
+  block_sum<8>(v, red);
+  if (threadIdx.x == 0) {
+    const double epair = E[HFB_EN_EPAIR] + E[HFB_EN_EPAIR + 1];
+    E[HFB_EN_EHF] = v[0] / 2.0 + E[HFB_EN_E3CORR] + E[HFB_EN_ECORRP] + E[HFB_EN_ECORC] - epair - E[HFB_EN_EZPE];
+    E[HFB_EN_TKE] = v[1];
+    for (int c = 0; c < 3; ++c) {
+      E[HFB_EN_ORB + c] = v[2 + c];
+      E[HFB_EN_SPIN + c] = v[5 + c];
+      E[HFB_EN_JTOT + c] = v[2 + c] + v[5 + c];
+    }
+  }
+}
+
+void push(DerivJobs& J, double* dst, int nterm, const double* s0, int a0, int o0, double c0, const double* s1 = nullptr,
+          int a1 = 0, int o1 = 1, double c1 = 0.0, const double* s2 = nullptr, int a2 = 0, int o2 = 1, double c2 = 0.0) {
+  DerivJob& j = J.j[J.njob++];
+  j.dst = dst;
+  j.nterm = nterm;
+  j.src[0] = s0; j.axis[0] = a0; j.order[0] = o0; j.coef[0] = c0;
+  j.src[1] = s1; j.axis[1] = a1; j.order[1] = o1; j.coef[1] = c1;
+  j.src[2] = s2; j.axis[2] = a2; j.order[2] = o2; j.coef[2] = c2;
+}
+
+int run_jobs(const hfb_ctx* c, const DerivJobs& J, hipStream_t st) {
+  const int64_t nvox = (int64_t)c->nx * c->ny * c->nz;
+  const int gx = (int)((nvox + 255) / 256);
+  hipLaunchKernelGGL(deriv_kernel, dim3(gx, J.njob), dim3(256), 0, st, J, c->nx, c->ny, c->nz, c->der1[0], c->der1[1],
+                     c->der1[2], c->der2[0], c->der2[1], c->der2[2]);
+  return launch_status();
+}
+
+}  // namespace
+
+// Derived density fields used by skyrme and integ_energy, each (2, ...):
+//   divj (2,n3) = div sodens   (meanfield.py:195-200, energies.py:170-176)
+//   lap  (2,n3) = Laplacian rho (meanfield.py:227-231, energies.py:95-100)
+//   grad (2,3,n3) = grad rho   (meanfield.py:262-268)
+//   curls(2,3,n3) = curl sdens (meanfield.py:279-289)
+//   curlj(2,3,n3) = curl current (meanfield.py:304-314, energies.py:196-209)
+extern "C" int hfb_density_derivs(hfb_ctx* c, const double* rho, const double* sdens, const double* current,
+                                  const double* sodens, double* divj, double* lap, double* grad, double* curls,
+                                  double* curlj, void* stream) {
+  if (!c) return HFB_E_NOCTX;
+  if (!rho || !sdens || !current || !sodens || !divj || !lap || !grad || !curls || !curlj) return HFB_E_ARG;
+  const int64_t n = (int64_t)c->nx * c->ny * c->nz;
+  DerivJobs J{};
+  for (int iq = 0; iq < 2; ++iq) {
+    const double* r = rho + iq * n;
+    const double* so = sodens + iq * 3 * n;
+    const double* sd = sdens + iq * 3 * n;
+    const double* cu = current + iq * 3 * n;
+    push(J, divj + iq * n, 3, so, 0, 1, 1.0, so + n, 1, 1, 1.0, so + 2 * n, 2, 1, 1.0);
+    push(J, lap + iq * n, 3, r, 0, 2, 1.0, r, 1, 2, 1.0, r, 2, 2, 1.0);
+    for (int a = 0; a < 3; ++a) push(J, grad + (iq * 3 + a) * n, 1, r, a, 1, 1.0);
+    // curl v = (dy vz - dz vy, dz vx - dx vz, dx vy - dy vx)
+    push(J, curls + (iq * 3 + 0) * n, 2, sd + 2 * n, 1, 1, 1.0, sd + 1 * n, 2, 1, -1.0);
+    push(J, curls + (iq * 3 + 1) * n, 2, sd + 0 * n, 2, 1, 1.0, sd + 2 * n, 0, 1, -1.0);
+    push(J, curls + (iq * 3 + 2) * n, 2, sd + 1 * n, 0, 1, 1.0, sd + 0 * n, 1, 1, -1.0);
+    push(J, curlj + (iq * 3 + 0) * n, 2, cu + 2 * n, 1, 1, 1.0, cu + 1 * n, 2, 1, -1.0);
+    push(J, curlj + (iq * 3 + 1) * n, 2, cu + 0 * n, 2, 1, 1.0, cu + 2 * n, 0, 1, -1.0);
+    push(J, curlj + (iq * 3 + 2) * n, 2, cu + 1 * n, 0, 1, 1.0, cu + 0 * n, 1, 1, -1.0);
+  }
+  return run_jobs(c, J, (hipStream_t)stream);
+}
+
+// divaq = div aq, dbmass = grad bmass (meanfield.py:329-344); computed for
+// completeness of the Meanfield state (hpsi01 does not read them).
+extern "C" int hfb_meanfield_derivs(hfb_ctx* c, const double* aq, const double* bmass, double* divaq, double* dbmass,
+                                    void* stream) {
+  if (!c) return HFB_E_NOCTX;
+  if (!aq || !bmass || !divaq || !dbmass) return HFB_E_ARG;
+  const int64_t n = (int64_t)c->nx * c->ny * c->nz;
+  DerivJobs J{};
+  for (int iq = 0; iq < 2; ++iq) {
+    const double* A = aq + iq * 3 * n;
+    push(J, divaq + iq * n, 3, A, 0, 1, 1.0, A + n, 1, 1, 1.0, A + 2 * n, 2, 1, 1.0);
+    for (int a = 0; a < 3; ++a) push(J, dbmass + (iq * 3 + a) * n, 1, bmass + iq * n, a, 1, 1.0);
+  }
+  return run_jobs(c, J, (hipStream_t)stream);
+}
+
+// Potentials of meanfield.py:163-391 from densities + derived fields +
+// wcoul.  The *32 outputs (float copies for the FP32 spinor kernels) may be
+// null.  E receives meanfield.ecorrp in slot HFB_EN_ECORRP.  epart: scratch of
+// hfb_skyrme_nblocks() doubles.
+extern "C" int hfb_skyrme_nblocks(hfb_ctx* c) {
+  if (!c) return HFB_E_NOCTX;
+  const int64_t n = (int64_t)c->nx * c->ny * c->nz;
+  int nb = (int)((n + 255) / 256);
+  return nb > 1024 ? 1024 : nb;
+}
+
+extern "C" int hfb_skyrme(hfb_ctx* c, const hfb_force* f, const double* rho, const double* tau, const double* chi,
+                          const double* current, const double* sdens, const double* divj, const double* lap,
+                          const double* grad, const double* curls, const double* curlj, const double* wcoul,
+                          double* upot, double* bmass, double* wlspot, double* aq, double* spot, double* v_pair,
+                          float* upot32, float* bmass32, float* wlspot32, float* aq32, float* spot32, float* v_pair32,
+                          double* epart, double* E, void* stream) {
+  if (!c) return HFB_E_NOCTX;
+  if (!f || !rho || !tau || !chi || !current || !sdens || !divj || !lap || !grad || !curls || !curlj || !upot ||
+      !bmass || !wlspot || !aq || !spot || !v_pair || !epart || !E)
+    return HFB_E_ARG;
+  if (f->tcoul && !wcoul) return HFB_E_ARG;
+  if (upot32 && (!bmass32 || !wlspot32 || !aq32 || !spot32 || !v_pair32)) return HFB_E_ARG;
+  SkyArgs a{};
+  a.f = *f;
+  a.nvox = (int64_t)c->nx * c->ny * c->nz;
+  a.rho = rho;
+  a.tau = tau;
+  a.chi = chi;
+  a.cur = current;
+  a.sdens = sdens;
+  a.divj = divj;
+  a.lap = lap;
+  a.grad = grad;
+  a.curls = curls;
+  a.curlj = curlj;
+  a.wcoul = wcoul;
+  a.upot = upot;
+  a.bmass = bmass;
+  a.wlspot = wlspot;
+  a.aq = aq;
+  a.spot = spot;
+  a.vpair = v_pair;
+  a.upot32 = upot32;
+  a.bmass32 = bmass32;
+  a.wlspot32 = wlspot32;
+  a.aq32 = aq32;
+  a.spot32 = spot32;
+  a.vpair32 = v_pair32;
+  a.epart = epart;
+  const int nb = hfb_skyrme_nblocks(c);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(skyrme_combine_kernel, dim3(nb), dim3(256), 0, st, a);
+  int rc = launch_status();
+  if (rc != HFB_OK) return rc;
+  hipLaunchKernelGGL(ecorrp_finish_kernel, dim3(1), dim3(256), 0, st, epart, nb, c->dx * c->dy * c->dz, f->ipair, E);
+  return launch_status();
+}
+
+// integ_energy: writes the functional integrals into E.  epart: scratch of
+// hfb_skyrme_nblocks()*20 doubles.  Pairing energies are read from
+// E[HFB_EN_EPAIR..+1].
+extern "C" int hfb_integ_energy(hfb_ctx* c, const hfb_force* f, const double* rho, const double* tau,
+                                const double* current, const double* sdens, const double* divj, const double* lap,
+                                const double* curlj, const double* wcoul, double* epart, double* E, void* stream) {
+  if (!c) return HFB_E_NOCTX;
+  if (!f || !rho || !tau || !current || !sdens || !divj || !lap || !curlj || !epart || !E) return HFB_E_ARG;
+  if (f->tcoul && !wcoul) return HFB_E_ARG;
+  EnArgs a{};
+  a.f = *f;
+  a.nvox = (int64_t)c->nx * c->ny * c->nz;
+  a.rho = rho;
+  a.tau = tau;
+  a.sdens = sdens;
+  a.divj = divj;
+  a.lap = lap;
+  a.curlj = curlj;
+  a.wcoul = wcoul;
+  a.part = epart;
+  const int nb = hfb_skyrme_nblocks(c);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(energy_kernel, dim3(nb), dim3(256), 0, st, a);
+  int rc = launch_status();
+  if (rc != HFB_OK) return rc;
+  hipLaunchKernelGGL(energy_finish_kernel, dim3(1), dim3(256), 0, st, a, nb, c->dx * c->dy * c->dz, current, c->nx,
+                     c->ny, c->nz, E);
+  return launch_status();
+}
+
+extern "C" int hfb_sum_energy(int nstates, const double* wocc, const double* wstates, const double* sp_kinetic,
+                              const double* sp_energy, const double* sp_orbital, const double* sp_spin, double* E,
+                              void* stream) {
+  if (nstates <= 0 || !wocc || !wstates || !sp_kinetic || !sp_energy || !sp_orbital || !sp_spin || !E)
+    return HFB_E_ARG;
+  hipLaunchKernelGGL(sum_energy_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, nstates, wocc, wstates,
+                     sp_kinetic, sp_energy, sp_orbital, sp_spin, E);
+  return launch_status();
+}

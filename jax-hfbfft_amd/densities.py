@@ -1,0 +1,45 @@
+"""Densities (densities.py:6-337): rho, chi, tau, current, spin and
+spin-orbit densities per isospin, accumulated on the device without
+materialising per-state fields."""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import torch
+
+from . import kernels
+
+
+@dataclass
+class Densities:
+    rho: torch.Tensor
+    chi: torch.Tensor
+    tau: torch.Tensor
+    current: torch.Tensor
+    sdens: torch.Tensor
+    sodens: torch.Tensor
+
+
+def init_densities(grids, device=None) -> Densities:
+    dev = torch.device("cuda") if device is None else torch.device(device)
+    s4 = (2, grids.nx, grids.ny, grids.nz)
+    s5 = (2, 3, grids.nx, grids.ny, grids.nz)
+    z = lambda s: torch.zeros(s, dtype=torch.float64, device=dev)
+    return Densities(z(s4), z(s4), z(s4), z(s5), z(s5), z(s5))
+
+
+def add_density(densities, grids, levels, mix=False, addnew=0.2, moment=None, want_sp=False):
+    """Sum of per-state contributions per isospin (densities.py:177-234).
+
+    ``mix`` applies the loop's linear mixing of rho, tau, chi
+    (static.py:1817-1823) in the same pass; ``want_sp`` also fills the
+    per-state kinetic energy, <L>, <S> and parity (inout.py:64-149) from the
+    same transforms."""
+    ctx = grids.ctx
+    n0, n1 = int(levels.npsi[0]), int(levels.npsi[1])
+    cm = moment.cmtot.to(device=levels.psi.device, dtype=torch.float64) if moment is not None else \
+        torch.zeros(3, dtype=torch.float64, device=levels.psi.device)
+    dout, spart = kernels.density(ctx, levels.psi, n0, n1, levels.wocc, levels.wstates, levels.wguv, levels.pairwg,
+                                  cm, True, want_sp)
+    kernels.density_finish(ctx, dout, densities, mix, addnew)
+    return densities, spart

@@ -1,0 +1,95 @@
+"""Energy functional integrals and single-particle sums (energies.py:6-371).
+
+``Energies`` keeps every quantity in one float64 device vector (slot
+layout of include/hfb_hip.h, HFB_EN_*); the reference's attribute names
+read it (one device-to-host copy per read)."""
+from __future__ import annotations
+
+import torch
+
+from . import kernels
+
+EN = kernels.EN
+_SCALARS = {"ehft": "EHFT", "ehf0": "EHF0", "ehf1": "EHF1", "ehf2": "EHF2", "ehf3": "EHF3", "ehfls": "EHFLS",
+            "ehflsodd": "EHFLSODD", "ehfc": "EHFC", "ecorc": "ECORC", "ehfint": "EHFINT", "e3corr": "E3CORR",
+            "e_zpe": "EZPE", "ehfCrho0": "CRHO0", "ehfCrho1": "CRHO1", "ehfCdrho0": "CDRHO0",
+            "ehfCdrho1": "CDRHO1", "ehfCtau0": "CTAU0", "ehfCtau1": "CTAU1", "ehfCdJ0": "CDJ0", "ehfCdJ1": "CDJ1",
+            "ehfCj0": "CJ0", "ehfCj1": "CJ1", "ehf": "EHF", "tke": "TKE"}
+
+
+class Energies:
+    def __init__(self, device=None):
+        dev = torch.device("cuda") if device is None else torch.device(device)
+        self.vec = torch.zeros(kernels.EN_LEN, dtype=torch.float64, device=dev)
+        self.efluct1prev = 0.0
+        self.efluct2prev = 0.0
+        self.ehfprev = 0.0
+        self.e_extern = 0.0
+
+    def __getattr__(self, name):
+        if name in _SCALARS:
+            return float(self.vec[EN[_SCALARS[name]]].item())
+        raise AttributeError(name)
+
+    @property
+    def efluct1(self):
+        return self.vec[EN["EFL1"]:EN["EFL1"] + 1]
+
+    @property
+    def efluct2(self):
+        return self.vec[EN["EFL2"]:EN["EFL2"] + 1]
+
+    @property
+    def efluct1q(self):
+        return self.vec[EN["EFL1Q"]:EN["EFL1Q"] + 2]
+
+    @property
+    def efluct2q(self):
+        return self.vec[EN["EFL2Q"]:EN["EFL2Q"] + 2]
+
+    @property
+    def orbital(self):
+        return self.vec[EN["ORB"]:EN["ORB"] + 3]
+
+    @property
+    def spin(self):
+        return self.vec[EN["SPIN"]:EN["SPIN"] + 3]
+
+    @property
+    def total_angmom(self):
+        return self.vec[EN["JTOT"]:EN["JTOT"] + 3]
+
+    def host(self):
+        """All slots as a host dict (one synchronising copy)."""
+        v = self.vec.cpu().numpy()
+        d = {k: float(v[EN[s]]) for k, s in _SCALARS.items()}
+        d.update(efluct1=float(v[EN["EFL1"]]), efluct2=float(v[EN["EFL2"]]), ecorrp=float(v[EN["ECORRP"]]),
+                 sumflu=float(v[EN["SUMFLU"]]), delesum=float(v[EN["DELESUM"]]),
+                 epair=v[EN["EPAIR"]:EN["EPAIR"] + 2].copy())
+        return d
+
+
+def init_energies(device=None):
+    return Energies(device)
+
+
+def integ_energy(coulomb, densities, energies, forces, grids, levels, params, pairs, derivs=None):
+    """energies.py:90-306.  ``derivs``: DensityDerivs of these densities, if
+    already computed for skyrme.  Pairing energies come from pairs.epair."""
+    from .meanfield import density_derivs
+    if derivs is None:
+        derivs = density_derivs(grids, densities)
+    E = energies.vec
+    E[EN["EPAIR"]:EN["EPAIR"] + 2] = pairs.epair
+    fc = forces.to_c(params, levels.mass_number)
+    kernels.integ_energy(grids.ctx, fc, densities, derivs, coulomb.wcoul, E)
+    return energies
+
+
+def sum_energy(energies, levels, meanfield, pairs):
+    """energies.py:309-371."""
+    E = energies.vec
+    E[EN["EPAIR"]:EN["EPAIR"] + 2] = pairs.epair
+    E[EN["ECORRP"]] = meanfield.ecorrp_dev
+    kernels.sum_energy(levels, E)
+    return energies

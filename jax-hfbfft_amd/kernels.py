@@ -12,6 +12,7 @@ import torch
 
 from . import _lib
 from ._lib import check, lib, need, ptr, stream_handle
+from ctypes import c_void_p
 
 
 def cderv(ctx, axis: int, kind: int, x: torch.Tensor, bmass: torch.Tensor | None = None):
@@ -89,3 +90,164 @@ def selftest_mfma_f64(a: torch.Tensor, b: torch.Tensor):
     d = torch.empty(512, dtype=torch.float64, device=a.device)
     check(lib().hfb_selftest_mfma_f64(ptr(a), ptr(b), ptr(d), stream_handle()), "selftest_mfma")
     return d[:256].reshape(16, 16), d[256:].reshape(64, 4)
+
+
+# ---------------------------------------------------------------------------
+# Gradient step (static.py:93-259)
+# ---------------------------------------------------------------------------
+def grstep_weights(psi_states, npsi0, tbcs, wocc, wstates, wguv, pairwg, v_pair, nvox, out):
+    """Per-state hpsi weights and preconditioner parameters into out['w','wuv','pw','pshift']."""
+    S = int(psi_states)
+    check(lib().hfb_grstep_weights(S, int(npsi0), int(bool(tbcs)), ptr(wocc), ptr(wstates), ptr(wguv), ptr(pairwg),
+                                   ptr(v_pair), int(nvox), ptr(out["w"]), ptr(out["wuv"]), ptr(out["pw"]),
+                                   ptr(out["pshift"]), stream_handle()), "grstep_weights")
+
+
+def precond_step(ctx, psi, ps1, lagrange, hsum, wxyz, use_lagrange, pw, pshift, e0dmp, h2ma, x0dmp,
+                 fluct=None):
+    """psi <- psi - x0dmp * laplace(residual) in place (static.py:93-146).
+
+    If ``fluct`` (S,2) float64 is given it receives (sum|r|^2, sum|lag - h_exp psi|^2)*wxyz."""
+    prec = _lib.prec_of(psi.dtype)
+    S = psi.shape[0]
+    need(psi, _lib.cplx_of(prec), (S, 2, ctx.nx, ctx.ny, ctx.nz), "psi")
+    need(ps1, psi.dtype, psi.shape, "ps1")
+    need(lagrange, psi.dtype, psi.shape, "lagrange")
+    work = ctx.work(("step", prec, S), tuple(psi.shape), psi.dtype)
+    fpart = None
+    nt = lib().hfb_step_ntiles(ctx.handle, prec)
+    if fluct is not None:
+        fpart = ctx.work(("step_part", prec, S), (2 * S, nt, 2), torch.float64)
+    check(lib().hfb_precond_step(ctx.handle, prec, ptr(psi), ptr(ps1), ptr(lagrange), S, ptr(hsum), float(wxyz),
+                                 int(bool(use_lagrange)), ptr(pw), ptr(pshift), float(e0dmp), float(h2ma),
+                                 float(x0dmp), ptr(fpart), ptr(work), stream_handle()), "precond_step")
+    if fluct is not None:
+        check(lib().hfb_reduce_partials(ptr(fpart), S, 2 * nt, 2, float(wxyz), ptr(fluct), stream_handle()),
+              "reduce_partials")
+    return psi
+
+
+def grstep_finish(hsum, wxyz, sp_energy, wocc, wstates, fluct, sp_efluct1, sp_efluct2, acc):
+    S = sp_energy.shape[0]
+    check(lib().hfb_grstep_finish(S, ptr(hsum), float(wxyz), ptr(sp_energy), ptr(wocc), ptr(wstates), ptr(fluct),
+                                  ptr(sp_efluct1), ptr(sp_efluct2), ptr(acc), stream_handle()), "grstep_finish")
+
+
+# ---------------------------------------------------------------------------
+# diagstep (static.py:261-577)
+# ---------------------------------------------------------------------------
+def gram(ctx, A, Bs, scale, out):
+    """out[m] = scale * A^H Bs[m] for a state block A (nst, 2, nx, ny, nz)."""
+    prec = _lib.prec_of(A.dtype)
+    nst = A.shape[0]
+    K = A[0].numel()
+    nm = len(Bs)
+    for b in Bs:
+        need(b, A.dtype, A.shape, "B")
+    need(out, torch.complex128, (nm, nst, nst), "gram_out")
+    nwork = lib().hfb_gram_workspace(nst, K)
+    part = ctx.work(("gram_part", nst), (nwork,), torch.float64)
+    barr = (c_void_p * nm)(*[b.data_ptr() for b in Bs])
+    check(lib().hfb_gram(prec, ptr(A), barr, nm, nst, K, float(scale), ptr(part), ptr(out), stream_handle()), "gram")
+    return out
+
+
+def rotate(A, U1, U2, out1, out2):
+    """out1 = A U1, out2 = A U2 over states (psi_2d @ unitary, static.py:316, 557)."""
+    prec = _lib.prec_of(A.dtype)
+    nst = A.shape[0]
+    K = A[0].numel()
+    check(lib().hfb_rotate(prec, ptr(A), ptr(U1), ptr(U2), ptr(out1), ptr(out2), nst, K, stream_handle()), "rotate")
+
+
+def diag_small(nst0, nst1, nmax, mats, tdiag, ipair, tbcs, construct, levels, static, E, uo, scratch):
+    """Loewdin, optional diagonalisation and HFB matrices for both isospins."""
+    m = (c_void_p * 8)(*[(t.data_ptr() if t is not None else None) for t in mats])
+    u = (c_void_p * 4)(*[t.data_ptr() for t in uo])
+    Ep = E.data_ptr()
+    sz = 8
+    check(lib().hfb_diag_small(int(nst0), int(nst1), int(nmax), m, int(bool(tdiag)), int(bool(ipair)),
+                               int(bool(tbcs)), int(bool(construct)), ptr(levels.wocc), ptr(levels.wstates),
+                               ptr(levels.wguv), ptr(levels.pairwg), ptr(levels.sp_norm), ptr(levels.sp_energy),
+                               ptr(levels.deltaf), ptr(static.hmatrix), ptr(static.gapmatrix), ptr(static.symcond),
+                               ptr(static.lambda_save), c_void_p(Ep + sz * EN["EFL1Q"]),
+                               c_void_p(Ep + sz * EN["EFL2Q"]), c_void_p(Ep + sz * EN["EFL1"]),
+                               c_void_p(Ep + sz * EN["EFL2"]), u, ptr(scratch), stream_handle()), "diag_small")
+
+
+# ---------------------------------------------------------------------------
+# Densities and single-particle properties
+# ---------------------------------------------------------------------------
+def density(ctx, psi, nst0, nst1, wocc, wstates, wguv, pairwg, cmtot, want_dens, want_sp):
+    """Returns (dout, spart) scratch tensors filled by the three axis passes."""
+    prec = _lib.prec_of(psi.dtype)
+    nd = ctypes.c_longlong()
+    nsp = ctypes.c_longlong()
+    check(lib().hfb_density_sizes(ctx.handle, prec, ctypes.byref(nd), ctypes.byref(nsp)), "density_sizes")
+    S = nst0 + nst1
+    dout = ctx.work("dens_out", (nd.value,), torch.float64) if want_dens else None
+    spart = ctx.work(("sp_part", S), (S * nsp.value,), torch.float64) if want_sp else None
+    check(lib().hfb_density(ctx.handle, prec, ptr(psi), int(nst0), int(nst1), ptr(wocc), ptr(wstates), ptr(wguv),
+                            ptr(pairwg), ptr(cmtot), int(bool(want_dens)), int(bool(want_sp)), ptr(dout), ptr(spart),
+                            stream_handle()), "density")
+    return dout, spart
+
+
+def density_finish(ctx, dout, dens, mix, addnew=0.2):
+    check(lib().hfb_density_finish(ctx.handle, ptr(dout), ptr(dens.rho), ptr(dens.chi), ptr(dens.tau),
+                                   ptr(dens.current), ptr(dens.sdens), ptr(dens.sodens), int(bool(mix)),
+                                   float(addnew), stream_handle()), "density_finish")
+
+
+def sp_finish(ctx, spart, nst0, nst1, h2m, levels):
+    check(lib().hfb_sp_finish(ctx.handle, ptr(spart), int(nst0), int(nst1), float(h2m[0]), float(h2m[1]),
+                              ptr(levels.sp_kinetic), ptr(levels.sp_orbital), ptr(levels.sp_spin),
+                              ptr(levels.sp_parity), stream_handle()), "sp_finish")
+
+
+# ---------------------------------------------------------------------------
+# Potentials and energies
+# ---------------------------------------------------------------------------
+EN = {"EHFT": 0, "EHF0": 1, "EHF1": 2, "EHF2": 3, "EHF3": 4, "EHFLS": 5, "EHFLSODD": 6, "EHFC": 7, "ECORC": 8,
+      "EHFINT": 9, "E3CORR": 10, "EZPE": 11, "CRHO0": 12, "CRHO1": 13, "CDRHO0": 14, "CDRHO1": 15, "CTAU0": 16,
+      "CTAU1": 17, "CDJ0": 18, "CDJ1": 19, "CJ0": 20, "CJ1": 21, "EHF": 22, "TKE": 23, "ORB": 24, "SPIN": 27,
+      "JTOT": 30, "EFL1": 33, "EFL2": 34, "EFL1Q": 35, "EFL2Q": 37, "ECORRP": 39, "SUMFLU": 40, "DELESUM": 41,
+      "EPAIR": 42}
+EN_LEN = 48
+
+
+def density_derivs(ctx, dens, der):
+    check(lib().hfb_density_derivs(ctx.handle, ptr(dens.rho), ptr(dens.sdens), ptr(dens.current), ptr(dens.sodens),
+                                   ptr(der["divj"]), ptr(der["lap"]), ptr(der["grad"]), ptr(der["curls"]),
+                                   ptr(der["curlj"]), stream_handle()), "density_derivs")
+
+
+def meanfield_derivs(ctx, mf):
+    check(lib().hfb_meanfield_derivs(ctx.handle, ptr(mf.aq), ptr(mf.bmass), ptr(mf.divaq), ptr(mf.dbmass),
+                                     stream_handle()), "meanfield_derivs")
+
+
+def skyrme(ctx, fc, dens, der, wcoul, mf, E, f32=None):
+    nb = lib().hfb_skyrme_nblocks(ctx.handle)
+    epart = ctx.work("sky_part", (nb,), torch.float64)
+    g = f32 or {}
+    check(lib().hfb_skyrme(ctx.handle, ctypes.byref(fc), ptr(dens.rho), ptr(dens.tau), ptr(dens.chi),
+                           ptr(dens.current), ptr(dens.sdens), ptr(der["divj"]), ptr(der["lap"]), ptr(der["grad"]),
+                           ptr(der["curls"]), ptr(der["curlj"]), ptr(wcoul), ptr(mf.upot), ptr(mf.bmass),
+                           ptr(mf.wlspot), ptr(mf.aq), ptr(mf.spot), ptr(mf.v_pair), ptr(g.get("upot")),
+                           ptr(g.get("bmass")), ptr(g.get("wlspot")), ptr(g.get("aq")), ptr(g.get("spot")),
+                           ptr(g.get("v_pair")), ptr(epart), ptr(E), stream_handle()), "skyrme")
+
+
+def integ_energy(ctx, fc, dens, der, wcoul, E):
+    nb = lib().hfb_skyrme_nblocks(ctx.handle)
+    epart = ctx.work("en_part", (nb * 20,), torch.float64)
+    check(lib().hfb_integ_energy(ctx.handle, ctypes.byref(fc), ptr(dens.rho), ptr(dens.tau), ptr(dens.current),
+                                 ptr(dens.sdens), ptr(der["divj"]), ptr(der["lap"]), ptr(der["curlj"]), ptr(wcoul),
+                                 ptr(epart), ptr(E), stream_handle()), "integ_energy")
+
+
+def sum_energy(levels, E):
+    check(lib().hfb_sum_energy(int(levels.nstmax), ptr(levels.wocc), ptr(levels.wstates), ptr(levels.sp_kinetic),
+                               ptr(levels.sp_energy), ptr(levels.sp_orbital), ptr(levels.sp_spin), ptr(E),
+                               stream_handle()), "sum_energy")

@@ -1,0 +1,327 @@
+"""Static Hartree-Fock(+BCS) solver (static.py of the reference).
+
+Same entry points and argument order as the reference: init_static,
+harmosc, grstep, diagstep, statichf, statichf_with_benchmark.  The state
+objects hold device tensors; every function launches the library's kernels
+on the current HIP stream and returns the (mutated) objects like the
+reference does."""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import kernels
+from .densities import add_density
+from .energies import integ_energy, sum_energy
+from .inout import sp_properties
+from .meanfield import density_derivs, skyrme
+from .pairs import pair
+
+
+@dataclass
+class Static:
+    tdiag: bool = False
+    tlarge: bool = False
+    tvaryx_0: bool = False
+    ttime: bool = False
+    tsort: bool = False
+    maxiter: int = 200
+    iternat: int = 100
+    iternat_start: int = 40
+    iteranneal: int = 0
+    pairenhance: float = 0.0
+    inibcs: int = 30
+    inidiag: int = 30
+    delstepbas: float = 2.0
+    e0bas: float = 10.0
+    outerpot: int = 0
+    radinx: float = 3.1
+    radiny: float = 3.1
+    radinz: float = 3.1
+    serr: float = 1.0e-20
+    x0dmp: float = 0.45
+    e0dmp: float = 100.0
+    x0dmpmin: float = 0.45
+    outertype: str = "N"
+    hmatrix: torch.Tensor = None
+    gapmatrix: torch.Tensor = None
+    symcond: torch.Tensor = None
+    lambda_save: torch.Tensor = None
+    acc: torch.Tensor = None  # device (sumflu, delesum)
+
+    @property
+    def sumflu(self):
+        return float(self.acc[0].item())
+
+    @property
+    def delesum(self):
+        return float(self.acc[1].item())
+
+
+def init_static(forces, levels, **kwargs):
+    """Damping defaults and c.m. scaling of h2m (static.py:53-90)."""
+    nst = max(int(levels.npsi[0]), int(levels.npsi[1]))
+    known = {k: v for k, v in kwargs.items() if k in Static.__dataclass_fields__}
+    st = Static(**known)
+    st.x0dmpmin = st.x0dmp
+    dev = levels.psi.device
+    z = lambda: torch.zeros((2, nst, nst), dtype=torch.complex128, device=dev)
+    st.hmatrix, st.gapmatrix, st.symcond, st.lambda_save = z(), z(), z(), z()
+    st.acc = torch.zeros(2, dtype=torch.float64, device=dev)
+    if forces.zpe == 0:
+        forces.h2m = forces.h2m * ((levels.mass_number - 1.0) / levels.mass_number)
+    return forces, st
+
+
+# ---------------------------------------------------------------------------
+# Initial states (static.py:580-738)
+# ---------------------------------------------------------------------------
+def shell_quantum_numbers(nps):
+    """(i, j, k) of the states of one isospin: shells ka = i+j+k in the
+    k-outer / j / i-inner order, each entry twice (static.py:613-659)."""
+    out = []
+    ka = 0
+    while len(out) < nps:
+        for k in range(ka + 1):
+            for j in range(ka + 1):
+                for i in range(ka + 1):
+                    if ka == i + j + k:
+                        out.extend([(i, j, k)] * 2)
+        ka += 1
+    return out[:nps]
+
+
+def harmosc_host(grids, npsi, nneut, nprot, radin):
+    """Gaussian times monomial initial states, normalised, as a host array
+    (static.py:580-738)."""
+    X = grids.x[:, None, None]
+    Y = grids.y[None, :, None]
+    Z = grids.z[None, None, :]
+    w = grids.wxyz
+    g = np.exp(-((X / radin[0]) ** 2 + (Y / radin[1]) ** 2 + (Z / radin[2]) ** 2))
+    g = g / np.sqrt(w * np.sum(g * g))
+    S = int(npsi[0] + npsi[1])
+    psi = np.zeros((S, 2, grids.nx, grids.ny, grids.nz), dtype=np.complex128)
+    start = 0
+    for iq in range(2):
+        nps = int(npsi[iq])
+        qn = shell_quantum_numbers(nps)
+        for m in range(nps):
+            s = start + m
+            if m == 0:
+                psi[s, 0] = g
+            else:
+                i, j, k = qn[m]
+                poly = (grids.x ** i)[:, None, None] * (grids.y ** j)[None, :, None] * (grids.z ** k)[None, None, :]
+                psi[s, m % 2] = g * poly
+            an = w * np.sum(np.abs(psi[s]) ** 2)
+            if m == 0 or an > 1e-12:
+                psi[s] /= np.sqrt(an)
+        start += nps
+    return psi
+
+
+def harmosc(grids, levels, params, static):
+    """static.py:580-738: initial states and occupations on the device."""
+    psi = harmosc_host(grids, levels.npsi, levels.nneut, levels.nprot, (static.radinx, static.radiny, static.radinz))
+    levels.psi.copy_(torch.from_numpy(psi).to(levels.psi.dtype))
+    levels.wocc.zero_()
+    levels.wocc[:levels.nneut] = 1.0
+    n0 = int(levels.npsi[0])
+    levels.wocc[n0:n0 + levels.nprot] = 1.0
+    levels.wguv.zero_()
+    return levels
+
+
+# ---------------------------------------------------------------------------
+# Gradient step (static.py:93-259)
+# ---------------------------------------------------------------------------
+def _ws(ctx, levels):
+    S = levels.nstmax
+    dt = levels.psi.dtype
+    shp = tuple(levels.psi.shape)
+    f64 = torch.float64
+    return dict(w=ctx.work(("w", S), (S,), f64), wuv=ctx.work(("wuv", S), (S,), f64),
+                pw=ctx.work(("pw", S), (S,), f64), pshift=ctx.work(("pshift", S), (S,), f64),
+                fluct=ctx.work(("fluct", S), (S, 2), f64), pout=ctx.work(("pout", dt, S), shp, dt))
+
+
+def grstep(forces, grids, levels, meanfield, params, static):
+    """Damped gradient step on all states (static.py:215-259)."""
+    ctx = grids.ctx
+    S = levels.nstmax
+    n0 = int(levels.npsi[0])
+    ws = _ws(ctx, levels)
+    kernels.grstep_weights(S, n0, forces.tbcs, levels.wocc, levels.wstates, levels.wguv, levels.pairwg,
+                           meanfield.v_pair, grids.nx * grids.ny * grids.nz, ws)
+    fields = meanfield.fields(levels.psi.dtype)
+    # h psi on the old states: ps1 (weighted) and psi_mf -> hampsi (static.py:152-173)
+    _, _, _, hsum = kernels.hpsi(ctx, levels.psi, n0, *fields, ws["w"], ws["wuv"], want_del=False,
+                                 out={"pout": ws["pout"], "pout_mf": levels.hampsi})
+    want_fl = params.mprint > 0 and params.iteration % params.mprint == 0
+    if static.e0dmp > 0.0:
+        use_lag = (params.iteration > 1) and (not forces.tbcs)
+        kernels.precond_step(ctx, levels.psi, ws["pout"], levels.lagrange, hsum, grids.wxyz, use_lag, ws["pw"],
+                             ws["pshift"], static.e0dmp, forces.h2ma, static.x0dmp,
+                             fluct=ws["fluct"] if want_fl else None)
+    else:  # static.py:181-185
+        spe_new = grids.wxyz * hsum[:, 0]
+        fac = (1.0 + static.x0dmp * (spe_new - levels.sp_energy)).to(levels.psi.real.dtype)
+        levels.psi.mul_(fac.view(-1, 1, 1, 1, 1)).sub_(static.x0dmp * ws["pout"])
+        want_fl = False
+    # h psi on the new states -> hmfpsi, delpsi (static.py:195-208); with no
+    # pairing v_pair == 0 and delpsi stays identically zero
+    out = {"pout_mf": levels.hmfpsi}
+    if forces.ipair != 0:
+        out["pout_del"] = levels.delpsi
+    kernels.hpsi(ctx, levels.psi, n0, *fields, ws["w"], ws["wuv"], want_pout=False, want_del=forces.ipair != 0,
+                 want_partial=False, out=out)
+    kernels.grstep_finish(hsum, grids.wxyz, levels.sp_energy, levels.wocc, levels.wstates,
+                          ws["fluct"] if want_fl else None, levels.sp_efluct1, levels.sp_efluct2, static.acc)
+    return levels, static
+
+
+# ---------------------------------------------------------------------------
+# Orthonormalisation / subspace (static.py:261-577)
+# ---------------------------------------------------------------------------
+def diagstep(energies, forces, grids, levels, static, diagonalize=False, construct=True, npsi_neutron=None,
+             npsi_proton=None):
+    """Loewdin orthonormalisation, optional subspace diagonalisation and the
+    HFB matrices / Lagrange field (static.py:261-461)."""
+    ctx = grids.ctx
+    n0 = int(levels.npsi[0]) if npsi_neutron is None else int(npsi_neutron)
+    n1 = int(levels.npsi[1]) if npsi_proton is None else int(npsi_proton)
+    nmax = static.hmatrix.shape[-1]
+    use_gap = forces.ipair != 0 and construct
+    mats, uo = [], []
+    for iq, (a, b) in enumerate(((0, n0), (n0, n0 + n1))):
+        nst = b - a
+        A = levels.psi[a:b]
+        Bs = [A, levels.hmfpsi[a:b]]
+        if diagonalize:
+            Bs.append(levels.hampsi[a:b])
+        if use_gap:
+            Bs.append(levels.delpsi[a:b])
+        M = ctx.work(("gram_out", iq, len(Bs), nst), (len(Bs), nst, nst), torch.complex128)
+        kernels.gram(ctx, A, Bs, grids.wxyz, M)
+        mats += [M[0], M[1], M[2] if diagonalize else None, M[len(Bs) - 1] if use_gap else None]
+        uo += [ctx.work(("U", iq, nst), (nst, nst), torch.complex128), ctx.work(("M2", iq, nst), (nst, nst),
+                                                                              torch.complex128)]
+    scratch = ctx.work(("diag_scratch", nmax), (4, nmax, nmax), torch.complex128)
+    kernels.diag_small(n0, n1, nmax, mats, diagonalize, forces.ipair != 0, forces.tbcs, construct, levels, static,
+                       energies.vec, uo, scratch)
+    spare = ctx.work(("psi_spare", levels.psi.dtype), tuple(levels.psi.shape), levels.psi.dtype)
+    for iq, (a, b) in enumerate(((0, n0), (n0, n0 + n1))):
+        kernels.rotate(levels.psi[a:b], uo[2 * iq], uo[2 * iq + 1], spare[a:b], levels.lagrange[a:b])
+    ctx._work[("psi_spare", levels.psi.dtype)] = levels.psi
+    levels.psi = spare
+    return energies, levels, static
+
+
+# ---------------------------------------------------------------------------
+# Drivers (static.py:1394-1927)
+# ---------------------------------------------------------------------------
+class LoopState:
+    """Per-run device scratch shared by the loop's calls."""
+
+    def __init__(self, grids, densities):
+        self.derivs = density_derivs(grids, densities)
+
+
+def prelude(coulomb, densities, energies, forces, grids, levels, meanfield, moment, params, static, pairs):
+    """Steps before the loop (static.py:1700-1744)."""
+    params.iteration = 0
+    diagstep(energies, forces, grids, levels, static, False, True)
+    add_density(densities, grids, levels)
+    derivs = density_derivs(grids, densities)
+    skyrme(coulomb, densities, forces, grids, meanfield, params, static, derivs=derivs, E=energies.vec,
+           mass_number=levels.mass_number)
+    grstep(forces, grids, levels, meanfield, params, static)
+    if forces.ipair != 0:
+        pair(levels, meanfield, forces, params, grids, pairs, E=energies.vec, check_status=False)
+    diagstep(energies, forces, grids, levels, static, False, True)
+    sp_properties(forces, grids, levels, moment)
+    integ_energy(coulomb, densities, energies, forces, grids, levels, params, pairs, derivs=derivs)
+    sum_energy(energies, levels, meanfield, pairs)
+    if static.tvaryx_0:
+        static.x0dmp *= 3.0
+    return derivs
+
+
+def iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, moment, params, static, pairs,
+            tbcssav, derivs=None, check=True):
+    """One SCF iteration: the loop body of static.py:1755-1882 without output."""
+    params.iteration = i
+    forces.tbcs = True if i <= static.inibcs else tbcssav
+    static.tdiag = not (i > static.inidiag)
+    grstep(forces, grids, levels, meanfield, params, static)
+    if forces.tbcs:
+        static.tdiag = True
+    diagstep(energies, forces, grids, levels, static, static.tdiag, True)
+    if forces.ipair != 0:
+        pair(levels, meanfield, forces, params, grids, pairs, E=energies.vec, check_status=False)
+    if forces.ipair != 0 and static.iternat > 0 and i > static.iternat_start and not forces.tbcs:
+        raise NotImplementedError("hfb_natural sub-iterations are outside the static HF+BCS path")
+    # density with the 0.2/0.8 mixing of rho, tau, chi and the per-state
+    # properties from the same transforms (static.py:1800-1833)
+    _, spart = add_density(densities, grids, levels, mix=True, addnew=0.2, moment=moment, want_sp=True)
+    derivs = density_derivs(grids, densities, out=derivs)
+    skyrme(coulomb, densities, forces, grids, meanfield, params, static, derivs=derivs, E=energies.vec,
+           mass_number=levels.mass_number)
+    if static.tsort:
+        raise NotImplementedError("sort_states is outside the static HF+BCS path")
+    sp_properties(forces, grids, levels, moment, spart=spart)
+    integ_energy(coulomb, densities, energies, forces, grids, levels, params, pairs, derivs=derivs)
+    sum_energy(energies, levels, meanfield, pairs)
+    converged = False
+    if check or static.tvaryx_0:
+        e = energies.host()
+        converged = e["efluct1"] < static.serr and i > 1
+        if static.tvaryx_0:
+            if ((e["ehf"] < energies.ehfprev and e["efluct1"] < energies.efluct1prev * (1.0 - 1.0e-5))
+                    or e["efluct2"] < energies.efluct2prev * (1.0 - 1.0e-5)):
+                static.x0dmp *= 1.005
+            else:
+                static.x0dmp *= 0.8
+            static.x0dmp = float(np.clip(static.x0dmp, static.x0dmpmin, static.x0dmpmin * 5.0))
+            energies.efluct1prev, energies.efluct2prev, energies.ehfprev = e["efluct1"], e["efluct2"], e["ehf"]
+    return converged, derivs
+
+
+def statichf(coulomb, densities, energies, forces, grids, levels, meanfield, moment, params, static, pairs,
+             output_writer=None, output_interval=5, callback=None):
+    """Prelude + SCF loop (static.py:1394-1601).  ``output_writer`` is not
+    required (the reference needs one, Q1); ``callback(i, state)`` runs after
+    every iteration."""
+    derivs = prelude(coulomb, densities, energies, forces, grids, levels, meanfield, moment, params, static, pairs)
+    tbcssav = forces.tbcs
+    for i in range(1, static.maxiter + 1):
+        conv, derivs = iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, moment, params,
+                               static, pairs, tbcssav, derivs)
+        if callback is not None:
+            callback(i, levels, energies)
+        if conv:
+            break
+    return coulomb, densities, energies, forces, grids, levels, meanfield, moment, params, static
+
+
+def statichf_with_benchmark(coulomb, densities, energies, forces, grids, levels, meanfield, moment, params, static,
+                            pairs, output_writer=None, output_interval=10):
+    """statichf with per-iteration wall times (static.py:1666-1927); returns
+    the reference's tuple plus the list of iteration durations."""
+    derivs = prelude(coulomb, densities, energies, forces, grids, levels, meanfield, moment, params, static, pairs)
+    tbcssav = forces.tbcs
+    times = []
+    for i in range(1, static.maxiter + 1):
+        t0 = time.perf_counter()
+        conv, derivs = iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, moment, params,
+                               static, pairs, tbcssav, derivs)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        if conv:
+            break
+    return (coulomb, densities, energies, forces, grids, levels, meanfield, moment, params, static), times

@@ -1,0 +1,211 @@
+"""Parity of each stage of the SCF iteration on the HIP path against the FP64
+oracle on the same inputs, then of whole iterations.
+
+Tolerances (FP64 path): kernels that reduce or transform agree to 1e-10
+relative; energies after k iterations to 1e-7 MeV (round-off of different
+summation orders, amplified mildly by the self-consistent loop)."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch.device("cuda:0")
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def T(x, dtype=None, dev="cuda"):
+    return torch.tensor(np.ascontiguousarray(x), dtype=dtype, device=dev)
+
+
+def small_case(O, pkg, n=24, nneut=8, nprot=8, npsi=None, ipair=0, tbcs=False, name="SV-bas", b=1.8, beta2=0.0,
+               precision="f64", v0=None):
+    cfg = dict(name=name, n=n, d=1.0, nneut=nneut, nprot=nprot, npsi=npsi, b=b, beta2=beta2, ipair=ipair, tbcs=tbcs,
+               v0=v0)
+    ref = O.setup(**cfg)
+    run = pkg.driver.HFRun.from_config(precision=precision, psi0=ref.levels.psi, **cfg)
+    return ref, run
+
+
+def test_initial_states_match(pkg, O, dev):
+    ref, run = small_case(O, pkg)
+    assert rel(run.levels.psi.cpu().numpy(), ref.levels.psi) == 0.0
+    # harmosc of the package restates the same construction as the oracle's
+    g = ref.grids
+    from jax_hfbfft_amd import static as S
+    psi = S.harmosc_host(g, ref.levels.npsi, 8, 8, (ref.static.radinx, ref.static.radiny, ref.static.radinz))
+    noisy = pkg.driver.add_noise(psi, 1e-3, 0)
+    assert rel(noisy, ref.levels.psi) < 1e-15
+
+
+def test_laplace_matches_oracle(pkg, O, dev):
+    ref, run = small_case(O, pkg)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((2, 2, 24, 24, 24)) + 1j * rng.standard_normal((2, 2, 24, 24, 24))
+    for wg, wguv, vmax in ((1.0, 0.0, 0.0), (0.3, 0.5, 4.0)):
+        out = pkg.levels.laplace(0, 0, run.grids, run.forces, run.levels, wg, wguv, T(x), vmax, 100.0)
+        r = O.laplace(ref.grids, ref.forces.h2ma, wg, wguv, x, vmax, 100.0)
+        assert rel(out.cpu().numpy(), r) < 1e-12
+
+
+def test_diagstep_matches_oracle(pkg, O, dev):
+    """Loewdin + subspace diagonalisation + HFB matrices on the same states."""
+    ref, run = small_case(O, pkg, ipair=5, tbcs=True, npsi=[12, 10])
+    rng = np.random.default_rng(5)
+    shp = ref.levels.psi.shape
+    for nm in ("hmfpsi", "hampsi", "delpsi", "lagrange"):
+        a = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+        setattr(ref.levels, nm, a * 0.1)
+        getattr(run.levels, nm).copy_(T(a * 0.1))
+    ref.levels.wocc = rng.uniform(0.1, 1.0, ref.levels.nstmax)
+    ref.levels.wguv = rng.uniform(0.0, 0.5, ref.levels.nstmax)
+    run.levels.wocc.copy_(T(ref.levels.wocc))
+    run.levels.wguv.copy_(T(ref.levels.wguv))
+    for diag in (False, True):
+        import copy
+        r2 = copy.deepcopy(ref)
+        lv = run.levels
+        saved = {k: getattr(lv, k).clone() for k in ("psi", "lagrange")}
+        O.diagstep(r2.energies, r2.forces, r2.grids, r2.levels, r2.static, diag, True)
+        pkg.static.diagstep(run.energies, run.forces, run.grids, lv, run.static, diag, True)
+        torch.cuda.synchronize()
+        assert rel(lv.sp_norm.cpu().numpy(), r2.levels.sp_norm) < 1e-12
+        assert rel(lv.sp_energy.cpu().numpy(), r2.levels.sp_energy) < 1e-10
+        assert rel(lv.deltaf.cpu().numpy(), r2.levels.deltaf) < 1e-10
+        e = run.energies.host()
+        assert abs(e["efluct1"] - r2.energies.efluct1[0]) < 1e-10 * max(1.0, r2.energies.efluct1[0])
+        assert abs(e["efluct2"] - r2.energies.efluct2[0]) < 1e-10 * max(1.0, r2.energies.efluct2[0])
+        # invariants of the rotated states: the projector onto the span and
+        # the Lagrange field (phases of degenerate eigenvectors are free)
+        for iq in range(2):
+            sl = r2.levels.block(iq)
+            P_ref = np.einsum("ia,ib->ab", r2.levels.psi[sl].reshape(sl.stop - sl.start, -1)[:, :50],
+                              np.conj(r2.levels.psi[sl].reshape(sl.stop - sl.start, -1)[:, :50]))
+            g = lv.psi[sl].cpu().numpy().reshape(sl.stop - sl.start, -1)[:, :50]
+            P_gpu = np.einsum("ia,ib->ab", g, np.conj(g))
+            assert rel(P_gpu, P_ref) < 1e-10
+            L_ref = np.einsum("ia,ib->ab", r2.levels.lagrange[sl].reshape(sl.stop - sl.start, -1)[:, :50],
+                              np.conj(r2.levels.psi[sl].reshape(sl.stop - sl.start, -1)[:, :50]))
+            gl = lv.lagrange[sl].cpu().numpy().reshape(sl.stop - sl.start, -1)[:, :50]
+            assert rel(np.einsum("ia,ib->ab", gl, np.conj(g)), L_ref) < 1e-9
+        if not diag:  # without diagonalisation the Loewdin states are unique
+            assert rel(lv.psi.cpu().numpy(), r2.levels.psi) < 1e-11
+            assert rel(lv.lagrange.cpu().numpy(), r2.levels.lagrange) < 1e-10
+        lv.psi, lv.lagrange = saved["psi"], saved["lagrange"]
+
+
+def test_density_sp_skyrme_energy(pkg, O, dev):
+    ref, run = small_case(O, pkg, ipair=6, npsi=[10, 9])
+    rng = np.random.default_rng(9)
+    S = ref.levels.nstmax
+    ref.levels.wocc = rng.uniform(0.2, 1.0, S)
+    ref.levels.wguv = rng.uniform(0.0, 0.5, S)
+    ref.levels.wstates = rng.uniform(0.8, 1.0, S)
+    ref.levels.pairwg = rng.uniform(0.5, 1.0, S)
+    for nm in ("wocc", "wguv", "wstates", "pairwg"):
+        getattr(run.levels, nm).copy_(T(getattr(ref.levels, nm)))
+    O.add_density(ref.densities, ref.grids, ref.levels)
+    _, spart = pkg.densities.add_density(run.densities, run.grids, run.levels, want_sp=True, moment=run.moment)
+    for nm in ("rho", "chi", "tau", "current", "sdens", "sodens"):
+        assert rel(getattr(run.densities, nm).cpu().numpy(), getattr(ref.densities, nm)) < 1e-11, nm
+    O.sp_properties(ref.forces, ref.grids, ref.levels)
+    pkg.inout.sp_properties(run.forces, run.grids, run.levels, run.moment, spart=spart)
+    for nm in ("sp_kinetic", "sp_orbital", "sp_spin", "sp_parity"):
+        assert rel(getattr(run.levels, nm).cpu().numpy(), getattr(ref.levels, nm)) < 1e-10, nm
+    # skyrme on these densities (DDDI pairing exercises the rearrangement term)
+    O.skyrme(ref.coulomb, ref.densities, ref.forces, ref.grids, ref.meanfield)
+    pkg.meanfield.skyrme(run.coulomb, run.densities, run.forces, run.grids, run.meanfield, run.params, run.static,
+                         E=run.energies.vec, mass_number=run.levels.mass_number)
+    for nm in ("upot", "bmass", "wlspot", "aq", "spot", "v_pair", "divaq", "dbmass"):
+        assert rel(getattr(run.meanfield, nm).cpu().numpy(), getattr(ref.meanfield, nm)) < 1e-10, nm
+    assert abs(run.meanfield.ecorrp - ref.meanfield.ecorrp) < 1e-9 * max(1.0, abs(ref.meanfield.ecorrp))
+    O.integ_energy(ref.coulomb, ref.densities, ref.energies, ref.forces, ref.grids, ref.levels, ref.pairs)
+    pkg.energies.integ_energy(run.coulomb, run.densities, run.energies, run.forces, run.grids, run.levels,
+                              run.params, run.pairs)
+    e = run.energies.host()
+    for nm in ("ehft", "ehf0", "ehf1", "ehf2", "ehf3", "ehfls", "ehflsodd", "ehfc", "ecorc", "ehfint", "e3corr",
+               "ehfCrho0", "ehfCrho1", "ehfCdrho0", "ehfCdrho1", "ehfCtau0", "ehfCtau1", "ehfCdJ0", "ehfCdJ1",
+               "ehfCj0", "ehfCj1"):
+        r = getattr(ref.energies, nm)
+        assert abs(e[nm] - r) <= 1e-10 * max(1.0, abs(r)), (nm, e[nm], r)
+
+
+def test_bcs_matches_oracle(pkg, O, dev):
+    ref, run = small_case(O, pkg, ipair=5, tbcs=True, npsi=[12, 10], nneut=8, nprot=6)
+    rng = np.random.default_rng(11)
+    S = ref.levels.nstmax
+    e = np.sort(rng.uniform(-40, 5, S))
+    ref.levels.sp_energy = e.copy()
+    run.levels.sp_energy.copy_(T(e))
+    for it in (3, 12):
+        ref.params.iteration = it
+        run.params.iteration = it
+        d = rng.uniform(0.5, 2.0, S)
+        ref.levels.deltaf = d.copy()
+        run.levels.deltaf.copy_(T(d))
+        O.pair(ref.levels, ref.meanfield, ref.forces, ref.params, ref.grids, ref.pairs)
+        pkg.pairs.pair(run.levels, run.meanfield, run.forces, run.params, run.grids, run.pairs)
+        assert rel(run.levels.wocc.cpu().numpy(), ref.levels.wocc) < 1e-12
+        assert rel(run.levels.wguv.cpu().numpy(), ref.levels.wguv) < 1e-12
+        assert rel(run.levels.deltaf.cpu().numpy(), ref.levels.deltaf) < 1e-15
+        for nm in ("eferm", "epair", "avdelt", "avdeltv2", "avg"):
+            assert rel(getattr(run.pairs, nm).cpu().numpy(), getattr(ref.pairs, nm)) < 1e-12, nm
+
+
+@pytest.mark.parametrize("case", ["O16", "O16-bcs"])
+def test_iterations_match_oracle(pkg, O, dev, case):
+    """Prelude + 4 SCF iterations, FP64 GPU vs FP64 oracle (both tbcs/tdiag
+    branches of the first phase; pairing for the -bcs case)."""
+    if case == "O16":
+        ref, run = small_case(O, pkg)
+    else:
+        ref, run = small_case(O, pkg, ipair=5, tbcs=True, npsi=[12, 12])
+    O.prelude(ref)
+    run.prelude()
+    e = run.energies_host()
+    assert abs(e["ehf"] - ref.energies.ehf) < 1e-8, ("prelude", e["ehf"], ref.energies.ehf)
+    for i in range(1, 5):
+        O.iteration(ref, i)
+        run.iteration(i)
+        e = run.energies_host()
+        print(case, i, e["ehf"], ref.energies.ehf, e["ehfint"], ref.energies.ehfint)
+        assert abs(e["ehf"] - ref.energies.ehf) < 1e-7, (i, e["ehf"], ref.energies.ehf)
+        assert abs(e["ehfint"] - ref.energies.ehfint) < 1e-7, (i, e["ehfint"], ref.energies.ehfint)
+    assert rel(run.densities.rho.cpu().numpy(), ref.densities.rho) < 1e-8
+
+
+def test_late_phase_iterations(pkg, O, dev):
+    """Iterations past inibcs/inidiag: residual against the Lagrange field,
+    no diagonalisation (static.py:99-104, 1763-1771)."""
+    ref, run = small_case(O, pkg)
+    ref.static.inibcs = ref.static.inidiag = 2
+    run.static.inibcs = run.static.inidiag = 2
+    O.prelude(ref)
+    run.prelude()
+    for i in range(1, 6):
+        O.iteration(ref, i)
+        run.iteration(i)
+    e = run.energies_host()
+    assert abs(e["ehf"] - ref.energies.ehf) < 1e-7
+    assert abs(e["efluct1"] - ref.energies.efluct1[0]) < 1e-7
+
+
+def test_mixed_precision_runs(pkg, O, dev):
+    ref, run = small_case(O, pkg, precision="mixed")
+    O.prelude(ref)
+    run.prelude()
+    for i in range(1, 4):
+        O.iteration(ref, i)
+        run.iteration(i)
+    e = run.energies_host()
+    print("mixed vs f64 oracle after 3 it:", e["ehf"] - ref.energies.ehf)
+    assert abs(e["ehf"] - ref.energies.ehf) < 1e-3
