@@ -1,0 +1,165 @@
+"""Benchmark: static Skyrme-HF iterations per second on one MI355X.
+
+Workload: Ca40 (20N, 20Z) on a 32^3 grid, dx = 1.0 fm, SV-bas couplings, no
+pairing, 20+20 two-spinor states from the oscillator start (b = 2.0 fm) with
+seeded noise 1e-3.  One step = one SCF iteration (the loop body of the
+reference's statichf: gradient step, orthonormalisation/diagonalisation,
+densities with mixing, potentials incl. Coulomb, single-particle properties,
+energies).  Two runs from the same initial states: mixed precision (FP32
+transforms and h psi, FP64 subspace algebra, potentials and energies) and
+all-FP64.  After warmup+steps iterations the mixed run is promoted to FP64
+and both runs do `--refine` more FP64 iterations; the binding energy E_B =
+ehf is then compared.
+
+Prints one JSON line.  `value` is the mixed run's iterations/s.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+METRIC = "HF iter/s at E_B err<1 keV vs FP64, mixed vs FP64 GPU; HBM % of peak"
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--refine", type=int, default=10)
+    ap.add_argument("--workload", default="Ca40")
+    ap.add_argument("--cpu-iters", type=int, default=3, help="oracle iterations timed for the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="launch kernels from Python instead of HIP graphs")
+    return ap.parse_args()
+
+
+def iter_bytes(S, nvox, c, r, P=11):
+    """Algorithmic HBM bytes of one iteration: P spinor-array passes plus
+    density, potential and Coulomb traffic (B_iter = P S 2N c + F)."""
+    F = (2 * 12 + 2 * 16 + 2 * 12) * nvox * r + 2 * 2 * 8 * nvox * 16
+    return P * S * 2 * nvox * c + F
+
+
+def hpsi_bytes(S, nvox, c, r):
+    """Compulsory bytes of one h psi over S states: read psi, write h psi and
+    the weighted result, read 12 potential fields per isospin once."""
+    return S * 2 * nvox * c * 3 + 2 * 12 * nvox * r
+
+
+def cpu_baseline(cfg, iters):
+    sys.path.insert(0, str(ROOT))
+    from oracle import hfb_oracle as O
+    cores = len(os.sched_getaffinity(0))
+    O.WORKERS = cores
+    try:
+        import threadpoolctl
+        threadpoolctl.threadpool_limits(cores)
+    except Exception:
+        pass
+    st = O.setup(**cfg)
+    O.prelude(st)
+    O.iteration(st, 1)  # first iteration excluded like the GPU warmup
+    t0 = time.perf_counter()
+    for i in range(2, 2 + iters):
+        O.iteration(st, i)
+    dt = time.perf_counter() - t0
+    return {"value": iters / dt, "unit": "iter/s", "cores": cores, "kind": "port",
+            "sample": f"{iters} SCF iterations (2..{1 + iters}) of the same workload on the FP64 NumPy/SciPy "
+                      f"oracle, scipy.fft workers={cores}"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    drv = pkg.driver
+    kernels = pkg.kernels
+    cfg = dict(drv.CONFIGS[args.workload])
+    dev = "cuda:0"
+
+    runs = {}
+    total = args.warmup + args.steps + args.refine
+    for prec in ("mixed", "f64"):
+        r = drv.HFRun.from_config(precision=prec, device=dev, **cfg)
+        r.prelude()
+        if not args.eager:
+            # every iteration replays a HIP graph captured ahead of time
+            r.enable_graphs(True)
+            r.precapture(range(1, (args.warmup + args.steps if prec == "mixed" else total) + 1))
+        runs[prec] = r
+    # warmup (untimed) -- iterations 1..W
+    for prec, r in runs.items():
+        r.run(args.warmup, start=1, check_every=0)
+    torch.cuda.synchronize()
+
+    timing = {}
+    for prec, r in runs.items():
+        kernels.TIMERS = {} if prec == "mixed" else None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r.run(args.steps, start=args.warmup + 1, check_every=0)
+        torch.cuda.synchronize()
+        timing[prec] = time.perf_counter() - t0
+        if prec == "mixed":
+            ev = kernels.TIMERS.get("hpsi", [])
+            hpsi_ms = [a.elapsed_time(b) for a, b in ev]
+        kernels.TIMERS = None
+
+    # FP64 refinement of the mixed run; the FP64 run continues in step
+    start = args.warmup + args.steps + 1
+    runs["mixed"].promote()
+    if not args.eager:
+        runs["mixed"].precapture(range(start, start + args.refine))
+    for prec, r in runs.items():
+        r.run(args.refine, start=start, check_every=0)
+    e_mix = runs["mixed"].energies_host()
+    e_64 = runs["f64"].energies_host()
+    err_kev = abs(e_mix["ehf"] - e_64["ehf"]) * 1e3
+
+    lv = runs["f64"].levels
+    S = lv.nstmax
+    nvox = runs["f64"].grids.nx * runs["f64"].grids.ny * runs["f64"].grids.nz
+    mixed_ips = args.steps / timing["mixed"]
+    f64_ips = args.steps / timing["f64"]
+    hp = sorted(hpsi_ms)
+    hp_avg = sum(hp) / max(1, len(hp))
+    ach = hpsi_bytes(S, nvox, 8, 4) / (hp_avg * 1e-3) / 1e9 if hp else None
+    roof = {"bound": "hbm", "kernel": "hpsi (hpsi_axis_kernel x3, FP32)", "achieved": ach, "peak": 8000.0,
+            "unit": "GB/s", "frac": (ach / 8000.0) if ach else None, "traffic": None,
+            "avg_launch_ms": hp_avg, "launches": len(hp),
+            "algorithmic_bytes": hpsi_bytes(S, nvox, 8, 4)}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, args.cpu_iters)
+    out = {
+        "metric": METRIC, "value": mixed_ips, "unit": "iter/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 / mixed_ips, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{args.workload} {cfg['n']}^3 dx={cfg['d']} {cfg['name']} "
+                               f"{cfg['nneut']}N+{cfg['nprot']}Z states, HO start b={cfg['b']} + noise 1e-3",
+                   "states": S, "grid": cfg["n"], "refine_iters": args.refine,
+                   "launch": "eager" if args.eager else "hip-graph replay"},
+        "fp64_iter_s": f64_ips, "speedup_vs_fp64": mixed_ips / f64_ips,
+        "eb_err_kev": err_kev, "ehf_mixed": e_mix["ehf"], "ehf_f64": e_64["ehf"],
+        "hbm_frac_iter_mixed": iter_bytes(S, nvox, 8, 8) * mixed_ips / HBM_PEAK,
+        "hbm_frac_iter_f64": iter_bytes(S, nvox, 16, 8) * f64_ips / HBM_PEAK,
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -236,7 +236,7 @@ int launch_dens(const hfb_ctx* c, DensArgs a, hipStream_t st) {
     constexpr int NN = N_;
     const size_t lds = sizeof(cx<T>) * (size_t)NSUB * 3 * NN * 2 * P;
     auto kern = dens_axis_kernel<T, NN, AXIS, P, NSUB>;
-    if (lds > 32 * 1024) hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (lds > 32 * 1024) allow_full_lds((const void*)kern);
     hipLaunchKernelGGL(kern, dim3(nt, 2), dim3(NSUB * 8 * P), lds, st, a, twiddles<T>(c, NN));
   })
   return launch_status();

@@ -37,108 +37,144 @@ __device__ __forceinline__ cx<double> ld64(const cx<T>* p) {
 struct GramArgs {
   const void* A;
   const void* B[4];
-  int nm;
+  int nm;       // number of right-hand blocks
+  int bsame;    // bitmask: B[m] == A (no second load)
   int nst;
   int64_t K;
-  int64_t kc;   // K per chunk, multiple of 16
+  int64_t kc;   // K per chunk, multiple of KS
+  int nchunk;
   int np;       // nst padded to 16
-  double* part; // [chunk][m][np][np] complex
+  double* part; // [m][np][np][chunk] complex
 };
 
-template <typename T, int NM>
-__global__ void __launch_bounds__(64) gram_kernel(GramArgs g) {
+constexpr int KS = 16;        // complex elements of K staged per step
+constexpr int LDR = KS + 1;   // padded LDS row stride (rows sit on different banks)
+constexpr int GRAM_NT = 256;  // 4 wavefronts
+
+// One workgroup per K-chunk computes every 16x16 tile of every C_m.  Each
+// K-step stages A and the distinct B_m rows (KS contiguous elements per row,
+// converted to double) in LDS, so every input element is read from HBM once.
+// The four wavefronts split the output tiles; MFMA operands come from LDS.
+template <typename T, int NM, int MAXT>
+__global__ void __launch_bounds__(GRAM_NT) gram_kernel(GramArgs g) {
+  extern __shared__ double glds[];
+  const int np = g.np;
+  const int ntile1 = np / 16;
+  const int ntl = ntile1 * ntile1;
+  cx<double>* sA = (cx<double>*)glds;                 // [np][LDR]
+  cx<double>* sB = sA + (size_t)np * LDR;             // [NM][np][LDR]
   const int chunk = blockIdx.x;
-  const int ntj = g.np / 16;
-  const int ti = blockIdx.y / ntj, tj = blockIdx.y % ntj;
-  const int l = threadIdx.x;
+  const int64_t k0 = (int64_t)chunk * g.kc;
+  const int64_t k1 = min(g.K, k0 + g.kc);
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int grp = l >> 4, c = l & 15;
-  const int i = ti * 16 + c;
-  const int j = tj * 16 + c;
-  const int64_t q = g.kc / 4;
-  const int64_t kb = (int64_t)chunk * g.kc + grp * q;
-  const int64_t kend = g.K;
-  const cx<T>* ar = (const cx<T>*)g.A + (int64_t)(i < g.nst ? i : 0) * g.K;
-  const cx<T>* br[NM];
+  // tiles of this wave: t = wave + 4*u over NM*ntl tiles
+  const int ntot = NM * ntl;
+  v4d cre[MAXT], cim[MAXT];
 #pragma unroll
-  for (int m = 0; m < NM; ++m) br[m] = (const cx<T>*)g.B[m] + (int64_t)(j < g.nst ? j : 0) * g.K;
-  const bool iok = i < g.nst, jok = j < g.nst;
-  v4d cre[NM], cim[NM];
-#pragma unroll
-  for (int m = 0; m < NM; ++m) {
-    cre[m] = v4d{0.0, 0.0, 0.0, 0.0};
-    cim[m] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int u = 0; u < MAXT; ++u) {
+    cre[u] = v4d{0.0, 0.0, 0.0, 0.0};
+    cim[u] = v4d{0.0, 0.0, 0.0, 0.0};
   }
-  for (int64_t s = 0; s < q; s += 4) {
-    cx<double> a[4], b[NM][4];
+  for (int64_t kb = k0; kb < k1; kb += KS) {
+    // stage: rows of A and of the distinct B_m
+    for (int e = threadIdx.x; e < np * KS; e += GRAM_NT) {
+      const int row = e / KS, kk = e % KS;
+      const int64_t k = kb + kk;
+      const bool ok = row < g.nst && k < k1;
+      const int le = row * LDR + kk;
+      sA[le] = ok ? ld64((const cx<T>*)g.A + (int64_t)row * g.K + k) : cx<double>{0.0, 0.0};
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int64_t k = kb + s + u;
-      const bool kin = k < kend && (s + u) < q;
-      a[u] = (iok && kin) ? ld64(ar + k) : cx<double>{0.0, 0.0};
-#pragma unroll
-      for (int m = 0; m < NM; ++m) b[m][u] = (jok && kin) ? ld64(br[m] + k) : cx<double>{0.0, 0.0};
+      for (int m = 0; m < NM; ++m)
+        if (!((g.bsame >> m) & 1))
+          sB[(size_t)m * np * LDR + le] = ok ? ld64((const cx<T>*)g.B[m] + (int64_t)row * g.K + k) : cx<double>{0.0, 0.0};
     }
+    __syncthreads();
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < MAXT; ++u) {
+      const int t = wave + 4 * u;
+      if (t < ntot) {
+        const int m = t / ntl, tt = t % ntl;
+        const int ti = tt / ntile1, tj = tt % ntile1;
+        const cx<double>* bb = ((g.bsame >> m) & 1) ? sA : sB + (size_t)m * np * LDR;
 #pragma unroll
-      for (int m = 0; m < NM; ++m) {
-        // conj(a) b = (ar br + ai bi) + i (ar bi - ai br)
-        cre[m] = mfma(a[u].x, b[m][u].x, cre[m]);
-        cre[m] = mfma(a[u].y, b[m][u].y, cre[m]);
-        cim[m] = mfma(a[u].x, b[m][u].y, cim[m]);
-        cim[m] = mfma(-a[u].y, b[m][u].x, cim[m]);
+        for (int s4 = 0; s4 < KS; s4 += 4) {
+          const cx<double> a = sA[(ti * 16 + c) * LDR + s4 + grp];
+          const cx<double> b = bb[(tj * 16 + c) * LDR + s4 + grp];
+          // conj(a) b = (ar br + ai bi) + i (ar bi - ai br)
+          cre[u] = mfma(a.x, b.x, cre[u]);
+          cre[u] = mfma(a.y, b.y, cre[u]);
+          cim[u] = mfma(a.x, b.y, cim[u]);
+          cim[u] = mfma(-a.y, b.x, cim[u]);
+        }
       }
     }
+    __syncthreads();
   }
-  const int np = g.np;
 #pragma unroll
-  for (int m = 0; m < NM; ++m) {
+  for (int u = 0; u < MAXT; ++u) {
+    const int t = wave + 4 * u;
+    if (t < ntot) {
+      const int m = t / ntl, tt = t % ntl;
+      const int ti = tt / ntile1, tj = tt % ntile1;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = ti * 16 + grp + 4 * r;
-      const int col = tj * 16 + c;
-      double* dst = g.part + 2 * ((((int64_t)chunk * NM + m) * np + row) * np + col);
-      dst[0] = cre[m][r];
-      dst[1] = cim[m][r];
+      for (int r = 0; r < 4; ++r) {
+        const int row = ti * 16 + grp + 4 * r;
+        const int col = tj * 16 + c;
+        double* dst = g.part + 2 * ((((int64_t)m * np + row) * np + col) * g.nchunk + chunk);
+        dst[0] = cre[u][r];
+        dst[1] = cim[u][r];
+      }
     }
   }
 }
 
-__global__ void gram_reduce_kernel(const double* __restrict__ part, int nchunk, int nm, int np, int nst,
-                                   double scale, double* __restrict__ out) {
-  // out: [m][nst][nst] complex
+// One wavefront per output element: lanes stride over the chunks.
+__global__ void __launch_bounds__(256) gram_reduce_kernel(const double* __restrict__ part, int nchunk, int nm, int np,
+                                                          int nst, double scale, double* __restrict__ out) {
   const int64_t total = (int64_t)nm * nst * nst;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wid = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t e = wid; e < total; e += nw) {
     const int m = e / ((int64_t)nst * nst);
     const int ij = e % ((int64_t)nst * nst);
     const int i = ij / nst, j = ij % nst;
+    const double* p = part + 2 * (((int64_t)m * np + i) * np + j) * nchunk;
     double re = 0.0, im = 0.0;
-    for (int ch = 0; ch < nchunk; ++ch) {
-      const double* p = part + 2 * ((((int64_t)ch * nm + m) * np + i) * np + j);
-      re += p[0];
-      im += p[1];
+    for (int ch = lane; ch < nchunk; ch += 64) {
+      re += p[2 * ch];
+      im += p[2 * ch + 1];
     }
-    out[2 * e] = re * scale;
-    out[2 * e + 1] = im * scale;
+    for (int o = 32; o > 0; o >>= 1) {
+      re += __shfl_down(re, o, 64);
+      im += __shfl_down(im, o, 64);
+    }
+    if (lane == 0) {
+      out[2 * e] = re * scale;
+      out[2 * e + 1] = im * scale;
+    }
   }
 }
 
 int gram_nchunk(int np, int64_t K) {
-  const int ntile = (np / 16) * (np / 16);
-  int nchunk = (1024 + ntile - 1) / ntile;
-  const int64_t maxch = (K + 63) / 64;
-  if (nchunk > maxch) nchunk = (int)maxch;
-  return nchunk < 1 ? 1 : nchunk;
+  (void)np;
+  int64_t nchunk = (K + 511) / 512;  // ~512 elements of K per workgroup
+  if (nchunk > 256) nchunk = 256;
+  return nchunk < 1 ? 1 : (int)nchunk;
 }
 
 // ---------------------------------------------------------------------------
 // Rotation: out1_j = sum_i psi_i U1[i][j], out2_j = sum_i psi_i U2[i][j]
 // ---------------------------------------------------------------------------
-template <typename T, int NJT>
-__global__ void __launch_bounds__(256) rotate_kernel(const cx<T>* __restrict__ psi, const double* __restrict__ U1,
-                                                     const double* __restrict__ U2, cx<T>* __restrict__ out1,
-                                                     cx<T>* __restrict__ out2, int nst, int64_t K) {
-  // U (nst x nst complex, row-major [i][j]) staged in LDS as padded (np x np)
+template <typename T, int NJT, int NI>
+__global__ void __launch_bounds__(256) rotate_kernel(const cx<T>* psi, const double* __restrict__ U1,
+                                                     const double* __restrict__ U2, cx<T>* out1, cx<T>* out2, int nst,
+                                                     int64_t K) {
+  // U (nst x nst complex, row-major [i][j]) staged in LDS as padded (np x np).
+  // Each wavefront owns whole 16-point blocks of K: it reads psi_i for all i
+  // there before writing out_j, so out1 may alias psi when one workgroup
+  // column covers all j (gridDim.y == 1).
   extern __shared__ double lds[];
   const int np = (nst + 15) & ~15;
   double* u1 = lds;                       // 2*np*np
@@ -159,9 +195,17 @@ __global__ void __launch_bounds__(256) rotate_kernel(const cx<T>* __restrict__ p
   const int grp = l >> 4, c = l & 15;
   const int jg0 = blockIdx.y * NJT;          // first j-tile of this block
   const int njt = min(NJT, np / 16 - jg0);
+  const int nis = np / 4;                     // MFMA steps over i
   const int64_t nblk = (K + 15) / 16;
   for (int64_t blk = (int64_t)blockIdx.x * 4 + wave; blk < nblk; blk += (int64_t)gridDim.x * 4) {
     const int64_t k = blk * 16 + c;
+    // B operand psi^T[i][k] for every i-step, loaded up front
+    cx<double> bv[NI];
+#pragma unroll
+    for (int s = 0; s < NI; ++s) {
+      const int i = 4 * s + grp;
+      bv[s] = (s < nis && i < nst && k < K) ? ld64(psi + (int64_t)i * K + k) : cx<double>{0.0, 0.0};
+    }
     v4d d1r[NJT], d1i[NJT], d2r[NJT], d2i[NJT];
 #pragma unroll
     for (int t = 0; t < NJT; ++t) {
@@ -170,26 +214,29 @@ __global__ void __launch_bounds__(256) rotate_kernel(const cx<T>* __restrict__ p
       d2r[t] = d1r[t];
       d2i[t] = d1r[t];
     }
-    for (int i0 = 0; i0 < np; i0 += 4) {
-      const int i = i0 + grp;  // B operand: psi^T[i][k]
-      cx<double> b = (i < nst && k < K) ? ld64(psi + (int64_t)i * K + k) : cx<double>{0.0, 0.0};
 #pragma unroll
-      for (int t = 0; t < NJT; ++t) {
-        if (t < njt) {
-          const int j = (jg0 + t) * 16 + c;  // A operand: U^T[j][i] = U[i][j]
-          const double* pa = u1 + 2 * ((int64_t)i * np + j);
-          const double ar = pa[0], ai = pa[1];
-          d1r[t] = mfma(ar, b.x, d1r[t]);
-          d1r[t] = mfma(-ai, b.y, d1r[t]);
-          d1i[t] = mfma(ar, b.y, d1i[t]);
-          d1i[t] = mfma(ai, b.x, d1i[t]);
-          if (U2) {
-            const double* pb = u2 + 2 * ((int64_t)i * np + j);
-            const double br = pb[0], bi = pb[1];
-            d2r[t] = mfma(br, b.x, d2r[t]);
-            d2r[t] = mfma(-bi, b.y, d2r[t]);
-            d2i[t] = mfma(br, b.y, d2i[t]);
-            d2i[t] = mfma(bi, b.x, d2i[t]);
+    for (int s = 0; s < NI; ++s) {
+      if (s < nis) {
+        const int i = 4 * s + grp;
+        const cx<double> b = bv[s];
+#pragma unroll
+        for (int t = 0; t < NJT; ++t) {
+          if (t < njt) {
+            const int j = (jg0 + t) * 16 + c;  // A operand: U^T[j][i] = U[i][j]
+            const double* pa = u1 + 2 * ((int64_t)i * np + j);
+            const double ar = pa[0], ai = pa[1];
+            d1r[t] = mfma(ar, b.x, d1r[t]);
+            d1r[t] = mfma(-ai, b.y, d1r[t]);
+            d1i[t] = mfma(ar, b.y, d1i[t]);
+            d1i[t] = mfma(ai, b.x, d1i[t]);
+            if (U2) {
+              const double* pb = u2 + 2 * ((int64_t)i * np + j);
+              const double br = pb[0], bi = pb[1];
+              d2r[t] = mfma(br, b.x, d2r[t]);
+              d2r[t] = mfma(-bi, b.y, d2r[t]);
+              d2i[t] = mfma(br, b.y, d2i[t]);
+              d2i[t] = mfma(bi, b.x, d2i[t]);
+            }
           }
         }
       }
@@ -219,46 +266,69 @@ extern "C" int hfb_gram_workspace(int nst, long long K) {
   return gram_nchunk(np, K) * 4 * np * np * 2;  // doubles for up to four matrices
 }
 
+template <typename T, int NM>
+int launch_gram(const GramArgs& g, hipStream_t st) {
+  const int ntile = (g.np / 16) * (g.np / 16) * NM;
+  const size_t lds = sizeof(cx<double>) * (size_t)g.np * LDR * (1 + NM);
+  const dim3 grid(g.nchunk);
+#define HFB_GR(MT)                                                                                               \
+  {                                                                                                              \
+    auto kern = gram_kernel<T, NM, MT>;                                                                          \
+    if (lds > 32 * 1024) allow_full_lds((const void*)kern); \
+    hipLaunchKernelGGL(kern, grid, dim3(GRAM_NT), lds, st, g);                                                  \
+  }
+  const int per_wave = (ntile + 3) / 4;
+  if (per_wave <= 1) HFB_GR(1)
+  else if (per_wave <= 2) HFB_GR(2)
+  else if (per_wave <= 4) HFB_GR(4)
+  else if (per_wave <= 8) HFB_GR(8)
+  else if (per_wave <= 16) HFB_GR(16)
+  else return HFB_E_SIZE;
+#undef HFB_GR
+  return launch_status();
+}
+
 extern "C" int hfb_gram(int prec, const void* A, const void* const* B, int nm, int nst, long long K, double scale,
                         double* part, double* out, void* stream) {
   if (!A || !B || !part || !out || nm < 1 || nm > 4 || nst <= 0 || K <= 0) return HFB_E_ARG;
-  for (int m = 0; m < nm; ++m)
-    if (!B[m]) return HFB_E_ARG;
   GramArgs g{};
   g.A = A;
-  for (int m = 0; m < nm; ++m) g.B[m] = B[m];
+  g.bsame = 0;
+  for (int m = 0; m < nm; ++m) {
+    if (!B[m]) return HFB_E_ARG;
+    g.B[m] = B[m];
+    if (B[m] == A) g.bsame |= 1 << m;
+  }
   g.nm = nm;
   g.nst = nst;
   g.K = K;
   g.np = (nst + 15) & ~15;
-  const int nchunk = gram_nchunk(g.np, K);
-  int64_t kc = (K + nchunk - 1) / nchunk;
-  kc = (kc + 15) & ~(int64_t)15;
+  g.nchunk = gram_nchunk(g.np, K);
+  int64_t kc = (K + g.nchunk - 1) / g.nchunk;
+  kc = (kc + KS - 1) / KS * KS;
   g.kc = kc;
   g.part = part;
   hipStream_t st = (hipStream_t)stream;
-  const dim3 grid(nchunk, (g.np / 16) * (g.np / 16));
-#define HFB_GR(TT, NMV) hipLaunchKernelGGL((gram_kernel<TT, NMV>), grid, dim3(64), 0, st, g)
+  int rc;
+#define HFB_GT(TT)                                   \
+  if (nm == 1) rc = launch_gram<TT, 1>(g, st);       \
+  else if (nm == 2) rc = launch_gram<TT, 2>(g, st);  \
+  else if (nm == 3) rc = launch_gram<TT, 3>(g, st);  \
+  else rc = launch_gram<TT, 4>(g, st);
   if (prec == HFB_F64) {
-    if (nm == 1) HFB_GR(double, 1);
-    else if (nm == 2) HFB_GR(double, 2);
-    else if (nm == 3) HFB_GR(double, 3);
-    else HFB_GR(double, 4);
+    HFB_GT(double)
   } else if (prec == HFB_F32) {
-    if (nm == 1) HFB_GR(float, 1);
-    else if (nm == 2) HFB_GR(float, 2);
-    else if (nm == 3) HFB_GR(float, 3);
-    else HFB_GR(float, 4);
+    HFB_GT(float)
   } else {
     return HFB_E_ARG;
   }
-#undef HFB_GR
-  int rc = launch_status();
+#undef HFB_GT
   if (rc != HFB_OK) return rc;
   const int64_t tot = (int64_t)nm * nst * nst;
-  const int nb = (int)((tot + 255) / 256);
-  hipLaunchKernelGGL(gram_reduce_kernel, dim3(nb < 1 ? 1 : nb), dim3(256), 0, st, part, nchunk, nm, g.np, nst, scale,
-                     out);
+  int nb = (int)((tot * 64 + 255) / 256);
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(gram_reduce_kernel, dim3(nb < 1 ? 1 : nb), dim3(256), 0, st, part, g.nchunk, nm, g.np, nst,
+                     scale, out);
   return launch_status();
 }
 
@@ -269,24 +339,33 @@ extern "C" int hfb_rotate(int prec, const void* psi, const double* U1, const dou
   const int np = (nst + 15) & ~15;
   constexpr int NJT = 4;
   const int jgroups = (np / 16 + NJT - 1) / NJT;
+  if (jgroups > 1 && out1 == psi) return HFB_E_ARG;  // in place only when one column of blocks covers all j
   const size_t lds = sizeof(double) * 2 * 2 * (size_t)np * np;
   if (lds > 160 * 1024) return HFB_E_SIZE;
   const int64_t nblk = (K + 15) / 16;
-  int gx = (int)((nblk + 4 * 8 - 1) / (4 * 8));  // ~8 point blocks per wave
+  int gx = (int)((nblk + 4 * 2 - 1) / (4 * 2));  // two 16-point blocks per wavefront
   if (gx < 1) gx = 1;
   hipStream_t st = (hipStream_t)stream;
+#define HFB_RT(TT, NI)                                                                                              \
+  {                                                                                                                 \
+    auto kern = rotate_kernel<TT, NJT, NI>;                                                                         \
+    if (lds > 32 * 1024) allow_full_lds((const void*)kern); \
+    hipLaunchKernelGGL(kern, dim3(gx, jgroups), dim3(256), lds, st, (const cx<TT>*)psi, U1, U2, (cx<TT>*)out1,        \
+                       (cx<TT>*)out2, nst, (int64_t)K);                                                             \
+  }
+  const int nis = np / 4;
+  if (nis > 32) return HFB_E_SIZE;
   if (prec == HFB_F64) {
-    if (lds > 64 * 1024)
-      hipFuncSetAttribute((const void*)rotate_kernel<double, NJT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((rotate_kernel<double, NJT>), dim3(gx, jgroups), dim3(256), lds, st, (const cx<double>*)psi, U1,
-                       U2, (cx<double>*)out1, (cx<double>*)out2, nst, (int64_t)K);
+    if (nis <= 8) HFB_RT(double, 8)
+    else if (nis <= 16) HFB_RT(double, 16)
+    else HFB_RT(double, 32)
   } else if (prec == HFB_F32) {
-    if (lds > 64 * 1024)
-      hipFuncSetAttribute((const void*)rotate_kernel<float, NJT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((rotate_kernel<float, NJT>), dim3(gx, jgroups), dim3(256), lds, st, (const cx<float>*)psi, U1,
-                       U2, (cx<float>*)out1, (cx<float>*)out2, nst, (int64_t)K);
+    if (nis <= 8) HFB_RT(float, 8)
+    else if (nis <= 16) HFB_RT(float, 16)
+    else HFB_RT(float, 32)
   } else {
     return HFB_E_ARG;
   }
+#undef HFB_RT
   return launch_status();
 }

@@ -27,8 +27,9 @@ struct DiagIso {
   const cx<double>* Gold;  // null unless pairing
   cx<double>* U;
   cx<double>* M2;
-  cx<double>* T1;  // scratch nst x nst
+  cx<double>* T1;  // scratch nst x nst (used when the work matrices do not fit in LDS)
   cx<double>* T2;
+  cx<double>* T3;
   cx<double>* Hm;  // static arrays of this isospin, leading dimension nmax
   cx<double>* Gm;
   cx<double>* Sym;
@@ -207,69 +208,173 @@ __device__ void mm(cx<double>* C, int ldc, const cx<double>* A, int lda, int opA
   __syncthreads();
 }
 
+constexpr int NLDS_ALL = 40;  // up to this many states all five work matrices sit in LDS
+
+// Inverse square root of the Hermitian positive matrix in Y (scaled copy of
+// S) by the coupled Newton-Schulz iteration Y <- Y T, Z <- T Z,
+// T = (3I - Z Y)/2, which converges to Z = Y0^{-1/2}.  Returns false when it
+// has not converged (nearly singular S); the caller then takes the
+// eigen-decomposition with the reference's clamp.
+__device__ bool newton_schulz(cx<double>*& Y, cx<double>*& Z, cx<double>* P, cx<double>*& Y2, cx<double>*& Z2, int n,
+                              double* red) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < n * n; e += blockDim.x) Z[e] = cx<double>{(e / n == e % n) ? 1.0 : 0.0, 0.0};
+  __syncthreads();
+  __shared__ double resid;
+  int remaining = -1;  // updates left once the residual is below 1e-6
+  for (int it = 0; it < 80; ++it) {
+    mm(P, n, Z, n, 0, Y, n, n);
+    double v = 0.0;
+    for (int e = tid; e < n * n; e += blockDim.x) {
+      const cx<double> d = csub(P[e], cx<double>{(e / n == e % n) ? 1.0 : 0.0, 0.0});
+      v = fmax(v, fabs(d.x) + fabs(d.y));
+    }
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+      double m = 0.0;
+      for (int w = 0; w < (int)blockDim.x / 64; ++w) m = fmax(m, red[w]);
+      resid = m;
+    }
+    __syncthreads();
+    // quadratic convergence: two updates after 1e-6 reach round-off
+    if (remaining < 0 && resid < 1e-6) remaining = 2;
+    for (int e = tid; e < n * n; e += blockDim.x) {  // T = (3I - P)/2
+      const double id = (e / n == e % n) ? 1.5 : 0.0;
+      P[e] = cx<double>{id - 0.5 * P[e].x, -0.5 * P[e].y};
+    }
+    __syncthreads();
+    mm(Y2, n, Y, n, 0, P, n, n);
+    mm(Z2, n, P, n, 0, Z, n, n);
+    cx<double>* t = Y;
+    Y = Y2;
+    Y2 = t;
+    t = Z;
+    Z = Z2;
+    Z2 = t;
+    if (remaining > 0 && --remaining == 0) return true;
+  }
+  return false;
+}
+
 __global__ void __launch_bounds__(256) diag_kernel(DiagArgs a) {
   extern __shared__ double lds_raw[];
   __shared__ JacobiShared js;
+  __shared__ double red[8];
   const int iq = blockIdx.x;
   const DiagIso d = a.iso[iq];
   const int n = d.nst;
   if (n <= 0) return;
   const int tid = threadIdx.x;
   const int nm = a.nmax;
-  cx<double>* A = (cx<double>*)lds_raw;
-  cx<double>* V = A + n * n;
+  cx<double>* Bf[5];
+  Bf[0] = (cx<double>*)lds_raw;
+  Bf[1] = Bf[0] + n * n;
+  if (n <= NLDS_ALL) {
+    Bf[2] = Bf[1] + n * n;
+    Bf[3] = Bf[2] + n * n;
+    Bf[4] = Bf[3] + n * n;
+  } else {
+    Bf[2] = d.T1;
+    Bf[3] = d.T2;
+    Bf[4] = d.T3;
+  }
 
-  // Loewdin (static.py:463-482)
+  // Loewdin S^{-1/2} (static.py:463-482)
   for (int i = tid; i < n; i += blockDim.x) a.sp_norm[d.start + i] = d.S[i * n + i].x;
-  load_lower_herm(A, d.S, n);
-  jacobi_eigh(A, V, n, js, false, d.T1);
-  __shared__ double fac[NMAXJ];
-  if (tid == 0) {
-    double wmax = js.ev[0];
-    for (int i = 1; i < n; ++i) wmax = fmax(wmax, js.ev[i]);
-    const double thr = fmax(1e-12, 2.220446049250313e-16 * wmax * n);
-    for (int i = 0; i < n; ++i) fac[i] = 1.0 / sqrt(fmax(js.ev[i], thr));
+  load_lower_herm(Bf[0], d.S, n);
+  __shared__ double cscale_s;
+  if (tid == 0) {  // Gershgorin bound on the largest eigenvalue
+    double c = 0.0;
+    for (int i = 0; i < n; ++i) {
+      double r = 0.0;
+      for (int j = 0; j < n; ++j) r += sqrt(cabs2(Bf[0][i * n + j]));
+      c = fmax(c, r);
+    }
+    cscale_s = c > 0.0 ? c : 1.0;
   }
   __syncthreads();
-  for (int e = tid; e < n * n; e += blockDim.x) {  // U = V diag(fac) V^H
-    const int i = e / n, j = e % n;
-    cx<double> acc = {0.0, 0.0};
-    for (int k = 0; k < n; ++k) acc = cadd(acc, cscale(cmul(V[i * n + k], cconj(V[j * n + k])), fac[k]));
-    d.U[e] = acc;
-  }
+  const double cs = cscale_s;
+  for (int e = tid; e < n * n; e += blockDim.x) Bf[0][e] = cscale(Bf[0][e], 1.0 / cs);
   __syncthreads();
-
-  // subspace diagonalisation (static.py:300-303): U <- U_rho U_lam
-  if (a.tdiag && d.Lam) {
-    load_lower_herm(A, d.Lam, n);
-    jacobi_eigh(A, V, n, js, true, d.T1);
-    mm(d.T1, n, d.U, n, 0, V, n, n);
-    for (int e = tid; e < n * n; e += blockDim.x) d.U[e] = d.T1[e];
+  cx<double>*Y = Bf[0], *Z = Bf[1], *Y2 = Bf[3], *Z2 = Bf[4];
+  const bool ok = newton_schulz(Y, Z, Bf[2], Y2, Z2, n, red);
+  cx<double>* Urho = Bf[4];
+  if (ok) {
+    const double f = 1.0 / sqrt(cs);
+    // Z may live in Bf[4]; scale in place or copy
+    for (int e = tid; e < n * n; e += blockDim.x) {
+      const cx<double> v = cscale(Z[e], f);
+      Urho[e] = v;
+    }
+    __syncthreads();
+  } else {
+    load_lower_herm(Bf[0], d.S, n);
+    jacobi_eigh(Bf[0], Bf[1], n, js, false, Bf[2]);
+    __shared__ double fac[NMAXJ];
+    if (tid == 0) {
+      double wmax = js.ev[0];
+      for (int i = 1; i < n; ++i) wmax = fmax(wmax, js.ev[i]);
+      const double thr = fmax(1e-12, 2.220446049250313e-16 * wmax * n);
+      for (int i = 0; i < n; ++i) fac[i] = 1.0 / sqrt(fmax(js.ev[i], thr));
+    }
+    __syncthreads();
+    const cx<double>* V = Bf[1];
+    for (int e = tid; e < n * n; e += blockDim.x) {  // U = V diag(fac) V^H
+      const int i = e / n, j = e % n;
+      cx<double> acc = {0.0, 0.0};
+      for (int k = 0; k < n; ++k) acc = cadd(acc, cscale(cmul(V[i * n + k], cconj(V[j * n + k])), fac[k]));
+      Urho[e] = acc;
+    }
     __syncthreads();
   }
 
+  // subspace diagonalisation (static.py:300-303): U <- U_rho U_lam
+  cx<double>* U = Urho;
+  if (a.tdiag && d.Lam) {
+    load_lower_herm(Bf[0], d.Lam, n);
+    jacobi_eigh(Bf[0], Bf[1], n, js, true, Bf[2]);
+    mm(Bf[3], n, Urho, n, 0, Bf[1], n, n);
+    U = Bf[3];
+  }
+  for (int e = tid; e < n * n; e += blockDim.x) d.U[e] = U[e];
+
   if (a.construct) {  // construct_hfb_matrices (static.py:484-577)
-    mm(d.T1, n, d.Hold, n, 0, d.U, n, n);
-    mm(d.Hm, nm, d.U, n, 1, d.T1, n, n);
+    cx<double>* H = Bf[2];
+    cx<double>* G = Bf[0];
+    for (int e = tid; e < n * n; e += blockDim.x) Bf[0][e] = d.Hold[e];
+    __syncthreads();
+    mm(Bf[1], n, Bf[0], n, 0, U, n, n);
+    mm(H, n, U, n, 1, Bf[1], n, n);
     if (a.ipair && d.Gold) {
-      mm(d.T1, n, d.Gold, n, 0, d.U, n, n);
-      mm(d.Gm, nm, d.U, n, 1, d.T1, n, n);
+      for (int e = tid; e < n * n; e += blockDim.x) Bf[0][e] = d.Gold[e];
+      __syncthreads();
+      mm(Bf[1], n, Bf[0], n, 0, U, n, n);
+      mm(G, n, U, n, 1, Bf[1], n, n);
     } else {
-      for (int e = tid; e < n * n; e += blockDim.x) d.Gm[(e / n) * nm + e % n] = {0.0, 0.0};
+      for (int e = tid; e < n * n; e += blockDim.x) G[e] = {0.0, 0.0};
       __syncthreads();
     }
+    for (int e = tid; e < n * n; e += blockDim.x) {
+      d.Hm[(e / n) * nm + e % n] = H[e];
+      d.Gm[(e / n) * nm + e % n] = G[e];
+    }
     for (int i = tid; i < n; i += blockDim.x) {
-      a.sp_energy[d.start + i] = d.Hm[i * nm + i].x;
-      if (a.ipair) a.deltaf[d.start + i] = d.Gm[i * nm + i].x * a.pairwg[d.start + i];
+      a.sp_energy[d.start + i] = H[i * n + i].x;
+      if (a.ipair) a.deltaf[d.start + i] = G[i * n + i].x * a.pairwg[d.start + i];
     }
     __syncthreads();
     if (a.tbcs) {
       for (int e = tid; e < n * n; e += blockDim.x)
-        if (e / n != e % n) d.Gm[(e / n) * nm + e % n] = {0.0, 0.0};
+        if (e / n != e % n) {
+          G[e] = {0.0, 0.0};
+          d.Gm[(e / n) * nm + e % n] = {0.0, 0.0};
+        }
       __syncthreads();
     }
-    // lambda, its parts and the fluctuation measures
-    double v[2] = {0.0, 0.0};
+    // lambda, its parts and the fluctuation measures; lambda+ kept in Bf[1]
+    double mx = 0.0, sm = 0.0;
     for (int e = tid; e < n * n; e += blockDim.x) {
       const int i = e / n, j = e % n;
       const int sj = d.start + j, si = d.start + i;
@@ -277,20 +382,19 @@ __global__ void __launch_bounds__(256) diag_kernel(DiagArgs a) {
       const double uj = a.wguv[sj] * a.pairwg[sj] * a.wstates[sj];
       const double wi = a.wocc[si] * a.wstates[si];
       const double ui = a.wguv[si] * a.pairwg[si] * a.wstates[si];
-      const cx<double> lij = csub(cscale(d.Hm[i * nm + j], wj), cscale(d.Gm[i * nm + j], uj));
-      const cx<double> lji = csub(cscale(d.Hm[j * nm + i], wi), cscale(d.Gm[j * nm + i], ui));
+      const cx<double> lij = csub(cscale(H[i * n + j], wj), cscale(G[i * n + j], uj));
+      const cx<double> lji = csub(cscale(H[j * n + i], wi), cscale(G[j * n + i], ui));
       const cx<double> lh = cconj(lji);  // (lambda^H)[i][j]
       const cx<double> asym = cscale(csub(lij, lh), 0.5);
       const cx<double> sym = cscale(cadd(lij, lh), 0.5);
       d.Sym[i * nm + j] = asym;
       d.Ls[i * nm + j] = sym;
+      Bf[1][e] = sym;
       const double aa = cabs2(asym);
-      v[0] = fmax(v[0], sqrt(aa));
-      v[1] += aa;
+      mx = fmax(mx, sqrt(aa));
+      sm += aa;
     }
-    // max and sum reductions
     __shared__ double rmax[16], rsum[16];
-    double mx = v[0], sm = v[1];
     for (int o = 32; o > 0; o >>= 1) {
       mx = fmax(mx, __shfl_down(mx, o, 64));
       sm += __shfl_down(sm, o, 64);
@@ -310,9 +414,14 @@ __global__ void __launch_bounds__(256) diag_kernel(DiagArgs a) {
       a.efl2q[iq] = sqrt(Ssum / ((double)n * n));
     }
     __syncthreads();
+  } else {
+    // construct=False: lagrange from the stored lambda (static.py:336-350)
+    for (int e = tid; e < n * n; e += blockDim.x) Bf[1][e] = d.Ls[(e / n) * nm + e % n];
+    __syncthreads();
   }
-  // M2 = U lambda+ (static.py:557-570 and the construct=False branch 336-350)
-  mm(d.M2, n, d.U, n, 0, d.Ls, nm, n);
+  // M2 = U lambda+ (static.py:557-570)
+  mm(Bf[0], n, U, n, 0, Bf[1], n, n);
+  for (int e = tid; e < n * n; e += blockDim.x) d.M2[e] = Bf[0][e];
 }
 
 __global__ void diag_finish_kernel(const double* efl1q, const double* efl2q, double* efl1, double* efl2) {
@@ -324,7 +433,7 @@ __global__ void diag_finish_kernel(const double* efl1q, const double* efl2q, dou
 
 // mats[iq*4 + m]: m=0 S, 1 H_old, 2 lambda_lin (or null), 3 G_old (or null),
 // each nst_iq x nst_iq complex; uo[iq*2]: U, uo[iq*2+1]: M2 outputs;
-// scratch: 4 * nmax^2 complex doubles.
+// scratch: 6 * nmax^2 complex doubles.
 extern "C" int hfb_diag_small(int nst0, int nst1, int nmax, const double* const* mats, int tdiag, int ipair,
                               int tbcs, int construct, const double* wocc, const double* wstates, const double* wguv,
                               const double* pairwg, double* sp_norm, double* sp_energy, double* deltaf,
@@ -351,8 +460,9 @@ extern "C" int hfb_diag_small(int nst0, int nst1, int nmax, const double* const*
     d.U = (cx<double>*)uo[iq * 2 + 0];
     d.M2 = (cx<double>*)uo[iq * 2 + 1];
     if (d.nst > 0 && (!d.U || !d.M2)) return HFB_E_ARG;
-    d.T1 = (cx<double>*)scratch + (iq * 2 + 0) * nm2;
-    d.T2 = (cx<double>*)scratch + (iq * 2 + 1) * nm2;
+    d.T1 = (cx<double>*)scratch + (iq * 3 + 0) * nm2;
+    d.T2 = (cx<double>*)scratch + (iq * 3 + 1) * nm2;
+    d.T3 = (cx<double>*)scratch + (iq * 3 + 2) * nm2;
     d.Hm = (cx<double>*)hmatrix + iq * nm2;
     d.Gm = (cx<double>*)gapmatrix + iq * nm2;
     d.Sym = (cx<double>*)symcond + iq * nm2;
@@ -372,9 +482,9 @@ extern "C" int hfb_diag_small(int nst0, int nst1, int nmax, const double* const*
   a.tbcs = tbcs;
   a.construct = construct;
   const int nbig = nst0 > nst1 ? nst0 : nst1;
-  const size_t lds = sizeof(cx<double>) * 2 * (size_t)nbig * nbig;
+  const size_t lds = sizeof(cx<double>) * (nbig <= NLDS_ALL ? 5 : 2) * (size_t)nbig * nbig;
   hipStream_t st = (hipStream_t)stream;
-  if (lds > 48 * 1024) hipFuncSetAttribute((const void*)diag_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (lds > 32 * 1024) allow_full_lds((const void*)diag_kernel);
   hipLaunchKernelGGL(diag_kernel, dim3(2), dim3(256), lds, st, a);
   int rc = launch_status();
   if (rc != HFB_OK) return rc;

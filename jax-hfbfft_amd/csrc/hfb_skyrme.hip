@@ -227,22 +227,24 @@ __global__ void energy_finish_kernel(EnArgs a, int nblk, double wxyz, const doub
   for (int b = threadIdx.x; b < nblk; b += blockDim.x)
     for (int j = 0; j < NES; ++j) s[j] += a.part[(int64_t)b * NES + j];
   block_sum<NES>(s, red);
-  if (threadIdx.x != 0) return;
   const hfb_force& f = a.f;
   // energies.py:157-167 index current[:,:,:,0,0] .. current[:,:,:,2,1]: the
   // (y, z) = (0..2, 0..1) corner of every (iq, component, x) -- kept as written.
   const int64_t nvox = (int64_t)nx * ny * nz;
-  double cj0 = 0.0, cj1 = 0.0;
-  for (int iq = 0; iq < 2; ++iq)
-    for (int c = 0; c < 3; ++c)
-      for (int x = 0; x < nx; ++x) {
-        const double* p = cur + (int64_t)(iq * 3 + c) * nvox + (int64_t)x * ny * nz;
-        for (int y = 0; y < 3; ++y) {
-          const double a0 = p[(int64_t)y * nz + 0], a1 = p[(int64_t)y * nz + 1];
-          cj0 += (a0 + a1) * (a0 + a1);
-          cj1 += (-a0 + a1) * (-a0 + a1);
-        }
-      }
+  double cj[2] = {0.0, 0.0};
+  for (int it = threadIdx.x; it < 2 * 3 * nx * 3; it += blockDim.x) {
+    const int y = it % 3;
+    const int x = (it / 3) % nx;
+    const int ic = it / (3 * nx);  // iq*3 + component
+    const double* p = cur + (int64_t)ic * nvox + (int64_t)x * ny * nz + (int64_t)y * nz;
+    const double a0 = p[0], a1 = p[1];
+    cj[0] += (a0 + a1) * (a0 + a1);
+    cj[1] += (-a0 + a1) * (-a0 + a1);
+  }
+  __shared__ double red2[4 * 2];
+  block_sum<2>(cj, red2);
+  if (threadIdx.x != 0) return;
+  const double cj0 = cj[0], cj1 = cj[1];
   const double ehf0 = wxyz * s[S_EHF0], ehf1 = wxyz * s[S_EHF1], ehf2 = wxyz * s[S_EHF2], ehf3 = wxyz * s[S_EHF3];
   const double ehflsodd = wxyz * s[S_LSODD];
   const double ehfls = wxyz * s[S_EHFLS] + ehflsodd;

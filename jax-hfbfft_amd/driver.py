@@ -60,6 +60,30 @@ CONFIGS = {
 }
 
 
+class _Segmenter:
+    """Captures one iteration into a chain of HIP graphs, cut around timed
+    library calls (kernels._timed) so replays can be bracketed by events."""
+
+    def __init__(self, pool):
+        self.pool = pool
+        self.graphs = []
+        self.labels = []
+        self._begin(None)
+
+    def _begin(self, label):
+        g = torch.cuda.CUDAGraph()
+        g.capture_begin(pool=self.pool, capture_error_mode="relaxed")
+        self.graphs.append(g)
+        self.labels.append(label)
+
+    def cut(self, label):
+        self.graphs[-1].capture_end()
+        self._begin(label)
+
+    def close(self):
+        self.graphs[-1].capture_end()
+
+
 class HFRun:
     def __init__(self, params, forces, grids, levels, meanfield, densities, coulomb, static, energies, pairs, moment):
         self.params, self.forces, self.grids, self.levels = params, forces, grids, levels
@@ -68,6 +92,8 @@ class HFRun:
         self.derivs = None
         self.tbcssav = forces.tbcs
         self.iteration_no = 0
+        self.graphs_enabled = False
+        self.capture_first = False
 
     @classmethod
     def from_config(cls, precision="f64", device="cuda:0", name="SV-bas", n=24, d=1.0, nneut=8, nprot=8, npsi=None,
@@ -93,7 +119,7 @@ class HFRun:
         levels.psi.copy_(torch.from_numpy(np.ascontiguousarray(psi0)).to(dtype))
         meanfield = init_meanfield(grids, dev, f32=(precision == "mixed"))
         return cls(params, forces, grids, levels, meanfield, init_densities(grids, dev), coulomb, static,
-                   init_energies(dev), Pairs(), init_moment([0.0, 0.0, 0.0]))
+                   init_energies(dev), Pairs(), init_moment(torch.zeros(3, dtype=torch.float64, device=dev)))
 
     @property
     def state(self):
@@ -104,10 +130,86 @@ class HFRun:
         self.derivs = st_mod.prelude(*self.state)
         self.tbcssav = self.forces.tbcs
 
+    def signature(self, i):
+        """Host-side switches of iteration i; iterations with equal signatures
+        launch identical kernel sequences (static.py:1759-1786, 99-122, pairs.py:135)."""
+        s, p, f = self.static, self.params, self.forces
+        tbcs = True if i <= s.inibcs else self.tbcssav
+        tdiag = tbcs or not (i > s.inidiag)
+        return (tbcs, tdiag, i > 1 and not tbcs, p.mprint > 0 and i % p.mprint == 0,
+                f.ipair != 0 and i <= 10, self.levels.psi.dtype, s.x0dmp)
+
     def iteration(self, i, check=False):
         self.iteration_no = i
-        conv, self.derivs = st_mod.iterate(i, *self.state, self.tbcssav, self.derivs, check=check)
-        return conv
+        if not self.graphs_enabled:
+            conv, self.derivs = st_mod.iterate(i, *self.state, self.tbcssav, self.derivs, check=check)
+            return conv
+        sig = self.signature(i)
+        g = self._graphs.get(sig)
+        if g is None and (sig in self._seen or self.capture_first):
+            g = self._capture(i)
+            self._graphs[sig] = g
+        if g is None:
+            self._seen.add(sig)
+            conv, self.derivs = st_mod.iterate(i, *self.state, self.tbcssav, self.derivs, check=check)
+            return conv
+        # replay: keep the host-side switches in step with the captured iteration
+        self.params.iteration = i
+        self.forces.tbcs = sig[0]
+        self.static.tdiag = sig[1]
+        from . import kernels
+        for graph, label in zip(g.graphs, g.labels):
+            if label is not None and kernels.TIMERS is not None:
+                ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                ev[0].record()
+                graph.replay()
+                ev[1].record()
+                kernels.TIMERS.setdefault(label, []).append(ev)
+            else:
+                graph.replay()
+        if check:
+            e = self.energies.host()
+            return e["efluct1"] < self.static.serr and i > 1
+        return False
+
+    def enable_graphs(self, on=True, capture_first=False):
+        """Replay each distinct iteration signature from HIP graphs.  By
+        default a signature is captured on its second occurrence (the first
+        runs eagerly and allocates its workspaces); with capture_first the
+        first occurrence is captured directly (new workspaces then come from
+        the graph pool)."""
+        self.graphs_enabled = on
+        self.capture_first = capture_first
+        if not hasattr(self, "_graphs"):
+            self._graphs, self._seen = {}, set()
+            self._pool = torch.cuda.graph_pool_handle()
+
+    def precapture(self, iterations):
+        """Capture the graphs of the given iteration numbers ahead of time
+        (capture records kernels without running them, so no state changes)."""
+        saved = (self.params.iteration, self.forces.tbcs, self.static.tdiag)
+        for i in iterations:
+            sig = self.signature(i)
+            if sig not in self._graphs:
+                self._graphs[sig] = self._capture(i)
+        self.params.iteration, self.forces.tbcs, self.static.tdiag = saved
+
+    def _capture(self, i):
+        from . import kernels
+        torch.cuda.synchronize()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            seg = _Segmenter(self._pool)
+            kernels.SEGMENTER = seg
+            try:
+                _, self.derivs = st_mod.iterate(i, *self.state, self.tbcssav, self.derivs, check=False)
+            finally:
+                kernels.SEGMENTER = None
+                seg.close()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        return seg
 
     def run(self, maxiter, start=1, check_every=10, callback=None):
         for i in range(start, start + maxiter):
@@ -126,6 +228,11 @@ class HFRun:
             return
         for name in ("psi", "hampsi", "lagrange", "hmfpsi", "delpsi"):
             setattr(lv, name, getattr(lv, name).to(torch.complex128))
+        if hasattr(self, "_graphs"):  # captured graphs refer to the old buffers
+            # retired graphs stay referenced: dropping every graph of a pool
+            # would release the pool that later captures still use
+            self._retired = getattr(self, "_retired", []) + list(self._graphs.values())
+            self._graphs, self._seen = {}, set()
 
     def energies_host(self):
         torch.cuda.synchronize()

@@ -10,9 +10,44 @@ import ctypes
 
 import torch
 
+from ctypes import c_void_p
+
 from . import _lib
 from ._lib import check, lib, need, ptr, stream_handle
-from ctypes import c_void_p
+
+# Optional timing of library calls with HIP events on the launching stream:
+# set TIMERS to a dict name -> list and every timed call appends an
+# (start, end) event pair.  While an iteration is being captured into HIP
+# graphs, SEGMENTER splits the capture around timed calls so that replays can
+# be bracketed by events outside the graphs.
+TIMERS = None
+SEGMENTER = None
+
+
+class _timed:
+    def __init__(self, name):
+        self.name = name
+        self.ev = None
+
+    def __enter__(self):
+        if self.name is None:
+            return self
+        if SEGMENTER is not None:
+            SEGMENTER.cut(self.name)
+        elif TIMERS is not None:
+            self.ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            self.ev[0].record()
+        return self
+
+    def __exit__(self, *exc):
+        if self.name is None:
+            return False
+        if SEGMENTER is not None:
+            SEGMENTER.cut(None)
+        elif self.ev is not None:
+            self.ev[1].record()
+            TIMERS.setdefault(self.name, []).append(self.ev)
+        return False
 
 
 def cderv(ctx, axis: int, kind: int, x: torch.Tensor, bmass: torch.Tensor | None = None):
@@ -35,7 +70,7 @@ def cderv(ctx, axis: int, kind: int, x: torch.Tensor, bmass: torch.Tensor | None
 
 
 def hpsi(ctx, psi, npsi0, upot, bmass, aq, spot, wlspot, v_pair, w, wuv,
-         want_pout=True, want_del=True, want_partial=True, out=None):
+         want_pout=True, want_del=True, want_partial=True, out=None, timer=None):
     """Batched hpsi01 (meanfield.py:40-159) on psi (S, 2, nx, ny, nz).
 
     Returns (pout, pout_mf, pout_del, sums) where sums (S, 2) double holds
@@ -56,19 +91,24 @@ def hpsi(ctx, psi, npsi0, upot, bmass, aq, spot, wlspot, v_pair, w, wuv,
     pmf = out.get("pout_mf")
     if pmf is None:
         pmf = torch.empty_like(psi)
-    pout = out.get("pout", torch.empty_like(psi) if want_pout else None)
-    pdel = out.get("pout_del", torch.empty_like(psi) if want_del else None)
+    pout = out.get("pout")
+    if pout is None and want_pout:
+        pout = torch.empty_like(psi)
+    pdel = out.get("pout_del")
+    if pdel is None and want_del:
+        pdel = torch.empty_like(psi)
     ntile = lib().hfb_hpsi_ntiles(ctx.handle, prec)
     if ntile < 0:
         check(ntile, "hpsi_ntiles")
     part = ctx.work(("hpsi_part", prec, S), (S, ntile, 2), torch.float64) if want_partial else None
     nt = ctypes.c_int()
-    check(lib().hfb_hpsi(ctx.handle, prec, ptr(psi), S, int(npsi0), ptr(upot), ptr(bmass), ptr(aq), ptr(spot),
-                         ptr(wlspot), ptr(v_pair), ptr(w), ptr(wuv), ptr(pout), ptr(pmf), ptr(pdel), ptr(part),
-                         ctypes.byref(nt), stream_handle()), "hpsi")
+    with _timed(timer):
+        check(lib().hfb_hpsi(ctx.handle, prec, ptr(psi), S, int(npsi0), ptr(upot), ptr(bmass), ptr(aq), ptr(spot),
+                             ptr(wlspot), ptr(v_pair), ptr(w), ptr(wuv), ptr(pout), ptr(pmf), ptr(pdel), ptr(part),
+                             ctypes.byref(nt), stream_handle()), "hpsi")
     sums = None
     if want_partial:
-        sums = torch.empty((S, 2), dtype=torch.float64, device=psi.device)
+        sums = ctx.work(("hpsi_sums", S), (S, 2), torch.float64)
         check(lib().hfb_reduce_partials(ptr(part), S, nt.value, 2, 1.0, ptr(sums), stream_handle()),
               "reduce_partials")
     return pout, pmf, pdel, sums

@@ -161,7 +161,7 @@ def grstep(forces, grids, levels, meanfield, params, static):
     fields = meanfield.fields(levels.psi.dtype)
     # h psi on the old states: ps1 (weighted) and psi_mf -> hampsi (static.py:152-173)
     _, _, _, hsum = kernels.hpsi(ctx, levels.psi, n0, *fields, ws["w"], ws["wuv"], want_del=False,
-                                 out={"pout": ws["pout"], "pout_mf": levels.hampsi})
+                                 out={"pout": ws["pout"], "pout_mf": levels.hampsi}, timer="hpsi")
     want_fl = params.mprint > 0 and params.iteration % params.mprint == 0
     if static.e0dmp > 0.0:
         use_lag = (params.iteration > 1) and (not forces.tbcs)
@@ -211,14 +211,18 @@ def diagstep(energies, forces, grids, levels, static, diagonalize=False, constru
         mats += [M[0], M[1], M[2] if diagonalize else None, M[len(Bs) - 1] if use_gap else None]
         uo += [ctx.work(("U", iq, nst), (nst, nst), torch.complex128), ctx.work(("M2", iq, nst), (nst, nst),
                                                                               torch.complex128)]
-    scratch = ctx.work(("diag_scratch", nmax), (4, nmax, nmax), torch.complex128)
+    scratch = ctx.work(("diag_scratch", nmax), (6, nmax, nmax), torch.complex128)
     kernels.diag_small(n0, n1, nmax, mats, diagonalize, forces.ipair != 0, forces.tbcs, construct, levels, static,
                        energies.vec, uo, scratch)
-    spare = ctx.work(("psi_spare", levels.psi.dtype), tuple(levels.psi.shape), levels.psi.dtype)
+    # psi <- psi U in place (each 16-point block of K is read, then written,
+    # by one wavefront); blocks of more than 64 states go through a copy
     for iq, (a, b) in enumerate(((0, n0), (n0, n0 + n1))):
-        kernels.rotate(levels.psi[a:b], uo[2 * iq], uo[2 * iq + 1], spare[a:b], levels.lagrange[a:b])
-    ctx._work[("psi_spare", levels.psi.dtype)] = levels.psi
-    levels.psi = spare
+        if b - a <= 64:
+            kernels.rotate(levels.psi[a:b], uo[2 * iq], uo[2 * iq + 1], levels.psi[a:b], levels.lagrange[a:b])
+        else:
+            spare = ctx.work(("psi_spare", levels.psi.dtype, b - a), tuple(levels.psi[a:b].shape), levels.psi.dtype)
+            kernels.rotate(levels.psi[a:b], uo[2 * iq], uo[2 * iq + 1], spare, levels.lagrange[a:b])
+            levels.psi[a:b].copy_(spare)
     return energies, levels, static
 
 
