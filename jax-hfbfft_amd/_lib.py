@@ -41,6 +41,8 @@ _SIGS = {
     "hfb_gram_workspace": (c_int, [c_int, ctypes.c_longlong]),
     "hfb_gram": (c_int, [c_int, c_void_p, POINTER(c_void_p), c_int, c_int, ctypes.c_longlong, c_double, c_void_p,
                          c_void_p, c_void_p]),
+    "hfb_gram2": (c_int, [c_int, POINTER(c_void_p), POINTER(c_void_p), c_int, POINTER(c_int), ctypes.c_longlong,
+                          c_double, POINTER(c_void_p), POINTER(c_void_p), c_void_p]),
     "hfb_rotate": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, ctypes.c_longlong,
                            c_void_p]),
     "hfb_diag_small": (c_int, [c_int, c_int, c_int, POINTER(c_void_p), c_int, c_int, c_int, c_int, c_void_p,

@@ -34,69 +34,94 @@ __device__ __forceinline__ cx<double> ld64(const cx<T>* p) {
   return {double(v.x), double(v.y)};
 }
 
-struct GramArgs {
+struct GramIso {
   const void* A;
   const void* B[4];
+  int nst;
+  int np;       // nst padded to 16
+  double* part; // [m][np][np][chunk] complex
+};
+
+struct GramArgs {
+  GramIso iso[2];
   int nm;       // number of right-hand blocks
   int bsame;    // bitmask: B[m] == A (no second load)
-  int nst;
   int64_t K;
   int64_t kc;   // K per chunk, multiple of KS
   int nchunk;
-  int np;       // nst padded to 16
-  double* part; // [m][np][np][chunk] complex
 };
 
 constexpr int KS = 16;        // complex elements of K staged per step
 constexpr int LDR = KS + 1;   // padded LDS row stride (rows sit on different banks)
 constexpr int GRAM_NT = 256;  // 4 wavefronts
 
-// One workgroup per K-chunk computes every 16x16 tile of every C_m.  Each
-// K-step stages A and the distinct B_m rows (KS contiguous elements per row,
-// converted to double) in LDS, so every input element is read from HBM once.
-// The four wavefronts split the output tiles; MFMA operands come from LDS.
-template <typename T, int NM, int MAXT>
-__global__ void __launch_bounds__(GRAM_NT) gram_kernel(GramArgs g) {
+// One workgroup per (K-chunk, isospin block) computes every 16x16 tile of
+// every C_m.  Each K-step stages A and the distinct B_m rows (KS contiguous
+// elements per row, converted to double) in LDS, so every input element is
+// read from HBM once; the loads of step t+1 are issued before the MFMAs of
+// step t.  The four wavefronts split the output tiles; MFMA operands come
+// from LDS.
+template <typename T, int NM, int MAXT, int NE>
+__global__ void __launch_bounds__(GRAM_NT) gram_kernel(GramArgs ga) {
   extern __shared__ double glds[];
+  const GramIso& g = ga.iso[blockIdx.y];
   const int np = g.np;
+  if (np == 0) return;
   const int ntile1 = np / 16;
   const int ntl = ntile1 * ntile1;
   cx<double>* sA = (cx<double>*)glds;                 // [np][LDR]
   cx<double>* sB = sA + (size_t)np * LDR;             // [NM][np][LDR]
   const int chunk = blockIdx.x;
-  const int64_t k0 = (int64_t)chunk * g.kc;
-  const int64_t k1 = min(g.K, k0 + g.kc);
+  const int64_t k0 = (int64_t)chunk * ga.kc;
+  const int64_t k1 = min(ga.K, k0 + ga.kc);
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int grp = l >> 4, c = l & 15;
-  // tiles of this wave: t = wave + 4*u over NM*ntl tiles
   const int ntot = NM * ntl;
+  const int bsame = ga.bsame;
   v4d cre[MAXT], cim[MAXT];
 #pragma unroll
   for (int u = 0; u < MAXT; ++u) {
     cre[u] = v4d{0.0, 0.0, 0.0, 0.0};
     cim[u] = v4d{0.0, 0.0, 0.0, 0.0};
   }
-  for (int64_t kb = k0; kb < k1; kb += KS) {
-    // stage: rows of A and of the distinct B_m
-    for (int e = threadIdx.x; e < np * KS; e += GRAM_NT) {
+  // register staging of one K-step: NE elements per thread and matrix
+  cx<double> ra[NE], rb[NM][NE];
+  auto fetch = [&](int64_t kb) {
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      const int e = threadIdx.x + q * GRAM_NT;
       const int row = e / KS, kk = e % KS;
       const int64_t k = kb + kk;
-      const bool ok = row < g.nst && k < k1;
-      const int le = row * LDR + kk;
-      sA[le] = ok ? ld64((const cx<T>*)g.A + (int64_t)row * g.K + k) : cx<double>{0.0, 0.0};
+      const bool ok = e < np * KS && row < g.nst && k < k1;
+      ra[q] = ok ? ld64((const cx<T>*)g.A + (int64_t)row * ga.K + k) : cx<double>{0.0, 0.0};
 #pragma unroll
       for (int m = 0; m < NM; ++m)
-        if (!((g.bsame >> m) & 1))
-          sB[(size_t)m * np * LDR + le] = ok ? ld64((const cx<T>*)g.B[m] + (int64_t)row * g.K + k) : cx<double>{0.0, 0.0};
+        if (!((bsame >> m) & 1))
+          rb[m][q] = ok ? ld64((const cx<T>*)g.B[m] + (int64_t)row * ga.K + k) : cx<double>{0.0, 0.0};
+    }
+  };
+  fetch(k0);
+  for (int64_t kb = k0; kb < k1; kb += KS) {
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      const int e = threadIdx.x + q * GRAM_NT;
+      if (e < np * KS) {
+        const int le = (e / KS) * LDR + e % KS;
+        sA[le] = ra[q];
+#pragma unroll
+        for (int m = 0; m < NM; ++m)
+          if (!((bsame >> m) & 1)) sB[(size_t)m * np * LDR + le] = rb[m][q];
+      }
     }
     __syncthreads();
+    if (kb + KS < k1) fetch(kb + KS);
 #pragma unroll
     for (int u = 0; u < MAXT; ++u) {
       const int t = wave + 4 * u;
       if (t < ntot) {
         const int m = t / ntl, tt = t % ntl;
         const int ti = tt / ntile1, tj = tt % ntile1;
-        const cx<double>* bb = ((g.bsame >> m) & 1) ? sA : sB + (size_t)m * np * LDR;
+        const cx<double>* bb = ((bsame >> m) & 1) ? sA : sB + (size_t)m * np * LDR;
 #pragma unroll
         for (int s4 = 0; s4 < KS; s4 += 4) {
           const cx<double> a = sA[(ti * 16 + c) * LDR + s4 + grp];
@@ -121,7 +146,7 @@ __global__ void __launch_bounds__(GRAM_NT) gram_kernel(GramArgs g) {
       for (int r = 0; r < 4; ++r) {
         const int row = ti * 16 + grp + 4 * r;
         const int col = tj * 16 + c;
-        double* dst = g.part + 2 * ((((int64_t)m * np + row) * np + col) * g.nchunk + chunk);
+        double* dst = g.part + 2 * ((((int64_t)m * np + row) * np + col) * ga.nchunk + chunk);
         dst[0] = cre[u][r];
         dst[1] = cim[u][r];
       }
@@ -159,7 +184,7 @@ __global__ void __launch_bounds__(256) gram_reduce_kernel(const double* __restri
 
 int gram_nchunk(int np, int64_t K) {
   (void)np;
-  int64_t nchunk = (K + 511) / 512;  // ~512 elements of K per workgroup
+  int64_t nchunk = (K + 255) / 256;  // ~256 elements of K per workgroup
   if (nchunk > 256) nchunk = 256;
   return nchunk < 1 ? 1 : (int)nchunk;
 }
@@ -167,7 +192,7 @@ int gram_nchunk(int np, int64_t K) {
 // ---------------------------------------------------------------------------
 // Rotation: out1_j = sum_i psi_i U1[i][j], out2_j = sum_i psi_i U2[i][j]
 // ---------------------------------------------------------------------------
-template <typename T, int NJT, int NI>
+template <typename T, int NJT>
 __global__ void __launch_bounds__(256) rotate_kernel(const cx<T>* psi, const double* __restrict__ U1,
                                                      const double* __restrict__ U2, cx<T>* out1, cx<T>* out2, int nst,
                                                      int64_t K) {
@@ -193,19 +218,12 @@ __global__ void __launch_bounds__(256) rotate_kernel(const cx<T>* psi, const dou
   const int wave = threadIdx.x >> 6;
   const int l = threadIdx.x & 63;
   const int grp = l >> 4, c = l & 15;
-  const int jg0 = blockIdx.y * NJT;          // first j-tile of this block
-  const int njt = min(NJT, np / 16 - jg0);
-  const int nis = np / 4;                     // MFMA steps over i
+  const int jg0 = blockIdx.y * NJT;  // first j-tile of this block
+  const int nis = np / 4;            // MFMA steps over i
   const int64_t nblk = (K + 15) / 16;
+  const bool two = U2 != nullptr;
   for (int64_t blk = (int64_t)blockIdx.x * 4 + wave; blk < nblk; blk += (int64_t)gridDim.x * 4) {
     const int64_t k = blk * 16 + c;
-    // B operand psi^T[i][k] for every i-step, loaded up front
-    cx<double> bv[NI];
-#pragma unroll
-    for (int s = 0; s < NI; ++s) {
-      const int i = 4 * s + grp;
-      bv[s] = (s < nis && i < nst && k < K) ? ld64(psi + (int64_t)i * K + k) : cx<double>{0.0, 0.0};
-    }
     v4d d1r[NJT], d1i[NJT], d2r[NJT], d2i[NJT];
 #pragma unroll
     for (int t = 0; t < NJT; ++t) {
@@ -214,29 +232,34 @@ __global__ void __launch_bounds__(256) rotate_kernel(const cx<T>* psi, const dou
       d2r[t] = d1r[t];
       d2i[t] = d1r[t];
     }
+    for (int s0 = 0; s0 < nis; s0 += 4) {
+      // B operand psi^T[i][k] for four i-steps, loaded together
+      cx<double> bv[4];
 #pragma unroll
-    for (int s = 0; s < NI; ++s) {
-      if (s < nis) {
-        const int i = 4 * s + grp;
-        const cx<double> b = bv[s];
+      for (int u = 0; u < 4; ++u) {
+        const int i = 4 * (s0 + u) + grp;
+        bv[u] = (s0 + u < nis && i < nst && k < K) ? ld64(psi + (int64_t)i * K + k) : cx<double>{0.0, 0.0};
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = 4 * (s0 + u) + grp;
+        const cx<double> b = bv[u];
 #pragma unroll
         for (int t = 0; t < NJT; ++t) {
-          if (t < njt) {
-            const int j = (jg0 + t) * 16 + c;  // A operand: U^T[j][i] = U[i][j]
-            const double* pa = u1 + 2 * ((int64_t)i * np + j);
-            const double ar = pa[0], ai = pa[1];
-            d1r[t] = mfma(ar, b.x, d1r[t]);
-            d1r[t] = mfma(-ai, b.y, d1r[t]);
-            d1i[t] = mfma(ar, b.y, d1i[t]);
-            d1i[t] = mfma(ai, b.x, d1i[t]);
-            if (U2) {
-              const double* pb = u2 + 2 * ((int64_t)i * np + j);
-              const double br = pb[0], bi = pb[1];
-              d2r[t] = mfma(br, b.x, d2r[t]);
-              d2r[t] = mfma(-bi, b.y, d2r[t]);
-              d2i[t] = mfma(br, b.y, d2i[t]);
-              d2i[t] = mfma(bi, b.x, d2i[t]);
-            }
+          const int j = (jg0 + t) * 16 + c;  // A operand: U^T[j][i] = U[i][j]
+          const double* pa = u1 + 2 * ((int64_t)i * np + j);
+          const double ar = pa[0], ai = pa[1];
+          d1r[t] = mfma(ar, b.x, d1r[t]);
+          d1r[t] = mfma(-ai, b.y, d1r[t]);
+          d1i[t] = mfma(ar, b.y, d1i[t]);
+          d1i[t] = mfma(ai, b.x, d1i[t]);
+          if (two) {
+            const double* pb = u2 + 2 * ((int64_t)i * np + j);
+            const double br = pb[0], bi = pb[1];
+            d2r[t] = mfma(br, b.x, d2r[t]);
+            d2r[t] = mfma(-bi, b.y, d2r[t]);
+            d2i[t] = mfma(br, b.y, d2i[t]);
+            d2i[t] = mfma(bi, b.x, d2i[t]);
           }
         }
       }
@@ -244,14 +267,12 @@ __global__ void __launch_bounds__(256) rotate_kernel(const cx<T>* psi, const dou
     if (k < K) {
 #pragma unroll
       for (int t = 0; t < NJT; ++t) {
-        if (t < njt) {
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int j = (jg0 + t) * 16 + grp + 4 * r;
-            if (j < nst) {
-              out1[(int64_t)j * K + k] = cx<T>{T(d1r[t][r]), T(d1i[t][r])};
-              if (U2) out2[(int64_t)j * K + k] = cx<T>{T(d2r[t][r]), T(d2i[t][r])};
-            }
+        for (int r = 0; r < 4; ++r) {
+          const int j = (jg0 + t) * 16 + grp + 4 * r;
+          if (j < nst) {
+            out1[(int64_t)j * K + k] = cx<T>{T(d1r[t][r]), T(d1i[t][r])};
+            if (two) out2[(int64_t)j * K + k] = cx<T>{T(d2r[t][r]), T(d2i[t][r])};
           }
         }
       }
@@ -267,54 +288,78 @@ extern "C" int hfb_gram_workspace(int nst, long long K) {
 }
 
 template <typename T, int NM>
-int launch_gram(const GramArgs& g, hipStream_t st) {
-  const int ntile = (g.np / 16) * (g.np / 16) * NM;
-  const size_t lds = sizeof(cx<double>) * (size_t)g.np * LDR * (1 + NM);
-  const dim3 grid(g.nchunk);
-#define HFB_GR(MT)                                                                                               \
-  {                                                                                                              \
-    auto kern = gram_kernel<T, NM, MT>;                                                                          \
-    if (lds > 32 * 1024) allow_full_lds((const void*)kern); \
-    hipLaunchKernelGGL(kern, grid, dim3(GRAM_NT), lds, st, g);                                                  \
+int launch_gram(const GramArgs& g, int npmax, hipStream_t st) {
+  const int ntile = (npmax / 16) * (npmax / 16) * NM;
+  const size_t lds = sizeof(cx<double>) * (size_t)npmax * LDR * (1 + NM);
+  const dim3 grid(g.nchunk, 2);
+  const int ne = (npmax * KS + GRAM_NT - 1) / GRAM_NT;
+#define HFB_GR(MT, NEV)                                                          \
+  {                                                                              \
+    auto kern = gram_kernel<T, NM, MT, NEV>;                                     \
+    if (lds > 32 * 1024) allow_full_lds((const void*)kern);                      \
+    hipLaunchKernelGGL(kern, grid, dim3(GRAM_NT), lds, st, g);                  \
   }
   const int per_wave = (ntile + 3) / 4;
-  if (per_wave <= 1) HFB_GR(1)
-  else if (per_wave <= 2) HFB_GR(2)
-  else if (per_wave <= 4) HFB_GR(4)
-  else if (per_wave <= 8) HFB_GR(8)
-  else if (per_wave <= 16) HFB_GR(16)
+  if (ne > 8) return HFB_E_SIZE;
+#define HFB_GRN(MT)                    \
+  if (ne <= 2) HFB_GR(MT, 2)           \
+  else if (ne <= 4) HFB_GR(MT, 4)      \
+  else HFB_GR(MT, 8)
+  if (per_wave <= 1) { HFB_GRN(1) }
+  else if (per_wave <= 2) { HFB_GRN(2) }
+  else if (per_wave <= 4) { HFB_GRN(4) }
+  else if (per_wave <= 8) { HFB_GRN(8) }
+  else if (per_wave <= 16) { HFB_GRN(16) }
   else return HFB_E_SIZE;
+#undef HFB_GRN
 #undef HFB_GR
   return launch_status();
 }
 
-extern "C" int hfb_gram(int prec, const void* A, const void* const* B, int nm, int nst, long long K, double scale,
-                        double* part, double* out, void* stream) {
-  if (!A || !B || !part || !out || nm < 1 || nm > 4 || nst <= 0 || K <= 0) return HFB_E_ARG;
+// Both isospin blocks in one launch: A[iq], B[iq*4 + m], out[iq] (nm, nst_iq,
+// nst_iq) complex, part[iq] workspaces of hfb_gram_workspace(nst_iq, K).
+// A block with nst 0 is skipped.
+extern "C" int hfb_gram2(int prec, const void* const* A, const void* const* B, int nm, const int* nst, long long K,
+                         double scale, double* const* part, double* const* out, void* stream) {
+  if (!A || !B || !part || !out || !nst || nm < 1 || nm > 4 || K <= 0) return HFB_E_ARG;
   GramArgs g{};
-  g.A = A;
   g.bsame = 0;
   for (int m = 0; m < nm; ++m) {
-    if (!B[m]) return HFB_E_ARG;
-    g.B[m] = B[m];
-    if (B[m] == A) g.bsame |= 1 << m;
+    bool same = true;
+    for (int iq = 0; iq < 2; ++iq)
+      if (nst[iq] > 0 && B[iq * 4 + m] != A[iq]) same = false;
+    if (same) g.bsame |= 1 << m;
   }
+  int npmax = 0;
+  for (int iq = 0; iq < 2; ++iq) {
+    GramIso& gi = g.iso[iq];
+    gi.nst = nst[iq];
+    gi.np = nst[iq] > 0 ? (nst[iq] + 15) & ~15 : 0;
+    if (nst[iq] > 0) {
+      if (!A[iq] || !part[iq] || !out[iq]) return HFB_E_ARG;
+      gi.A = A[iq];
+      for (int m = 0; m < nm; ++m) {
+        if (!B[iq * 4 + m]) return HFB_E_ARG;
+        gi.B[m] = B[iq * 4 + m];
+      }
+      gi.part = part[iq];
+    }
+    npmax = gi.np > npmax ? gi.np : npmax;
+  }
+  if (npmax == 0) return HFB_OK;
   g.nm = nm;
-  g.nst = nst;
   g.K = K;
-  g.np = (nst + 15) & ~15;
-  g.nchunk = gram_nchunk(g.np, K);
+  g.nchunk = gram_nchunk(npmax, K);
   int64_t kc = (K + g.nchunk - 1) / g.nchunk;
   kc = (kc + KS - 1) / KS * KS;
   g.kc = kc;
-  g.part = part;
   hipStream_t st = (hipStream_t)stream;
   int rc;
-#define HFB_GT(TT)                                   \
-  if (nm == 1) rc = launch_gram<TT, 1>(g, st);       \
-  else if (nm == 2) rc = launch_gram<TT, 2>(g, st);  \
-  else if (nm == 3) rc = launch_gram<TT, 3>(g, st);  \
-  else rc = launch_gram<TT, 4>(g, st);
+#define HFB_GT(TT)                                          \
+  if (nm == 1) rc = launch_gram<TT, 1>(g, npmax, st);       \
+  else if (nm == 2) rc = launch_gram<TT, 2>(g, npmax, st);  \
+  else if (nm == 3) rc = launch_gram<TT, 3>(g, npmax, st);  \
+  else rc = launch_gram<TT, 4>(g, npmax, st);
   if (prec == HFB_F64) {
     HFB_GT(double)
   } else if (prec == HFB_F32) {
@@ -324,12 +369,29 @@ extern "C" int hfb_gram(int prec, const void* A, const void* const* B, int nm, i
   }
 #undef HFB_GT
   if (rc != HFB_OK) return rc;
-  const int64_t tot = (int64_t)nm * nst * nst;
-  int nb = (int)((tot * 64 + 255) / 256);
-  if (nb > 4096) nb = 4096;
-  hipLaunchKernelGGL(gram_reduce_kernel, dim3(nb < 1 ? 1 : nb), dim3(256), 0, st, part, g.nchunk, nm, g.np, nst,
-                     scale, out);
-  return launch_status();
+  for (int iq = 0; iq < 2 && rc == HFB_OK; ++iq) {
+    if (nst[iq] <= 0) continue;
+    const int64_t tot = (int64_t)nm * nst[iq] * nst[iq];
+    int nb = (int)((tot * 64 + 255) / 256);
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3(nb < 1 ? 1 : nb), dim3(256), 0, st, part[iq], g.nchunk, nm,
+                       g.iso[iq].np, nst[iq], scale, out[iq]);
+    rc = launch_status();
+  }
+  return rc;
+}
+
+// One isospin block (kept for callers of a single block).
+extern "C" int hfb_gram(int prec, const void* A, const void* const* B, int nm, int nst, long long K, double scale,
+                        double* part, double* out, void* stream) {
+  if (!B) return HFB_E_ARG;
+  const void* As[2] = {A, nullptr};
+  const void* Bs[8] = {nullptr};
+  for (int m = 0; m < nm && m < 4; ++m) Bs[m] = B[m];
+  const int ns[2] = {nst, 0};
+  double* parts[2] = {part, nullptr};
+  double* outs[2] = {out, nullptr};
+  return hfb_gram2(prec, As, Bs, nm, ns, K, scale, parts, outs, stream);
 }
 
 extern "C" int hfb_rotate(int prec, const void* psi, const double* U1, const double* U2, void* out1, void* out2,
@@ -337,35 +399,37 @@ extern "C" int hfb_rotate(int prec, const void* psi, const double* U1, const dou
   if (!psi || !U1 || !out1 || nst <= 0 || K <= 0) return HFB_E_ARG;
   if (U2 && !out2) return HFB_E_ARG;
   const int np = (nst + 15) & ~15;
-  constexpr int NJT = 4;
-  const int jgroups = (np / 16 + NJT - 1) / NJT;
+  const int ntj = np / 16;
+  const int njt = ntj < 4 ? ntj : 4;  // j-tiles per workgroup column
+  const int jgroups = (ntj + njt - 1) / njt;
   if (jgroups > 1 && out1 == psi) return HFB_E_ARG;  // in place only when one column of blocks covers all j
+  if (ntj % njt != 0) return HFB_E_SIZE;
   const size_t lds = sizeof(double) * 2 * 2 * (size_t)np * np;
-  if (lds > 160 * 1024) return HFB_E_SIZE;
+  if (lds > 152 * 1024) return HFB_E_SIZE;
   const int64_t nblk = (K + 15) / 16;
   int gx = (int)((nblk + 4 * 2 - 1) / (4 * 2));  // two 16-point blocks per wavefront
   if (gx < 1) gx = 1;
   hipStream_t st = (hipStream_t)stream;
-#define HFB_RT(TT, NI)                                                                                              \
-  {                                                                                                                 \
-    auto kern = rotate_kernel<TT, NJT, NI>;                                                                         \
-    if (lds > 32 * 1024) allow_full_lds((const void*)kern); \
-    hipLaunchKernelGGL(kern, dim3(gx, jgroups), dim3(256), lds, st, (const cx<TT>*)psi, U1, U2, (cx<TT>*)out1,        \
-                       (cx<TT>*)out2, nst, (int64_t)K);                                                             \
+#define HFB_RT(TT, NJ)                                                                                          \
+  {                                                                                                             \
+    auto kern = rotate_kernel<TT, NJ>;                                                                          \
+    if (lds > 32 * 1024) allow_full_lds((const void*)kern);                                                     \
+    hipLaunchKernelGGL(kern, dim3(gx, jgroups), dim3(256), lds, st, (const cx<TT>*)psi, U1, U2, (cx<TT>*)out1,    \
+                       (cx<TT>*)out2, nst, (int64_t)K);                                                         \
   }
-  const int nis = np / 4;
-  if (nis > 32) return HFB_E_SIZE;
+#define HFB_RTJ(TT)                 \
+  if (njt == 1) HFB_RT(TT, 1)       \
+  else if (njt == 2) HFB_RT(TT, 2)  \
+  else if (njt == 3) HFB_RT(TT, 3)  \
+  else HFB_RT(TT, 4)
   if (prec == HFB_F64) {
-    if (nis <= 8) HFB_RT(double, 8)
-    else if (nis <= 16) HFB_RT(double, 16)
-    else HFB_RT(double, 32)
+    HFB_RTJ(double)
   } else if (prec == HFB_F32) {
-    if (nis <= 8) HFB_RT(float, 8)
-    else if (nis <= 16) HFB_RT(float, 16)
-    else HFB_RT(float, 32)
+    HFB_RTJ(float)
   } else {
     return HFB_E_ARG;
   }
+#undef HFB_RTJ
 #undef HFB_RT
   return launch_status();
 }

@@ -485,7 +485,10 @@ extern "C" int hfb_diag_small(int nst0, int nst1, int nmax, const double* const*
   const size_t lds = sizeof(cx<double>) * (nbig <= NLDS_ALL ? 5 : 2) * (size_t)nbig * nbig;
   hipStream_t st = (hipStream_t)stream;
   if (lds > 32 * 1024) allow_full_lds((const void*)diag_kernel);
-  hipLaunchKernelGGL(diag_kernel, dim3(2), dim3(256), lds, st, a);
+  // one wavefront per isospin for small blocks: the Jacobi sweeps are
+  // barrier-bound and a single-wave barrier is nearly free
+  const int nthr = nbig <= 32 ? 64 : 256;
+  hipLaunchKernelGGL(diag_kernel, dim3(2), dim3(nthr), lds, st, a);
   int rc = launch_status();
   if (rc != HFB_OK) return rc;
   if (construct) {

@@ -192,6 +192,38 @@ def gram(ctx, A, Bs, scale, out):
     return out
 
 
+def gram2(ctx, blocks, scale, outs):
+    """Both isospin blocks in one launch: blocks[iq] = (A, [B_m]) and
+    outs[iq] (nm, nst, nst) complex128 receives scale * A^H B_m."""
+    A0 = blocks[0][0]
+    prec = _lib.prec_of(A0.dtype)
+    K = A0[0].numel()
+    nm = len(blocks[0][1])
+    As = (c_void_p * 2)()
+    Bs = (c_void_p * 8)()
+    parts = (c_void_p * 2)()
+    op = (c_void_p * 2)()
+    ns = (ctypes.c_int * 2)()
+    for iq, (A, Bl) in enumerate(blocks):
+        nst = A.shape[0]
+        ns[iq] = nst
+        if nst == 0:
+            continue
+        if len(Bl) != nm:
+            raise ValueError("both isospin blocks need the same number of right-hand blocks")
+        for b in Bl:
+            need(b, A.dtype, A.shape, "B")
+        need(outs[iq], torch.complex128, (nm, nst, nst), "gram_out")
+        As[iq] = A.data_ptr()
+        for m, b in enumerate(Bl):
+            Bs[iq * 4 + m] = b.data_ptr()
+        part = ctx.work(("gram_part", iq, nst), (lib().hfb_gram_workspace(nst, K),), torch.float64)
+        parts[iq] = part.data_ptr()
+        op[iq] = outs[iq].data_ptr()
+    check(lib().hfb_gram2(prec, As, Bs, nm, ns, K, float(scale), parts, op, stream_handle()), "gram2")
+    return outs
+
+
 def rotate(A, U1, U2, out1, out2):
     """out1 = A U1, out2 = A U2 over states (psi_2d @ unitary, static.py:316, 557)."""
     prec = _lib.prec_of(A.dtype)

@@ -197,7 +197,7 @@ def diagstep(energies, forces, grids, levels, static, diagonalize=False, constru
     n1 = int(levels.npsi[1]) if npsi_proton is None else int(npsi_proton)
     nmax = static.hmatrix.shape[-1]
     use_gap = forces.ipair != 0 and construct
-    mats, uo = [], []
+    mats, uo, blocks, outs = [], [], [], []
     for iq, (a, b) in enumerate(((0, n0), (n0, n0 + n1))):
         nst = b - a
         A = levels.psi[a:b]
@@ -207,10 +207,13 @@ def diagstep(energies, forces, grids, levels, static, diagonalize=False, constru
         if use_gap:
             Bs.append(levels.delpsi[a:b])
         M = ctx.work(("gram_out", iq, len(Bs), nst), (len(Bs), nst, nst), torch.complex128)
-        kernels.gram(ctx, A, Bs, grids.wxyz, M)
+        blocks.append((A, Bs))
+        outs.append(M)
         mats += [M[0], M[1], M[2] if diagonalize else None, M[len(Bs) - 1] if use_gap else None]
         uo += [ctx.work(("U", iq, nst), (nst, nst), torch.complex128), ctx.work(("M2", iq, nst), (nst, nst),
                                                                               torch.complex128)]
+    # overlap, H_old, lambda_lin and G_old of both isospins in one pass over the states
+    kernels.gram2(ctx, blocks, grids.wxyz, outs)
     scratch = ctx.work(("diag_scratch", nmax), (6, nmax, nmax), torch.complex128)
     kernels.diag_small(n0, n1, nmax, mats, diagonalize, forces.ipair != 0, forces.tbcs, construct, levels, static,
                        energies.vec, uo, scratch)
