@@ -105,6 +105,193 @@ int hfb_reduce_partials(const double* partial, int nstates, int ntile, int nv, d
  * a (16x4) and b (4x16) row-major doubles in, d = a*b (16x16) out. */
 int hfb_selftest_mfma_f64(const double* a, const double* b, double* d, void* stream);
 
+
+/* ---------------------------------------------------------------------
+ * Coulomb (coulomb.py:54-75, poisson)
+ * --------------------------------------------------------------------- */
+
+/* wcoul (nx,ny,nz) = Re IFFT3(e2 * wxyz * qhat * FFT3(rho[1] zero-padded on
+ * the doubled grid))[:nx,:ny,:nz]; rho is the (2,nx,ny,nz) density, only
+ * rho[1] is read.  work: (2nx)(2ny)(2nz) complex doubles. */
+int hfb_poisson(hfb_ctx* ctx, const double* rho, double* wcoul, double e2, void* work, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Gradient step (static.py:93-259: grstep, grstep_helper, e0dmp_gt_zero;
+ * levels.py:397-437: laplace)
+ * --------------------------------------------------------------------- */
+
+/* Per-state weights of grstep_helper (static.py:174-185): w = wstates*wocc,
+ * wuv = wstates*wguv*pairwg (w=1, wuv=0 when tbcs), and the preconditioner
+ * parameters of e0dmp_gt_zero (static.py:131-143): pw = weight of the
+ * (E0 - h2ma k^2) term, pshift = 1/2 max(wuv,.1) max_r v_pair[iq].
+ * v_pair: (2, nvox) double. */
+int hfb_grstep_weights(int nstates, int npsi0, int tbcs, const double* wocc, const double* wstates,
+                       const double* wguv, const double* pairwg, const double* v_pair, long long nvox,
+                       double* w, double* wuv, double* pw, double* pshift, void* stream);
+
+/* Number of partial-sum tiles per field written by hfb_precond_step. */
+int hfb_step_ntiles(hfb_ctx* ctx, int prec);
+
+/* In place psi <- psi - x0dmp * laplace(r) for nstates two-spinors
+ * (e0dmp_gt_zero, static.py:93-146, with laplace levels.py:397-437):
+ *   h_exp = wxyz * hsum[2s+1]   (Re<psi|ps1>, from hfb_hpsi's partials)
+ *   r = ps1 - lagrange          if use_lagrange   (iteration > 1 and !tbcs)
+ *   r = ps1 - h_exp psi         otherwise
+ *   laplace: FFT3 r / (pw (e0dmp - h2ma k^2) + pshift), Nyquist kept.
+ * fpart (optional, double): per state and tile (|r|^2, |lagrange - h_exp
+ * psi|^2) partial sums for sp_efluct1/2 (static.py:120-129); reduce with
+ * hfb_reduce_partials.  work: one spinor block of prec's complex type. */
+int hfb_precond_step(hfb_ctx* ctx, int prec, void* psi, const void* ps1, const void* lagrange, int nstates,
+                     const double* hsum, double wxyz, int use_lagrange, const double* pw, const double* pshift,
+                     double e0dmp, double h2ma, double x0dmp, double* fpart, void* work, void* stream);
+
+/* Scalar tail of grstep (static.py:204-213, 250-257): sp_energy <- wxyz *
+ * hsum[2s]; denerg = (spe_old - spe_new)/|spe_new|; sp_efluct1/2 <-
+ * sqrt(fsum) when fsum is given (mprint iterations); acc[0] += sum
+ * sp_efluct1 (static.sumflu), acc[1] += sum wocc*wstates*denerg
+ * (static.delesum). */
+int hfb_grstep_finish(int nstates, const double* hsum, double wxyz, double* sp_energy, const double* wocc,
+                      const double* wstates, const double* fsum, double* sp_efluct1, double* sp_efluct2,
+                      double* acc, void* stream);
+
+/* ---------------------------------------------------------------------
+ * diagstep (static.py:261-577): Gram matrices on MFMA, small dense
+ * eigenproblems, rotation of the state block
+ * --------------------------------------------------------------------- */
+
+/* Doubles of partial-sum workspace hfb_gram / hfb_gram2 need for one block
+ * of nst states of K = 2*nx*ny*nz complex entries. */
+int hfb_gram_workspace(int nst, long long K);
+
+/* out[m] = scale * A^H B[m] (m < nm <= 4), A and B[m] (nst, K) blocks of
+ * prec's complex type, out (nm, nst, nst) complex128: the psi_2d^H psi_2d,
+ * psi_2d^H hampsi, psi_2d^H hmfpsi and psi_2d^H delpsi products of
+ * loewdin_orthonormalize (static.py:470), diagstep (static.py:305-310) and
+ * construct_hfb_matrices (static.py:503-529), with scale = wxyz. */
+int hfb_gram(int prec, const void* A, const void* const* B, int nm, int nst, long long K, double scale,
+             double* part, double* out, void* stream);
+
+/* hfb_gram for both isospin blocks in one launch: A[iq], B[iq*4 + m],
+ * nst[iq], part[iq], out[iq]; a block with nst[iq] == 0 is skipped. */
+int hfb_gram2(int prec, const void* const* A, const void* const* B, int nm, const int* nst, long long K,
+              double scale, double* const* part, double* const* out, void* stream);
+
+/* Loewdin orthonormalisation, optional subspace diagonalisation and the HFB
+ * matrices for both isospins (static.py:463-482 loewdin_orthonormalize with
+ * its eigenvalue clamp; static.py:296-316 lower-triangle eigh of
+ * lambda_lin; static.py:484-577 construct_hfb_matrices).  mats[iq*4 + m]:
+ * m=0 S, 1 H_old, 2 lambda_lin (or null), 3 G_old (or null), each
+ * nst_iq x nst_iq complex128 from hfb_gram2.  uo[iq*2] receives the
+ * unitary U (real and imaginary planes for hfb_rotate), uo[iq*2+1] the
+ * matrix lambda+ that maps the rotated states to lagrange.  Writes sp_norm,
+ * sp_energy, deltaf (ipair), hmatrix, gapmatrix, symcond, lambda_save
+ * (2, nmax, nmax) complex128 and efluct1q/2q (2), efluct1/2 (1).
+ * scratch: 6 * nmax^2 complex doubles.  nmax <= 64. */
+int hfb_diag_small(int nst0, int nst1, int nmax, const double* const* mats, int tdiag, int ipair, int tbcs,
+                   int construct, const double* wocc, const double* wstates, const double* wguv,
+                   const double* pairwg, double* sp_norm, double* sp_energy, double* deltaf, double* hmatrix,
+                   double* gapmatrix, double* symcond, double* lambda_save, double* efluct1q, double* efluct2q,
+                   double* efluct1, double* efluct2, double* const* uo, double* scratch, void* stream);
+
+/* out1 = psi_2d U1 and out2 = psi_2d U2 over the nst states of one block
+ * (static.py:316 psi_2d @ unitary; static.py:557 lagrange = psi_new @
+ * lambda+), psi (nst, K).  out1 may alias psi (in place, nst <= 64);
+ * U2/out2 may be null. */
+int hfb_rotate(int prec, const void* psi, const double* U1, const double* U2, void* out1, void* out2, int nst,
+               long long K, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Densities and single-particle properties (densities.py:177-337,
+ * inout.py:64-149)
+ * --------------------------------------------------------------------- */
+
+/* Sizes in doubles of the dout scratch of hfb_density and of its per-state
+ * partial sums. */
+int hfb_density_sizes(hfb_ctx* ctx, int prec, long long* dout_doubles, long long* spart_doubles_per_state);
+
+/* One sweep over the states of both isospins (nst0 + nst1 of them) that
+ * accumulates, with weights w = wocc*wstates and wuv = wguv*pairwg*wstates,
+ * rho, chi, tau, current, sdens and sodens (add_density with
+ * compute_single_contribution, densities.py:177-337) into dout when
+ * want_dens, and the per-state sums of sp_properties (inout.py:64-149:
+ * kinetic, orbital about cmtot[3], spin, parity) into spart when want_sp. */
+int hfb_density(hfb_ctx* ctx, int prec, const void* psi, int nst0, int nst1, const double* wocc,
+                const double* wstates, const double* wguv, const double* pairwg, const double* cmtot,
+                int want_dens, int want_sp, double* dout, double* spart, void* stream);
+
+/* dout -> the Densities fields rho, chi, tau (2,n3), current, sdens, sodens
+ * (2,3,n3).  mix: x <- addnew*x_new + (1-addnew)*x for rho, tau, chi
+ * (static.py:1841-1848); the other fields are replaced. */
+int hfb_density_finish(hfb_ctx* ctx, const double* dout, double* rho, double* chi, double* tau, double* current,
+                       double* sdens, double* sodens, int mix, double addnew, void* stream);
+
+/* spart -> sp_kinetic (S), sp_orbital (S,3), sp_spin (S,3), sp_parity (S)
+ * with the wxyz and h2m_q factors of inout.py:140-148. */
+int hfb_sp_finish(hfb_ctx* ctx, const double* spart, int nst0, int nst1, double h2m0, double h2m1,
+                  double* sp_kinetic, double* sp_orbital, double* sp_spin, double* sp_parity, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Potentials and energies (meanfield.py:163-391, energies.py:90-371)
+ * --------------------------------------------------------------------- */
+
+/* Derived density fields shared by skyrme and integ_energy, each per
+ * isospin: divj = div sodens (meanfield.py:195-200), lap = Laplacian rho
+ * (meanfield.py:227-231), grad = grad rho (meanfield.py:262-268), curls =
+ * curl sdens (meanfield.py:279-289), curlj = curl current
+ * (meanfield.py:304-314), with the sder/sder2 matrices of grids.py:36-63. */
+int hfb_density_derivs(hfb_ctx* ctx, const double* rho, const double* sdens, const double* current,
+                       const double* sodens, double* divj, double* lap, double* grad, double* curls,
+                       double* curlj, void* stream);
+
+/* divaq = div aq and dbmass = grad bmass (meanfield.py:329-344). */
+int hfb_meanfield_derivs(hfb_ctx* ctx, const double* aq, const double* bmass, double* divaq, double* dbmass,
+                         void* stream);
+
+/* Doubles of the epart scratch of hfb_skyrme (x20 for hfb_integ_energy). */
+int hfb_skyrme_nblocks(hfb_ctx* ctx);
+
+/* Mean-field potentials upot, bmass, v_pair (2,n3), wlspot, aq, spot
+ * (2,3,n3) from the densities, the derived fields and wcoul (skyrme,
+ * meanfield.py:163-391, incl. the DDDI rearrangement and ecorrp into
+ * E[HFB_EN_ECORRP]).  The *32 outputs are float copies for the FP32
+ * spinor kernels and may be null. */
+int hfb_skyrme(hfb_ctx* ctx, const hfb_force* f, const double* rho, const double* tau, const double* chi,
+               const double* current, const double* sdens, const double* divj, const double* lap,
+               const double* grad, const double* curls, const double* curlj, const double* wcoul, double* upot,
+               double* bmass, double* wlspot, double* aq, double* spot, double* v_pair, float* upot32,
+               float* bmass32, float* wlspot32, float* aq32, float* spot32, float* v_pair32, double* epart,
+               double* E, void* stream);
+
+/* Energy-functional integrals of integ_energy (energies.py:90-306) into the
+ * slots of E, incl. the C-coupling split and ehfint; pairing energies are
+ * read from E[HFB_EN_EPAIR..+1]. */
+int hfb_integ_energy(hfb_ctx* ctx, const hfb_force* f, const double* rho, const double* tau,
+                     const double* current, const double* sdens, const double* divj, const double* lap,
+                     const double* curlj, const double* wcoul, double* epart, double* E, void* stream);
+
+/* ehf, tke, orbital/spin/total angular momentum of sum_energy
+ * (energies.py:309-371) into E. */
+int hfb_sum_energy(int nstates, const double* wocc, const double* wstates, const double* sp_kinetic,
+                   const double* sp_energy, const double* sp_orbital, const double* sp_spin, double* E,
+                   void* stream);
+
+/* ---------------------------------------------------------------------
+ * BCS pairing (pairs.py:25-220)
+ * --------------------------------------------------------------------- */
+
+/* pair for both isospins: if const_gap >= 0 every deltaf is first set to
+ * it (pairgap, pairs.py:135-137); then per isospin the Fermi energy of
+ * sum clip(v2) wstates = N_q (the reference's brentq on [-100,100],
+ * pairs.py:47-61; here a bisection to the same 1e-14 tolerance), v2, uv,
+ * wocc, soft cutoffs, and eferm, epair, avdelt, avdeltv2, avg (2) into
+ * their arrays and epair into E[HFB_EN_EPAIR..+1] (pairs.py:66-131).
+ * status[iq] != 0 flags a bracket without a sign change (the reference's
+ * brentq ValueError). */
+int hfb_bcs(int nst0, int nst1, double nneut, double nprot, const double* pair_cutoff,
+            const double* state_cutoff, double softcut, double const_gap, const double* sp_energy,
+            double* deltaf, double* wocc, double* wguv, double* pairwg, double* wstates, double* eferm,
+            double* epair, double* avdelt, double* avdeltv2, double* avg, double* E, int* status, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
+from . import _lib
 from . import static as st_mod
 from .coulomb import init_coulomb
 from .densities import init_densities
@@ -102,6 +103,9 @@ class HFRun:
         if precision not in ("f64", "mixed"):
             raise ValueError(f"precision must be 'f64' or 'mixed', not {precision!r}")
         dev = torch.device(device)
+        if dev.type != "cuda":
+            raise RuntimeError(f"HFRun runs on a HIP device (got {dev}); there is no CPU path")
+        _lib.require_gpu()
         torch.cuda.set_device(dev)
         params = init_params(imode=1, mprint=mprint)
         forces = init_forces(params, name=name, ipair=ipair, tbcs=tbcs, v0prot=v0, v0neut=v0)

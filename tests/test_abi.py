@@ -1,0 +1,156 @@
+"""The C ABI (include/hfb_hip.h) and the host layer, without a GPU.
+
+Loads libhfbhip.so, checks that every entry declared in the header is
+exported and bound by the ctypes layer, that the mirrored structs and enums
+agree with the header, that the argument checks return the documented codes
+(no device work is launched by those calls), and that the product path
+refuses to run without a HIP device instead of falling back to the CPU."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "hfb_hip.h"
+
+
+def header_entries():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int)\s+(hfb_\w+)\s*\(", text, flags=re.M)))
+
+
+def header_enum(name):
+    text = HEADER.read_text()
+    m = re.search(rf"\b{name}\s*=\s*(-?\d+)", text)
+    return int(m.group(1))
+
+
+@pytest.fixture(scope="module")
+def libmod(pkg):
+    so = ROOT / "jax-hfbfft_amd" / "libhfbhip.so"
+    if not so.exists():
+        import __graft_entry__ as ge
+        ge.build()
+    return pkg._lib
+
+
+def test_header_declares_entries():
+    names = header_entries()
+    assert len(names) >= 29
+    for must in ("hfb_ctx_create", "hfb_hpsi", "hfb_poisson", "hfb_precond_step", "hfb_gram2", "hfb_density",
+                 "hfb_skyrme", "hfb_integ_energy", "hfb_bcs"):
+        assert must in names
+
+
+def test_library_exports_every_header_entry(libmod):
+    L = ctypes.CDLL(str(libmod.LIB_PATH))
+    missing = [n for n in header_entries() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_ctypes_layer_binds_every_header_entry(libmod):
+    assert sorted(libmod.EXPORTED) == header_entries()
+    L = libmod.lib()
+    for n in header_entries():
+        assert getattr(L, n).restype is not None
+
+
+def test_error_codes_and_strerror(libmod):
+    L = libmod.lib()
+    for name, code in (("HFB_OK", 0), ("HFB_E_ARG", -1), ("HFB_E_SIZE", -2), ("HFB_E_HIP", -3), ("HFB_E_NOCTX", -4)):
+        assert header_enum(name) == code
+        assert L.hfb_strerror(code)
+    assert L.hfb_strerror(0) != L.hfb_strerror(-1)
+    # null-context calls return before touching the device
+    assert L.hfb_hpsi_ntiles(None, 0) == -4
+    assert L.hfb_step_ntiles(None, 0) == -4
+    assert L.hfb_skyrme_nblocks(None) == -4
+    assert L.hfb_ctx_destroy(None) in (0, -4)
+    with pytest.raises(RuntimeError):
+        libmod.check(-2, "probe")
+
+
+def test_force_struct_matches_header(pkg):
+    text = HEADER.read_text()
+    body = re.search(r"typedef struct hfb_force \{(.*?)\} hfb_force;", text, re.S).group(1)
+    doubles = []
+    ints = []
+    for line in body.strip().splitlines():
+        line = line.strip().rstrip(";")
+        typ, rest = line.split(None, 1)
+        names = [x.strip() for x in rest.split(",")]
+        (doubles if typ == "double" else ints).extend(names)
+    fields = [f[0] for f in pkg.forces.HfbForce._fields_]
+    flat = [n if not n.startswith("h2m") else "h2m" for n in doubles]
+    assert fields == flat + ints
+    assert ctypes.sizeof(pkg.forces.HfbForce) == 8 * (len(doubles) + 1) + 4 * len(ints)
+
+
+def test_energy_slots_match_header(pkg):
+    text = HEADER.read_text()
+    assert header_enum("HFB_EN_LEN") == pkg.kernels.EN_LEN
+    # the slot order of the enum
+    order = ["EHFT", "EHF0", "EHF1", "EHF2", "EHF3", "EHFLS", "EHFLSODD", "EHFC", "ECORC", "EHFINT", "E3CORR",
+             "EZPE", "CRHO0", "CRHO1", "CDRHO0", "CDRHO1", "CTAU0", "CTAU1", "CDJ0", "CDJ1", "CJ0", "CJ1", "EHF",
+             "TKE", "ORB"]
+    for i, k in enumerate(order):
+        assert pkg.kernels.EN[k] == i
+        assert f"HFB_EN_{k}" in text
+    assert pkg.kernels.EN["SPIN"] == 27 and pkg.kernels.EN["JTOT"] == 30 and pkg.kernels.EN["EFL1"] == 33
+    assert pkg.kernels.EN["EFL1Q"] == 35 and pkg.kernels.EN["EFL2Q"] == 37 and pkg.kernels.EN["ECORRP"] == 39
+    assert pkg.kernels.EN["EPAIR"] == 42
+
+
+def test_forces_match_oracle(pkg, O):
+    """The product's coupling derivation (forces.py) agrees with the oracle's."""
+    P = pkg.params.init_params()
+    for name in ("SLy4", "SV-bas"):
+        f = pkg.forces.init_forces(P, name=name)
+        g = O.init_forces(name)
+        for k in ("b0", "b0p", "b1", "b1p", "b2", "b2p", "b3", "b3p", "b4", "b4p", "power", "slate", "Crho0",
+                  "Crho1", "Crho0D", "Crho1D", "Cdrho0", "Cdrho1", "Ctau0", "Ctau1", "CdJ0", "CdJ1", "h2ma"):
+            assert getattr(f, k) == pytest.approx(getattr(g, k), rel=1e-14), (name, k)
+        assert np.allclose(f.h2m, g.h2m)
+    f5 = pkg.forces.init_forces(P, name="SV-bas", ipair=5)
+    g5 = O.init_forces("SV-bas", ipair=5)
+    assert (f5.v0neut, f5.v0prot) == (g5.v0neut, g5.v0prot)
+    with pytest.raises(KeyError):
+        pkg.forces.init_forces(P, name="no-such-force")
+
+
+def test_init_values_match_oracle(pkg, O):
+    """Build-defined initialisation (noise, deformed radii) is the same on
+    both sides, so CPU and GPU runs start from identical states."""
+    psi = np.ones((2, 2, 4, 4, 4), complex)
+    a = pkg.driver.add_noise(psi, 1e-3, 0)
+    b = O.add_noise(psi, 1e-3, 0)
+    assert np.array_equal(a, b)
+    assert pkg.driver.deformed_radii(2.0, 0.2) == pytest.approx(O.deformed_radii(2.0, 0.2))
+    assert pkg.static.shell_quantum_numbers(40) == O.shell_quantum_numbers(40)
+
+
+def test_grids_reject_odd(pkg, O):
+    """grids.py:88-110 errors; the centred coordinates match the oracle's."""
+    P = pkg.params.init_params()
+    with pytest.raises(ValueError):
+        pkg.grids.init_grids(P, nx=23, ny=24, nz=24, dx=1.0, dy=1.0, dz=1.0)
+    g = pkg.grids.init_grids(P, nx=24, ny=32, nz=24, dx=1.0, dy=0.8, dz=1.0)
+    h = O.init_grids(24, 32, 24, 1.0, 0.8, 1.0)
+    for a in ("x", "y", "z"):
+        assert np.allclose(np.asarray(getattr(g, a)), getattr(h, a))
+    assert g.wxyz == pytest.approx(h.wxyz)
+
+
+@pytest.mark.skipif(torch.cuda.is_available(), reason="checks the behaviour without a HIP device")
+def test_product_path_refuses_cpu(pkg):
+    """No CPU fallback: the product raises when there is no HIP device."""
+    with pytest.raises(RuntimeError):
+        pkg._lib.require_gpu()
+    with pytest.raises(RuntimeError):
+        pkg.driver.HFRun.from_config(precision="f64", device="cpu", name="SV-bas", n=24, d=1.0, nneut=8, nprot=8)
+    with pytest.raises(RuntimeError):
+        pkg.driver.HFRun.from_config(precision="f64", device="cuda:0", name="SV-bas", n=24, d=1.0, nneut=8,
+                                     nprot=8)
