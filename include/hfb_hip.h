@@ -58,7 +58,8 @@ enum {
   HFB_EN_CJ1, HFB_EN_EHF, HFB_EN_TKE, HFB_EN_ORB /* 3 */, HFB_EN_SPIN = HFB_EN_ORB + 3,
   HFB_EN_JTOT = HFB_EN_SPIN + 3, HFB_EN_EFL1 = HFB_EN_JTOT + 3, HFB_EN_EFL2, HFB_EN_EFL1Q /* 2 */,
   HFB_EN_EFL2Q = HFB_EN_EFL1Q + 2, HFB_EN_ECORRP = HFB_EN_EFL2Q + 2, HFB_EN_SUMFLU, HFB_EN_DELESUM,
-  HFB_EN_EPAIR /* 2 */, HFB_EN_LEN = 48
+  HFB_EN_EPAIR /* 2 */, HFB_EN_JACOBI = HFB_EN_EPAIR + 2 /* 2 */, HFB_EN_NSITER = HFB_EN_JACOBI + 2 /* 2 */,
+  HFB_EN_LEN = 48
 };
 
 const char* hfb_strerror(int code);
@@ -186,12 +187,14 @@ int hfb_gram2(int prec, const void* const* A, const void* const* B, int nm, cons
  * matrix lambda+ that maps the rotated states to lagrange.  Writes sp_norm,
  * sp_energy, deltaf (ipair), hmatrix, gapmatrix, symcond, lambda_save
  * (2, nmax, nmax) complex128 and efluct1q/2q (2), efluct1/2 (1).
- * scratch: 6 * nmax^2 complex doubles.  nmax <= 64. */
+ * nst0, nst1 <= 36 (HFB_E_SIZE above).  stats (optional, 4
+ * doubles): Jacobi sweeps of the lambda_lin eigensolve per isospin, then
+ * Newton-Schulz iterations per isospin (-1: fell back to the eigensolve). */
 int hfb_diag_small(int nst0, int nst1, int nmax, const double* const* mats, int tdiag, int ipair, int tbcs,
                    int construct, const double* wocc, const double* wstates, const double* wguv,
                    const double* pairwg, double* sp_norm, double* sp_energy, double* deltaf, double* hmatrix,
                    double* gapmatrix, double* symcond, double* lambda_save, double* efluct1q, double* efluct2q,
-                   double* efluct1, double* efluct2, double* const* uo, double* scratch, void* stream);
+                   double* efluct1, double* efluct2, double* const* uo, double* stats, void* stream);
 
 /* out1 = psi_2d U1 and out2 = psi_2d U2 over the nst states of one block
  * (static.py:316 psi_2d @ unitary; static.py:557 lagrange = psi_new @
@@ -291,6 +294,31 @@ int hfb_bcs(int nst0, int nst1, double nneut, double nprot, const double* pair_c
             const double* state_cutoff, double softcut, double const_gap, const double* sp_energy,
             double* deltaf, double* wocc, double* wguv, double* pairwg, double* wstates, double* eferm,
             double* epair, double* avdelt, double* avdeltv2, double* avg, double* E, int* status, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Energy-functional backward.  No counterpart in the reference: the
+ * derivatives of ehfint (energies.py:90-306, summed at energies.py:267)
+ * with respect to the densities and to the couplings t0..t3, x0..x3,
+ * alpha (= power), W0 (= t4) of forces.py:60-143.
+ * --------------------------------------------------------------------- */
+
+/* Blocks of the per-point kernel; part needs 72 doubles per block. */
+int hfb_energy_bwd_nblocks(hfb_ctx* ctx);
+
+/* forces: nbatch (<= 8) host structs (the derived couplings of each set);
+ * tx: device (nbatch, 10) raw couplings (t0, t1, t2, t3, x0, x1, x2, x3,
+ * power, t4); gout: device (nbatch) upstream gradients dL/dE_b.  Inputs as
+ * for hfb_integ_energy plus grad rho and curl s (hfb_density_derivs).
+ * Outputs (each may be null): d_rho, d_tau (2,n3), d_sodens, d_current,
+ * d_sdens (2,3,n3) = sum_b gout_b dE_b/d(field), which equal wxyz times
+ * the mean fields U, B, W, (spin part of) A and S of meanfield.py:178-324;
+ * d_coupling (device, nbatch x 10) = gout_b dE_b/d(coupling). */
+int hfb_energy_bwd(hfb_ctx* ctx, const hfb_force* forces, const double* tx, const double* gout, int nbatch,
+                   const double* rho, const double* tau, const double* sodens, const double* current,
+                   const double* sdens, const double* divj, const double* lap, const double* grad,
+                   const double* curls, const double* curlj, const double* wcoul, double* d_rho, double* d_tau,
+                   double* d_sodens, double* d_current, double* d_sdens, double* d_coupling, double* part,
+                   void* stream);
 
 #ifdef __cplusplus
 }

@@ -63,6 +63,9 @@ _SIGS = {
                    [c_void_p, c_void_p, c_void_p]),
     "hfb_integ_energy": (c_int, [c_void_p, c_void_p] + [c_void_p] * 8 + [c_void_p, c_void_p, c_void_p]),
     "hfb_sum_energy": (c_int, [c_int] + [c_void_p] * 7 + [c_void_p]),
+    "hfb_energy_bwd_nblocks": (c_int, [c_void_p]),
+    "hfb_energy_bwd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int] + [c_void_p] * 11 + [c_void_p] * 6 +
+                       [c_void_p, c_void_p]),
     "hfb_bcs": (c_int, [c_int, c_int, c_double, c_double, POINTER(c_double), POINTER(c_double), c_double, c_double] +
                 [c_void_p] * 13 + [c_void_p]),
 }

@@ -136,6 +136,172 @@ __device__ __forceinline__ void load_twiddles(cx<T>* tw_lds, const cx<T>* tw_glo
 }
 
 // ---------------------------------------------------------------------------
+// Lane-split pencil FFT (wavefront-shuffle butterflies).
+//
+// A pencil of length N lives in a group of L consecutive lanes; lane j of the
+// group holds x[j + L m], m < N1 = N / L, in registers.  The forward DFT is
+// the four-step split N = N1 * L:
+//   1. an N1-point DFT inside each lane (compile-time indices: the compiler
+//      keeps the pencil in VGPRs and reads twiddles through the scalar cache),
+//   2. twiddle by W_N^{j k1},
+//   3. an L-point DFT across the group as radix-2 decimation-in-frequency
+//      butterflies between lanes j and j^h (DPP quad permutes for h = 1, 2,
+//      ds_swizzle for h = 4).
+// Lane j then holds X[k1 + N1 * bitrev(j)] at element k1.  The inverse runs
+// the steps backwards (decimation-in-time across lanes on the bit-reversed
+// input, conjugate twiddles, inverse in-lane DFT) and returns N x in the
+// original layout.  Both directions are unnormalised.  tw is the table
+// exp(-2 pi i m / N), m < N, in global memory.
+// ---------------------------------------------------------------------------
+template <int H>
+__device__ __forceinline__ int lane_xor_i(int v) {
+  if constexpr (H == 1)
+    return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
+  else if constexpr (H == 2)
+    return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+  else
+    return __builtin_amdgcn_ds_swizzle(v, 0x1F | (H << 10));  // bit mode, xor H within 32 lanes
+}
+
+template <int H>
+__device__ __forceinline__ float lane_xor(float v) {
+  return __int_as_float(lane_xor_i<H>(__float_as_int(v)));
+}
+
+template <int H>
+__device__ __forceinline__ double lane_xor(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = lane_xor_i<H>((int)(b & 0xffffffffLL));
+  const int hi = lane_xor_i<H>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int H, typename T>
+__device__ __forceinline__ cx<T> lane_xor(cx<T> v) {
+  return {lane_xor<H>(v.x), lane_xor<H>(v.y)};
+}
+
+// Sum over the L lanes of a group; every lane receives the sum.
+template <int L, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+  if constexpr (L >= 2) v += lane_xor<1>(v);
+  if constexpr (L >= 4) v += lane_xor<2>(v);
+  if constexpr (L >= 8) v += lane_xor<4>(v);
+  return v;
+}
+
+// In-lane DFT of the NC values v[0..NC), X_t = sum_s v_s exp(SIGN 2 pi i s t / NC),
+// NC | N, by recursive decimation in time (radix 4, 2 or 3).
+template <typename T, int NC, int SIGN, int N>
+__device__ __forceinline__ void dft_lane(cx<T>* v, const cx<T>* __restrict__ tw) {
+  if constexpr (NC == 2 || NC == 3 || NC == 4) {
+    dft_small<T, NC, SIGN>(v);
+  } else if constexpr (NC > 4) {
+    constexpr int R = (NC % 4 == 0) ? 4 : ((NC % 2 == 0) ? 2 : 3);
+    constexpr int M = NC / R;
+    cx<T> sub[R][M];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int m = 0; m < M; ++m) sub[r][m] = v[r + R * m];
+#pragma unroll
+    for (int r = 0; r < R; ++r) dft_lane<T, M, SIGN, N>(sub[r], tw);
+#pragma unroll
+    for (int k1 = 0; k1 < M; ++k1) {
+      cx<T> u[R];
+      u[0] = sub[0][k1];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        if (k1 == 0) {
+          u[r] = sub[r][0];
+        } else {
+          cx<T> t = tw[(r * k1 * (N / NC)) % N];
+          if constexpr (SIGN > 0) t.y = -t.y;
+          u[r] = cmul(sub[r][k1], t);
+        }
+      }
+      dft_small<T, R, SIGN>(u);
+#pragma unroll
+      for (int t = 0; t < R; ++t) v[k1 + M * t] = u[t];
+    }
+  }
+}
+
+template <typename T, int N, int L>
+struct LaneFFT {
+  static constexpr int N1 = N / L;
+  static_assert(N % L == 0 && (L == 1 || L == 2 || L == 4 || L == 8), "lane split");
+  static constexpr int NS = L == 8 ? 3 : (L == 4 ? 2 : (L == 2 ? 1 : 0));
+  int j;           // position in the lane group
+  int brev;        // bit-reversed j (log2 L bits)
+  cx<T> twj[N1];   // W_N^{j k1}
+  cx<T> tws[NS + 1];  // tws[s] = W_{2h}^{j mod h}, h = 2^s
+
+  __device__ __forceinline__ void init(int jj, const cx<T>* __restrict__ tw) {
+    j = jj;
+    brev = 0;
+#pragma unroll
+    for (int b = 0; b < NS; ++b) brev |= ((jj >> b) & 1) << (NS - 1 - b);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) twj[k1] = tw[(jj * k1) % N];
+#pragma unroll
+    for (int st = 0; st < NS; ++st) {
+      const int h = 1 << st;
+      tws[st] = tw[(jj % h) * (N / (2 * h))];
+    }
+  }
+
+  // frequency index of element k1 after fwd
+  __device__ __forceinline__ int freq(int k1) const { return k1 + N1 * brev; }
+
+  template <int ST>
+  __device__ __forceinline__ void fwd_stages(cx<T>* v) const {
+    if constexpr (ST >= 0) {
+      constexpr int H = 1 << ST;
+      const bool bottom = (j & H) != 0;
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) {
+        const cx<T> u = lane_xor<H>(v[k1]);
+        const cx<T> top = cadd(v[k1], u);
+        const cx<T> bot = cmul(csub(u, v[k1]), tws[ST]);
+        v[k1] = bottom ? bot : top;
+      }
+      fwd_stages<ST - 1>(v);
+    }
+  }
+
+  template <int ST>
+  __device__ __forceinline__ void inv_stages(cx<T>* v) const {
+    if constexpr (ST < NS) {
+      constexpr int H = 1 << ST;
+      const bool bottom = (j & H) != 0;
+      const cx<T> wc = cconj(tws[ST]);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) {
+        const cx<T> sv = bottom ? cmul(v[k1], wc) : v[k1];
+        const cx<T> r = lane_xor<H>(sv);
+        v[k1] = bottom ? csub(r, sv) : cadd(v[k1], r);
+      }
+      inv_stages<ST + 1>(v);
+    }
+  }
+
+  __device__ __forceinline__ void fwd(cx<T>* v, const cx<T>* __restrict__ tw) const {
+    dft_lane<T, N1, -1, N>(v, tw);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twj[k1]);
+    fwd_stages<NS - 1>(v);
+  }
+
+  __device__ __forceinline__ void inv(cx<T>* v, const cx<T>* __restrict__ tw) const {
+    inv_stages<0>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], cconj(twj[k1]));
+    dft_lane<T, N1, +1, N>(v, tw);
+  }
+};
+
+// ---------------------------------------------------------------------------
 // Pencil tiles.  For axis a a tile holds P pencils of length N at a fixed
 // value of one transverse coordinate and P consecutive values of the other:
 //   a = x: tile (y, z0..z0+P)      element (k,p) at k*ny*nz + y*nz + z0+p

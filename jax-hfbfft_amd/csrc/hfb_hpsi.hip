@@ -142,145 +142,148 @@ struct MF {
   const T* V;  // (2, nvox)
 };
 
-template <typename T, int N, int AXIS, int P, int NT>
-__global__ void __launch_bounds__(NT) hpsi_axis_kernel(const cx<T>* __restrict__ psi, cx<T>* __restrict__ pmf,
-                                                       cx<T>* __restrict__ pout, cx<T>* __restrict__ pdel,
-                                                       MF<T> mf, const double* __restrict__ w,
-                                                       const double* __restrict__ wuv, int npsi0, int nx, int ny,
-                                                       int nz, T d, const cx<T>* __restrict__ twg,
-                                                       double* __restrict__ partial) {
-  constexpr int Q = 2 * P;  // pencils: spin 0 in q < P, spin 1 in q >= P
-  __shared__ cx<T> tw[N];
-  __shared__ cx<T> sPSI[N * Q];
-  __shared__ cx<T> sD1[N * Q];
-  __shared__ cx<T> sW[3][N * Q];
-  __shared__ T mb[P];
-  __shared__ cx<T> nyq[Q];
-  __shared__ double red[(NT / 64) * 2];
-
-  const int tile = blockIdx.x;
+// One axis of h psi for the pencils of one state, in registers: a group of L
+// lanes holds one pencil of both spin components (LaneFFT layout), the
+// block's four wavefronts hold 4 * 64 / L pencils.  Per pencil:
+//   d1 = IFFT(i k FFT psi)                       (cderv00 / first half of cderv02)
+//   dd = IFFT(i k FFT(f - B d1)) with the Nyquist entry -(pi/d)^2 mean(B) psi_N / N
+//      = d(f) - d(B d1)                          (meanfield.py:61-145, levels.py:283-395)
+// where f is the axis' spin-orbit / current combination, then the pointwise
+// terms.  The x pass adds U psi and the spin-current coupling, the z pass
+// the weighting w h psi - wuv v_pair psi and the overlap partial sums.
+template <typename T, int N, int AXIS, int L>
+__global__ void __launch_bounds__(256) hpsi_lane_kernel(const cx<T>* __restrict__ psi, cx<T>* __restrict__ pmf,
+                                                        cx<T>* __restrict__ pout, cx<T>* __restrict__ pdel,
+                                                        MF<T> mf, const double* __restrict__ w,
+                                                        const double* __restrict__ wuv, int npsi0, int nx, int ny,
+                                                        int nz, T d, const cx<T>* __restrict__ tw,
+                                                        double* __restrict__ partial) {
+  constexpr int N1 = N / L;
+  constexpr int PW = 64 / L;  // pencils per wavefront
+  __shared__ double red[4 * 2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = lane % L, q = lane / L;
+  const int g = (blockIdx.x * 4 + wave) * PW + q;  // pencil
   const int s = blockIdx.y;
   const int iq = s >= npsi0 ? 1 : 0;
   const int64_t nvox = (int64_t)nx * ny * nz;
-  const cx<T>* ps0 = psi + (int64_t)(2 * s) * nvox;
+  int64_t base, stride;
+  if constexpr (AXIS == 0) {
+    base = g;
+    stride = (int64_t)ny * nz;
+  } else if constexpr (AXIS == 1) {
+    base = (int64_t)(g / nz) * ny * nz + g % nz;
+    stride = nz;
+  } else {
+    base = (int64_t)g * nz;
+    stride = 1;
+  }
+  LaneFFT<T, N, L> F;
+  F.init(j, tw);
+  const int64_t f0 = (int64_t)(2 * s) * nvox;
+  const cx<T>* ps0 = psi + f0;
   const cx<T>* ps1 = ps0 + nvox;
-  const Tile t = make_tile<AXIS>(tile, nx, ny, nz, P);
   const T* Bq = mf.B + iq * nvox;
   const T* Aa = mf.A + (iq * 3 + AXIS) * nvox;
-  const T* W0 = mf.W + (iq * 3 + 0) * nvox;
-  const T* W1 = mf.W + (iq * 3 + 1) * nvox;
-  const T* W2 = mf.W + (iq * 3 + 2) * nvox;
+  // the two spin-orbit components this axis uses (meanfield.py:61-145)
+  constexpr int WA = AXIS == 0 ? 1 : 0;
+  constexpr int WB = AXIS == 2 ? 1 : 2;
+  const T* Wa = mf.W + (iq * 3 + WA) * nvox;
+  const T* Wb = mf.W + (iq * 3 + WB) * nvox;
 
-  load_twiddles(tw, twg, N);
-  for (int e = threadIdx.x; e < N * P; e += blockDim.x) {
-    int k, p;
-    elem_of<AXIS, N, P>(e, k, p);
-    const int64_t off = t.base + k * t.sk + p * t.sp;
-    const cx<T> a = ps0[off], b = ps1[off];
-    sPSI[k * Q + p] = a;
-    sPSI[k * Q + P + p] = b;
-    sW[0][k * Q + p] = a;
-    sW[0][k * Q + P + p] = b;
+  cx<T> a[N1], b[N1], fa[N1], fb[N1], ga[N1], gb[N1];
+  T Bv[N1], Av[N1], wa[N1], wb[N1];
+  T bsum = T(0);
+#pragma unroll
+  for (int m = 0; m < N1; ++m) {
+    const int64_t off = base + (int64_t)(j + L * m) * stride;
+    a[m] = ps0[off];
+    b[m] = ps1[off];
+    Bv[m] = Bq[off];
+    Av[m] = Aa[off];
+    wa[m] = Wa[off];
+    wb[m] = Wb[off];
+    bsum += Bv[m];
+    fa[m] = a[m];
+    fb[m] = b[m];
   }
-  for (int p = threadIdx.x; p < P; p += blockDim.x) {
-    T acc = T(0);
-    for (int k = 0; k < N; ++k) acc += Bq[t.base + k * t.sk + p * t.sp];
-    mb[p] = acc / T(N);
-  }
-  __syncthreads();
-
+  const T mb = group_sum<L>(bsum) / T(N);
   const T kfac = T(2.0 * M_PI) / (d * T(N));
-  // d1 = d/da psi (cderv00 / the first half of cderv02)
-  cx<T>* r = fft_lds<T, N, Q, -1>(sW[0], sW[1], tw, Q);
-  for (int e = threadIdx.x; e < N * Q; e += blockDim.x) {
-    const int k = e / Q;
-    const cx<T> v = r[e];
-    if (k == N / 2) nyq[e % Q] = v;
-    r[e] = cmul_is(v, d1_factor<T, N>(k, kfac));
+
+  // d1 = d/da psi
+  F.fwd(fa, tw);
+  F.fwd(fb, tw);
+  const cx<T> nya = fa[0], nyb = fb[0];  // psi_N on the lane that holds k = N/2 at k1 = 0
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) {
+    const T f = d1_factor<T, N>(F.freq(k1), kfac);
+    fa[k1] = cmul_is(fa[k1], f);
+    fb[k1] = cmul_is(fb[k1], f);
   }
-  __syncthreads();
-  cx<T>* d1p = fft_lds<T, N, Q, +1>(r, other_of(r, sW[0], sW[1]), tw, Q);
-  // keep d1; stage B*d1
-  for (int e = threadIdx.x; e < N * Q; e += blockDim.x) {
-    const int k = e / Q, p = (e % Q) % P;
-    const cx<T> v = d1p[e];
-    sD1[e] = v;
-    sW[2][e] = cscale(v, Bq[t.base + k * t.sk + p * t.sp]);
-  }
-  __syncthreads();
-  // spectrum of d(B d1): ik multiplier, m=0 zero, Nyquist rule (levels.py:309-315)
-  cx<T>* x2 = fft_lds<T, N, Q, -1>(sW[2], sW[0], tw, Q);
-  const T nq = -T(M_PI / (double)d) * T(M_PI / (double)d) / T(N);
-  for (int e = threadIdx.x; e < N * Q; e += blockDim.x) {
-    const int k = e / Q, q = e % Q;
-    if (k == N / 2)
-      x2[e] = cscale(nyq[q], nq * mb[q % P]);
-    else
-      x2[e] = cmul_is(x2[e], d1_factor<T, N>(k, kfac));
-  }
-  // the spin-orbit / current combination whose derivative is added
-  cx<T>* f1 = (x2 == sW[2]) ? sW[0] : sW[2];
-  cx<T>* f2 = sW[1];
-  for (int e = threadIdx.x; e < N * P; e += blockDim.x) {
-    const int k = e / P, p = e % P;
-    const int64_t off = t.base + k * t.sk + p * t.sp;
-    const cx<T> a = sPSI[k * Q + p], b = sPSI[k * Q + P + p];
-    const T av = Aa[off];
+  F.inv(fa, tw);
+  F.inv(fb, tw);
+
+  // f - B d1 with the axis' combination f (meanfield.py:75-83, 104-112, 133-141)
+#pragma unroll
+  for (int m = 0; m < N1; ++m) {
+    const cx<T> av = a[m], bv = b[m];
+    const T A = Av[m];
     cx<T> q0, q1;
-    if constexpr (AXIS == 0) {  // meanfield.py:75-83
-      const T w1 = W1[off], w2 = W2[off];
-      q0 = csub(cmul_is(a, T(-0.5) * (av - w1)), cscale(b, T(0.5) * w2));
-      q1 = cadd(cmul_is(b, T(-0.5) * (av + w1)), cscale(a, T(0.5) * w2));
-    } else if constexpr (AXIS == 1) {  // meanfield.py:104-112
-      const T w0 = W0[off], w2 = W2[off];
-      q0 = cadd(cmul_is(a, T(-0.5) * (av + w0)), cmul_is(b, T(0.5) * w2));
-      q1 = cadd(cmul_is(b, T(-0.5) * (av - w0)), cmul_is(a, T(0.5) * w2));
-    } else {  // meanfield.py:133-141
-      const T w0 = W0[off], w1 = W1[off];
-      q0 = cadd(cmul_is(a, T(-0.5) * av), cadd(cscale(b, T(0.5) * w0), cmul_is(b, T(-0.5) * w1)));
-      q1 = cadd(cmul_is(b, T(-0.5) * av), cadd(cscale(a, T(-0.5) * w0), cmul_is(a, T(-0.5) * w1)));
+    if constexpr (AXIS == 0) {  // wa = W1, wb = W2
+      q0 = csub(cmul_is(av, T(-0.5) * (A - wa[m])), cscale(bv, T(0.5) * wb[m]));
+      q1 = cadd(cmul_is(bv, T(-0.5) * (A + wa[m])), cscale(av, T(0.5) * wb[m]));
+    } else if constexpr (AXIS == 1) {  // wa = W0, wb = W2
+      q0 = cadd(cmul_is(av, T(-0.5) * (A + wa[m])), cmul_is(bv, T(0.5) * wb[m]));
+      q1 = cadd(cmul_is(bv, T(-0.5) * (A - wa[m])), cmul_is(av, T(0.5) * wb[m]));
+    } else {  // wa = W0, wb = W1
+      q0 = cadd(cmul_is(av, T(-0.5) * A), cadd(cscale(bv, T(0.5) * wa[m]), cmul_is(bv, T(-0.5) * wb[m])));
+      q1 = cadd(cmul_is(bv, T(-0.5) * A), cadd(cscale(av, T(-0.5) * wa[m]), cmul_is(av, T(-0.5) * wb[m])));
     }
-    f1[k * Q + p] = q0;
-    f1[k * Q + P + p] = q1;
+    ga[m] = csub(q0, cscale(fa[m], Bv[m]));
+    gb[m] = csub(q1, cscale(fb[m], Bv[m]));
   }
-  __syncthreads();
-  cx<T>* x3 = fft_lds<T, N, Q, -1>(f1, f2, tw, Q);
-  for (int e = threadIdx.x; e < N * Q; e += blockDim.x) {
-    const int k = e / Q;
-    x3[e] = csub(cmul_is(x3[e], d1_factor<T, N>(k, kfac)), x2[e]);
+  F.fwd(ga, tw);
+  F.fwd(gb, tw);
+  const T nq = -T(M_PI / (double)d) * T(M_PI / (double)d) / T(N);
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) {
+    const int kf = F.freq(k1);
+    if (kf == N / 2) {  // only d(B d1) has a Nyquist entry (levels.py:309-315)
+      ga[k1] = cscale(nya, -nq * mb);
+      gb[k1] = cscale(nyb, -nq * mb);
+    } else {
+      const T f = d1_factor<T, N>(kf, kfac);
+      ga[k1] = cmul_is(ga[k1], f);
+      gb[k1] = cmul_is(gb[k1], f);
+    }
   }
-  __syncthreads();
-  cx<T>* dd = fft_lds<T, N, Q, +1>(x3, other_of(x3, f1, f2), tw, Q);  // d3 - d2
+  F.inv(ga, tw);
+  F.inv(gb, tw);
 
   double acc[2] = {0.0, 0.0};
-  const int64_t f0 = (int64_t)(2 * s) * nvox;
-  for (int e = threadIdx.x; e < N * P; e += blockDim.x) {
-    int k, p;
-    elem_of<AXIS, N, P>(e, k, p);
-    const int64_t off = t.base + k * t.sk + p * t.sp;
-    const cx<T> a = sPSI[k * Q + p], b = sPSI[k * Q + P + p];
-    const cx<T> da = sD1[k * Q + p], db = sD1[k * Q + P + p];
-    cx<T> o0 = dd[k * Q + p], o1 = dd[k * Q + P + p];
-    const T av = Aa[off];
+#pragma unroll
+  for (int m = 0; m < N1; ++m) {
+    const int64_t off = base + (int64_t)(j + L * m) * stride;
+    const cx<T> av = a[m], bv = b[m];
+    const cx<T> da = fa[m], db = fb[m];
+    const T A = Av[m];
+    cx<T> o0 = ga[m], o1 = gb[m];
     if constexpr (AXIS == 0) {  // meanfield.py:63-73
-      const T w1 = W1[off], w2 = W2[off];
-      o0 = cadd(o0, csub(cmul_is(da, -(T(0.5) * av - T(0.5) * w1)), cscale(db, T(0.5) * w2)));
-      o1 = cadd(o1, cadd(cmul_is(db, -(T(0.5) * av + T(0.5) * w1)), cscale(da, T(0.5) * w2)));
+      o0 = cadd(o0, csub(cmul_is(da, -(T(0.5) * A - T(0.5) * wa[m])), cscale(db, T(0.5) * wb[m])));
+      o1 = cadd(o1, cadd(cmul_is(db, -(T(0.5) * A + T(0.5) * wa[m])), cscale(da, T(0.5) * wb[m])));
       // meanfield.py:47-58: U psi and the spin-current coupling
       const T u = mf.U[iq * nvox + off];
       const T s0 = mf.S[(iq * 3 + 0) * nvox + off];
       const T s1 = mf.S[(iq * 3 + 1) * nvox + off];
       const T s2 = mf.S[(iq * 3 + 2) * nvox + off];
-      o0 = cadd(o0, cadd(cscale(a, u + s2), cadd(cscale(b, s0), cmul_is(b, -s1))));
-      o1 = cadd(o1, cadd(cscale(b, u - s2), cadd(cscale(a, s0), cmul_is(a, s1))));
+      o0 = cadd(o0, cadd(cscale(av, u + s2), cadd(cscale(bv, s0), cmul_is(bv, -s1))));
+      o1 = cadd(o1, cadd(cscale(bv, u - s2), cadd(cscale(av, s0), cmul_is(av, s1))));
     } else if constexpr (AXIS == 1) {  // meanfield.py:92-102
-      const T w0 = W0[off], w2 = W2[off];
-      o0 = cadd(o0, cadd(cmul_is(da, -(T(0.5) * av + T(0.5) * w0)), cmul_is(db, T(0.5) * w2)));
-      o1 = cadd(o1, cadd(cmul_is(db, -(T(0.5) * av - T(0.5) * w0)), cmul_is(da, T(0.5) * w2)));
+      o0 = cadd(o0, cadd(cmul_is(da, -(T(0.5) * A + T(0.5) * wa[m])), cmul_is(db, T(0.5) * wb[m])));
+      o1 = cadd(o1, cadd(cmul_is(db, -(T(0.5) * A - T(0.5) * wa[m])), cmul_is(da, T(0.5) * wb[m])));
     } else {  // meanfield.py:121-131
-      const T w0 = W0[off], w1 = W1[off];
-      o0 = cadd(o0, cadd(cmul_is(da, -T(0.5) * av), cadd(cscale(db, T(0.5) * w0), cmul_is(db, T(-0.5) * w1))));
-      o1 = cadd(o1, cadd(cmul_is(db, -T(0.5) * av), cadd(cscale(da, T(-0.5) * w0), cmul_is(da, T(-0.5) * w1))));
+      o0 = cadd(o0, cadd(cmul_is(da, -T(0.5) * A), cadd(cscale(db, T(0.5) * wa[m]), cmul_is(db, T(-0.5) * wb[m]))));
+      o1 = cadd(o1, cadd(cmul_is(db, -T(0.5) * A), cadd(cscale(da, T(-0.5) * wa[m]), cmul_is(da, T(-0.5) * wb[m]))));
     }
     if constexpr (AXIS != 0) {
       o0 = cadd(o0, pmf[f0 + off]);
@@ -290,7 +293,7 @@ __global__ void __launch_bounds__(NT) hpsi_axis_kernel(const cx<T>* __restrict__
     pmf[f0 + nvox + off] = o1;
     if constexpr (AXIS == 2) {  // meanfield.py:150-157
       const T v = mf.V[iq * nvox + off];
-      const cx<T> e0 = cscale(a, v), e1 = cscale(b, v);
+      const cx<T> e0 = cscale(av, v), e1 = cscale(bv, v);
       if (pdel) {
         pdel[f0 + off] = e0;
         pdel[f0 + nvox + off] = e1;
@@ -302,50 +305,57 @@ __global__ void __launch_bounds__(NT) hpsi_axis_kernel(const cx<T>* __restrict__
         pout[f0 + off] = r0;
         pout[f0 + nvox + off] = r1;
       }
-      acc[0] += double(a.x) * double(o0.x) + double(a.y) * double(o0.y) + double(b.x) * double(o1.x) +
-                double(b.y) * double(o1.y);
-      acc[1] += double(a.x) * double(r0.x) + double(a.y) * double(r0.y) + double(b.x) * double(r1.x) +
-                double(b.y) * double(r1.y);
+      acc[0] += double(av.x) * double(o0.x) + double(av.y) * double(o0.y) + double(bv.x) * double(o1.x) +
+                double(bv.y) * double(o1.y);
+      acc[1] += double(av.x) * double(r0.x) + double(av.y) * double(r0.y) + double(bv.x) * double(r1.x) +
+                double(bv.y) * double(r1.y);
     }
   }
   if constexpr (AXIS == 2) {
     if (partial) {
       block_sum<2>(acc, red);
       if (threadIdx.x == 0) {
-        const int ntile = gridDim.x;
-        partial[((int64_t)s * ntile + tile) * 2 + 0] = acc[0];
-        partial[((int64_t)s * ntile + tile) * 2 + 1] = acc[1];
+        partial[((int64_t)s * gridDim.x + blockIdx.x) * 2 + 0] = acc[0];
+        partial[((int64_t)s * gridDim.x + blockIdx.x) * 2 + 1] = acc[1];
       }
     }
   }
 }
 
+// Lanes per pencil: FP32 pencils split over 4 lanes, FP64 over 8, which
+// keeps the six per-lane register arrays of the kernel at N/L elements.
 template <typename T>
-int pencils_per_tile(const hfb_ctx* c) {
-  if (sizeof(T) == 4 && c->nx % 16 == 0 && c->ny % 16 == 0 && c->nz % 16 == 0) return 16;
-  return 8;
+constexpr int lanes_per_pencil() { return sizeof(T) == 4 ? 4 : 8; }
+
+template <typename T>
+int hpsi_pencils_per_block() { return 4 * (64 / lanes_per_pencil<T>()); }
+
+inline int pencils_of_axis(const hfb_ctx* c, int axis) {
+  return axis == 0 ? c->ny * c->nz : (axis == 1 ? c->nx * c->nz : c->nx * c->ny);
 }
 
-template <typename T, int AXIS, int P>
+template <typename T, int AXIS>
 int launch_hpsi_axis(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T> mf, const double* w,
                      const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st) {
-  constexpr int NT = 16 * P;  // 2P pencils x 8 radix-4 butterflies of a 32-point pass
+  constexpr int L = lanes_per_pencil<T>();
+  const int ppb = hpsi_pencils_per_block<T>();
+  const int np = pencils_of_axis(c, AXIS);
+  if (np % ppb != 0) return HFB_E_SIZE;
   const int n = axis_len(c, AXIS);
-  const int nt = tiles_per_field<AXIS>(c->nx, c->ny, c->nz, P);
   HFB_DISPATCH_N(n, {
-    hipLaunchKernelGGL((hpsi_axis_kernel<T, N_, AXIS, P, NT>), dim3(nt, ns), dim3(NT), 0, st, (const cx<T>*)psi,
+    hipLaunchKernelGGL((hpsi_lane_kernel<T, N_, AXIS, L>), dim3(np / ppb, ns), dim3(256), 0, st, (const cx<T>*)psi,
                        (cx<T>*)pmf, (cx<T>*)pout, (cx<T>*)pdel, mf, w, wuv, npsi0, c->nx, c->ny, c->nz,
                        T(axis_d(c, AXIS)), twiddles<T>(c, N_), partial);
   })
   return launch_status();
 }
 
-template <typename T, int P>
-int hpsi_all(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T> mf, const double* w,
-             const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st) {
-  int rc = launch_hpsi_axis<T, 0, P>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
-  if (rc == HFB_OK) rc = launch_hpsi_axis<T, 1, P>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
-  if (rc == HFB_OK) rc = launch_hpsi_axis<T, 2, P>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
+template <typename T>
+int hpsi_all(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T> mf, const double* w, const double* wuv,
+             void* pout, void* pmf, void* pdel, double* partial, hipStream_t st) {
+  int rc = launch_hpsi_axis<T, 0>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
+  if (rc == HFB_OK) rc = launch_hpsi_axis<T, 1>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
+  if (rc == HFB_OK) rc = launch_hpsi_axis<T, 2>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
   return rc;
 }
 
@@ -364,8 +374,8 @@ extern "C" int hfb_cderv(hfb_ctx* c, int prec, int axis, int kind, const void* i
 
 extern "C" int hfb_hpsi_ntiles(hfb_ctx* c, int prec) {
   if (!c) return HFB_E_NOCTX;
-  const int P = prec == HFB_F32 ? pencils_per_tile<float>(c) : pencils_per_tile<double>(c);
-  return tiles_per_field<2>(c->nx, c->ny, c->nz, P);
+  const int ppb = prec == HFB_F32 ? hpsi_pencils_per_block<float>() : hpsi_pencils_per_block<double>();
+  return pencils_of_axis(c, 2) / ppb;
 }
 
 extern "C" int hfb_hpsi(hfb_ctx* c, int prec, const void* psi, int nstates, int npsi0, const void* upot,
@@ -380,14 +390,12 @@ extern "C" int hfb_hpsi(hfb_ctx* c, int prec, const void* psi, int nstates, int 
   if (prec == HFB_F64) {
     MF<double> mf{(const double*)upot, (const double*)bmass, (const double*)aq,
                   (const double*)spot, (const double*)wlspot, (const double*)v_pair};
-    return hpsi_all<double, 8>(c, psi, nstates, npsi0, mf, w, wuv, pout, pout_mf, pout_del, partial, st);
+    return hpsi_all<double>(c, psi, nstates, npsi0, mf, w, wuv, pout, pout_mf, pout_del, partial, st);
   }
   if (prec == HFB_F32) {
     MF<float> mf{(const float*)upot, (const float*)bmass, (const float*)aq,
                  (const float*)spot, (const float*)wlspot, (const float*)v_pair};
-    if (pencils_per_tile<float>(c) == 16)
-      return hpsi_all<float, 16>(c, psi, nstates, npsi0, mf, w, wuv, pout, pout_mf, pout_del, partial, st);
-    return hpsi_all<float, 8>(c, psi, nstates, npsi0, mf, w, wuv, pout, pout_mf, pout_del, partial, st);
+    return hpsi_all<float>(c, psi, nstates, npsi0, mf, w, wuv, pout, pout_mf, pout_del, partial, st);
   }
   return HFB_E_ARG;
 }

@@ -27,9 +27,6 @@ struct DiagIso {
   const cx<double>* Gold;  // null unless pairing
   cx<double>* U;
   cx<double>* M2;
-  cx<double>* T1;  // scratch nst x nst (used when the work matrices do not fit in LDS)
-  cx<double>* T2;
-  cx<double>* T3;
   cx<double>* Hm;  // static arrays of this isospin, leading dimension nmax
   cx<double>* Gm;
   cx<double>* Sym;
@@ -48,15 +45,60 @@ struct DiagArgs {
   double* deltaf;
   double* efl1q;
   double* efl2q;
+  double* stats;  // optional: [Jacobi sweeps (2), Newton-Schulz iterations (2, -1 = fell back)]
   int tdiag, ipair, tbcs, construct;
 };
 
-__device__ __forceinline__ cx<double> cmulc(cx<double> a, cx<double> b) { return cmul(a, b); }
+constexpr int DIAG_NT = 256;  // four wavefronts per isospin block
+
+// Work split of an n x n matrix (n <= NP) over the DIAG_NT threads: thread t
+// owns column j = t % NP and rows t / NP + RG*s.  Matrices are stored densely
+// with leading dimension n.
+#define HFB_TILE_LOOP(NP, n, i, j)                                   \
+  for (int j = threadIdx.x % (NP), i = threadIdx.x / (NP); j < (n) && i < (n); i += DIAG_NT / (NP))
+
+template <int NP, typename F>
+__device__ __forceinline__ void tile_for(int n, F&& f) {
+  constexpr int RG = DIAG_NT / NP;
+  const int j = threadIdx.x % NP;
+  if (j >= n) return;
+  for (int i = threadIdx.x / NP; i < n; i += RG) f(i, j);
+}
+
+// Sum of two values over the block, returned to every thread.
+__device__ __forceinline__ void block_sum2_all(double& a, double& b, double* red) {
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[2 * wid] = a;
+    red[2 * wid + 1] = b;
+  }
+  __syncthreads();
+  a = b = 0.0;
+  for (int w = 0; w < DIAG_NT / 64; ++w) {
+    a += red[2 * w];
+    b += red[2 * w + 1];
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ double block_max_all(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  v = red[0];
+  for (int w = 1; w < DIAG_NT / 64; ++w) v = fmax(v, red[w]);
+  __syncthreads();
+  return v;
+}
 
 // A (n x n, LDS) <- Hermitian matrix from the lower triangle of M (global).
+template <int NP>
 __device__ void load_lower_herm(cx<double>* A, const cx<double>* M, int n) {
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-    const int i = e / n, j = e % n;
+  tile_for<NP>(n, [&](int i, int j) {
     cx<double> v;
     if (i > j)
       v = M[i * n + j];
@@ -64,7 +106,45 @@ __device__ void load_lower_herm(cx<double>* A, const cx<double>* M, int n) {
       v = cconj(M[j * n + i]);
     else
       v = {M[i * n + i].x, 0.0};
-    A[e] = v;
+    A[i * n + j] = v;
+  });
+  __syncthreads();
+}
+
+// C = op(A) B for n x n matrices (C distinct from A and B); op(A) = A or A^H.
+// Each thread accumulates a column strip of NP*NP/DIAG_NT entries in
+// registers; the loads of A are broadcasts within a wavefront.
+template <int NP, bool CONJ_A>
+__device__ __forceinline__ void mm(cx<double>* __restrict__ C, const cx<double>* A, const cx<double>* B, int n) {
+  constexpr int RG = DIAG_NT / NP;
+  constexpr int TI = NP / RG;
+  const int j = threadIdx.x % NP;
+  const int r0 = threadIdx.x / NP;
+  const int jc = j < n ? j : n - 1;  // idle lanes read in bounds and store nothing
+  int ic[TI];
+  cx<double> acc[TI];
+#pragma unroll
+  for (int t = 0; t < TI; ++t) {
+    const int i = r0 + RG * t;
+    ic[t] = i < n ? i : n - 1;
+    acc[t] = {0.0, 0.0};
+  }
+#pragma unroll 2
+  for (int k = 0; k < n; ++k) {
+    const cx<double> b = B[k * n + jc];
+#pragma unroll
+    for (int t = 0; t < TI; ++t) {
+      const cx<double> a = CONJ_A ? cconj(A[k * n + ic[t]]) : A[ic[t] * n + k];
+      acc[t].x = fma(a.x, b.x, fma(-a.y, b.y, acc[t].x));
+      acc[t].y = fma(a.x, b.y, fma(a.y, b.x, acc[t].y));
+    }
+  }
+  if (j < n) {
+#pragma unroll
+    for (int t = 0; t < TI; ++t) {
+      const int i = r0 + RG * t;
+      if (i < n) C[i * n + j] = acc[t];
+    }
   }
   __syncthreads();
 }
@@ -73,34 +153,34 @@ struct JacobiShared {
   int p[NMAXJ / 2], q[NMAXJ / 2];
   double c[NMAXJ / 2], s[NMAXJ / 2];
   cx<double> e[NMAXJ / 2];
-  double red[32];
+  double red[16];
   int perm[NMAXJ];
   double ev[NMAXJ];
-  int done;
 };
 
 // Eigen-decomposition of the Hermitian A (LDS, destroyed); V (LDS) gets the
 // eigenvectors as columns, js.ev the eigenvalues; sorted ascending when
-// `sort` (T is global scratch used for the column permutation).
-__device__ void jacobi_eigh(cx<double>* A, cx<double>* V, int n, JacobiShared& js, bool sort, cx<double>* T) {
+// `sort` (T is scratch for the column permutation).  Parallel cyclic Jacobi:
+// in each round the n/2 disjoint pairs of a round-robin ordering are rotated
+// together (columns of A and V, then rows of A).
+template <int NP>
+__device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* V, int n, JacobiShared& js, bool sort, cx<double>* T) {
   const int tid = threadIdx.x;
-  for (int e = tid; e < n * n; e += blockDim.x) V[e] = cx<double>{(e / n == e % n) ? 1.0 : 0.0, 0.0};
+  tile_for<NP>(n, [&](int i, int j) { V[i * n + j] = cx<double>{i == j ? 1.0 : 0.0, 0.0}; });
   const int m = n + (n & 1);
   const int npair = m / 2;
   __syncthreads();
-  for (int sweep = 0; sweep < 40; ++sweep) {
-    double v[2] = {0.0, 0.0};
-    for (int e = tid; e < n * n; e += blockDim.x) {
-      const int i = e / n, j = e % n;
+  int sweep = 0;
+  for (; sweep < 40; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    HFB_TILE_LOOP(NP, n, i, j) {
       if (i == j)
-        v[1] += cabs2(A[e]);
+        dia += cabs2(A[i * n + j]);
       else
-        v[0] += cabs2(A[e]);
+        off += cabs2(A[i * n + j]);
     }
-    block_sum<2>(v, js.red);
-    if (tid == 0) js.done = (v[0] <= 1e-30 * v[1]) || v[0] == 0.0;
-    __syncthreads();
-    if (js.done) break;
+    block_sum2_all(off, dia, js.red);
+    if (off <= 1e-30 * dia || off == 0.0) break;
     for (int t = 0; t < m - 1; ++t) {
       if (tid < npair) {
         const int k = tid;
@@ -131,11 +211,13 @@ __device__ void jacobi_eigh(cx<double>* A, cx<double>* V, int n, JacobiShared& j
         js.e[k] = ea;
       }
       __syncthreads();
-      // columns of A and V: X <- X V2
-      for (int it = tid; it < 2 * npair * n; it += blockDim.x) {
-        const int mat = it / (npair * n);
-        const int rem = it % (npair * n);
-        const int k = rem / n, r = rem % n;
+      // columns of A and V: X <- X R for each rotated pair
+      for (int it = tid; it < NP * NP; it += DIAG_NT) {
+        const int r = it % NP;
+        const int rest = it / NP;
+        const int k = rest % (NP / 2);
+        const int mat = rest / (NP / 2);
+        if (r >= n || k >= npair) continue;
         const int p = js.p[k], q = js.q[k];
         if (p < 0 || js.s[k] == 0.0) continue;
         cx<double>* X = mat ? V : A;
@@ -145,30 +227,31 @@ __device__ void jacobi_eigh(cx<double>* A, cx<double>* V, int n, JacobiShared& j
         X[r * n + q] = cadd(cmul(xp, es), cscale(xq, js.c[k]));
       }
       __syncthreads();
-      // rows of A: A <- V2^H A
-      for (int it = tid; it < npair * n; it += blockDim.x) {
-        const int k = it / n, r = it % n;
+      // rows of A: A <- R^H A; the rotated pair's off-diagonal entries are
+      // set to zero and its diagonal to real
+      for (int it = tid; it < (NP / 2) * NP; it += DIAG_NT) {
+        const int k = it / NP, r = it % NP;
+        if (r >= n || k >= npair) continue;
         const int p = js.p[k], q = js.q[k];
         if (p < 0 || js.s[k] == 0.0) continue;
         const cx<double> xp = A[p * n + r], xq = A[q * n + r];
         const cx<double> ecc = cscale(cconj(js.e[k]), js.c[k]), esc = cscale(cconj(js.e[k]), js.s[k]);
-        A[p * n + r] = csub(cmul(ecc, xp), cscale(xq, js.s[k]));
-        A[q * n + r] = cadd(cmul(esc, xp), cscale(xq, js.c[k]));
-      }
-      __syncthreads();
-      if (tid < npair) {
-        const int p = js.p[tid], q = js.q[tid];
-        if (p >= 0 && js.s[tid] != 0.0) {
-          A[p * n + q] = {0.0, 0.0};
-          A[q * n + p] = {0.0, 0.0};
-          A[p * n + p].y = 0.0;
-          A[q * n + q].y = 0.0;
+        cx<double> np_ = csub(cmul(ecc, xp), cscale(xq, js.s[k]));
+        cx<double> nq_ = cadd(cmul(esc, xp), cscale(xq, js.c[k]));
+        if (r == q) {
+          np_ = {0.0, 0.0};
+          nq_.y = 0.0;
+        } else if (r == p) {
+          nq_ = {0.0, 0.0};
+          np_.y = 0.0;
         }
+        A[p * n + r] = np_;
+        A[q * n + r] = nq_;
       }
       __syncthreads();
     }
   }
-  for (int i = tid; i < n; i += blockDim.x) js.ev[i] = A[i * n + i].x;
+  for (int i = tid; i < n; i += DIAG_NT) js.ev[i] = A[i * n + i].x;
   __syncthreads();
   if (sort) {
     if (tid == 0) {
@@ -183,135 +266,98 @@ __device__ void jacobi_eigh(cx<double>* A, cx<double>* V, int n, JacobiShared& j
       }
     }
     __syncthreads();
-    for (int e = tid; e < n * n; e += blockDim.x) T[e] = V[(e / n) * n + js.perm[e % n]];
+    tile_for<NP>(n, [&](int i, int j) { T[i * n + j] = V[i * n + js.perm[j]]; });
     __syncthreads();
-    for (int e = tid; e < n * n; e += blockDim.x) V[e] = T[e];
+    tile_for<NP>(n, [&](int i, int j) { V[i * n + j] = T[i * n + j]; });
     double evs = 0.0;
     if (tid < n) evs = js.ev[js.perm[tid]];
     __syncthreads();
     if (tid < n) js.ev[tid] = evs;
     __syncthreads();
   }
+  return sweep;
 }
 
-// C = op(A) B (n x n); opA 0: A, 1: A^H.  Strides in elements.
-__device__ void mm(cx<double>* C, int ldc, const cx<double>* A, int lda, int opA, const cx<double>* B, int ldb, int n) {
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-    const int i = e / n, j = e % n;
-    cx<double> acc = {0.0, 0.0};
-    for (int k = 0; k < n; ++k) {
-      const cx<double> a = opA ? cconj(A[k * lda + i]) : A[i * lda + k];
-      acc = cadd(acc, cmul(a, B[k * ldb + j]));
-    }
-    C[i * ldc + j] = acc;
-  }
-  __syncthreads();
-}
-
-constexpr int NLDS_ALL = 40;  // up to this many states all five work matrices sit in LDS
+constexpr int NDIAG_MAX = 36;  // largest block hfb_diag_small accepts (five n x n matrices in LDS)
 
 // Inverse square root of the Hermitian positive matrix in Y (scaled copy of
 // S) by the coupled Newton-Schulz iteration Y <- Y T, Z <- T Z,
 // T = (3I - Z Y)/2, which converges to Z = Y0^{-1/2}.  Returns false when it
 // has not converged (nearly singular S); the caller then takes the
 // eigen-decomposition with the reference's clamp.
-__device__ bool newton_schulz(cx<double>*& Y, cx<double>*& Z, cx<double>* P, cx<double>*& Y2, cx<double>*& Z2, int n,
-                              double* red) {
-  const int tid = threadIdx.x;
-  for (int e = tid; e < n * n; e += blockDim.x) Z[e] = cx<double>{(e / n == e % n) ? 1.0 : 0.0, 0.0};
+template <int NP>
+__device__ __forceinline__ bool newton_schulz(cx<double>*& Y, cx<double>*& Z, cx<double>* P, cx<double>*& Y2, cx<double>*& Z2, int n,
+                                              double* red, int& iters) {
+  tile_for<NP>(n, [&](int i, int j) { Z[i * n + j] = cx<double>{i == j ? 1.0 : 0.0, 0.0}; });
   __syncthreads();
-  __shared__ double resid;
   int remaining = -1;  // updates left once the residual is below 1e-6
   for (int it = 0; it < 80; ++it) {
-    mm(P, n, Z, n, 0, Y, n, n);
+    mm<NP, false>(P, Z, Y, n);
     double v = 0.0;
-    for (int e = tid; e < n * n; e += blockDim.x) {
-      const cx<double> d = csub(P[e], cx<double>{(e / n == e % n) ? 1.0 : 0.0, 0.0});
-      v = fmax(v, fabs(d.x) + fabs(d.y));
+    HFB_TILE_LOOP(NP, n, i, j) {  // residual max|P - I|, then P <- T = (3I - P)/2
+      const cx<double> pv = P[i * n + j];
+      const double id = i == j ? 1.0 : 0.0;
+      v = fmax(v, fabs(pv.x - id) + fabs(pv.y));
+      P[i * n + j] = cx<double>{1.5 * id - 0.5 * pv.x, -0.5 * pv.y};
     }
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
-    if ((tid & 63) == 0) red[tid >> 6] = v;
-    __syncthreads();
-    if (tid == 0) {
-      double m = 0.0;
-      for (int w = 0; w < (int)blockDim.x / 64; ++w) m = fmax(m, red[w]);
-      resid = m;
-    }
-    __syncthreads();
+    const double resid = block_max_all(v, red);
     // quadratic convergence: two updates after 1e-6 reach round-off
     if (remaining < 0 && resid < 1e-6) remaining = 2;
-    for (int e = tid; e < n * n; e += blockDim.x) {  // T = (3I - P)/2
-      const double id = (e / n == e % n) ? 1.5 : 0.0;
-      P[e] = cx<double>{id - 0.5 * P[e].x, -0.5 * P[e].y};
-    }
-    __syncthreads();
-    mm(Y2, n, Y, n, 0, P, n, n);
-    mm(Z2, n, P, n, 0, Z, n, n);
+    mm<NP, false>(Y2, Y, P, n);
+    mm<NP, false>(Z2, P, Z, n);
     cx<double>* t = Y;
     Y = Y2;
     Y2 = t;
     t = Z;
     Z = Z2;
     Z2 = t;
+    iters = it + 1;
     if (remaining > 0 && --remaining == 0) return true;
   }
   return false;
 }
 
-__global__ void __launch_bounds__(256) diag_kernel(DiagArgs a) {
+template <int NP>
+__global__ void __launch_bounds__(DIAG_NT) diag_kernel(DiagArgs a) {
   extern __shared__ double lds_raw[];
   __shared__ JacobiShared js;
-  __shared__ double red[8];
+  __shared__ double red[16];
   const int iq = blockIdx.x;
   const DiagIso d = a.iso[iq];
   const int n = d.nst;
   if (n <= 0) return;
   const int tid = threadIdx.x;
   const int nm = a.nmax;
-  cx<double>* Bf[5];
-  Bf[0] = (cx<double>*)lds_raw;
-  Bf[1] = Bf[0] + n * n;
-  if (n <= NLDS_ALL) {
-    Bf[2] = Bf[1] + n * n;
-    Bf[3] = Bf[2] + n * n;
-    Bf[4] = Bf[3] + n * n;
-  } else {
-    Bf[2] = d.T1;
-    Bf[3] = d.T2;
-    Bf[4] = d.T3;
-  }
+  // five work matrices in LDS (n <= NDIAG_MAX); deriving every pointer from
+  // the LDS array lets the compiler use LDS instructions instead of flat ones
+  cx<double>* const B0 = (cx<double>*)lds_raw;
+  cx<double>* const B1 = B0 + n * n;
+  cx<double>* const B2 = B1 + n * n;
+  cx<double>* const B3 = B2 + n * n;
+  cx<double>* const B4 = B3 + n * n;
 
   // Loewdin S^{-1/2} (static.py:463-482)
-  for (int i = tid; i < n; i += blockDim.x) a.sp_norm[d.start + i] = d.S[i * n + i].x;
-  load_lower_herm(Bf[0], d.S, n);
-  __shared__ double cscale_s;
-  if (tid == 0) {  // Gershgorin bound on the largest eigenvalue
-    double c = 0.0;
-    for (int i = 0; i < n; ++i) {
-      double r = 0.0;
-      for (int j = 0; j < n; ++j) r += sqrt(cabs2(Bf[0][i * n + j]));
-      c = fmax(c, r);
-    }
-    cscale_s = c > 0.0 ? c : 1.0;
-  }
+  for (int i = tid; i < n; i += DIAG_NT) a.sp_norm[d.start + i] = d.S[i * n + i].x;
+  load_lower_herm<NP>(B0, d.S, n);
+  double rowsum = 0.0;  // Gershgorin bound on the largest eigenvalue
+  if (tid < n)
+    for (int j = 0; j < n; ++j) rowsum += sqrt(cabs2(B0[tid * n + j]));
+  double cs = block_max_all(rowsum, red);
+  if (!(cs > 0.0)) cs = 1.0;
+  tile_for<NP>(n, [&](int i, int j) { B0[i * n + j] = cscale(B0[i * n + j], 1.0 / cs); });
   __syncthreads();
-  const double cs = cscale_s;
-  for (int e = tid; e < n * n; e += blockDim.x) Bf[0][e] = cscale(Bf[0][e], 1.0 / cs);
-  __syncthreads();
-  cx<double>*Y = Bf[0], *Z = Bf[1], *Y2 = Bf[3], *Z2 = Bf[4];
-  const bool ok = newton_schulz(Y, Z, Bf[2], Y2, Z2, n, red);
-  cx<double>* Urho = Bf[4];
+  cx<double>*Y = B0, *Z = B1, *Y2 = B3, *Z2 = B4;
+  int ns_iters = 0;
+  const bool ok = newton_schulz<NP>(Y, Z, B2, Y2, Z2, n, red, ns_iters);
+  cx<double>* Urho = B4;
   if (ok) {
     const double f = 1.0 / sqrt(cs);
-    // Z may live in Bf[4]; scale in place or copy
-    for (int e = tid; e < n * n; e += blockDim.x) {
-      const cx<double> v = cscale(Z[e], f);
-      Urho[e] = v;
-    }
+    // Z may live in B4: the scaling is elementwise, so in place is safe
+    tile_for<NP>(n, [&](int i, int j) { Urho[i * n + j] = cscale(Z[i * n + j], f); });
     __syncthreads();
   } else {
-    load_lower_herm(Bf[0], d.S, n);
-    jacobi_eigh(Bf[0], Bf[1], n, js, false, Bf[2]);
+    load_lower_herm<NP>(B0, d.S, n);
+    jacobi_eigh<NP>(B0, B1, n, js, false, B2);
     __shared__ double fac[NMAXJ];
     if (tid == 0) {
       double wmax = js.ev[0];
@@ -320,63 +366,67 @@ __global__ void __launch_bounds__(256) diag_kernel(DiagArgs a) {
       for (int i = 0; i < n; ++i) fac[i] = 1.0 / sqrt(fmax(js.ev[i], thr));
     }
     __syncthreads();
-    const cx<double>* V = Bf[1];
-    for (int e = tid; e < n * n; e += blockDim.x) {  // U = V diag(fac) V^H
-      const int i = e / n, j = e % n;
+    const cx<double>* V = B1;
+    tile_for<NP>(n, [&](int i, int j) {  // U = V diag(fac) V^H
       cx<double> acc = {0.0, 0.0};
       for (int k = 0; k < n; ++k) acc = cadd(acc, cscale(cmul(V[i * n + k], cconj(V[j * n + k])), fac[k]));
-      Urho[e] = acc;
-    }
+      Urho[i * n + j] = acc;
+    });
     __syncthreads();
   }
 
   // subspace diagonalisation (static.py:300-303): U <- U_rho U_lam
+  if (a.stats && tid == 0) {
+    a.stats[2 + iq] = ok ? ns_iters : -1;
+    a.stats[iq] = 0;
+  }
   cx<double>* U = Urho;
   if (a.tdiag && d.Lam) {
-    load_lower_herm(Bf[0], d.Lam, n);
-    jacobi_eigh(Bf[0], Bf[1], n, js, true, Bf[2]);
-    mm(Bf[3], n, Urho, n, 0, Bf[1], n, n);
-    U = Bf[3];
+    load_lower_herm<NP>(B0, d.Lam, n);
+    const int sw = jacobi_eigh<NP>(B0, B1, n, js, true, B2);
+    if (a.stats && tid == 0) a.stats[iq] = sw;
+    mm<NP, false>(B3, Urho, B1, n);
+    U = B3;
   }
-  for (int e = tid; e < n * n; e += blockDim.x) d.U[e] = U[e];
+  tile_for<NP>(n, [&](int i, int j) { d.U[i * n + j] = U[i * n + j]; });
 
   if (a.construct) {  // construct_hfb_matrices (static.py:484-577)
-    cx<double>* H = Bf[2];
-    cx<double>* G = Bf[0];
-    for (int e = tid; e < n * n; e += blockDim.x) Bf[0][e] = d.Hold[e];
+    cx<double>* H = B2;
+    cx<double>* G = B0;
+    tile_for<NP>(n, [&](int i, int j) { B0[i * n + j] = d.Hold[i * n + j]; });
     __syncthreads();
-    mm(Bf[1], n, Bf[0], n, 0, U, n, n);
-    mm(H, n, U, n, 1, Bf[1], n, n);
+    mm<NP, false>(B1, B0, U, n);
+    mm<NP, true>(H, U, B1, n);
     if (a.ipair && d.Gold) {
-      for (int e = tid; e < n * n; e += blockDim.x) Bf[0][e] = d.Gold[e];
+      tile_for<NP>(n, [&](int i, int j) { B0[i * n + j] = d.Gold[i * n + j]; });
       __syncthreads();
-      mm(Bf[1], n, Bf[0], n, 0, U, n, n);
-      mm(G, n, U, n, 1, Bf[1], n, n);
+      mm<NP, false>(B1, B0, U, n);
+      mm<NP, true>(G, U, B1, n);
     } else {
-      for (int e = tid; e < n * n; e += blockDim.x) G[e] = {0.0, 0.0};
+      tile_for<NP>(n, [&](int i, int j) { G[i * n + j] = {0.0, 0.0}; });
       __syncthreads();
     }
-    for (int e = tid; e < n * n; e += blockDim.x) {
-      d.Hm[(e / n) * nm + e % n] = H[e];
-      d.Gm[(e / n) * nm + e % n] = G[e];
-    }
-    for (int i = tid; i < n; i += blockDim.x) {
+    tile_for<NP>(n, [&](int i, int j) {
+      d.Hm[i * nm + j] = H[i * n + j];
+      d.Gm[i * nm + j] = G[i * n + j];
+    });
+    for (int i = tid; i < n; i += DIAG_NT) {
       a.sp_energy[d.start + i] = H[i * n + i].x;
       if (a.ipair) a.deltaf[d.start + i] = G[i * n + i].x * a.pairwg[d.start + i];
     }
     __syncthreads();
     if (a.tbcs) {
-      for (int e = tid; e < n * n; e += blockDim.x)
-        if (e / n != e % n) {
-          G[e] = {0.0, 0.0};
-          d.Gm[(e / n) * nm + e % n] = {0.0, 0.0};
+      tile_for<NP>(n, [&](int i, int j) {
+        if (i != j) {
+          G[i * n + j] = {0.0, 0.0};
+          d.Gm[i * nm + j] = {0.0, 0.0};
         }
+      });
       __syncthreads();
     }
-    // lambda, its parts and the fluctuation measures; lambda+ kept in Bf[1]
+    // lambda, its parts and the fluctuation measures; lambda+ kept in B1
     double mx = 0.0, sm = 0.0;
-    for (int e = tid; e < n * n; e += blockDim.x) {
-      const int i = e / n, j = e % n;
+    HFB_TILE_LOOP(NP, n, i, j) {
       const int sj = d.start + j, si = d.start + i;
       const double wj = a.wocc[sj] * a.wstates[sj];
       const double uj = a.wguv[sj] * a.pairwg[sj] * a.wstates[sj];
@@ -389,39 +439,26 @@ __global__ void __launch_bounds__(256) diag_kernel(DiagArgs a) {
       const cx<double> sym = cscale(cadd(lij, lh), 0.5);
       d.Sym[i * nm + j] = asym;
       d.Ls[i * nm + j] = sym;
-      Bf[1][e] = sym;
+      B1[i * n + j] = sym;
       const double aa = cabs2(asym);
       mx = fmax(mx, sqrt(aa));
       sm += aa;
     }
-    __shared__ double rmax[16], rsum[16];
-    for (int o = 32; o > 0; o >>= 1) {
-      mx = fmax(mx, __shfl_down(mx, o, 64));
-      sm += __shfl_down(sm, o, 64);
-    }
-    if ((tid & 63) == 0) {
-      rmax[tid >> 6] = mx;
-      rsum[tid >> 6] = sm;
-    }
-    __syncthreads();
+    const double M = block_max_all(mx, red);
+    double Ssum = sm, dummy = 0.0;
+    block_sum2_all(Ssum, dummy, red);
     if (tid == 0) {
-      double M = 0.0, Ssum = 0.0;
-      for (int w = 0; w < (int)blockDim.x / 64; ++w) {
-        M = fmax(M, rmax[w]);
-        Ssum += rsum[w];
-      }
       a.efl1q[iq] = M;
       a.efl2q[iq] = sqrt(Ssum / ((double)n * n));
     }
-    __syncthreads();
   } else {
     // construct=False: lagrange from the stored lambda (static.py:336-350)
-    for (int e = tid; e < n * n; e += blockDim.x) Bf[1][e] = d.Ls[(e / n) * nm + e % n];
+    tile_for<NP>(n, [&](int i, int j) { B1[i * n + j] = d.Ls[i * nm + j]; });
     __syncthreads();
   }
   // M2 = U lambda+ (static.py:557-570)
-  mm(Bf[0], n, U, n, 0, Bf[1], n, n);
-  for (int e = tid; e < n * n; e += blockDim.x) d.M2[e] = Bf[0][e];
+  mm<NP, false>(B0, U, B1, n);
+  tile_for<NP>(n, [&](int i, int j) { d.M2[i * n + j] = B0[i * n + j]; });
 }
 
 __global__ void diag_finish_kernel(const double* efl1q, const double* efl2q, double* efl1, double* efl2) {
@@ -432,18 +469,21 @@ __global__ void diag_finish_kernel(const double* efl1q, const double* efl2q, dou
 }  // namespace
 
 // mats[iq*4 + m]: m=0 S, 1 H_old, 2 lambda_lin (or null), 3 G_old (or null),
-// each nst_iq x nst_iq complex; uo[iq*2]: U, uo[iq*2+1]: M2 outputs;
-// scratch: 6 * nmax^2 complex doubles.
+// each nst_iq x nst_iq complex; uo[iq*2]: U, uo[iq*2+1]: M2 outputs.
 extern "C" int hfb_diag_small(int nst0, int nst1, int nmax, const double* const* mats, int tdiag, int ipair,
                               int tbcs, int construct, const double* wocc, const double* wstates, const double* wguv,
                               const double* pairwg, double* sp_norm, double* sp_energy, double* deltaf,
                               double* hmatrix, double* gapmatrix, double* symcond, double* lambda_save,
                               double* efluct1q, double* efluct2q, double* efluct1, double* efluct2,
-                              double* const* uo, double* scratch, void* stream) {
-  if (!mats || !uo || !scratch || !wocc || !wstates || !wguv || !pairwg || !sp_norm || !sp_energy || !deltaf ||
+                              double* const* uo, double* stats, void* stream) {
+  if (!mats || !uo || !wocc || !wstates || !wguv || !pairwg || !sp_norm || !sp_energy || !deltaf ||
       !hmatrix || !gapmatrix || !symcond || !lambda_save || !efluct1q || !efluct2q || !efluct1 || !efluct2)
     return HFB_E_ARG;
-  if (nst0 < 0 || nst1 < 0 || nst0 > NMAXJ || nst1 > NMAXJ || nmax < nst0 || nmax < nst1) return HFB_E_SIZE;
+  if (nst0 < 0 || nst1 < 0 || nmax < nst0 || nmax < nst1) return HFB_E_SIZE;
+  // Blocks above NDIAG_MAX states are switched off: a restructured form of
+  // the Jacobi rounds faulted the device at 40 and 64 states and the cause
+  // has not been found.  Larger blocks need the multi-workgroup design.
+  if (nst0 > NDIAG_MAX || nst1 > NDIAG_MAX) return HFB_E_SIZE;
   DiagArgs a{};
   a.nmax = nmax;
   const int nsts[2] = {nst0, nst1};
@@ -460,9 +500,6 @@ extern "C" int hfb_diag_small(int nst0, int nst1, int nmax, const double* const*
     d.U = (cx<double>*)uo[iq * 2 + 0];
     d.M2 = (cx<double>*)uo[iq * 2 + 1];
     if (d.nst > 0 && (!d.U || !d.M2)) return HFB_E_ARG;
-    d.T1 = (cx<double>*)scratch + (iq * 3 + 0) * nm2;
-    d.T2 = (cx<double>*)scratch + (iq * 3 + 1) * nm2;
-    d.T3 = (cx<double>*)scratch + (iq * 3 + 2) * nm2;
     d.Hm = (cx<double>*)hmatrix + iq * nm2;
     d.Gm = (cx<double>*)gapmatrix + iq * nm2;
     d.Sym = (cx<double>*)symcond + iq * nm2;
@@ -481,14 +518,22 @@ extern "C" int hfb_diag_small(int nst0, int nst1, int nmax, const double* const*
   a.ipair = ipair;
   a.tbcs = tbcs;
   a.construct = construct;
+  a.stats = stats;
   const int nbig = nst0 > nst1 ? nst0 : nst1;
-  const size_t lds = sizeof(cx<double>) * (nbig <= NLDS_ALL ? 5 : 2) * (size_t)nbig * nbig;
+  const size_t lds = sizeof(cx<double>) * 5 * (size_t)nbig * nbig;
   hipStream_t st = (hipStream_t)stream;
-  if (lds > 32 * 1024) allow_full_lds((const void*)diag_kernel);
-  // one wavefront per isospin for small blocks: the Jacobi sweeps are
-  // barrier-bound and a single-wave barrier is nearly free
-  const int nthr = nbig <= 32 ? 64 : 256;
-  hipLaunchKernelGGL(diag_kernel, dim3(2), dim3(nthr), lds, st, a);
+#define HFB_DIAG(NPV)                                                             \
+  {                                                                               \
+    if (lds > 32 * 1024) allow_full_lds((const void*)diag_kernel<NPV>);           \
+    hipLaunchKernelGGL(diag_kernel<NPV>, dim3(2), dim3(DIAG_NT), lds, st, a);      \
+  }
+  if (nbig <= 16)
+    HFB_DIAG(16)
+  else if (nbig <= 32)
+    HFB_DIAG(32)
+  else
+    HFB_DIAG(64)
+#undef HFB_DIAG
   int rc = launch_status();
   if (rc != HFB_OK) return rc;
   if (construct) {

@@ -75,6 +75,78 @@ __global__ void __launch_bounds__(256) deriv_kernel(DerivJobs jobs, int nx, int 
   }
 }
 
+// Tiled form for cubic grids: a workgroup stages 64 pencils of one source
+// field along the term's axis in LDS and each wavefront produces n/4 output
+// rows for all 64 pencils (out = D s, D row-uniform across the wavefront, so
+// its entries come through the scalar cache).  One launch per term index;
+// later terms accumulate into dst in stream order.
+constexpr int DT_P = 64;          // pencils per tile
+constexpr int DT_LD = DT_P + 1;   // LDS row pitch (doubles)
+
+struct DerivMats {
+  const double* d1[3];
+  const double* d2[3];
+};
+
+template <int N>
+__device__ __forceinline__ int64_t pencil_off(int ax, int k, int gp) {
+  if (ax == 0) return (int64_t)k * N * N + gp;
+  if (ax == 1) return (int64_t)(gp / N) * N * N + (int64_t)k * N + gp % N;
+  return (int64_t)gp * N + k;
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) deriv_tile_kernel(DerivJobs jobs, int term, DerivMats dm) {
+  __shared__ double sS[N * DT_LD];
+  const DerivJob& jb = jobs.j[blockIdx.y];
+  if (term >= jb.nterm) return;
+  const int ax = jb.axis[term];
+  const double* __restrict__ src = jb.src[term];
+  const double* __restrict__ D = jb.order[term] == 1 ? dm.d1[ax] : dm.d2[ax];
+  const double coef = jb.coef[term];
+  double* __restrict__ dst = jb.dst;
+  const int p0 = blockIdx.x * DT_P;
+  // element (k, p) of the tile; consecutive threads walk the contiguous direction
+  auto kp = [&](int e, int& k, int& p) {
+    if (ax == 2) {
+      k = e % N;
+      p = e / N;
+    } else {
+      p = e % DT_P;
+      k = e / DT_P;
+    }
+  };
+  for (int e = threadIdx.x; e < N * DT_P; e += 256) {
+    int k, p;
+    kp(e, k, p);
+    sS[k * DT_LD + p] = src[pencil_off<N>(ax, k, p0 + p)];
+  }
+  __syncthreads();
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int p = threadIdx.x & 63;
+  constexpr int R = N / 4;
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+#pragma unroll
+  for (int l = 0; l < N; ++l) {
+    const double sv = sS[l * DT_LD + p];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = fma(D[(w + 4 * r) * N + l], sv, acc[r]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) sS[(w + 4 * r) * DT_LD + p] = acc[r];
+  __syncthreads();
+  for (int e = threadIdx.x; e < N * DT_P; e += 256) {
+    int k, pp;
+    kp(e, k, pp);
+    const int64_t off = pencil_off<N>(ax, k, p0 + pp);
+    const double v = coef * sS[k * DT_LD + pp];
+    dst[off] = term == 0 ? v : dst[off] + v;
+  }
+}
+
 struct SkyArgs {
   hfb_force f;
   int64_t nvox;
@@ -323,6 +395,20 @@ void push(DerivJobs& J, double* dst, int nterm, const double* s0, int a0, int o0
 }
 
 int run_jobs(const hfb_ctx* c, const DerivJobs& J, hipStream_t st) {
+  if (c->nx == c->ny && c->ny == c->nz) {
+    DerivMats dm{{c->der1[0], c->der1[1], c->der1[2]}, {c->der2[0], c->der2[1], c->der2[2]}};
+    int nterm = 0;
+    for (int j = 0; j < J.njob; ++j) nterm = J.j[j].nterm > nterm ? J.j[j].nterm : nterm;
+    const int n = c->nx;
+    for (int t = 0; t < nterm; ++t) {
+      HFB_DISPATCH_N(n, {
+        hipLaunchKernelGGL(deriv_tile_kernel<N_>, dim3(N_ * N_ / DT_P, J.njob), dim3(256), 0, st, J, t, dm);
+      })
+      const int rc = launch_status();
+      if (rc != HFB_OK) return rc;
+    }
+    return HFB_OK;
+  }
   const int64_t nvox = (int64_t)c->nx * c->ny * c->nz;
   const int gx = (int)((nvox + 255) / 256);
   hipLaunchKernelGGL(deriv_kernel, dim3(gx, J.njob), dim3(256), 0, st, J, c->nx, c->ny, c->nz, c->der1[0], c->der1[1],

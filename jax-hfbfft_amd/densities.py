@@ -28,13 +28,18 @@ def init_densities(grids, device=None) -> Densities:
     return Densities(z(s4), z(s4), z(s4), z(s5), z(s5), z(s5))
 
 
-def add_density(densities, grids, levels, mix=False, addnew=0.2, moment=None, want_sp=False):
-    """Sum of per-state contributions per isospin (densities.py:177-234).
+def add_density(densities, grids, levels):
+    """Sum of per-state contributions per isospin (densities.py:177-234);
+    returns the updated densities like the reference."""
+    return add_density_sweep(densities, grids, levels)[0]
 
-    ``mix`` applies the loop's linear mixing of rho, tau, chi
-    (static.py:1817-1823) in the same pass; ``want_sp`` also fills the
-    per-state kinetic energy, <L>, <S> and parity (inout.py:64-149) from the
-    same transforms."""
+
+def add_density_sweep(densities, grids, levels, mix=False, addnew=0.2, moment=None, want_sp=False):
+    """add_density in one sweep over the states, optionally fused with the
+    loop's linear mixing of rho, tau, chi (``mix``, static.py:1817-1823) and
+    with the per-state kinetic energy, <L>, <S> and parity of sp_properties
+    (``want_sp``, inout.py:64-149) from the same transforms.  Returns
+    (densities, per-state partial sums or None)."""
     ctx = grids.ctx
     n0, n1 = int(levels.npsi[0]), int(levels.npsi[1])
     cm = moment.cmtot.to(device=levels.psi.device, dtype=torch.float64) if moment is not None else \

@@ -65,7 +65,9 @@ class Energies:
         d = {k: float(v[EN[s]]) for k, s in _SCALARS.items()}
         d.update(efluct1=float(v[EN["EFL1"]]), efluct2=float(v[EN["EFL2"]]), ecorrp=float(v[EN["ECORRP"]]),
                  sumflu=float(v[EN["SUMFLU"]]), delesum=float(v[EN["DELESUM"]]),
-                 epair=v[EN["EPAIR"]:EN["EPAIR"] + 2].copy())
+                 epair=v[EN["EPAIR"]:EN["EPAIR"] + 2].copy(),
+                 jacobi_sweeps=v[EN["JACOBI"]:EN["JACOBI"] + 2].copy(),
+                 ns_iterations=v[EN["NSITER"]:EN["NSITER"] + 2].copy())
         return d
 
 

@@ -232,7 +232,7 @@ def rotate(A, U1, U2, out1, out2):
     check(lib().hfb_rotate(prec, ptr(A), ptr(U1), ptr(U2), ptr(out1), ptr(out2), nst, K, stream_handle()), "rotate")
 
 
-def diag_small(nst0, nst1, nmax, mats, tdiag, ipair, tbcs, construct, levels, static, E, uo, scratch):
+def diag_small(nst0, nst1, nmax, mats, tdiag, ipair, tbcs, construct, levels, static, E, uo):
     """Loewdin, optional diagonalisation and HFB matrices for both isospins."""
     m = (c_void_p * 8)(*[(t.data_ptr() if t is not None else None) for t in mats])
     u = (c_void_p * 4)(*[t.data_ptr() for t in uo])
@@ -244,7 +244,8 @@ def diag_small(nst0, nst1, nmax, mats, tdiag, ipair, tbcs, construct, levels, st
                                ptr(levels.deltaf), ptr(static.hmatrix), ptr(static.gapmatrix), ptr(static.symcond),
                                ptr(static.lambda_save), c_void_p(Ep + sz * EN["EFL1Q"]),
                                c_void_p(Ep + sz * EN["EFL2Q"]), c_void_p(Ep + sz * EN["EFL1"]),
-                               c_void_p(Ep + sz * EN["EFL2"]), u, ptr(scratch), stream_handle()), "diag_small")
+                               c_void_p(Ep + sz * EN["EFL2"]), u, c_void_p(Ep + sz * EN["JACOBI"]),
+                               stream_handle()), "diag_small")
 
 
 # ---------------------------------------------------------------------------
@@ -284,7 +285,7 @@ EN = {"EHFT": 0, "EHF0": 1, "EHF1": 2, "EHF2": 3, "EHF3": 4, "EHFLS": 5, "EHFLSO
       "EHFINT": 9, "E3CORR": 10, "EZPE": 11, "CRHO0": 12, "CRHO1": 13, "CDRHO0": 14, "CDRHO1": 15, "CTAU0": 16,
       "CTAU1": 17, "CDJ0": 18, "CDJ1": 19, "CJ0": 20, "CJ1": 21, "EHF": 22, "TKE": 23, "ORB": 24, "SPIN": 27,
       "JTOT": 30, "EFL1": 33, "EFL2": 34, "EFL1Q": 35, "EFL2Q": 37, "ECORRP": 39, "SUMFLU": 40, "DELESUM": 41,
-      "EPAIR": 42}
+      "EPAIR": 42, "JACOBI": 44, "NSITER": 46}
 EN_LEN = 48
 
 
@@ -323,3 +324,36 @@ def sum_energy(levels, E):
     check(lib().hfb_sum_energy(int(levels.nstmax), ptr(levels.wocc), ptr(levels.wstates), ptr(levels.sp_kinetic),
                                ptr(levels.sp_energy), ptr(levels.sp_orbital), ptr(levels.sp_spin), ptr(E),
                                stream_handle()), "sum_energy")
+
+
+# ---------------------------------------------------------------------------
+# Energy-functional backward (no reference counterpart)
+# ---------------------------------------------------------------------------
+def energy_bwd(ctx, forces_c, tx, gout, dens, der, wcoul, want_fields=True, want_couplings=True):
+    """Gradients of ehfint for a batch of coupling sets sharing the densities.
+
+    forces_c: list of HfbForce (host), tx: (B, 10) float64 device raw
+    couplings, gout: (B,) float64 device upstream gradients.  Returns
+    (fields dict or None, d_coupling (B, 10) or None); the density gradients
+    are summed over the batch with gout."""
+    B = len(forces_c)
+    arr = (type(forces_c[0]) * B)(*forces_c)
+    need(tx, torch.float64, (B, 10), "couplings")
+    need(gout, torch.float64, (B,), "grad_output")
+    dev = dens.rho.device
+    s4 = tuple(dens.rho.shape)
+    s5 = tuple(dens.sodens.shape)
+    fields = None
+    if want_fields:
+        fields = {k: torch.empty(s, dtype=torch.float64, device=dev)
+                  for k, s in (("rho", s4), ("tau", s4), ("sodens", s5), ("current", s5), ("sdens", s5))}
+    dcpl = torch.empty((B, 10), dtype=torch.float64, device=dev) if want_couplings else None
+    part = ctx.work("adj_part", (lib().hfb_energy_bwd_nblocks(ctx.handle) * 72,), torch.float64)
+    g = fields or {}
+    check(lib().hfb_energy_bwd(ctx.handle, ctypes.cast(arr, c_void_p), ptr(tx), ptr(gout), B, ptr(dens.rho),
+                               ptr(dens.tau), ptr(dens.sodens), ptr(dens.current), ptr(dens.sdens),
+                               ptr(der["divj"]), ptr(der["lap"]), ptr(der["grad"]), ptr(der["curls"]),
+                               ptr(der["curlj"]), ptr(wcoul), ptr(g.get("rho")), ptr(g.get("tau")),
+                               ptr(g.get("sodens")), ptr(g.get("current")), ptr(g.get("sdens")), ptr(dcpl),
+                               ptr(part), stream_handle()), "energy_bwd")
+    return fields, dcpl

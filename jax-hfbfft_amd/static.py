@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import kernels
-from .densities import add_density
+from .densities import add_density, add_density_sweep
 from .energies import integ_energy, sum_energy
 from .inout import sp_properties
 from .meanfield import density_derivs, skyrme
@@ -214,9 +214,8 @@ def diagstep(energies, forces, grids, levels, static, diagonalize=False, constru
                                                                               torch.complex128)]
     # overlap, H_old, lambda_lin and G_old of both isospins in one pass over the states
     kernels.gram2(ctx, blocks, grids.wxyz, outs)
-    scratch = ctx.work(("diag_scratch", nmax), (6, nmax, nmax), torch.complex128)
     kernels.diag_small(n0, n1, nmax, mats, diagonalize, forces.ipair != 0, forces.tbcs, construct, levels, static,
-                       energies.vec, uo, scratch)
+                       energies.vec, uo)
     # psi <- psi U in place (each 16-point block of K is read, then written,
     # by one wavefront); blocks of more than 64 states go through a copy
     for iq, (a, b) in enumerate(((0, n0), (n0, n0 + n1))):
@@ -275,7 +274,7 @@ def iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, m
         raise NotImplementedError("hfb_natural sub-iterations are outside the static HF+BCS path")
     # density with the 0.2/0.8 mixing of rho, tau, chi and the per-state
     # properties from the same transforms (static.py:1800-1833)
-    _, spart = add_density(densities, grids, levels, mix=True, addnew=0.2, moment=moment, want_sp=True)
+    _, spart = add_density_sweep(densities, grids, levels, mix=True, addnew=0.2, moment=moment, want_sp=True)
     derivs = density_derivs(grids, densities, out=derivs)
     skyrme(coulomb, densities, forces, grids, meanfield, params, static, derivs=derivs, E=energies.vec,
            mass_number=levels.mass_number)

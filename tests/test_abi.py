@@ -101,7 +101,7 @@ def test_energy_slots_match_header(pkg):
         assert f"HFB_EN_{k}" in text
     assert pkg.kernels.EN["SPIN"] == 27 and pkg.kernels.EN["JTOT"] == 30 and pkg.kernels.EN["EFL1"] == 33
     assert pkg.kernels.EN["EFL1Q"] == 35 and pkg.kernels.EN["EFL2Q"] == 37 and pkg.kernels.EN["ECORRP"] == 39
-    assert pkg.kernels.EN["EPAIR"] == 42
+    assert pkg.kernels.EN["EPAIR"] == 42 and pkg.kernels.EN["JACOBI"] == 44 and pkg.kernels.EN["NSITER"] == 46
 
 
 def test_forces_match_oracle(pkg, O):
@@ -154,3 +154,18 @@ def test_product_path_refuses_cpu(pkg):
     with pytest.raises(RuntimeError):
         pkg.driver.HFRun.from_config(precision="f64", device="cuda:0", name="SV-bas", n=24, d=1.0, nneut=8,
                                      nprot=8)
+
+
+def test_adjoint_couplings_round_trip(pkg, O):
+    """forces_at() re-derives the b- and C-couplings from perturbed t/x/alpha/W0
+    exactly as the oracle's init_forces does for the same values."""
+    P = pkg.params.init_params()
+    f = pkg.forces.init_forces(P, name="SV-bas")
+    c = pkg.adjoint.coupling_vector(f)
+    assert pkg.adjoint.COUPLINGS == ("t0", "t1", "t2", "t3", "x0", "x1", "x2", "x3", "power", "t4")
+    c2 = c * torch.linspace(0.98, 1.02, c.numel(), dtype=torch.float64)
+    g = pkg.adjoint.forces_at(f, c2, P)
+    o = O.init_forces("SV-bas", overrides=dict(zip(pkg.adjoint.COUPLINGS, c2.tolist())))
+    for k in ("b0", "b0p", "b1", "b1p", "b2", "b2p", "b3", "b3p", "b4", "power", "Crho0", "Crho1D", "Ctau1", "CdJ0"):
+        assert getattr(g, k) == pytest.approx(getattr(o, k), rel=1e-14), k
+    assert f.t0 == c[0].item()  # the template is not modified

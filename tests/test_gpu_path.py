@@ -57,9 +57,17 @@ def test_laplace_matches_oracle(pkg, O, dev):
         assert rel(out.cpu().numpy(), r) < 1e-12
 
 
-def test_diagstep_matches_oracle(pkg, O, dev):
-    """Loewdin + subspace diagonalisation + HFB matrices on the same states."""
-    ref, run = small_case(O, pkg, ipair=5, tbcs=True, npsi=[12, 10])
+_FAULTED = pytest.mark.skip(reason="blocks above 36 states are switched off in hfb_diag_small: this case faulted "
+                                   "the device (with a since-reverted form of the Jacobi rounds) and the cause is "
+                                   "not found")
+
+
+@pytest.mark.parametrize("npsi", [[12, 10], [36, 30], pytest.param([40, 33], marks=_FAULTED),
+                                  pytest.param([64, 50], marks=_FAULTED)])
+def test_diagstep_matches_oracle(pkg, O, dev, npsi):
+    """Loewdin + subspace diagonalisation + HFB matrices on the same states;
+    block sizes of O16-BCS, Cr48 and the largest block the kernel takes."""
+    ref, run = small_case(O, pkg, ipair=5, tbcs=True, npsi=npsi)
     rng = np.random.default_rng(5)
     shp = ref.levels.psi.shape
     for nm in ("hmfpsi", "hampsi", "delpsi", "lagrange"):
@@ -114,7 +122,7 @@ def test_density_sp_skyrme_energy(pkg, O, dev):
     for nm in ("wocc", "wguv", "wstates", "pairwg"):
         getattr(run.levels, nm).copy_(T(getattr(ref.levels, nm)))
     O.add_density(ref.densities, ref.grids, ref.levels)
-    _, spart = pkg.densities.add_density(run.densities, run.grids, run.levels, want_sp=True, moment=run.moment)
+    _, spart = pkg.densities.add_density_sweep(run.densities, run.grids, run.levels, want_sp=True, moment=run.moment)
     for nm in ("rho", "chi", "tau", "current", "sdens", "sodens"):
         assert rel(getattr(run.densities, nm).cpu().numpy(), getattr(ref.densities, nm)) < 1e-11, nm
     O.sp_properties(ref.forces, ref.grids, ref.levels)
