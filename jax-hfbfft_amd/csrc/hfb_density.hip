@@ -1,20 +1,21 @@
 // Densities summed over states (densities.py:177-337) fused with the
 // per-state single-particle properties (inout.py:64-149).
 //
-// One launch per axis a.  A workgroup owns one pencil tile of one isospin
-// and walks the states of that isospin, NSUB states at a time (one state
-// per sub-group of NTS threads).  For each state the tile of both spin
-// components is transformed along a in LDS; the spectrum gives the state's
-// kinetic-energy contribution sum k^2 |psi_k|^2 / n (Nyquist kept, the
-// Parseval form of inout.py:86-91,130) and the inverse transform of
-// i k psi_k gives d_a psi (Nyquist zeroed, cderv*00/01's d1).  Density
-// contributions accumulate in registers of the points each thread owns;
-// sub-groups are combined in LDS at the end, so no per-state field is ever
-// written to HBM.  The x pass also adds the pointwise rho, chi, spin
-// density and the per-state spin and parity sums.
+// One launch per axis a.  A workgroup owns PW pencils along a of one
+// isospin and one chunk of that isospin's states; its four wavefronts take
+// the chunk's states in turn (state s goes to wavefront s mod 4).  A pencil
+// of both spin components sits in a group of L lanes (the LaneFFT layout of
+// hfb_common.h): the spectrum gives the state's kinetic-energy contribution
+// sum k^2 |psi_k|^2 / n (Nyquist kept, the Parseval form of
+// inout.py:86-91,130) and the inverse transform of i k psi_k gives d_a psi
+// (Nyquist zeroed, cderv*00's d1).  Density contributions accumulate in
+// registers over the wavefront's states and are combined over the four
+// wavefronts in LDS; the chunks are summed by hfb_density_finish, so no
+// per-state field is ever written to HBM.  The x pass also adds the
+// pointwise rho, chi, spin density and the per-state spin and parity sums.
 //
-// Density partials (double) are written per axis into slots of
-// `dout` = [2 iso][17 slot][nvox]:
+// Density partials (double) go per chunk and axis into slots of
+// `dout` = [DENS_G chunk][2 iso][17 slot][nvox]:
 //   0 rho 1 chi 2 sx 3 sy 4 sz 5 tau_x 6 j_x 7 Jy(x) 8 Jz(x)     (x pass)
 //   9 tau_y 10 j_y 11 Jx(y) 12 Jz(y)                              (y pass)
 //   13 tau_z 14 j_z 15 Jx(z) 16 Jy(z)                             (z pass)
@@ -55,189 +56,212 @@ __device__ __forceinline__ double re_ab(double ax, double ay, double bx, double 
   return ax * bx + ay * by;
 }
 
-template <typename T, int N, int AXIS, int P, int NSUB>
-__global__ void __launch_bounds__(NSUB * 8 * P) dens_axis_kernel(DensArgs a, const cx<T>* __restrict__ twg) {
-  constexpr int Q = 2 * P;
-  constexpr int NTS = 8 * P;          // threads per sub-group
-  constexpr int NT = NSUB * NTS;
-  constexpr int PPT = (N * P + NTS - 1) / NTS;  // tile points per thread
-  constexpr int NACC = AXIS == 0 ? 9 : 4;
-  extern __shared__ unsigned char lds_raw[];
-  __shared__ cx<T> tw[N];
-  __shared__ double red[NT / 64 * 8];
-  cx<T>* base = (cx<T>*)lds_raw;
-  const int sg = threadIdx.x / NTS;
-  const int lt = threadIdx.x % NTS;
-  cx<T>* sPSI = base + (size_t)sg * 3 * N * Q;
-  cx<T>* sW0 = sPSI + N * Q;
-  cx<T>* sW1 = sW0 + N * Q;
+constexpr int DENS_G = 2;  // state chunks per isospin (blocks along the states)
+constexpr int DENS_L = 8;  // lanes per pencil
+constexpr int DENS_PW = 64 / DENS_L;  // pencils per block
 
+__host__ __device__ __forceinline__ int pencils_axis(int ax, int nx, int ny, int nz) {
+  return ax == 0 ? ny * nz : (ax == 1 ? nx * nz : nx * ny);
+}
+
+template <typename T, int N, int AXIS>
+__global__ void __launch_bounds__(256) dens_lane_kernel(DensArgs a, const cx<T>* __restrict__ tw) {
+  constexpr int L = DENS_L, N1 = N / L, PW = DENS_PW;
+  constexpr int NACC = AXIS == 0 ? 9 : 4;
+  __shared__ double comb[4][PW * N];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = lane % L, q = lane / L;
   const int tile = blockIdx.x;
+  const int g = tile * PW + q;  // pencil
   const int iq = blockIdx.y;
+  const int chunk = blockIdx.z;
   const int nst = iq == 0 ? a.nst0 : a.nst1;
   const int start = iq == 0 ? 0 : a.nst0;
+  const int per = (nst + DENS_G - 1) / DENS_G;
+  const int cs = chunk * per, ce = min(nst, cs + per);
   const int64_t nvox = (int64_t)a.nx * a.ny * a.nz;
-  const Tile t = make_tile<AXIS>(tile, a.nx, a.ny, a.nz, P);
+  int64_t base, stride;
+  int c0, c1;  // the two transverse coordinates of the pencil
+  if constexpr (AXIS == 0) {
+    base = g;
+    stride = (int64_t)a.ny * a.nz;
+    c0 = g / a.nz;  // y
+    c1 = g % a.nz;  // z
+  } else if constexpr (AXIS == 1) {
+    base = (int64_t)(g / a.nz) * a.ny * a.nz + g % a.nz;
+    stride = a.nz;
+    c0 = g / a.nz;  // x
+    c1 = g % a.nz;  // z
+  } else {
+    base = (int64_t)g * a.nz;
+    stride = 1;
+    c0 = g / a.ny;  // x
+    c1 = g % a.ny;  // y
+  }
   const double dax = AXIS == 0 ? a.dx : (AXIS == 1 ? a.dy : a.dz);
   const T kfac = T(2.0 * M_PI / (dax * N));
   const double kf = 2.0 * M_PI / (dax * N);
   const cx<T>* psi = (const cx<T>*)a.psi;
-  load_twiddles(tw, twg, N);
+  LaneFFT<T, N, L> F;
+  F.init(j, tw);
 
-  double acc[PPT][NACC];
+  double acc[N1][NACC];
 #pragma unroll
-  for (int i = 0; i < PPT; ++i)
+  for (int i = 0; i < N1; ++i)
 #pragma unroll
-    for (int j = 0; j < NACC; ++j) acc[i][j] = 0.0;
+    for (int k = 0; k < NACC; ++k) acc[i][k] = 0.0;
 
-  for (int s0 = 0; s0 < nst; s0 += NSUB) {
-    const int sl = s0 + sg;
-    const bool active = sl < nst;
-    const int s = start + (active ? sl : 0);
+  for (int sl = cs + wave; sl < ce; sl += 4) {
+    const int s = start + sl;
     const cx<T>* p0 = psi + (int64_t)(2 * s) * nvox;
     const cx<T>* p1 = p0 + nvox;
-    for (int e = lt; e < N * P; e += NTS) {
-      int k, p;
-      elem_of<AXIS, N, P>(e, k, p);
-      const int64_t off = t.base + k * t.sk + p * t.sp;
-      cx<T> u = {T(0), T(0)}, v = {T(0), T(0)};
-      if (active) {
-        u = p0[off];
-        v = p1[off];
-      }
-      sPSI[k * Q + p] = u;
-      sPSI[k * Q + P + p] = v;
-      sW0[k * Q + p] = u;
-      sW0[k * Q + P + p] = v;
+    cx<T> u0[N1], u1[N1], g0[N1], g1[N1];
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {
+      const int64_t off = base + (int64_t)(j + L * m) * stride;
+      u0[m] = p0[off];
+      u1[m] = p1[off];
+      g0[m] = u0[m];
+      g1[m] = u1[m];
     }
-    __syncthreads();
-    cx<T>* r = fft_lds<T, N, Q, -1>(sW0, sW1, tw, Q, lt, NTS);
+    F.fwd(g0, tw);
+    F.fwd(g1, tw);
     double sp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int e = lt; e < N * Q; e += NTS) {
-      const int k = e / Q;
-      const cx<T> v = r[e];
-      const double m = double(k <= N / 2 ? k : N - k) * kf;
-      sp[0] += m * m * (double(v.x) * double(v.x) + double(v.y) * double(v.y));
-      r[e] = cmul_is(v, d1_factor<T, N>(k, kfac));
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      const int kq = F.freq(k1);
+      const double mk = double(kq <= N / 2 ? kq : N - kq) * kf;
+      sp[0] += mk * mk *
+               (double(g0[k1].x) * double(g0[k1].x) + double(g0[k1].y) * double(g0[k1].y) +
+                double(g1[k1].x) * double(g1[k1].x) + double(g1[k1].y) * double(g1[k1].y));
+      const T f = d1_factor<T, N>(kq, kfac);
+      g0[k1] = cmul_is(g0[k1], f);
+      g1[k1] = cmul_is(g1[k1], f);
     }
     sp[0] *= 1.0 / N;
-    __syncthreads();
-    cx<T>* d1 = fft_lds<T, N, Q, +1>(r, r == sW0 ? sW1 : sW0, tw, Q, lt, NTS);
-    const double w = active ? a.wocc[s] * a.wstates[s] : 0.0;
-    const double wuv = active ? a.wguv[s] * a.pairwg[s] * a.wstates[s] : 0.0;
+    F.inv(g0, tw);
+    F.inv(g1, tw);
+    const double w = a.wocc[s] * a.wstates[s];
+    const double wuv = a.wguv[s] * a.pairwg[s] * a.wstates[s];
 #pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-      const int e = lt + i * NTS;
-      if (e >= N * P) break;
-      int k, p;
-      elem_of<AXIS, N, P>(e, k, p);
-      const cx<T> u0 = sPSI[k * Q + p], u1 = sPSI[k * Q + P + p];
-      const cx<T> g0 = d1[k * Q + p], g1 = d1[k * Q + P + p];
-      const double ax = u0.x, ay = u0.y, bx = u1.x, by = u1.y;
-      const double cx0 = g0.x, cy0 = g0.y, cx1 = g1.x, cy1 = g1.y;
-      const double tau = cx0 * cx0 + cy0 * cy0 + cx1 * cx1 + cy1 * cy1;
-      const double ia = im_ab(cx0, cy0, ax, ay) + im_ab(cx1, cy1, bx, by);  // Im(d psi conj psi)
+    for (int m = 0; m < N1; ++m) {
+      const int kk = j + L * m;
       int x, y, z;
-      coords_of<AXIS>(tile, k, p, a.nx, a.ny, a.nz, P, x, y, z);
+      if constexpr (AXIS == 0) {
+        x = kk; y = c0; z = c1;
+      } else if constexpr (AXIS == 1) {
+        x = c0; y = kk; z = c1;
+      } else {
+        x = c0; y = c1; z = kk;
+      }
       const double xc = (x - 0.5 * (a.nx - 1)) * a.dx - a.cm[0];
       const double yc = (y - 0.5 * (a.ny - 1)) * a.dy - a.cm[1];
       const double zc = (z - 0.5 * (a.nz - 1)) * a.dz - a.cm[2];
-      acc[i][0] += w * tau;
-      acc[i][1] += w * ia;
+      const double ax = u0[m].x, ay = u0[m].y, bx = u1[m].x, by = u1[m].y;
+      const double cx0 = g0[m].x, cy0 = g0[m].y, cx1 = g1[m].x, cy1 = g1[m].y;
+      const double tau = cx0 * cx0 + cy0 * cy0 + cx1 * cx1 + cy1 * cy1;
+      const double ia = im_ab(cx0, cy0, ax, ay) + im_ab(cx1, cy1, bx, by);  // Im(d psi conj psi)
+      acc[m][0] += w * tau;
+      acc[m][1] += w * ia;
       if constexpr (AXIS == 0) {
         // densities.py:270-277
-        acc[i][2] += -w * (im_ab(cx0, cy0, ax, ay) - im_ab(cx1, cy1, bx, by));   // J_y
-        acc[i][3] += -w * (re_ab(ax, ay, cx1, cy1) - re_ab(bx, by, cx0, cy0));   // J_z
+        acc[m][2] += -w * (im_ab(cx0, cy0, ax, ay) - im_ab(cx1, cy1, bx, by));   // J_y
+        acc[m][3] += -w * (re_ab(ax, ay, cx1, cy1) - re_ab(bx, by, cx0, cy0));   // J_z
         const double n0 = ax * ax + ay * ay, n1 = bx * bx + by * by;
         const double rsx = ax * bx + ay * by;  // Re(conj psi0 psi1)
         const double rsy = ax * by - ay * bx;  // Im(conj psi0 psi1)
-        acc[i][4] += w * (n0 + n1);
-        acc[i][5] += 0.5 * wuv * (n0 + n1);
-        acc[i][6] += 2.0 * w * rsx;
-        acc[i][7] += 2.0 * w * rsy;
-        acc[i][8] += w * (n0 - n1);
+        acc[m][4] += w * (n0 + n1);
+        acc[m][5] += 0.5 * wuv * (n0 + n1);
+        acc[m][6] += 2.0 * w * rsx;
+        acc[m][7] += 2.0 * w * rsy;
+        acc[m][8] += w * (n0 - n1);
         sp[2] += zc * ia;
         sp[3] -= yc * ia;
         sp[4] += 2.0 * rsx;
         sp[5] += 2.0 * rsy;
         sp[6] += n0 - n1;
-        if (active) {  // parity: Re sum conj psi(r) psi(-r)  (inout.py:134-137)
-          const int64_t mo = ((int64_t)(a.nx - 1 - x) * a.ny + (a.ny - 1 - y)) * a.nz + (a.nz - 1 - z);
-          const cx<T> f0 = p0[mo], f1 = p1[mo];
-          sp[7] += ax * double(f0.x) + ay * double(f0.y) + bx * double(f1.x) + by * double(f1.y);
-        }
+        // parity: Re sum conj psi(r) psi(-r)  (inout.py:134-137)
+        const int64_t mo = ((int64_t)(a.nx - 1 - x) * a.ny + (a.ny - 1 - y)) * a.nz + (a.nz - 1 - z);
+        const cx<T> f0 = p0[mo], f1 = p1[mo];
+        sp[7] += ax * double(f0.x) + ay * double(f0.y) + bx * double(f1.x) + by * double(f1.y);
       } else if constexpr (AXIS == 1) {
         // densities.py:289-296
-        acc[i][2] += w * (im_ab(cx0, cy0, ax, ay) - im_ab(cx1, cy1, bx, by));    // J_x
-        acc[i][3] += -w * (im_ab(cx1, cy1, ax, ay) + im_ab(cx0, cy0, bx, by));   // J_z
+        acc[m][2] += w * (im_ab(cx0, cy0, ax, ay) - im_ab(cx1, cy1, bx, by));    // J_x
+        acc[m][3] += -w * (im_ab(cx1, cy1, ax, ay) + im_ab(cx0, cy0, bx, by));   // J_z
         sp[1] -= zc * ia;
         sp[3] += xc * ia;
       } else {
         // densities.py:308-315
-        acc[i][2] += w * (re_ab(cx1, cy1, ax, ay) - re_ab(cx0, cy0, bx, by));    // J_x
-        acc[i][3] += w * (im_ab(cx1, cy1, ax, ay) + im_ab(cx0, cy0, bx, by));    // J_y
+        acc[m][2] += w * (re_ab(cx1, cy1, ax, ay) - re_ab(cx0, cy0, bx, by));    // J_x
+        acc[m][3] += w * (im_ab(cx1, cy1, ax, ay) + im_ab(cx0, cy0, bx, by));    // J_y
         sp[1] += yc * ia;
         sp[2] -= xc * ia;
       }
     }
-    if (a.want_sp) {
-      // sub-group sums (each sub-group = NTS/64 waves)
+    if (a.want_sp) {  // the wavefront covers this tile's pencils for state s
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        double v = sp[j];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-        if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * 8 + j] = v;
+      for (int k = 0; k < 8; ++k) {
+        double v = sp[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        sp[k] = v;
       }
-      __syncthreads();
-      if (lt < 8 && active) {
-        double v = 0.0;
-        for (int wv = 0; wv < NTS / 64; ++wv) v += red[(sg * (NTS / 64) + wv) * 8 + lt];
-        a.spart[((int64_t)s * a.ntile + tile) * 8 + lt] = v;
+      if (lane < 8) {
+        double v = sp[0];
+#pragma unroll
+        for (int k = 1; k < 8; ++k)
+          if (lane == k) v = sp[k];
+        a.spart[((int64_t)s * a.ntile + tile) * 8 + lane] = v;
       }
     }
-    __syncthreads();
   }
   if (!a.want_dens) return;
-  // combine the sub-groups' accumulators through LDS, slot by slot
-  double* comb = (double*)lds_raw;  // NSUB * N*P doubles (fits in the tile buffers)
+  // combine the four wavefronts' accumulators in LDS, slot by slot
   const int slot0 = AXIS == 0 ? 0 : (AXIS == 1 ? 9 : 13);
-  // accumulator j -> slot: x: tau,j,Jy,Jz,rho,chi,sx,sy,sz -> 5,6,7,8,0,1,2,3,4
-  for (int j = 0; j < NACC; ++j) {
+#pragma unroll
+  for (int k = 0; k < NACC; ++k) {
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < PPT; ++i) {
-      const int e = lt + i * NTS;
-      if (e < N * P) comb[sg * N * P + e] = acc[i][j];
-    }
+    for (int m = 0; m < N1; ++m) comb[wave][q * N + j + L * m] = acc[m][k];
     __syncthreads();
     int slot;
     if constexpr (AXIS == 0)
-      slot = j < 4 ? 5 + j : j - 4;
+      slot = k < 4 ? 5 + k : k - 4;
     else
-      slot = slot0 + j;
-    double* dst = a.dout + ((int64_t)iq * NSLOT + slot) * nvox;
-    for (int e = threadIdx.x; e < N * P; e += NT) {
-      double v = 0.0;
-#pragma unroll
-      for (int g = 0; g < NSUB; ++g) v += comb[g * N * P + e];
-      int k, p;
-      elem_of<AXIS, N, P>(e, k, p);
-      dst[t.base + k * t.sk + p * t.sp] = v;
+      slot = slot0 + k;
+    double* dst = a.dout + (((int64_t)chunk * 2 + iq) * NSLOT + slot) * nvox;
+    for (int e = threadIdx.x; e < PW * N; e += 256) {
+      int qq, kk;
+      if constexpr (AXIS == 2) {  // contiguous along the pencil
+        qq = e / N;
+        kk = e % N;
+      } else {  // contiguous across pencils
+        qq = e % PW;
+        kk = e / PW;
+      }
+      const int pe = qq * N + kk;
+      const double v = comb[0][pe] + comb[1][pe] + comb[2][pe] + comb[3][pe];
+      const int gg = tile * PW + qq;
+      int64_t bb;
+      if constexpr (AXIS == 0)
+        bb = gg;
+      else if constexpr (AXIS == 1)
+        bb = (int64_t)(gg / a.nz) * a.ny * a.nz + gg % a.nz;
+      else
+        bb = (int64_t)gg * a.nz;
+      dst[bb + (int64_t)kk * stride] = v;
     }
   }
 }
 
-template <typename T, int AXIS, int P, int NSUB>
+template <typename T, int AXIS>
 int launch_dens(const hfb_ctx* c, DensArgs a, hipStream_t st) {
   const int n = axis_len(c, AXIS);
-  const int nt = tiles_per_field<AXIS>(c->nx, c->ny, c->nz, P);
-  a.ntile = nt;
+  const int np = pencils_axis(AXIS, c->nx, c->ny, c->nz);
+  if (np % DENS_PW != 0) return HFB_E_SIZE;
+  a.ntile = np / DENS_PW;
   HFB_DISPATCH_N(n, {
-    constexpr int NN = N_;
-    const size_t lds = sizeof(cx<T>) * (size_t)NSUB * 3 * NN * 2 * P;
-    auto kern = dens_axis_kernel<T, NN, AXIS, P, NSUB>;
-    if (lds > 32 * 1024) allow_full_lds((const void*)kern);
-    hipLaunchKernelGGL(kern, dim3(nt, 2), dim3(NSUB * 8 * P), lds, st, a, twiddles<T>(c, NN));
+    hipLaunchKernelGGL((dens_lane_kernel<T, N_, AXIS>), dim3(a.ntile, 2, DENS_G), dim3(256), 0, st, a,
+                       twiddles<T>(c, N_));
   })
   return launch_status();
 }
@@ -250,9 +274,16 @@ __global__ void dens_finish_kernel(const double* __restrict__ d, int64_t nvox, d
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
     const int iq = e / nvox;
     const int64_t v = e % nvox;
-    const double* s = d + (int64_t)iq * NSLOT * nvox + v;
-    double r = s[0 * nvox], ch = s[1 * nvox];
-    double ta = s[5 * nvox] + s[9 * nvox] + s[13 * nvox];
+    double sl[NSLOT];
+#pragma unroll
+    for (int k = 0; k < NSLOT; ++k) {
+      double acc = 0.0;
+      for (int gch = 0; gch < DENS_G; ++gch) acc += d[(((int64_t)gch * 2 + iq) * NSLOT + k) * nvox + v];
+      sl[k] = acc;
+    }
+    const double* s = sl;
+    double r = s[0], ch = s[1];
+    double ta = s[5] + s[9] + s[13];
     if (mix) {
       const double ac = 1.0 - addnew;
       r = addnew * r + ac * rho[e];
@@ -263,15 +294,15 @@ __global__ void dens_finish_kernel(const double* __restrict__ d, int64_t nvox, d
     chi[e] = ch;
     tau[e] = ta;
     const int64_t b3 = (int64_t)iq * 3 * nvox + v;
-    cur[b3] = s[6 * nvox];
-    cur[b3 + nvox] = s[10 * nvox];
-    cur[b3 + 2 * nvox] = s[14 * nvox];
-    sdens[b3] = s[2 * nvox];
-    sdens[b3 + nvox] = s[3 * nvox];
-    sdens[b3 + 2 * nvox] = s[4 * nvox];
-    sodens[b3] = s[11 * nvox] + s[15 * nvox];
-    sodens[b3 + nvox] = s[7 * nvox] + s[16 * nvox];
-    sodens[b3 + 2 * nvox] = s[8 * nvox] + s[12 * nvox];
+    cur[b3] = s[6];
+    cur[b3 + nvox] = s[10];
+    cur[b3 + 2 * nvox] = s[14];
+    sdens[b3] = s[2];
+    sdens[b3 + nvox] = s[3];
+    sdens[b3 + 2 * nvox] = s[4];
+    sodens[b3] = s[11] + s[15];
+    sodens[b3 + nvox] = s[7] + s[16];
+    sodens[b3 + 2 * nvox] = s[8] + s[12];
   }
 }
 
@@ -310,13 +341,11 @@ __global__ void sp_finish_kernel(const double* __restrict__ spart, int ns, int n
 
 extern "C" int hfb_density_sizes(hfb_ctx* c, int prec, long long* dout_doubles, long long* spart_doubles_per_state) {
   if (!c) return HFB_E_NOCTX;
-  const int P = 8;
   const long long nvox = (long long)c->nx * c->ny * c->nz;
-  if (dout_doubles) *dout_doubles = 2LL * NSLOT * nvox;
+  if (dout_doubles) *dout_doubles = (long long)DENS_G * 2LL * NSLOT * nvox;
   if (spart_doubles_per_state)
-    *spart_doubles_per_state = 8LL * (tiles_per_field<0>(c->nx, c->ny, c->nz, P) +
-                                      tiles_per_field<1>(c->nx, c->ny, c->nz, P) +
-                                      tiles_per_field<2>(c->nx, c->ny, c->nz, P));
+    *spart_doubles_per_state = 8LL * (pencils_axis(0, c->nx, c->ny, c->nz) + pencils_axis(1, c->nx, c->ny, c->nz) +
+                                      pencils_axis(2, c->nx, c->ny, c->nz)) / DENS_PW;
   (void)prec;
   return HFB_OK;
 }
@@ -328,7 +357,6 @@ extern "C" int hfb_density(hfb_ctx* c, int prec, const void* psi, int nst0, int 
   if (!c) return HFB_E_NOCTX;
   if (!psi || !wocc || !wstates || !wguv || !pairwg || nst0 < 0 || nst1 < 0) return HFB_E_ARG;
   if ((want_dens && !dout) || (want_sp && !spart)) return HFB_E_ARG;
-  constexpr int P = 8, NSUB = 4;
   DensArgs a{};
   a.nx = c->nx;
   a.ny = c->ny;
@@ -350,20 +378,20 @@ extern "C" int hfb_density(hfb_ctx* c, int prec, const void* psi, int nst0, int 
   a.want_dens = want_dens;
   a.want_sp = want_sp;
   const int ns = nst0 + nst1;
-  const int nt0 = tiles_per_field<0>(c->nx, c->ny, c->nz, P);
-  const int nt1 = tiles_per_field<1>(c->nx, c->ny, c->nz, P);
+  const int nt0 = pencils_axis(0, c->nx, c->ny, c->nz) / DENS_PW;
+  const int nt1 = pencils_axis(1, c->nx, c->ny, c->nz) / DENS_PW;
   hipStream_t st = (hipStream_t)stream;
   int rc = HFB_OK;
   for (int ax = 0; ax < 3 && rc == HFB_OK; ++ax) {
     a.spart = spart ? spart + (ax == 0 ? 0 : (ax == 1 ? (int64_t)ns * nt0 * 8 : (int64_t)ns * (nt0 + nt1) * 8)) : nullptr;
     if (prec == HFB_F64) {
-      if (ax == 0) rc = launch_dens<double, 0, P, NSUB>(c, a, st);
-      if (ax == 1) rc = launch_dens<double, 1, P, NSUB>(c, a, st);
-      if (ax == 2) rc = launch_dens<double, 2, P, NSUB>(c, a, st);
+      if (ax == 0) rc = launch_dens<double, 0>(c, a, st);
+      if (ax == 1) rc = launch_dens<double, 1>(c, a, st);
+      if (ax == 2) rc = launch_dens<double, 2>(c, a, st);
     } else if (prec == HFB_F32) {
-      if (ax == 0) rc = launch_dens<float, 0, P, NSUB>(c, a, st);
-      if (ax == 1) rc = launch_dens<float, 1, P, NSUB>(c, a, st);
-      if (ax == 2) rc = launch_dens<float, 2, P, NSUB>(c, a, st);
+      if (ax == 0) rc = launch_dens<float, 0>(c, a, st);
+      if (ax == 1) rc = launch_dens<float, 1>(c, a, st);
+      if (ax == 2) rc = launch_dens<float, 2>(c, a, st);
     } else {
       return HFB_E_ARG;
     }
@@ -388,12 +416,11 @@ extern "C" int hfb_sp_finish(hfb_ctx* c, const double* spart, int nst0, int nst1
                              void* stream) {
   if (!c) return HFB_E_NOCTX;
   if (!spart || !sp_kinetic || !sp_orbital || !sp_spin || !sp_parity) return HFB_E_ARG;
-  constexpr int P = 8;
   const int ns = nst0 + nst1;
   if (ns <= 0) return HFB_E_ARG;
   hipLaunchKernelGGL(sp_finish_kernel, dim3(ns), dim3(256), 0, (hipStream_t)stream, spart, ns, nst0,
-                     tiles_per_field<0>(c->nx, c->ny, c->nz, P), tiles_per_field<1>(c->nx, c->ny, c->nz, P),
-                     tiles_per_field<2>(c->nx, c->ny, c->nz, P), c->dx * c->dy * c->dz, h2m0, h2m1, sp_kinetic,
-                     sp_orbital, sp_spin, sp_parity);
+                     pencils_axis(0, c->nx, c->ny, c->nz) / DENS_PW, pencils_axis(1, c->nx, c->ny, c->nz) / DENS_PW,
+                     pencils_axis(2, c->nx, c->ny, c->nz) / DENS_PW, c->dx * c->dy * c->dz, h2m0, h2m1,
+                     sp_kinetic, sp_orbital, sp_spin, sp_parity);
   return launch_status();
 }

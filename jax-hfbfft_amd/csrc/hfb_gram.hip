@@ -280,11 +280,206 @@ __global__ void __launch_bounds__(256) rotate_kernel(const cx<T>* psi, const dou
   }
 }
 
+// ---------------------------------------------------------------------------
+// Small blocks (nst <= 64) on the vector FP64 units.  On gfx950 the FP64
+// matrix and vector rates are equal, and 16x16 MFMA tiles padded from 20 or
+// 36 states waste 1.6-2.9x of the work, so blocks up to 64 states use FMAs:
+// a wavefront owns a 4x4 tile of one C_m (complex, in registers) and a chunk
+// of K, lanes stride over k (coalesced rows); the lane sums are combined by
+// shuffles and the chunks by gram_reduce_kernel in a fixed order.  For
+// B_m == A only tiles with ti <= tj are computed; the reduction mirrors the
+// rest (C^H = C).
+// ---------------------------------------------------------------------------
+constexpr int GV_T = 4;      // tile edge
+constexpr int GV_KPL = 32;   // k per lane per chunk
+constexpr int GV_CHUNK = 64 * GV_KPL;
+
+int gram_vec_nchunk(int64_t K) {
+  int64_t nc = (K + GV_CHUNK - 1) / GV_CHUNK;
+  return nc < 1 ? 1 : (nc > 1024 ? 1024 : (int)nc);
+}
+
+struct GramVecArgs {
+  GramIso iso[2];
+  int nm;
+  int bsame;
+  int64_t K;
+  int64_t kc;      // K per chunk
+  int nchunk;
+  int ntile[2];    // tiles per isospin (over all m)
+};
+
+// tile index -> (m, ti, tj) for one isospin: matrices in order; for a
+// matrix with B == A the upper-triangular tiles (ti <= tj) row by row.
+__device__ __forceinline__ void gv_tile(int t, int nt, int nm, int bsame, int& m, int& ti, int& tj) {
+  m = 0;
+  for (; m < nm; ++m) {
+    const bool sym = (bsame >> m) & 1;
+    const int cnt = sym ? nt * (nt + 1) / 2 : nt * nt;
+    if (t < cnt) break;
+    t -= cnt;
+  }
+  const bool sym = (bsame >> m) & 1;
+  if (!sym) {
+    ti = t / nt;
+    tj = t % nt;
+    return;
+  }
+  ti = 0;
+  while (t >= nt - ti) {
+    t -= nt - ti;
+    ++ti;
+  }
+  tj = ti + t;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) gram_vec_kernel(GramVecArgs ga) {
+  const int iq = blockIdx.y;
+  const GramIso& g = ga.iso[iq];
+  const int nst = g.nst;
+  if (nst == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int item = blockIdx.x * 4 + (threadIdx.x >> 6);  // (tile, chunk)
+  const int ntile = ga.ntile[iq];
+  if (item >= ntile * ga.nchunk) return;
+  const int tile = item / ga.nchunk, chunk = item % ga.nchunk;
+  const int nt = (nst + GV_T - 1) / GV_T;
+  int m, ti, tj;
+  gv_tile(tile, nt, ga.nm, ga.bsame, m, ti, tj);
+  const cx<T>* A = (const cx<T>*)g.A;
+  const cx<T>* B = ((ga.bsame >> m) & 1) ? A : (const cx<T>*)g.B[m];
+  const int64_t K = ga.K;
+  const int64_t k0 = (int64_t)chunk * ga.kc, k1 = min(K, k0 + ga.kc);
+  int ri[GV_T], rj[GV_T];
+#pragma unroll
+  for (int u = 0; u < GV_T; ++u) {
+    ri[u] = min(ti * GV_T + u, nst - 1);  // clamped rows are computed and discarded
+    rj[u] = min(tj * GV_T + u, nst - 1);
+  }
+  double cr[GV_T][GV_T], ci[GV_T][GV_T];
+#pragma unroll
+  for (int a = 0; a < GV_T; ++a)
+#pragma unroll
+    for (int b = 0; b < GV_T; ++b) cr[a][b] = ci[a][b] = 0.0;
+  for (int64_t k = k0 + lane; k < k1; k += 64) {
+    cx<double> av[GV_T], bv[GV_T];
+#pragma unroll
+    for (int u = 0; u < GV_T; ++u) {
+      av[u] = ld64(A + (int64_t)ri[u] * K + k);
+      bv[u] = ld64(B + (int64_t)rj[u] * K + k);
+    }
+#pragma unroll
+    for (int a = 0; a < GV_T; ++a)
+#pragma unroll
+      for (int b = 0; b < GV_T; ++b) {  // conj(a) b
+        cr[a][b] = fma(av[a].x, bv[b].x, fma(av[a].y, bv[b].y, cr[a][b]));
+        ci[a][b] = fma(av[a].x, bv[b].y, fma(-av[a].y, bv[b].x, ci[a][b]));
+      }
+  }
+#pragma unroll
+  for (int a = 0; a < GV_T; ++a)
+#pragma unroll
+    for (int b = 0; b < GV_T; ++b)
+      for (int o = 32; o > 0; o >>= 1) {
+        cr[a][b] += __shfl_xor(cr[a][b], o, 64);
+        ci[a][b] += __shfl_xor(ci[a][b], o, 64);
+      }
+  if (lane == 0) {
+    const int np = nt * GV_T;
+#pragma unroll
+    for (int a = 0; a < GV_T; ++a)
+#pragma unroll
+      for (int b = 0; b < GV_T; ++b) {
+        const int row = ti * GV_T + a, col = tj * GV_T + b;
+        double* dst = g.part + 2 * ((((int64_t)m * np + row) * np + col) * ga.nchunk + chunk);
+        dst[0] = cr[a][b];
+        dst[1] = ci[a][b];
+      }
+  }
+}
+
+// Sum the chunks of each output element (one wavefront per element, fixed
+// order); entries below the diagonal of a B == A matrix are the conjugates
+// of the computed upper ones.
+__global__ void __launch_bounds__(256) gram_vec_reduce_kernel(const double* __restrict__ part, int nchunk, int nm,
+                                                              int np, int nst, int bsame, double scale,
+                                                              double* __restrict__ out) {
+  const int64_t total = (int64_t)nm * nst * nst;
+  const int lane = threadIdx.x & 63;
+  const int64_t wid = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t e = wid; e < total; e += nw) {
+    const int m = e / ((int64_t)nst * nst);
+    const int ij = e % ((int64_t)nst * nst);
+    int i = ij / nst, j = ij % nst;
+    const bool mirror = ((bsame >> m) & 1) && i > j;
+    if (mirror) {
+      const int t = i;
+      i = j;
+      j = t;
+    }
+    const double* p = part + 2 * (((int64_t)m * np + i) * np + j) * nchunk;
+    double re = 0.0, im = 0.0;
+    for (int ch = lane; ch < nchunk; ch += 64) {
+      re += p[2 * ch];
+      im += p[2 * ch + 1];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      re += __shfl_down(re, o, 64);
+      im += __shfl_down(im, o, 64);
+    }
+    if (lane == 0) {
+      out[2 * e] = re * scale;
+      out[2 * e + 1] = (mirror ? -im : im) * scale;
+    }
+  }
+}
+
+// Rotation of a small block (nst <= 32) on the vector units: each lane owns
+// one k, reads psi_i[k] for every i into registers, then writes
+// out1_j[k] = sum_i psi_i[k] U1[i][j] (and out2 with U2).  A whole column is
+// read before anything is written, so out1 may alias psi.  U is uniform
+// across the wavefront and is read through the scalar cache.
+template <typename T, int NI>
+__global__ void __launch_bounds__(256) rotate_vec_kernel(const cx<T>* psi, const double* __restrict__ U1,
+                                                         const double* __restrict__ U2, cx<T>* out1, cx<T>* out2,
+                                                         int nst, int64_t K) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  cx<double> v[NI];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) v[i] = i < nst ? ld64(psi + (int64_t)i * K + k) : cx<double>{0.0, 0.0};
+  const cx<double>* u1 = (const cx<double>*)U1;
+  const cx<double>* u2 = (const cx<double>*)U2;
+  for (int j = 0; j < nst; ++j) {
+    double r1 = 0.0, i1 = 0.0, r2 = 0.0, i2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      if (i < nst) {
+        const cx<double> a = u1[i * nst + j];
+        r1 = fma(v[i].x, a.x, fma(-v[i].y, a.y, r1));
+        i1 = fma(v[i].x, a.y, fma(v[i].y, a.x, i1));
+        if (u2) {
+          const cx<double> b = u2[i * nst + j];
+          r2 = fma(v[i].x, b.x, fma(-v[i].y, b.y, r2));
+          i2 = fma(v[i].x, b.y, fma(v[i].y, b.x, i2));
+        }
+      }
+    }
+    out1[(int64_t)j * K + k] = cx<T>{T(r1), T(i1)};
+    if (u2) out2[(int64_t)j * K + k] = cx<T>{T(r2), T(i2)};
+  }
+}
+
 }  // namespace
 
 extern "C" int hfb_gram_workspace(int nst, long long K) {
   const int np = (nst + 15) & ~15;
-  return gram_nchunk(np, K) * 4 * np * np * 2;  // doubles for up to four matrices
+  const int mfma = gram_nchunk(np, K) * 4 * np * np * 2;  // doubles for up to four matrices
+  const int np4 = (nst + GV_T - 1) / GV_T * GV_T;
+  const int vec = gram_vec_nchunk(K) * 4 * np4 * np4 * 2;
+  return mfma > vec ? mfma : vec;
 }
 
 template <typename T, int NM>
@@ -347,6 +542,43 @@ extern "C" int hfb_gram2(int prec, const void* const* A, const void* const* B, i
     npmax = gi.np > npmax ? gi.np : npmax;
   }
   if (npmax == 0) return HFB_OK;
+  if (npmax <= 64) {  // small blocks: vector FP64 path
+    GramVecArgs v{};
+    v.iso[0] = g.iso[0];
+    v.iso[1] = g.iso[1];
+    v.nm = nm;
+    v.bsame = g.bsame;
+    v.K = K;
+    v.nchunk = gram_vec_nchunk(K);
+    v.kc = (K + v.nchunk - 1) / v.nchunk;
+    int items = 0;
+    for (int iq = 0; iq < 2; ++iq) {
+      const int nt = (nst[iq] + GV_T - 1) / GV_T;
+      int cnt = 0;
+      for (int m = 0; m < nm; ++m) cnt += ((v.bsame >> m) & 1) ? nt * (nt + 1) / 2 : nt * nt;
+      v.ntile[iq] = nst[iq] > 0 ? cnt : 0;
+      items = v.ntile[iq] * v.nchunk > items ? v.ntile[iq] * v.nchunk : items;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == HFB_F64)
+      hipLaunchKernelGGL(gram_vec_kernel<double>, dim3((items + 3) / 4, 2), dim3(256), 0, st, v);
+    else if (prec == HFB_F32)
+      hipLaunchKernelGGL(gram_vec_kernel<float>, dim3((items + 3) / 4, 2), dim3(256), 0, st, v);
+    else
+      return HFB_E_ARG;
+    int rc = launch_status();
+    for (int iq = 0; iq < 2 && rc == HFB_OK; ++iq) {
+      if (nst[iq] <= 0) continue;
+      const int np4 = (nst[iq] + GV_T - 1) / GV_T * GV_T;
+      const int64_t tot = (int64_t)nm * nst[iq] * nst[iq];
+      int nb = (int)((tot * 64 + 255) / 256);
+      if (nb > 4096) nb = 4096;
+      hipLaunchKernelGGL(gram_vec_reduce_kernel, dim3(nb < 1 ? 1 : nb), dim3(256), 0, st, part[iq], v.nchunk, nm,
+                         np4, nst[iq], v.bsame, scale, out[iq]);
+      rc = launch_status();
+    }
+    return rc;
+  }
   g.nm = nm;
   g.K = K;
   g.nchunk = gram_nchunk(npmax, K);
@@ -398,6 +630,28 @@ extern "C" int hfb_rotate(int prec, const void* psi, const double* U1, const dou
                           int nst, long long K, void* stream) {
   if (!psi || !U1 || !out1 || nst <= 0 || K <= 0) return HFB_E_ARG;
   if (U2 && !out2) return HFB_E_ARG;
+  if (nst <= 32) {  // small blocks: register columns on the vector units
+    const int nb = (int)((K + 255) / 256);
+    hipStream_t st0 = (hipStream_t)stream;
+#define HFB_RV(TT, NIV)                                                                                   \
+  hipLaunchKernelGGL((rotate_vec_kernel<TT, NIV>), dim3(nb), dim3(256), 0, st0, (const cx<TT>*)psi, U1, U2, \
+                     (cx<TT>*)out1, (cx<TT>*)out2, nst, (int64_t)K)
+    if (prec == HFB_F64) {
+      if (nst <= 8) HFB_RV(double, 8);
+      else if (nst <= 16) HFB_RV(double, 16);
+      else if (nst <= 24) HFB_RV(double, 24);
+      else HFB_RV(double, 32);
+    } else if (prec == HFB_F32) {
+      if (nst <= 8) HFB_RV(float, 8);
+      else if (nst <= 16) HFB_RV(float, 16);
+      else if (nst <= 24) HFB_RV(float, 24);
+      else HFB_RV(float, 32);
+    } else {
+      return HFB_E_ARG;
+    }
+#undef HFB_RV
+    return launch_status();
+  }
   const int np = (nst + 15) & ~15;
   const int ntj = np / 16;
   const int njt = ntj < 4 ? ntj : 4;  // j-tiles per workgroup column

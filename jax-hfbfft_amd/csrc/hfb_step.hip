@@ -5,7 +5,11 @@
 // field: z (residual r formed from ps1 and lagrange or h_exp*psi, forward),
 // y (forward), x (forward, divide by the preconditioner, inverse), y
 // (inverse), z (inverse, psi <- psi - x0dmp * P r).  A scratch array of
-// the spinor block's size carries the transform between passes.
+// the spinor block's size carries the transform between passes; along an
+// axis that has been transformed, element k of a pencil holds frequency k.
+// Each pencil sits in a group of L lanes (LaneFFT layout): a forward pass
+// stores element k1 of lane j at frequency position k1 + N1 bitrev(j), an
+// inverse pass reads from there.
 #include <math.h>
 
 #include "hfb_ctx.h"
@@ -32,111 +36,127 @@ struct StepArgs {
   double* fpart;         // (2S fields, ntile, 2) partials of |r|^2, |lag - h_exp psi|^2
 };
 
-template <typename T, int N, int AXIS, int MODE, int P>
-__global__ void __launch_bounds__(8 * P) step_pass(StepArgs a, const cx<T>* __restrict__ twg) {
-  constexpr int NT = 8 * P;
-  __shared__ cx<T> tw[N];
-  __shared__ cx<T> b0[N * P], b1[N * P];
-  __shared__ double red[(NT + 63) / 64 * 2];
-  const int tile = blockIdx.x;
-  const int f = blockIdx.y;  // field = 2*state + spin
+constexpr int STEP_L = 4;                 // lanes per pencil
+constexpr int STEP_PPB = 4 * (64 / STEP_L);  // pencils per block
+
+inline int step_pencils(const hfb_ctx* c, int axis) {
+  return axis == 0 ? c->ny * c->nz : (axis == 1 ? c->nx * c->nz : c->nx * c->ny);
+}
+
+template <typename T, int N, int AXIS, int MODE>
+__global__ void __launch_bounds__(256) step_lane(StepArgs a, const cx<T>* __restrict__ tw) {
+  constexpr int L = STEP_L, N1 = N / L, PW = 64 / L;
+  __shared__ double red[4 * 2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = lane % L, q = lane / L;
+  const int g = (blockIdx.x * 4 + wave) * PW + q;  // pencil
+  const int f = blockIdx.y;                         // field = 2*state + spin
   const int s = f >> 1;
   const int64_t nvox = (int64_t)a.nx * a.ny * a.nz;
   const int64_t fo = (int64_t)f * nvox;
-  const Tile t = make_tile<AXIS>(tile, a.nx, a.ny, a.nz, P);
+  int64_t base, stride;
+  int c0, c1;
+  if constexpr (AXIS == 0) {
+    base = g;
+    stride = (int64_t)a.ny * a.nz;
+    c0 = g / a.nz;
+    c1 = g % a.nz;
+  } else if constexpr (AXIS == 1) {
+    base = (int64_t)(g / a.nz) * a.ny * a.nz + g % a.nz;
+    stride = a.nz;
+    c0 = g / a.nz;
+    c1 = g % a.nz;
+  } else {
+    base = (int64_t)g * a.nz;
+    stride = 1;
+    c0 = g / a.ny;
+    c1 = g % a.ny;
+  }
+  LaneFFT<T, N, L> F;
+  F.init(j, tw);
   cx<T>* psi = (cx<T>*)a.psi;
   cx<T>* R = (cx<T>*)a.work;
-  load_twiddles(tw, twg, N);
+  constexpr bool IN_SPECTRAL = MODE == SM_INV || MODE == SM_OUT;  // input pencil holds frequencies
+  cx<T> v[N1];
   double acc[2] = {0.0, 0.0};
-  for (int e = threadIdx.x; e < N * P; e += blockDim.x) {
-    int k, p;
-    elem_of<AXIS, N, P>(e, k, p);
-    const int64_t off = fo + t.base + k * t.sk + p * t.sp;
-    cx<T> v;
+#pragma unroll
+  for (int m = 0; m < N1; ++m) {
+    const int kk = IN_SPECTRAL ? F.freq(m) : j + L * m;
+    const int64_t off = fo + base + (int64_t)kk * stride;
     if constexpr (MODE == SM_IN) {
       const T hexp = T(a.wxyz * a.hsum[2 * s + 1]);
       const cx<T> p1 = ((const cx<T>*)a.ps1)[off];
       const cx<T> ps = psi[off];
       const cx<T> lg = ((const cx<T>*)a.lag)[off];
       const cx<T> hp = cscale(ps, hexp);
-      v = a.use_lag ? csub(p1, lg) : csub(p1, hp);
+      v[m] = a.use_lag ? csub(p1, lg) : csub(p1, hp);
       const cx<T> e2 = csub(lg, hp);
-      acc[0] += double(v.x) * double(v.x) + double(v.y) * double(v.y);
+      acc[0] += double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
       acc[1] += double(e2.x) * double(e2.x) + double(e2.y) * double(e2.y);
     } else {
-      v = R[off];
+      v[m] = R[off];
     }
-    b0[k * P + p] = v;
   }
   if constexpr (MODE == SM_IN) {
     if (a.fpart) {
       block_sum<2>(acc, red);
       if (threadIdx.x == 0) {
-        a.fpart[((int64_t)f * gridDim.x + tile) * 2 + 0] = acc[0];
-        a.fpart[((int64_t)f * gridDim.x + tile) * 2 + 1] = acc[1];
+        a.fpart[((int64_t)f * gridDim.x + blockIdx.x) * 2 + 0] = acc[0];
+        a.fpart[((int64_t)f * gridDim.x + blockIdx.x) * 2 + 1] = acc[1];
       }
     }
   }
-  __syncthreads();
-  cx<T>* r;
-  if constexpr (MODE == SM_INV || MODE == SM_OUT)
-    r = fft_lds<T, N, P, +1>(b0, b1, tw, P);
+  if constexpr (IN_SPECTRAL)
+    F.inv(v, tw);
   else
-    r = fft_lds<T, N, P, -1>(b0, b1, tw, P);
-  if constexpr (MODE == SM_DIV) {
+    F.fwd(v, tw);
+  if constexpr (MODE == SM_DIV) {  // levels.py:407-434 (k^2 with the Nyquist mode kept)
     const double ntot = double(nvox);
-    for (int e = threadIdx.x; e < N * P; e += blockDim.x) {
-      const int k = e / P, p = e % P;
-      int x, y, z;
-      coords_of<AXIS>(tile, k, p, a.nx, a.ny, a.nz, P, x, y, z);
-      const int mx = x <= a.nx / 2 ? x : a.nx - x;
-      const int my = y <= a.ny / 2 ? y : a.ny - y;
-      const int mz = z <= a.nz / 2 ? z : a.nz - z;
-      const double k2 = (mx * a.kfx) * (mx * a.kfx) + (my * a.kfy) * (my * a.kfy) + (mz * a.kfz) * (mz * a.kfz);
+    const int my = AXIS == 0 ? (c0 <= a.ny / 2 ? c0 : a.ny - c0) : 0;
+    const int mz = AXIS == 0 ? (c1 <= a.nz / 2 ? c1 : a.nz - c1) : 0;
+    const double kyz = (my * a.kfy) * (my * a.kfy) + (mz * a.kfz) * (mz * a.kfz);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {
+      const int kx = F.freq(m);
+      const int mx = kx <= N / 2 ? kx : N - kx;
+      const double k2 = (mx * a.kfx) * (mx * a.kfx) + kyz;
       const double den = (a.pw[s] * (a.e0 + a.h2ma * k2) + a.pshift[s]) * ntot;
-      r[e] = cscale(r[e], T(1.0 / den));
+      v[m] = cscale(v[m], T(1.0 / den));
     }
-    __syncthreads();
-    r = fft_lds<T, N, P, +1>(r, r == b0 ? b1 : b0, tw, P);
+    F.inv(v, tw);
   }
-  for (int e = threadIdx.x; e < N * P; e += blockDim.x) {
-    int k, p;
-    elem_of<AXIS, N, P>(e, k, p);
-    const int64_t off = fo + t.base + k * t.sk + p * t.sp;
-    if constexpr (MODE == SM_OUT) {
-      const cx<T> ps = psi[off];
-      const cx<T> pr = r[k * P + p];
-      psi[off] = csub(ps, cscale(pr, T(a.x0)));
-    } else {
-      R[off] = r[k * P + p];
-    }
+  // output: spectral positions after a forward pass, spatial after an inverse
+  constexpr bool OUT_SPECTRAL = MODE == SM_IN || MODE == SM_FWD;
+#pragma unroll
+  for (int m = 0; m < N1; ++m) {
+    const int kk = OUT_SPECTRAL ? F.freq(m) : j + L * m;
+    const int64_t off = fo + base + (int64_t)kk * stride;
+    if constexpr (MODE == SM_OUT)
+      psi[off] = csub(psi[off], cscale(v[m], T(a.x0)));
+    else
+      R[off] = v[m];
   }
 }
 
-template <typename T>
-int step_P(const hfb_ctx* c) {
-  if (c->nx % 16 == 0 && c->ny % 16 == 0 && c->nz % 16 == 0) return 16;
-  return 8;
-}
-
-template <typename T, int AXIS, int MODE, int P>
+template <typename T, int AXIS, int MODE>
 int launch_step(const hfb_ctx* c, const StepArgs& a, int nfields, hipStream_t st) {
   const int n = axis_len(c, AXIS);
-  const int nt = tiles_per_field<AXIS>(c->nx, c->ny, c->nz, P);
+  const int np = step_pencils(c, AXIS);
+  if (np % STEP_PPB != 0) return HFB_E_SIZE;
   HFB_DISPATCH_N(n, {
-    hipLaunchKernelGGL((step_pass<T, N_, AXIS, MODE, P>), dim3(nt, nfields), dim3(8 * P), 0, st, a,
+    hipLaunchKernelGGL((step_lane<T, N_, AXIS, MODE>), dim3(np / STEP_PPB, nfields), dim3(256), 0, st, a,
                        twiddles<T>(c, N_));
   })
   return launch_status();
 }
 
-template <typename T, int P>
+template <typename T>
 int step_all(const hfb_ctx* c, const StepArgs& a, int ns, hipStream_t st) {
-  int rc = launch_step<T, 2, SM_IN, P>(c, a, 2 * ns, st);
-  if (rc == HFB_OK) rc = launch_step<T, 1, SM_FWD, P>(c, a, 2 * ns, st);
-  if (rc == HFB_OK) rc = launch_step<T, 0, SM_DIV, P>(c, a, 2 * ns, st);
-  if (rc == HFB_OK) rc = launch_step<T, 1, SM_INV, P>(c, a, 2 * ns, st);
-  if (rc == HFB_OK) rc = launch_step<T, 2, SM_OUT, P>(c, a, 2 * ns, st);
+  int rc = launch_step<T, 2, SM_IN>(c, a, 2 * ns, st);
+  if (rc == HFB_OK) rc = launch_step<T, 1, SM_FWD>(c, a, 2 * ns, st);
+  if (rc == HFB_OK) rc = launch_step<T, 0, SM_DIV>(c, a, 2 * ns, st);
+  if (rc == HFB_OK) rc = launch_step<T, 1, SM_INV>(c, a, 2 * ns, st);
+  if (rc == HFB_OK) rc = launch_step<T, 2, SM_OUT>(c, a, 2 * ns, st);
   return rc;
 }
 
@@ -211,8 +231,8 @@ __global__ void grstep_finish_kernel(int ns, const double* __restrict__ hsum, do
 
 extern "C" int hfb_step_ntiles(hfb_ctx* c, int prec) {
   if (!c) return HFB_E_NOCTX;
-  const int P = prec == HFB_F32 ? step_P<float>(c) : step_P<double>(c);
-  return tiles_per_field<2>(c->nx, c->ny, c->nz, P);
+  (void)prec;
+  return step_pencils(c, 2) / STEP_PPB;
 }
 
 extern "C" int hfb_grstep_weights(int nstates, int npsi0, int tbcs, const double* wocc, const double* wstates,
@@ -252,14 +272,8 @@ extern "C" int hfb_precond_step(hfb_ctx* c, int prec, void* psi, const void* ps1
   a.kfz = 2.0 * M_PI / (c->dz * c->nz);
   a.fpart = fpart;
   hipStream_t st = (hipStream_t)stream;
-  if (prec == HFB_F64) {
-    if (step_P<double>(c) == 16) return step_all<double, 16>(c, a, nstates, st);
-    return step_all<double, 8>(c, a, nstates, st);
-  }
-  if (prec == HFB_F32) {
-    if (step_P<float>(c) == 16) return step_all<float, 16>(c, a, nstates, st);
-    return step_all<float, 8>(c, a, nstates, st);
-  }
+  if (prec == HFB_F64) return step_all<double>(c, a, nstates, st);
+  if (prec == HFB_F32) return step_all<float>(c, a, nstates, st);
   return HFB_E_ARG;
 }
 

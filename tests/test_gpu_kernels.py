@@ -121,3 +121,53 @@ def test_poisson_matches_oracle(pkg, O, dev, n):
     ref = O.poisson(g, coul, rho)
     out = pkg.kernels.poisson(ctx, torch.tensor(rho, device=dev), O.E2)
     assert relerr(out.cpu().numpy(), ref) < 1e-12
+
+
+@pytest.mark.parametrize("nsts", [(8, 8), (20, 20), (36, 30), (64, 50)])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_gram2_matches_matmul(pkg, dev, nsts, dtype):
+    """scale * A^H B_m for both isospin blocks (the vector path, used up to
+    64 states), with B_0 = A (Hermitian, half computed) and two other
+    blocks, against numpy in FP64."""
+    g = pkg.grids.init_grids(pkg.params.init_params(), nx=24, ny=24, nz=24, dx=1.0, dy=1.0, dz=1.0)
+    ctx = g.ctx
+    rng = np.random.default_rng(sum(nsts))
+    blocks, outs, ref = [], [], []
+    for iq, nst in enumerate(nsts):
+        arrs = [rng.standard_normal((nst, 2, 24, 24, 24)) + 1j * rng.standard_normal((nst, 2, 24, 24, 24))
+                for _ in range(3)]
+        ts = [torch.tensor(a, dtype=dtype, device=dev) for a in arrs]
+        blocks.append((ts[0], [ts[0], ts[1], ts[2]]))
+        outs.append(torch.empty((3, nst, nst), dtype=torch.complex128, device=dev))
+        A2 = ts[0].cpu().numpy().astype(np.complex128).reshape(nst, -1)
+        ref.append([0.7 * A2.conj() @ t.cpu().numpy().astype(np.complex128).reshape(nst, -1).T for t in ts])
+    pkg.kernels.gram2(ctx, blocks, 0.7, outs)
+    torch.cuda.synchronize()
+    for iq in range(2):
+        for m in range(3):
+            r = ref[iq][m]
+            got = outs[iq][m].cpu().numpy()
+            assert np.abs(got - r).max() <= 1e-12 * np.abs(r).max(), (iq, m)
+
+
+@pytest.mark.parametrize("nst", [8, 20, 30, 36, 64])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_rotate_matches_matmul(pkg, dev, nst, dtype):
+    """out1 = psi U1 and out2 = psi U2 in place (vector path up to 32
+    states, MFMA path above) against numpy."""
+    rng = np.random.default_rng(nst)
+    K = 2 * 24 ** 3
+    psi = rng.standard_normal((nst, K)) + 1j * rng.standard_normal((nst, K))
+    U1 = rng.standard_normal((nst, nst)) + 1j * rng.standard_normal((nst, nst))
+    U2 = rng.standard_normal((nst, nst)) + 1j * rng.standard_normal((nst, nst))
+    tp = torch.tensor(psi, dtype=dtype, device=dev)
+    tu1 = torch.tensor(U1, dtype=torch.complex128, device=dev)
+    tu2 = torch.tensor(U2, dtype=torch.complex128, device=dev)
+    out2 = torch.empty_like(tp)
+    pkg.kernels.rotate(tp, tu1, tu2, tp, out2)
+    torch.cuda.synchronize()
+    p64 = torch.tensor(psi, dtype=dtype).numpy().astype(np.complex128)
+    for got, U in ((tp, U1), (out2, U2)):
+        r = (p64.T @ U).T
+        tol = 1e-12 if dtype == torch.complex128 else 1e-6
+        assert np.abs(got.cpu().numpy() - r).max() <= tol * np.abs(r).max()
