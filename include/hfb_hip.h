@@ -187,14 +187,17 @@ int hfb_gram2(int prec, const void* const* A, const void* const* B, int nm, cons
  * matrix lambda+ that maps the rotated states to lagrange.  Writes sp_norm,
  * sp_energy, deltaf (ipair), hmatrix, gapmatrix, symcond, lambda_save
  * (2, nmax, nmax) complex128 and efluct1q/2q (2), efluct1/2 (1).
- * nst0, nst1 <= 36 (HFB_E_SIZE above).  stats (optional, 4
+ * nst0, nst1 <= 128 (HFB_E_SIZE above).  work: null when both blocks
+ * have at most 36 states (the work matrices then live in LDS), else
+ * 10 * nmax^2 complex doubles.  stats (optional, 4
  * doubles): Jacobi sweeps of the lambda_lin eigensolve per isospin, then
  * Newton-Schulz iterations per isospin (-1: fell back to the eigensolve). */
 int hfb_diag_small(int nst0, int nst1, int nmax, const double* const* mats, int tdiag, int ipair, int tbcs,
                    int construct, const double* wocc, const double* wstates, const double* wguv,
                    const double* pairwg, double* sp_norm, double* sp_energy, double* deltaf, double* hmatrix,
                    double* gapmatrix, double* symcond, double* lambda_save, double* efluct1q, double* efluct2q,
-                   double* efluct1, double* efluct2, double* const* uo, double* stats, void* stream);
+                   double* efluct1, double* efluct2, double* const* uo, double* work, double* stats,
+                   void* stream);
 
 /* out1 = psi_2d U1 and out2 = psi_2d U2 over the nst states of one block
  * (static.py:316 psi_2d @ unitary; static.py:557 lagrange = psi_new @

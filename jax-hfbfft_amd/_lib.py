@@ -48,7 +48,7 @@ _SIGS = {
     "hfb_diag_small": (c_int, [c_int, c_int, c_int, POINTER(c_void_p), c_int, c_int, c_int, c_int, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_void_p),
-                               c_void_p, c_void_p]),
+                               c_void_p, c_void_p, c_void_p]),
     "hfb_density_sizes": (c_int, [c_void_p, c_int, POINTER(ctypes.c_longlong), POINTER(ctypes.c_longlong)]),
     "hfb_density": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),

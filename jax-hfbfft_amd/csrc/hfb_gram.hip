@@ -1,32 +1,25 @@
-// Subspace matrices and rotations of diagstep (static.py:261-577) on the
-// FP64 matrix cores (v_mfma_f64_16x16x4f64).
+// Subspace matrices and rotations of diagstep (static.py:261-577), FP64
+// arithmetic on the vector units.
 //
 // Gram: C_m = A^H B_m for up to four right-hand blocks sharing A = psi
 // (overlap psi^H psi, H_old = psi^H hmfpsi, lambda_lin = psi^H hampsi,
-// G_old = psi^H delpsi), K = 2*nx*ny*nz per state.  Each wavefront owns a
-// 16x16 output tile and a K-chunk; chunks are summed in a fixed order by a
-// second kernel, so results are deterministic.
+// G_old = psi^H delpsi), K = 2*nx*ny*nz per state.
 //
 // Rotation: out_j = sum_i psi_i U[i][j] for two matrices at once (psi U and
-// psi (U lambda+), i.e. the rotated states and the Lagrange field) computed
-// as D = U^T psi^T so that each output row is a contiguous run of points.
+// psi (U lambda+), i.e. the rotated states and the Lagrange field).
 //
-// MFMA f64 16x16x4 lane layout (checked by hfb_selftest_mfma_f64):
-//   A: lane l holds A[l%16][l/16];  B: lane l holds B[l/16][l%16];
-//   D: lane l, element r holds D[l/16 + 4r][l%16].
+// On gfx950 the FP64 matrix cores and the FP64 vector units have the same
+// peak rate, and 16x16 matrix tiles padded from 20 or 36 states do 1.6-2.9x
+// the work, so both run as register-tiled FMAs.
 #include <math.h>
 
 #include "hfb_ctx.h"
 
 using namespace hfb;
 
-typedef double v4d __attribute__((ext_vector_type(4)));
 
 namespace {
 
-__device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
 
 template <typename T>
 __device__ __forceinline__ cx<double> ld64(const cx<T>* p) {
@@ -38,265 +31,32 @@ struct GramIso {
   const void* A;
   const void* B[4];
   int nst;
-  int np;       // nst padded to 16
+  int np;       // nst padded to the tile edge
   double* part; // [m][np][np][chunk] complex
 };
 
-struct GramArgs {
-  GramIso iso[2];
-  int nm;       // number of right-hand blocks
-  int bsame;    // bitmask: B[m] == A (no second load)
-  int64_t K;
-  int64_t kc;   // K per chunk, multiple of KS
-  int nchunk;
-};
-
-constexpr int KS = 16;        // complex elements of K staged per step
-constexpr int LDR = KS + 1;   // padded LDS row stride (rows sit on different banks)
-constexpr int GRAM_NT = 256;  // 4 wavefronts
-
-// One workgroup per (K-chunk, isospin block) computes every 16x16 tile of
-// every C_m.  Each K-step stages A and the distinct B_m rows (KS contiguous
-// elements per row, converted to double) in LDS, so every input element is
-// read from HBM once; the loads of step t+1 are issued before the MFMAs of
-// step t.  The four wavefronts split the output tiles; MFMA operands come
-// from LDS.
-template <typename T, int NM, int MAXT, int NE>
-__global__ void __launch_bounds__(GRAM_NT) gram_kernel(GramArgs ga) {
-  extern __shared__ double glds[];
-  const GramIso& g = ga.iso[blockIdx.y];
-  const int np = g.np;
-  if (np == 0) return;
-  const int ntile1 = np / 16;
-  const int ntl = ntile1 * ntile1;
-  cx<double>* sA = (cx<double>*)glds;                 // [np][LDR]
-  cx<double>* sB = sA + (size_t)np * LDR;             // [NM][np][LDR]
-  const int chunk = blockIdx.x;
-  const int64_t k0 = (int64_t)chunk * ga.kc;
-  const int64_t k1 = min(ga.K, k0 + ga.kc);
-  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int grp = l >> 4, c = l & 15;
-  const int ntot = NM * ntl;
-  const int bsame = ga.bsame;
-  v4d cre[MAXT], cim[MAXT];
-#pragma unroll
-  for (int u = 0; u < MAXT; ++u) {
-    cre[u] = v4d{0.0, 0.0, 0.0, 0.0};
-    cim[u] = v4d{0.0, 0.0, 0.0, 0.0};
-  }
-  // register staging of one K-step: NE elements per thread and matrix
-  cx<double> ra[NE], rb[NM][NE];
-  auto fetch = [&](int64_t kb) {
-#pragma unroll
-    for (int q = 0; q < NE; ++q) {
-      const int e = threadIdx.x + q * GRAM_NT;
-      const int row = e / KS, kk = e % KS;
-      const int64_t k = kb + kk;
-      const bool ok = e < np * KS && row < g.nst && k < k1;
-      ra[q] = ok ? ld64((const cx<T>*)g.A + (int64_t)row * ga.K + k) : cx<double>{0.0, 0.0};
-#pragma unroll
-      for (int m = 0; m < NM; ++m)
-        if (!((bsame >> m) & 1))
-          rb[m][q] = ok ? ld64((const cx<T>*)g.B[m] + (int64_t)row * ga.K + k) : cx<double>{0.0, 0.0};
-    }
-  };
-  fetch(k0);
-  for (int64_t kb = k0; kb < k1; kb += KS) {
-#pragma unroll
-    for (int q = 0; q < NE; ++q) {
-      const int e = threadIdx.x + q * GRAM_NT;
-      if (e < np * KS) {
-        const int le = (e / KS) * LDR + e % KS;
-        sA[le] = ra[q];
-#pragma unroll
-        for (int m = 0; m < NM; ++m)
-          if (!((bsame >> m) & 1)) sB[(size_t)m * np * LDR + le] = rb[m][q];
-      }
-    }
-    __syncthreads();
-    if (kb + KS < k1) fetch(kb + KS);
-#pragma unroll
-    for (int u = 0; u < MAXT; ++u) {
-      const int t = wave + 4 * u;
-      if (t < ntot) {
-        const int m = t / ntl, tt = t % ntl;
-        const int ti = tt / ntile1, tj = tt % ntile1;
-        const cx<double>* bb = ((bsame >> m) & 1) ? sA : sB + (size_t)m * np * LDR;
-#pragma unroll
-        for (int s4 = 0; s4 < KS; s4 += 4) {
-          const cx<double> a = sA[(ti * 16 + c) * LDR + s4 + grp];
-          const cx<double> b = bb[(tj * 16 + c) * LDR + s4 + grp];
-          // conj(a) b = (ar br + ai bi) + i (ar bi - ai br)
-          cre[u] = mfma(a.x, b.x, cre[u]);
-          cre[u] = mfma(a.y, b.y, cre[u]);
-          cim[u] = mfma(a.x, b.y, cim[u]);
-          cim[u] = mfma(-a.y, b.x, cim[u]);
-        }
-      }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int u = 0; u < MAXT; ++u) {
-    const int t = wave + 4 * u;
-    if (t < ntot) {
-      const int m = t / ntl, tt = t % ntl;
-      const int ti = tt / ntile1, tj = tt % ntile1;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = ti * 16 + grp + 4 * r;
-        const int col = tj * 16 + c;
-        double* dst = g.part + 2 * ((((int64_t)m * np + row) * np + col) * ga.nchunk + chunk);
-        dst[0] = cre[u][r];
-        dst[1] = cim[u][r];
-      }
-    }
-  }
-}
-
-// One wavefront per output element: lanes stride over the chunks.
-__global__ void __launch_bounds__(256) gram_reduce_kernel(const double* __restrict__ part, int nchunk, int nm, int np,
-                                                          int nst, double scale, double* __restrict__ out) {
-  const int64_t total = (int64_t)nm * nst * nst;
-  const int lane = threadIdx.x & 63;
-  const int64_t wid = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t e = wid; e < total; e += nw) {
-    const int m = e / ((int64_t)nst * nst);
-    const int ij = e % ((int64_t)nst * nst);
-    const int i = ij / nst, j = ij % nst;
-    const double* p = part + 2 * (((int64_t)m * np + i) * np + j) * nchunk;
-    double re = 0.0, im = 0.0;
-    for (int ch = lane; ch < nchunk; ch += 64) {
-      re += p[2 * ch];
-      im += p[2 * ch + 1];
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      re += __shfl_down(re, o, 64);
-      im += __shfl_down(im, o, 64);
-    }
-    if (lane == 0) {
-      out[2 * e] = re * scale;
-      out[2 * e + 1] = im * scale;
-    }
-  }
-}
-
-int gram_nchunk(int np, int64_t K) {
-  (void)np;
-  int64_t nchunk = (K + 255) / 256;  // ~256 elements of K per workgroup
-  if (nchunk > 256) nchunk = 256;
-  return nchunk < 1 ? 1 : (int)nchunk;
-}
 
 // ---------------------------------------------------------------------------
-// Rotation: out1_j = sum_i psi_i U1[i][j], out2_j = sum_i psi_i U2[i][j]
-// ---------------------------------------------------------------------------
-template <typename T, int NJT>
-__global__ void __launch_bounds__(256) rotate_kernel(const cx<T>* psi, const double* __restrict__ U1,
-                                                     const double* __restrict__ U2, cx<T>* out1, cx<T>* out2, int nst,
-                                                     int64_t K) {
-  // U (nst x nst complex, row-major [i][j]) staged in LDS as padded (np x np).
-  // Each wavefront owns whole 16-point blocks of K: it reads psi_i for all i
-  // there before writing out_j, so out1 may alias psi when one workgroup
-  // column covers all j (gridDim.y == 1).
-  extern __shared__ double lds[];
-  const int np = (nst + 15) & ~15;
-  double* u1 = lds;                       // 2*np*np
-  double* u2 = lds + 2 * (int64_t)np * np;
-  for (int e = threadIdx.x; e < np * np; e += blockDim.x) {
-    const int i = e / np, j = e % np;
-    const bool ok = i < nst && j < nst;
-    u1[2 * e] = ok ? U1[2 * (i * nst + j)] : 0.0;
-    u1[2 * e + 1] = ok ? U1[2 * (i * nst + j) + 1] : 0.0;
-    if (U2) {
-      u2[2 * e] = ok ? U2[2 * (i * nst + j)] : 0.0;
-      u2[2 * e + 1] = ok ? U2[2 * (i * nst + j) + 1] : 0.0;
-    }
-  }
-  __syncthreads();
-  const int wave = threadIdx.x >> 6;
-  const int l = threadIdx.x & 63;
-  const int grp = l >> 4, c = l & 15;
-  const int jg0 = blockIdx.y * NJT;  // first j-tile of this block
-  const int nis = np / 4;            // MFMA steps over i
-  const int64_t nblk = (K + 15) / 16;
-  const bool two = U2 != nullptr;
-  for (int64_t blk = (int64_t)blockIdx.x * 4 + wave; blk < nblk; blk += (int64_t)gridDim.x * 4) {
-    const int64_t k = blk * 16 + c;
-    v4d d1r[NJT], d1i[NJT], d2r[NJT], d2i[NJT];
-#pragma unroll
-    for (int t = 0; t < NJT; ++t) {
-      d1r[t] = v4d{0.0, 0.0, 0.0, 0.0};
-      d1i[t] = d1r[t];
-      d2r[t] = d1r[t];
-      d2i[t] = d1r[t];
-    }
-    for (int s0 = 0; s0 < nis; s0 += 4) {
-      // B operand psi^T[i][k] for four i-steps, loaded together
-      cx<double> bv[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = 4 * (s0 + u) + grp;
-        bv[u] = (s0 + u < nis && i < nst && k < K) ? ld64(psi + (int64_t)i * K + k) : cx<double>{0.0, 0.0};
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = 4 * (s0 + u) + grp;
-        const cx<double> b = bv[u];
-#pragma unroll
-        for (int t = 0; t < NJT; ++t) {
-          const int j = (jg0 + t) * 16 + c;  // A operand: U^T[j][i] = U[i][j]
-          const double* pa = u1 + 2 * ((int64_t)i * np + j);
-          const double ar = pa[0], ai = pa[1];
-          d1r[t] = mfma(ar, b.x, d1r[t]);
-          d1r[t] = mfma(-ai, b.y, d1r[t]);
-          d1i[t] = mfma(ar, b.y, d1i[t]);
-          d1i[t] = mfma(ai, b.x, d1i[t]);
-          if (two) {
-            const double* pb = u2 + 2 * ((int64_t)i * np + j);
-            const double br = pb[0], bi = pb[1];
-            d2r[t] = mfma(br, b.x, d2r[t]);
-            d2r[t] = mfma(-bi, b.y, d2r[t]);
-            d2i[t] = mfma(br, b.y, d2i[t]);
-            d2i[t] = mfma(bi, b.x, d2i[t]);
-          }
-        }
-      }
-    }
-    if (k < K) {
-#pragma unroll
-      for (int t = 0; t < NJT; ++t) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int j = (jg0 + t) * 16 + grp + 4 * r;
-          if (j < nst) {
-            out1[(int64_t)j * K + k] = cx<T>{T(d1r[t][r]), T(d1i[t][r])};
-            if (two) out2[(int64_t)j * K + k] = cx<T>{T(d2r[t][r]), T(d2i[t][r])};
-          }
-        }
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Small blocks (nst <= 64) on the vector FP64 units.  On gfx950 the FP64
-// matrix and vector rates are equal, and 16x16 MFMA tiles padded from 20 or
-// 36 states waste 1.6-2.9x of the work, so blocks up to 64 states use FMAs:
-// a wavefront owns a 4x4 tile of one C_m (complex, in registers) and a chunk
-// of K, lanes stride over k (coalesced rows); the lane sums are combined by
-// shuffles and the chunks by gram_reduce_kernel in a fixed order.  For
-// B_m == A only tiles with ti <= tj are computed; the reduction mirrors the
-// rest (C^H = C).
+// Gram on the vector FP64 units: a wavefront owns a 4x4 tile of one C_m
+// (complex, in registers) and a chunk of K; lanes stride over k (coalesced
+// rows); the lane
+// sums are combined by shuffles and the chunks by gram_vec_reduce_kernel in
+// a fixed order.  For B_m == A only tiles with ti <= tj are computed; the
+// reduction mirrors the rest (C^H = C).
 // ---------------------------------------------------------------------------
 constexpr int GV_T = 4;      // tile edge
 constexpr int GV_KPL = 32;   // k per lane per chunk
 constexpr int GV_CHUNK = 64 * GV_KPL;
 
-int gram_vec_nchunk(int64_t K) {
+// K-chunks: at least GV_KPL elements of K per lane, and about 16K
+// wavefront items over the tiles of one matrix (fewer chunks for big blocks)
+int gram_vec_nchunk(int nst, int64_t K) {
+  const int nt = (nst + GV_T - 1) / GV_T;
   int64_t nc = (K + GV_CHUNK - 1) / GV_CHUNK;
-  return nc < 1 ? 1 : (nc > 1024 ? 1024 : (int)nc);
+  int64_t cap = 16384 / ((int64_t)nt * nt);
+  if (cap < 4) cap = 4;
+  if (nc > cap) nc = cap;
+  return nc < 1 ? 1 : (int)nc;
 }
 
 struct GramVecArgs {
@@ -362,12 +122,13 @@ __global__ void __launch_bounds__(256) gram_vec_kernel(GramVecArgs ga) {
   for (int a = 0; a < GV_T; ++a)
 #pragma unroll
     for (int b = 0; b < GV_T; ++b) cr[a][b] = ci[a][b] = 0.0;
+  const bool sym = B == A && ti == tj;
   for (int64_t k = k0 + lane; k < k1; k += 64) {
     cx<double> av[GV_T], bv[GV_T];
 #pragma unroll
     for (int u = 0; u < GV_T; ++u) {
       av[u] = ld64(A + (int64_t)ri[u] * K + k);
-      bv[u] = ld64(B + (int64_t)rj[u] * K + k);
+      bv[u] = sym ? av[u] : ld64(B + (int64_t)rj[u] * K + k);
     }
 #pragma unroll
     for (int a = 0; a < GV_T; ++a)
@@ -436,79 +197,82 @@ __global__ void __launch_bounds__(256) gram_vec_reduce_kernel(const double* __re
   }
 }
 
-// Rotation of a small block (nst <= 32) on the vector units: each lane owns
-// one k, reads psi_i[k] for every i into registers, then writes
-// out1_j[k] = sum_i psi_i[k] U1[i][j] (and out2 with U2).  A whole column is
-// read before anything is written, so out1 may alias psi.  U is uniform
-// across the wavefront and is read through the scalar cache.
-template <typename T, int NI>
-__global__ void __launch_bounds__(256) rotate_vec_kernel(const cx<T>* psi, const double* __restrict__ U1,
-                                                         const double* __restrict__ U2, cx<T>* out1, cx<T>* out2,
-                                                         int nst, int64_t K) {
-  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k >= K) return;
-  cx<double> v[NI];
+// Rotation out_j[k] = sum_i psi_i[k] U[i][j] (two matrices U1, U2 at
+// once).  A workgroup stages psi_i[k] for every i and its 64*KPL
+// consecutive k in LDS; each wavefront then produces outputs j in groups of
+// RJ for those k, KPL values of k per lane.  The group index is made
+// wavefront-uniform, so U1[i][j], U2[i][j] are read through the scalar
+// cache into scalar registers (no LDS bandwidth spent on broadcasts).  A
+// workgroup reads all of its columns of psi before its first store, so out1
+// may alias psi.
+template <typename T, int RJ, int KPL>
+__global__ void __launch_bounds__(256) rotate_kernel(const cx<T>* psi, const double* __restrict__ U1,
+                                                     const double* __restrict__ U2, cx<T>* out1, cx<T>* out2,
+                                                     int nst, int64_t K) {
+  constexpr int KT = 64 * KPL;  // k per workgroup
+  extern __shared__ unsigned char rot_lds[];
+  cx<T>* sp = (cx<T>*)rot_lds;  // [nst][KT]
+  const bool two = U2 != nullptr;
+  const int64_t k0 = (int64_t)blockIdx.x * KT;
+  for (int e = threadIdx.x; e < nst * KT; e += blockDim.x) {
+    const int i = e / KT, kk = e % KT;
+    const int64_t k = k0 + kk;
+    sp[e] = k < K ? psi[(int64_t)i * K + k] : cx<T>{T(0), T(0)};
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int j0 = wave * RJ; j0 < nst; j0 += 4 * RJ) {
+    double r1[KPL][RJ], i1[KPL][RJ], r2[KPL][RJ], i2[KPL][RJ];
 #pragma unroll
-  for (int i = 0; i < NI; ++i) v[i] = i < nst ? ld64(psi + (int64_t)i * K + k) : cx<double>{0.0, 0.0};
-  const cx<double>* u1 = (const cx<double>*)U1;
-  const cx<double>* u2 = (const cx<double>*)U2;
-  for (int j = 0; j < nst; ++j) {
-    double r1 = 0.0, i1 = 0.0, r2 = 0.0, i2 = 0.0;
+    for (int c = 0; c < KPL; ++c)
 #pragma unroll
-    for (int i = 0; i < NI; ++i) {
-      if (i < nst) {
-        const cx<double> a = u1[i * nst + j];
-        r1 = fma(v[i].x, a.x, fma(-v[i].y, a.y, r1));
-        i1 = fma(v[i].x, a.y, fma(v[i].y, a.x, i1));
-        if (u2) {
-          const cx<double> b = u2[i * nst + j];
-          r2 = fma(v[i].x, b.x, fma(-v[i].y, b.y, r2));
-          i2 = fma(v[i].x, b.y, fma(v[i].y, b.x, i2));
+      for (int t = 0; t < RJ; ++t) r1[c][t] = i1[c][t] = r2[c][t] = i2[c][t] = 0.0;
+    for (int i = 0; i < nst; ++i) {
+      double vx[KPL], vy[KPL];
+#pragma unroll
+      for (int c = 0; c < KPL; ++c) {
+        const cx<T> pv = sp[i * KT + lane + 64 * c];
+        vx[c] = double(pv.x);
+        vy[c] = double(pv.y);
+      }
+      const double* u1 = U1 + 2 * ((int64_t)i * nst);
+      const double* u2 = two ? U2 + 2 * ((int64_t)i * nst) : u1;
+#pragma unroll
+      for (int t = 0; t < RJ; ++t) {
+        const int j = min(j0 + t, nst - 1);
+        const double ax = u1[2 * j], ay = u1[2 * j + 1];
+        const double bx = u2[2 * j], by = u2[2 * j + 1];
+#pragma unroll
+        for (int c = 0; c < KPL; ++c) {
+          r1[c][t] = fma(vx[c], ax, fma(-vy[c], ay, r1[c][t]));
+          i1[c][t] = fma(vx[c], ay, fma(vy[c], ax, i1[c][t]));
+          r2[c][t] = fma(vx[c], bx, fma(-vy[c], by, r2[c][t]));
+          i2[c][t] = fma(vx[c], by, fma(vy[c], bx, i2[c][t]));
         }
       }
     }
-    out1[(int64_t)j * K + k] = cx<T>{T(r1), T(i1)};
-    if (u2) out2[(int64_t)j * K + k] = cx<T>{T(r2), T(i2)};
+#pragma unroll
+    for (int c = 0; c < KPL; ++c) {
+      const int64_t k = k0 + lane + 64 * c;
+      if (k >= K) continue;
+#pragma unroll
+      for (int t = 0; t < RJ; ++t) {
+        const int j = j0 + t;
+        if (j < nst) {
+          out1[(int64_t)j * K + k] = cx<T>{T(r1[c][t]), T(i1[c][t])};
+          if (two) out2[(int64_t)j * K + k] = cx<T>{T(r2[c][t]), T(i2[c][t])};
+        }
+      }
+    }
   }
 }
 
 }  // namespace
 
 extern "C" int hfb_gram_workspace(int nst, long long K) {
-  const int np = (nst + 15) & ~15;
-  const int mfma = gram_nchunk(np, K) * 4 * np * np * 2;  // doubles for up to four matrices
   const int np4 = (nst + GV_T - 1) / GV_T * GV_T;
-  const int vec = gram_vec_nchunk(K) * 4 * np4 * np4 * 2;
-  return mfma > vec ? mfma : vec;
-}
-
-template <typename T, int NM>
-int launch_gram(const GramArgs& g, int npmax, hipStream_t st) {
-  const int ntile = (npmax / 16) * (npmax / 16) * NM;
-  const size_t lds = sizeof(cx<double>) * (size_t)npmax * LDR * (1 + NM);
-  const dim3 grid(g.nchunk, 2);
-  const int ne = (npmax * KS + GRAM_NT - 1) / GRAM_NT;
-#define HFB_GR(MT, NEV)                                                          \
-  {                                                                              \
-    auto kern = gram_kernel<T, NM, MT, NEV>;                                     \
-    if (lds > 32 * 1024) allow_full_lds((const void*)kern);                      \
-    hipLaunchKernelGGL(kern, grid, dim3(GRAM_NT), lds, st, g);                  \
-  }
-  const int per_wave = (ntile + 3) / 4;
-  if (ne > 8) return HFB_E_SIZE;
-#define HFB_GRN(MT)                    \
-  if (ne <= 2) HFB_GR(MT, 2)           \
-  else if (ne <= 4) HFB_GR(MT, 4)      \
-  else HFB_GR(MT, 8)
-  if (per_wave <= 1) { HFB_GRN(1) }
-  else if (per_wave <= 2) { HFB_GRN(2) }
-  else if (per_wave <= 4) { HFB_GRN(4) }
-  else if (per_wave <= 8) { HFB_GRN(8) }
-  else if (per_wave <= 16) { HFB_GRN(16) }
-  else return HFB_E_SIZE;
-#undef HFB_GRN
-#undef HFB_GR
-  return launch_status();
+  return gram_vec_nchunk(nst, K) * 4 * np4 * np4 * 2;  // doubles for up to four matrices
 }
 
 // Both isospin blocks in one launch: A[iq], B[iq*4 + m], out[iq] (nm, nst_iq,
@@ -517,19 +281,19 @@ int launch_gram(const GramArgs& g, int npmax, hipStream_t st) {
 extern "C" int hfb_gram2(int prec, const void* const* A, const void* const* B, int nm, const int* nst, long long K,
                          double scale, double* const* part, double* const* out, void* stream) {
   if (!A || !B || !part || !out || !nst || nm < 1 || nm > 4 || K <= 0) return HFB_E_ARG;
-  GramArgs g{};
-  g.bsame = 0;
+  GramVecArgs v{};
+  v.bsame = 0;
   for (int m = 0; m < nm; ++m) {
     bool same = true;
     for (int iq = 0; iq < 2; ++iq)
       if (nst[iq] > 0 && B[iq * 4 + m] != A[iq]) same = false;
-    if (same) g.bsame |= 1 << m;
+    if (same) v.bsame |= 1 << m;
   }
   int npmax = 0;
   for (int iq = 0; iq < 2; ++iq) {
-    GramIso& gi = g.iso[iq];
+    GramIso& gi = v.iso[iq];
     gi.nst = nst[iq];
-    gi.np = nst[iq] > 0 ? (nst[iq] + 15) & ~15 : 0;
+    gi.np = nst[iq] > 0 ? (nst[iq] + GV_T - 1) / GV_T * GV_T : 0;
     if (nst[iq] > 0) {
       if (!A[iq] || !part[iq] || !out[iq]) return HFB_E_ARG;
       gi.A = A[iq];
@@ -542,14 +306,10 @@ extern "C" int hfb_gram2(int prec, const void* const* A, const void* const* B, i
     npmax = gi.np > npmax ? gi.np : npmax;
   }
   if (npmax == 0) return HFB_OK;
-  if (npmax <= 64) {  // small blocks: vector FP64 path
-    GramVecArgs v{};
-    v.iso[0] = g.iso[0];
-    v.iso[1] = g.iso[1];
+  {  // FP64 vector path (gram_vec_kernel)
     v.nm = nm;
-    v.bsame = g.bsame;
     v.K = K;
-    v.nchunk = gram_vec_nchunk(K);
+    v.nchunk = gram_vec_nchunk(npmax, K);
     v.kc = (K + v.nchunk - 1) / v.nchunk;
     int items = 0;
     for (int iq = 0; iq < 2; ++iq) {
@@ -579,38 +339,6 @@ extern "C" int hfb_gram2(int prec, const void* const* A, const void* const* B, i
     }
     return rc;
   }
-  g.nm = nm;
-  g.K = K;
-  g.nchunk = gram_nchunk(npmax, K);
-  int64_t kc = (K + g.nchunk - 1) / g.nchunk;
-  kc = (kc + KS - 1) / KS * KS;
-  g.kc = kc;
-  hipStream_t st = (hipStream_t)stream;
-  int rc;
-#define HFB_GT(TT)                                          \
-  if (nm == 1) rc = launch_gram<TT, 1>(g, npmax, st);       \
-  else if (nm == 2) rc = launch_gram<TT, 2>(g, npmax, st);  \
-  else if (nm == 3) rc = launch_gram<TT, 3>(g, npmax, st);  \
-  else rc = launch_gram<TT, 4>(g, npmax, st);
-  if (prec == HFB_F64) {
-    HFB_GT(double)
-  } else if (prec == HFB_F32) {
-    HFB_GT(float)
-  } else {
-    return HFB_E_ARG;
-  }
-#undef HFB_GT
-  if (rc != HFB_OK) return rc;
-  for (int iq = 0; iq < 2 && rc == HFB_OK; ++iq) {
-    if (nst[iq] <= 0) continue;
-    const int64_t tot = (int64_t)nm * nst[iq] * nst[iq];
-    int nb = (int)((tot * 64 + 255) / 256);
-    if (nb > 4096) nb = 4096;
-    hipLaunchKernelGGL(gram_reduce_kernel, dim3(nb < 1 ? 1 : nb), dim3(256), 0, st, part[iq], g.nchunk, nm,
-                       g.iso[iq].np, nst[iq], scale, out[iq]);
-    rc = launch_status();
-  }
-  return rc;
 }
 
 // One isospin block (kept for callers of a single block).
@@ -630,60 +358,41 @@ extern "C" int hfb_rotate(int prec, const void* psi, const double* U1, const dou
                           int nst, long long K, void* stream) {
   if (!psi || !U1 || !out1 || nst <= 0 || K <= 0) return HFB_E_ARG;
   if (U2 && !out2) return HFB_E_ARG;
-  if (nst <= 32) {  // small blocks: register columns on the vector units
-    const int nb = (int)((K + 255) / 256);
-    hipStream_t st0 = (hipStream_t)stream;
-#define HFB_RV(TT, NIV)                                                                                   \
-  hipLaunchKernelGGL((rotate_vec_kernel<TT, NIV>), dim3(nb), dim3(256), 0, st0, (const cx<TT>*)psi, U1, U2, \
-                     (cx<TT>*)out1, (cx<TT>*)out2, nst, (int64_t)K)
-    if (prec == HFB_F64) {
-      if (nst <= 8) HFB_RV(double, 8);
-      else if (nst <= 16) HFB_RV(double, 16);
-      else if (nst <= 24) HFB_RV(double, 24);
-      else HFB_RV(double, 32);
-    } else if (prec == HFB_F32) {
-      if (nst <= 8) HFB_RV(float, 8);
-      else if (nst <= 16) HFB_RV(float, 16);
-      else if (nst <= 24) HFB_RV(float, 24);
-      else HFB_RV(float, 32);
-    } else {
-      return HFB_E_ARG;
-    }
-#undef HFB_RV
-    return launch_status();
-  }
-  const int np = (nst + 15) & ~15;
-  const int ntj = np / 16;
-  const int njt = ntj < 4 ? ntj : 4;  // j-tiles per workgroup column
-  const int jgroups = (ntj + njt - 1) / njt;
-  if (jgroups > 1 && out1 == psi) return HFB_E_ARG;  // in place only when one column of blocks covers all j
-  if (ntj % njt != 0) return HFB_E_SIZE;
-  const size_t lds = sizeof(double) * 2 * 2 * (size_t)np * np;
-  if (lds > 152 * 1024) return HFB_E_SIZE;
-  const int64_t nblk = (K + 15) / 16;
-  int gx = (int)((nblk + 4 * 2 - 1) / (4 * 2));  // two 16-point blocks per wavefront
-  if (gx < 1) gx = 1;
+  if (prec != HFB_F64 && prec != HFB_F32) return HFB_E_ARG;
+  const size_t csz = prec == HFB_F64 ? sizeof(cx<double>) : sizeof(cx<float>);
+  // two k per lane while the staged tile fits in LDS, else one
+  const int kpl = csz * (size_t)nst * 128 <= 150 * 1024 ? 2 : 1;
+  const size_t lds = csz * (size_t)nst * 64 * kpl;
+  if (lds > 150 * 1024) return HFB_E_SIZE;
+  const int nb = (int)((K + 64 * kpl - 1) / (64 * kpl));
   hipStream_t st = (hipStream_t)stream;
-#define HFB_RT(TT, NJ)                                                                                          \
-  {                                                                                                             \
-    auto kern = rotate_kernel<TT, NJ>;                                                                          \
-    if (lds > 32 * 1024) allow_full_lds((const void*)kern);                                                     \
-    hipLaunchKernelGGL(kern, dim3(gx, jgroups), dim3(256), lds, st, (const cx<TT>*)psi, U1, U2, (cx<TT>*)out1,    \
-                       (cx<TT>*)out2, nst, (int64_t)K);                                                         \
+  // outputs per wavefront pass: the four wavefronts cover nst in one pass
+  // up to 32 states, and loop over groups of 8 beyond
+  const int rj = nst <= 12 ? 3 : nst <= 20 ? 5 : 8;
+#define HFB_ROT(TT, RJV, KP)                                                                                   \
+  {                                                                                                            \
+    auto kern = rotate_kernel<TT, RJV, KP>;                                                                    \
+    if (lds > 32 * 1024) allow_full_lds((const void*)kern);                                                    \
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), lds, st, (const cx<TT>*)psi, U1, U2, (cx<TT>*)out1,           \
+                       (cx<TT>*)out2, nst, (int64_t)K);                                                        \
   }
-#define HFB_RTJ(TT)                 \
-  if (njt == 1) HFB_RT(TT, 1)       \
-  else if (njt == 2) HFB_RT(TT, 2)  \
-  else if (njt == 3) HFB_RT(TT, 3)  \
-  else HFB_RT(TT, 4)
+#define HFB_ROTK(TT, RJV) \
+  if (kpl == 2) HFB_ROT(TT, RJV, 2) else HFB_ROT(TT, RJV, 1)
+#define HFB_ROTP(TT)                  \
+  if (rj == 3) {                      \
+    HFB_ROTK(TT, 3)                   \
+  } else if (rj == 5) {               \
+    HFB_ROTK(TT, 5)                   \
+  } else {                            \
+    HFB_ROTK(TT, 8)                   \
+  }
   if (prec == HFB_F64) {
-    HFB_RTJ(double)
-  } else if (prec == HFB_F32) {
-    HFB_RTJ(float)
+    HFB_ROTP(double)
   } else {
-    return HFB_E_ARG;
+    HFB_ROTP(float)
   }
-#undef HFB_RTJ
-#undef HFB_RT
+#undef HFB_ROTP
+#undef HFB_ROTK
+#undef HFB_ROT
   return launch_status();
 }

@@ -17,7 +17,7 @@ using namespace hfb;
 
 namespace {
 
-constexpr int NMAXJ = 64;
+constexpr int NMAXJ = 128;
 
 struct DiagIso {
   int nst, start;
@@ -27,6 +27,7 @@ struct DiagIso {
   const cx<double>* Gold;  // null unless pairing
   cx<double>* U;
   cx<double>* M2;
+  cx<double>* W;  // 5 n x n work matrices in global memory (blocks above NDIAG_LDS states)
   cx<double>* Hm;  // static arrays of this isospin, leading dimension nmax
   cx<double>* Gm;
   cx<double>* Sym;
@@ -49,23 +50,23 @@ struct DiagArgs {
   int tdiag, ipair, tbcs, construct;
 };
 
-constexpr int DIAG_NT = 256;  // four wavefronts per isospin block
 
-// Work split of an n x n matrix (n <= NP) over the DIAG_NT threads: thread t
+// Work split of an n x n matrix (n <= NP) over the NT threads: thread t
 // owns column j = t % NP and rows t / NP + RG*s.  Matrices are stored densely
 // with leading dimension n.
-#define HFB_TILE_LOOP(NP, n, i, j)                                   \
-  for (int j = threadIdx.x % (NP), i = threadIdx.x / (NP); j < (n) && i < (n); i += DIAG_NT / (NP))
+#define HFB_TILE_LOOP(NP, NT, n, i, j)                               \
+  for (int j = threadIdx.x % (NP), i = threadIdx.x / (NP); j < (n) && i < (n); i += (NT) / (NP))
 
-template <int NP, typename F>
+template <int NP, int NT, typename F>
 __device__ __forceinline__ void tile_for(int n, F&& f) {
-  constexpr int RG = DIAG_NT / NP;
+  constexpr int RG = NT / NP;
   const int j = threadIdx.x % NP;
   if (j >= n) return;
   for (int i = threadIdx.x / NP; i < n; i += RG) f(i, j);
 }
 
 // Sum of two values over the block, returned to every thread.
+template <int NT>
 __device__ __forceinline__ void block_sum2_all(double& a, double& b, double* red) {
   for (int o = 32; o > 0; o >>= 1) {
     a += __shfl_xor(a, o, 64);
@@ -78,27 +79,28 @@ __device__ __forceinline__ void block_sum2_all(double& a, double& b, double* red
   }
   __syncthreads();
   a = b = 0.0;
-  for (int w = 0; w < DIAG_NT / 64; ++w) {
+  for (int w = 0; w < NT / 64; ++w) {
     a += red[2 * w];
     b += red[2 * w + 1];
   }
   __syncthreads();
 }
 
+template <int NT>
 __device__ __forceinline__ double block_max_all(double v, double* red) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   v = red[0];
-  for (int w = 1; w < DIAG_NT / 64; ++w) v = fmax(v, red[w]);
+  for (int w = 1; w < NT / 64; ++w) v = fmax(v, red[w]);
   __syncthreads();
   return v;
 }
 
 // A (n x n, LDS) <- Hermitian matrix from the lower triangle of M (global).
-template <int NP>
+template <int NP, int NT>
 __device__ void load_lower_herm(cx<double>* A, const cx<double>* M, int n) {
-  tile_for<NP>(n, [&](int i, int j) {
+  tile_for<NP, NT>(n, [&](int i, int j) {
     cx<double> v;
     if (i > j)
       v = M[i * n + j];
@@ -112,38 +114,42 @@ __device__ void load_lower_herm(cx<double>* A, const cx<double>* M, int n) {
 }
 
 // C = op(A) B for n x n matrices (C distinct from A and B); op(A) = A or A^H.
-// Each thread accumulates a column strip of NP*NP/DIAG_NT entries in
+// Each thread accumulates a column strip of NP*NP/NT entries in
 // registers; the loads of A are broadcasts within a wavefront.
-template <int NP, bool CONJ_A>
+template <int NP, int NT, bool CONJ_A>
 __device__ __forceinline__ void mm(cx<double>* __restrict__ C, const cx<double>* A, const cx<double>* B, int n) {
-  constexpr int RG = DIAG_NT / NP;
+  constexpr int RG = NT / NP;
   constexpr int TI = NP / RG;
+  constexpr int TC = TI < 8 ? TI : 8;  // rows accumulated per pass
   const int j = threadIdx.x % NP;
   const int r0 = threadIdx.x / NP;
   const int jc = j < n ? j : n - 1;  // idle lanes read in bounds and store nothing
-  int ic[TI];
-  cx<double> acc[TI];
+#pragma unroll 1
+  for (int t0 = 0; t0 < TI; t0 += TC) {
+    int ic[TC];
+    cx<double> acc[TC];
 #pragma unroll
-  for (int t = 0; t < TI; ++t) {
-    const int i = r0 + RG * t;
-    ic[t] = i < n ? i : n - 1;
-    acc[t] = {0.0, 0.0};
-  }
-#pragma unroll 2
-  for (int k = 0; k < n; ++k) {
-    const cx<double> b = B[k * n + jc];
-#pragma unroll
-    for (int t = 0; t < TI; ++t) {
-      const cx<double> a = CONJ_A ? cconj(A[k * n + ic[t]]) : A[ic[t] * n + k];
-      acc[t].x = fma(a.x, b.x, fma(-a.y, b.y, acc[t].x));
-      acc[t].y = fma(a.x, b.y, fma(a.y, b.x, acc[t].y));
+    for (int t = 0; t < TC; ++t) {
+      const int i = r0 + RG * (t0 + t);
+      ic[t] = i < n ? i : n - 1;
+      acc[t] = {0.0, 0.0};
     }
-  }
-  if (j < n) {
+#pragma unroll 2
+    for (int k = 0; k < n; ++k) {
+      const cx<double> b = B[k * n + jc];
 #pragma unroll
-    for (int t = 0; t < TI; ++t) {
-      const int i = r0 + RG * t;
-      if (i < n) C[i * n + j] = acc[t];
+      for (int t = 0; t < TC; ++t) {
+        const cx<double> a = CONJ_A ? cconj(A[k * n + ic[t]]) : A[ic[t] * n + k];
+        acc[t].x = fma(a.x, b.x, fma(-a.y, b.y, acc[t].x));
+        acc[t].y = fma(a.x, b.y, fma(a.y, b.x, acc[t].y));
+      }
+    }
+    if (j < n) {
+#pragma unroll
+      for (int t = 0; t < TC; ++t) {
+        const int i = r0 + RG * (t0 + t);
+        if (i < n) C[i * n + j] = acc[t];
+      }
     }
   }
   __syncthreads();
@@ -153,7 +159,7 @@ struct JacobiShared {
   int p[NMAXJ / 2], q[NMAXJ / 2];
   double c[NMAXJ / 2], s[NMAXJ / 2];
   cx<double> e[NMAXJ / 2];
-  double red[16];
+  double red[32];
   int perm[NMAXJ];
   double ev[NMAXJ];
 };
@@ -163,23 +169,23 @@ struct JacobiShared {
 // `sort` (T is scratch for the column permutation).  Parallel cyclic Jacobi:
 // in each round the n/2 disjoint pairs of a round-robin ordering are rotated
 // together (columns of A and V, then rows of A).
-template <int NP>
+template <int NP, int NT>
 __device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* V, int n, JacobiShared& js, bool sort, cx<double>* T) {
   const int tid = threadIdx.x;
-  tile_for<NP>(n, [&](int i, int j) { V[i * n + j] = cx<double>{i == j ? 1.0 : 0.0, 0.0}; });
+  tile_for<NP, NT>(n, [&](int i, int j) { V[i * n + j] = cx<double>{i == j ? 1.0 : 0.0, 0.0}; });
   const int m = n + (n & 1);
   const int npair = m / 2;
   __syncthreads();
   int sweep = 0;
   for (; sweep < 40; ++sweep) {
     double off = 0.0, dia = 0.0;
-    HFB_TILE_LOOP(NP, n, i, j) {
+    HFB_TILE_LOOP(NP, NT, n, i, j) {
       if (i == j)
         dia += cabs2(A[i * n + j]);
       else
         off += cabs2(A[i * n + j]);
     }
-    block_sum2_all(off, dia, js.red);
+    block_sum2_all<NT>(off, dia, js.red);
     if (off <= 1e-30 * dia || off == 0.0) break;
     for (int t = 0; t < m - 1; ++t) {
       if (tid < npair) {
@@ -212,7 +218,7 @@ __device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* V, int n, 
       }
       __syncthreads();
       // columns of A and V: X <- X R for each rotated pair
-      for (int it = tid; it < NP * NP; it += DIAG_NT) {
+      for (int it = tid; it < NP * NP; it += NT) {
         const int r = it % NP;
         const int rest = it / NP;
         const int k = rest % (NP / 2);
@@ -229,7 +235,7 @@ __device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* V, int n, 
       __syncthreads();
       // rows of A: A <- R^H A; the rotated pair's off-diagonal entries are
       // set to zero and its diagonal to real
-      for (int it = tid; it < (NP / 2) * NP; it += DIAG_NT) {
+      for (int it = tid; it < (NP / 2) * NP; it += NT) {
         const int k = it / NP, r = it % NP;
         if (r >= n || k >= npair) continue;
         const int p = js.p[k], q = js.q[k];
@@ -251,7 +257,7 @@ __device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* V, int n, 
       __syncthreads();
     }
   }
-  for (int i = tid; i < n; i += DIAG_NT) js.ev[i] = A[i * n + i].x;
+  for (int i = tid; i < n; i += NT) js.ev[i] = A[i * n + i].x;
   __syncthreads();
   if (sort) {
     if (tid == 0) {
@@ -266,9 +272,9 @@ __device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* V, int n, 
       }
     }
     __syncthreads();
-    tile_for<NP>(n, [&](int i, int j) { T[i * n + j] = V[i * n + js.perm[j]]; });
+    tile_for<NP, NT>(n, [&](int i, int j) { T[i * n + j] = V[i * n + js.perm[j]]; });
     __syncthreads();
-    tile_for<NP>(n, [&](int i, int j) { V[i * n + j] = T[i * n + j]; });
+    tile_for<NP, NT>(n, [&](int i, int j) { V[i * n + j] = T[i * n + j]; });
     double evs = 0.0;
     if (tid < n) evs = js.ev[js.perm[tid]];
     __syncthreads();
@@ -278,33 +284,34 @@ __device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* V, int n, 
   return sweep;
 }
 
-constexpr int NDIAG_MAX = 36;  // largest block hfb_diag_small accepts (five n x n matrices in LDS)
+constexpr int NDIAG_LDS = 36;   // blocks up to this size keep the five work matrices in LDS
+constexpr int NDIAG_MAX = 128;  // largest block hfb_diag_small accepts
 
 // Inverse square root of the Hermitian positive matrix in Y (scaled copy of
 // S) by the coupled Newton-Schulz iteration Y <- Y T, Z <- T Z,
 // T = (3I - Z Y)/2, which converges to Z = Y0^{-1/2}.  Returns false when it
 // has not converged (nearly singular S); the caller then takes the
 // eigen-decomposition with the reference's clamp.
-template <int NP>
+template <int NP, int NT>
 __device__ __forceinline__ bool newton_schulz(cx<double>*& Y, cx<double>*& Z, cx<double>* P, cx<double>*& Y2, cx<double>*& Z2, int n,
                                               double* red, int& iters) {
-  tile_for<NP>(n, [&](int i, int j) { Z[i * n + j] = cx<double>{i == j ? 1.0 : 0.0, 0.0}; });
+  tile_for<NP, NT>(n, [&](int i, int j) { Z[i * n + j] = cx<double>{i == j ? 1.0 : 0.0, 0.0}; });
   __syncthreads();
   int remaining = -1;  // updates left once the residual is below 1e-6
   for (int it = 0; it < 80; ++it) {
-    mm<NP, false>(P, Z, Y, n);
+    mm<NP, NT, false>(P, Z, Y, n);
     double v = 0.0;
-    HFB_TILE_LOOP(NP, n, i, j) {  // residual max|P - I|, then P <- T = (3I - P)/2
+    HFB_TILE_LOOP(NP, NT, n, i, j) {  // residual max|P - I|, then P <- T = (3I - P)/2
       const cx<double> pv = P[i * n + j];
       const double id = i == j ? 1.0 : 0.0;
       v = fmax(v, fabs(pv.x - id) + fabs(pv.y));
       P[i * n + j] = cx<double>{1.5 * id - 0.5 * pv.x, -0.5 * pv.y};
     }
-    const double resid = block_max_all(v, red);
+    const double resid = block_max_all<NT>(v, red);
     // quadratic convergence: two updates after 1e-6 reach round-off
     if (remaining < 0 && resid < 1e-6) remaining = 2;
-    mm<NP, false>(Y2, Y, P, n);
-    mm<NP, false>(Z2, P, Z, n);
+    mm<NP, NT, false>(Y2, Y, P, n);
+    mm<NP, NT, false>(Z2, P, Z, n);
     cx<double>* t = Y;
     Y = Y2;
     Y2 = t;
@@ -317,47 +324,52 @@ __device__ __forceinline__ bool newton_schulz(cx<double>*& Y, cx<double>*& Z, cx
   return false;
 }
 
-template <int NP>
-__global__ void __launch_bounds__(DIAG_NT) diag_kernel(DiagArgs a) {
+template <int NP, int NT>
+__global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
   extern __shared__ double lds_raw[];
   __shared__ JacobiShared js;
-  __shared__ double red[16];
+  __shared__ double red[32];
   const int iq = blockIdx.x;
   const DiagIso d = a.iso[iq];
   const int n = d.nst;
   if (n <= 0) return;
   const int tid = threadIdx.x;
   const int nm = a.nmax;
-  // five work matrices in LDS (n <= NDIAG_MAX); deriving every pointer from
-  // the LDS array lets the compiler use LDS instructions instead of flat ones
-  cx<double>* const B0 = (cx<double>*)lds_raw;
+  // five work matrices, in LDS for small blocks and in global memory (L2
+  // resident) for large ones; each instance sees one kind of pointer only,
+  // so the compiler uses LDS or global instructions, never flat ones
+  cx<double>* B0;
+  if constexpr (NP <= 64)
+    B0 = (cx<double>*)lds_raw;
+  else
+    B0 = d.W;
   cx<double>* const B1 = B0 + n * n;
   cx<double>* const B2 = B1 + n * n;
   cx<double>* const B3 = B2 + n * n;
   cx<double>* const B4 = B3 + n * n;
 
   // Loewdin S^{-1/2} (static.py:463-482)
-  for (int i = tid; i < n; i += DIAG_NT) a.sp_norm[d.start + i] = d.S[i * n + i].x;
-  load_lower_herm<NP>(B0, d.S, n);
+  for (int i = tid; i < n; i += NT) a.sp_norm[d.start + i] = d.S[i * n + i].x;
+  load_lower_herm<NP, NT>(B0, d.S, n);
   double rowsum = 0.0;  // Gershgorin bound on the largest eigenvalue
   if (tid < n)
     for (int j = 0; j < n; ++j) rowsum += sqrt(cabs2(B0[tid * n + j]));
-  double cs = block_max_all(rowsum, red);
+  double cs = block_max_all<NT>(rowsum, red);
   if (!(cs > 0.0)) cs = 1.0;
-  tile_for<NP>(n, [&](int i, int j) { B0[i * n + j] = cscale(B0[i * n + j], 1.0 / cs); });
+  tile_for<NP, NT>(n, [&](int i, int j) { B0[i * n + j] = cscale(B0[i * n + j], 1.0 / cs); });
   __syncthreads();
   cx<double>*Y = B0, *Z = B1, *Y2 = B3, *Z2 = B4;
   int ns_iters = 0;
-  const bool ok = newton_schulz<NP>(Y, Z, B2, Y2, Z2, n, red, ns_iters);
+  const bool ok = newton_schulz<NP, NT>(Y, Z, B2, Y2, Z2, n, red, ns_iters);
   cx<double>* Urho = B4;
   if (ok) {
     const double f = 1.0 / sqrt(cs);
     // Z may live in B4: the scaling is elementwise, so in place is safe
-    tile_for<NP>(n, [&](int i, int j) { Urho[i * n + j] = cscale(Z[i * n + j], f); });
+    tile_for<NP, NT>(n, [&](int i, int j) { Urho[i * n + j] = cscale(Z[i * n + j], f); });
     __syncthreads();
   } else {
-    load_lower_herm<NP>(B0, d.S, n);
-    jacobi_eigh<NP>(B0, B1, n, js, false, B2);
+    load_lower_herm<NP, NT>(B0, d.S, n);
+    jacobi_eigh<NP, NT>(B0, B1, n, js, false, B2);
     __shared__ double fac[NMAXJ];
     if (tid == 0) {
       double wmax = js.ev[0];
@@ -367,7 +379,7 @@ __global__ void __launch_bounds__(DIAG_NT) diag_kernel(DiagArgs a) {
     }
     __syncthreads();
     const cx<double>* V = B1;
-    tile_for<NP>(n, [&](int i, int j) {  // U = V diag(fac) V^H
+    tile_for<NP, NT>(n, [&](int i, int j) {  // U = V diag(fac) V^H
       cx<double> acc = {0.0, 0.0};
       for (int k = 0; k < n; ++k) acc = cadd(acc, cscale(cmul(V[i * n + k], cconj(V[j * n + k])), fac[k]));
       Urho[i * n + j] = acc;
@@ -382,41 +394,41 @@ __global__ void __launch_bounds__(DIAG_NT) diag_kernel(DiagArgs a) {
   }
   cx<double>* U = Urho;
   if (a.tdiag && d.Lam) {
-    load_lower_herm<NP>(B0, d.Lam, n);
-    const int sw = jacobi_eigh<NP>(B0, B1, n, js, true, B2);
+    load_lower_herm<NP, NT>(B0, d.Lam, n);
+    const int sw = jacobi_eigh<NP, NT>(B0, B1, n, js, true, B2);
     if (a.stats && tid == 0) a.stats[iq] = sw;
-    mm<NP, false>(B3, Urho, B1, n);
+    mm<NP, NT, false>(B3, Urho, B1, n);
     U = B3;
   }
-  tile_for<NP>(n, [&](int i, int j) { d.U[i * n + j] = U[i * n + j]; });
+  tile_for<NP, NT>(n, [&](int i, int j) { d.U[i * n + j] = U[i * n + j]; });
 
   if (a.construct) {  // construct_hfb_matrices (static.py:484-577)
     cx<double>* H = B2;
     cx<double>* G = B0;
-    tile_for<NP>(n, [&](int i, int j) { B0[i * n + j] = d.Hold[i * n + j]; });
+    tile_for<NP, NT>(n, [&](int i, int j) { B0[i * n + j] = d.Hold[i * n + j]; });
     __syncthreads();
-    mm<NP, false>(B1, B0, U, n);
-    mm<NP, true>(H, U, B1, n);
+    mm<NP, NT, false>(B1, B0, U, n);
+    mm<NP, NT, true>(H, U, B1, n);
     if (a.ipair && d.Gold) {
-      tile_for<NP>(n, [&](int i, int j) { B0[i * n + j] = d.Gold[i * n + j]; });
+      tile_for<NP, NT>(n, [&](int i, int j) { B0[i * n + j] = d.Gold[i * n + j]; });
       __syncthreads();
-      mm<NP, false>(B1, B0, U, n);
-      mm<NP, true>(G, U, B1, n);
+      mm<NP, NT, false>(B1, B0, U, n);
+      mm<NP, NT, true>(G, U, B1, n);
     } else {
-      tile_for<NP>(n, [&](int i, int j) { G[i * n + j] = {0.0, 0.0}; });
+      tile_for<NP, NT>(n, [&](int i, int j) { G[i * n + j] = {0.0, 0.0}; });
       __syncthreads();
     }
-    tile_for<NP>(n, [&](int i, int j) {
+    tile_for<NP, NT>(n, [&](int i, int j) {
       d.Hm[i * nm + j] = H[i * n + j];
       d.Gm[i * nm + j] = G[i * n + j];
     });
-    for (int i = tid; i < n; i += DIAG_NT) {
+    for (int i = tid; i < n; i += NT) {
       a.sp_energy[d.start + i] = H[i * n + i].x;
       if (a.ipair) a.deltaf[d.start + i] = G[i * n + i].x * a.pairwg[d.start + i];
     }
     __syncthreads();
     if (a.tbcs) {
-      tile_for<NP>(n, [&](int i, int j) {
+      tile_for<NP, NT>(n, [&](int i, int j) {
         if (i != j) {
           G[i * n + j] = {0.0, 0.0};
           d.Gm[i * nm + j] = {0.0, 0.0};
@@ -426,7 +438,7 @@ __global__ void __launch_bounds__(DIAG_NT) diag_kernel(DiagArgs a) {
     }
     // lambda, its parts and the fluctuation measures; lambda+ kept in B1
     double mx = 0.0, sm = 0.0;
-    HFB_TILE_LOOP(NP, n, i, j) {
+    HFB_TILE_LOOP(NP, NT, n, i, j) {
       const int sj = d.start + j, si = d.start + i;
       const double wj = a.wocc[sj] * a.wstates[sj];
       const double uj = a.wguv[sj] * a.pairwg[sj] * a.wstates[sj];
@@ -444,21 +456,21 @@ __global__ void __launch_bounds__(DIAG_NT) diag_kernel(DiagArgs a) {
       mx = fmax(mx, sqrt(aa));
       sm += aa;
     }
-    const double M = block_max_all(mx, red);
+    const double M = block_max_all<NT>(mx, red);
     double Ssum = sm, dummy = 0.0;
-    block_sum2_all(Ssum, dummy, red);
+    block_sum2_all<NT>(Ssum, dummy, red);
     if (tid == 0) {
       a.efl1q[iq] = M;
       a.efl2q[iq] = sqrt(Ssum / ((double)n * n));
     }
   } else {
     // construct=False: lagrange from the stored lambda (static.py:336-350)
-    tile_for<NP>(n, [&](int i, int j) { B1[i * n + j] = d.Ls[i * nm + j]; });
+    tile_for<NP, NT>(n, [&](int i, int j) { B1[i * n + j] = d.Ls[i * nm + j]; });
     __syncthreads();
   }
   // M2 = U lambda+ (static.py:557-570)
-  mm<NP, false>(B0, U, B1, n);
-  tile_for<NP>(n, [&](int i, int j) { d.M2[i * n + j] = B0[i * n + j]; });
+  mm<NP, NT, false>(B0, U, B1, n);
+  tile_for<NP, NT>(n, [&](int i, int j) { d.M2[i * n + j] = B0[i * n + j]; });
 }
 
 __global__ void diag_finish_kernel(const double* efl1q, const double* efl2q, double* efl1, double* efl2) {
@@ -475,15 +487,14 @@ extern "C" int hfb_diag_small(int nst0, int nst1, int nmax, const double* const*
                               const double* pairwg, double* sp_norm, double* sp_energy, double* deltaf,
                               double* hmatrix, double* gapmatrix, double* symcond, double* lambda_save,
                               double* efluct1q, double* efluct2q, double* efluct1, double* efluct2,
-                              double* const* uo, double* stats, void* stream) {
+                              double* const* uo, double* work, double* stats, void* stream) {
   if (!mats || !uo || !wocc || !wstates || !wguv || !pairwg || !sp_norm || !sp_energy || !deltaf ||
       !hmatrix || !gapmatrix || !symcond || !lambda_save || !efluct1q || !efluct2q || !efluct1 || !efluct2)
     return HFB_E_ARG;
   if (nst0 < 0 || nst1 < 0 || nmax < nst0 || nmax < nst1) return HFB_E_SIZE;
-  // Blocks above NDIAG_MAX states are switched off: a restructured form of
-  // the Jacobi rounds faulted the device at 40 and 64 states and the cause
-  // has not been found.  Larger blocks need the multi-workgroup design.
   if (nst0 > NDIAG_MAX || nst1 > NDIAG_MAX) return HFB_E_SIZE;
+  const int nbig0 = nst0 > nst1 ? nst0 : nst1;
+  if (nbig0 > NDIAG_LDS && !work) return HFB_E_ARG;
   DiagArgs a{};
   a.nmax = nmax;
   const int nsts[2] = {nst0, nst1};
@@ -500,6 +511,7 @@ extern "C" int hfb_diag_small(int nst0, int nst1, int nmax, const double* const*
     d.U = (cx<double>*)uo[iq * 2 + 0];
     d.M2 = (cx<double>*)uo[iq * 2 + 1];
     if (d.nst > 0 && (!d.U || !d.M2)) return HFB_E_ARG;
+    d.W = work ? (cx<double>*)work + (int64_t)iq * 5 * nm2 : nullptr;
     d.Hm = (cx<double>*)hmatrix + iq * nm2;
     d.Gm = (cx<double>*)gapmatrix + iq * nm2;
     d.Sym = (cx<double>*)symcond + iq * nm2;
@@ -520,19 +532,23 @@ extern "C" int hfb_diag_small(int nst0, int nst1, int nmax, const double* const*
   a.construct = construct;
   a.stats = stats;
   const int nbig = nst0 > nst1 ? nst0 : nst1;
-  const size_t lds = sizeof(cx<double>) * 5 * (size_t)nbig * nbig;
   hipStream_t st = (hipStream_t)stream;
-#define HFB_DIAG(NPV)                                                             \
-  {                                                                               \
-    if (lds > 32 * 1024) allow_full_lds((const void*)diag_kernel<NPV>);           \
-    hipLaunchKernelGGL(diag_kernel<NPV>, dim3(2), dim3(DIAG_NT), lds, st, a);      \
+#define HFB_DIAG(NPV, NTV, LDSV)                                                            \
+  {                                                                                         \
+    if ((LDSV) > 32 * 1024) allow_full_lds((const void*)diag_kernel<NPV, NTV>);             \
+    hipLaunchKernelGGL((diag_kernel<NPV, NTV>), dim3(2), dim3(NTV), (LDSV), st, a);         \
   }
-  if (nbig <= 16)
-    HFB_DIAG(16)
-  else if (nbig <= 32)
-    HFB_DIAG(32)
-  else
-    HFB_DIAG(64)
+  if (nbig <= NDIAG_LDS) {
+    const size_t lds = sizeof(cx<double>) * 5 * (size_t)nbig * nbig;
+    if (nbig <= 16)
+      HFB_DIAG(16, 256, lds)
+    else if (nbig <= 32)
+      HFB_DIAG(32, 256, lds)
+    else
+      HFB_DIAG(64, 256, lds)
+  } else {
+    HFB_DIAG(128, 1024, 0)
+  }
 #undef HFB_DIAG
   int rc = launch_status();
   if (rc != HFB_OK) return rc;

@@ -232,7 +232,7 @@ def rotate(A, U1, U2, out1, out2):
     check(lib().hfb_rotate(prec, ptr(A), ptr(U1), ptr(U2), ptr(out1), ptr(out2), nst, K, stream_handle()), "rotate")
 
 
-def diag_small(nst0, nst1, nmax, mats, tdiag, ipair, tbcs, construct, levels, static, E, uo):
+def diag_small(nst0, nst1, nmax, mats, tdiag, ipair, tbcs, construct, levels, static, E, uo, work=None):
     """Loewdin, optional diagonalisation and HFB matrices for both isospins."""
     m = (c_void_p * 8)(*[(t.data_ptr() if t is not None else None) for t in mats])
     u = (c_void_p * 4)(*[t.data_ptr() for t in uo])
@@ -244,7 +244,7 @@ def diag_small(nst0, nst1, nmax, mats, tdiag, ipair, tbcs, construct, levels, st
                                ptr(levels.deltaf), ptr(static.hmatrix), ptr(static.gapmatrix), ptr(static.symcond),
                                ptr(static.lambda_save), c_void_p(Ep + sz * EN["EFL1Q"]),
                                c_void_p(Ep + sz * EN["EFL2Q"]), c_void_p(Ep + sz * EN["EFL1"]),
-                               c_void_p(Ep + sz * EN["EFL2"]), u, c_void_p(Ep + sz * EN["JACOBI"]),
+                               c_void_p(Ep + sz * EN["EFL2"]), u, ptr(work), c_void_p(Ep + sz * EN["JACOBI"]),
                                stream_handle()), "diag_small")
 
 

@@ -214,17 +214,15 @@ def diagstep(energies, forces, grids, levels, static, diagonalize=False, constru
                                                                               torch.complex128)]
     # overlap, H_old, lambda_lin and G_old of both isospins in one pass over the states
     kernels.gram2(ctx, blocks, grids.wxyz, outs)
+    # blocks above 36 states keep the subspace work matrices in global memory
+    work = ctx.work(("diag_work", nmax), (10 * nmax * nmax,), torch.complex128) if max(n0, n1) > 36 else None
     kernels.diag_small(n0, n1, nmax, mats, diagonalize, forces.ipair != 0, forces.tbcs, construct, levels, static,
-                       energies.vec, uo)
-    # psi <- psi U in place (each 16-point block of K is read, then written,
-    # by one wavefront); blocks of more than 64 states go through a copy
+                       energies.vec, uo, work)
+    # psi <- psi U and lagrange <- psi M2 in place (every workgroup reads its
+    # columns of psi before it writes them)
     for iq, (a, b) in enumerate(((0, n0), (n0, n0 + n1))):
-        if b - a <= 64:
+        if b > a:
             kernels.rotate(levels.psi[a:b], uo[2 * iq], uo[2 * iq + 1], levels.psi[a:b], levels.lagrange[a:b])
-        else:
-            spare = ctx.work(("psi_spare", levels.psi.dtype, b - a), tuple(levels.psi[a:b].shape), levels.psi.dtype)
-            kernels.rotate(levels.psi[a:b], uo[2 * iq], uo[2 * iq + 1], spare, levels.lagrange[a:b])
-            levels.psi[a:b].copy_(spare)
     return energies, levels, static
 
 

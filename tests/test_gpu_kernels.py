@@ -78,7 +78,7 @@ def random_meanfield(O, g, rng):
     return mf
 
 
-@pytest.mark.parametrize("n", [24, 32])
+@pytest.mark.parametrize("n", [24, 32, 48])
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
 def test_hpsi_matches_oracle(pkg, O, dev, n, dtype):
     from jax_hfbfft_amd import _lib
@@ -123,12 +123,11 @@ def test_poisson_matches_oracle(pkg, O, dev, n):
     assert relerr(out.cpu().numpy(), ref) < 1e-12
 
 
-@pytest.mark.parametrize("nsts", [(8, 8), (20, 20), (36, 30), (64, 50)])
+@pytest.mark.parametrize("nsts", [(8, 8), (20, 20), (36, 30), (64, 50), (126, 82)])
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
 def test_gram2_matches_matmul(pkg, dev, nsts, dtype):
-    """scale * A^H B_m for both isospin blocks (the vector path, used up to
-    64 states), with B_0 = A (Hermitian, half computed) and two other
-    blocks, against numpy in FP64."""
+    """scale * A^H B_m for both isospin blocks, with B_0 = A (Hermitian, half
+    computed) and two other blocks, against numpy in FP64."""
     g = pkg.grids.init_grids(pkg.params.init_params(), nx=24, ny=24, nz=24, dx=1.0, dy=1.0, dz=1.0)
     ctx = g.ctx
     rng = np.random.default_rng(sum(nsts))
@@ -150,11 +149,11 @@ def test_gram2_matches_matmul(pkg, dev, nsts, dtype):
             assert np.abs(got - r).max() <= 1e-12 * np.abs(r).max(), (iq, m)
 
 
-@pytest.mark.parametrize("nst", [8, 20, 30, 36, 64])
+@pytest.mark.parametrize("nst", [8, 20, 30, 36, 64, 126])
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
 def test_rotate_matches_matmul(pkg, dev, nst, dtype):
-    """out1 = psi U1 and out2 = psi U2 in place (vector path up to 32
-    states, MFMA path above) against numpy."""
+    """out1 = psi U1 and out2 = psi U2 in place (register columns up to 32
+    states, LDS-staged columns above) against numpy."""
     rng = np.random.default_rng(nst)
     K = 2 * 24 ** 3
     psi = rng.standard_normal((nst, K)) + 1j * rng.standard_normal((nst, K))

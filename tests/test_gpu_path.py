@@ -57,16 +57,16 @@ def test_laplace_matches_oracle(pkg, O, dev):
         assert rel(out.cpu().numpy(), r) < 1e-12
 
 
-_FAULTED = pytest.mark.skip(reason="blocks above 36 states are switched off in hfb_diag_small: this case faulted "
-                                   "the device (with a since-reverted form of the Jacobi rounds) and the cause is "
-                                   "not found")
+_FAULTED = pytest.mark.skip(reason="this case faulted the device with a since-reverted form of the Jacobi rounds "
+                                   "in hfb_diag_small and the cause is not found; it stays off the device")
 
 
 @pytest.mark.parametrize("npsi", [[12, 10], [36, 30], pytest.param([40, 33], marks=_FAULTED),
-                                  pytest.param([64, 50], marks=_FAULTED)])
+                                  pytest.param([64, 50], marks=_FAULTED), [90, 60], [126, 82]])
 def test_diagstep_matches_oracle(pkg, O, dev, npsi):
     """Loewdin + subspace diagonalisation + HFB matrices on the same states;
-    block sizes of O16-BCS, Cr48 and the largest block the kernel takes."""
+    block sizes of O16-BCS and Cr48 (work matrices in LDS) and of the
+    global-memory path up to Pb208's 126/82."""
     ref, run = small_case(O, pkg, ipair=5, tbcs=True, npsi=npsi)
     rng = np.random.default_rng(5)
     shp = ref.levels.psi.shape

@@ -1,0 +1,115 @@
+"""Whole-workload runs at their full sizes.
+
+Ca40 (32^3, 20+20 states) and Cr48 (32^3, 36+36 states, BCS pairing,
+deformed start): prelude + 3 SCF iterations, FP64 HIP path against the FP64
+oracle on the same start, energies to 1e-7 MeV and densities to 1e-8
+relative (as for O16).
+
+Pb208 (48^3, dx = 0.8 fm, 126+82 states) is too large for the oracle to run
+in a test, so it is checked through properties that do not depend on a
+reference run: orthonormal states per isospin after each diagonalisation
+(Gram = 1 to 1e-10), particle numbers integrated from the densities, energy
+components that are finite and a total energy that falls over the first
+iterations, and a mixed-precision run that follows the FP64 run."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch.device("cuda:0")
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("workload", ["Ca40", "Cr48"])
+def test_workload_iterations_match_oracle(pkg, O, dev, workload):
+    cfg = dict(pkg.driver.CONFIGS[workload])
+    ref = O.setup(**cfg)
+    run = pkg.driver.HFRun.from_config(precision="f64", psi0=ref.levels.psi, **cfg)
+    O.prelude(ref)
+    run.prelude()
+    e = run.energies_host()
+    print(workload, "prelude", e["ehf"], ref.energies.ehf, flush=True)
+    assert abs(e["ehf"] - ref.energies.ehf) < 1e-8
+    for i in range(1, 4):
+        O.iteration(ref, i)
+        run.iteration(i)
+        e = run.energies_host()
+        print(workload, i, e["ehf"], ref.energies.ehf, flush=True)
+        assert abs(e["ehf"] - ref.energies.ehf) < 1e-7, (i, e["ehf"], ref.energies.ehf)
+        assert abs(e["ehfint"] - ref.energies.ehfint) < 1e-7, (i, e["ehfint"], ref.energies.ehfint)
+    assert rel(run.densities.rho.cpu().numpy(), ref.densities.rho) < 1e-8
+    if cfg.get("ipair"):
+        assert rel(run.levels.wocc.cpu().numpy(), ref.levels.wocc) < 1e-8
+        assert abs(float(run.pairs.eferm[0]) - ref.pairs.eferm[0]) < 1e-7
+
+
+def _overlap_error(run):
+    lv, g = run.levels, run.grids
+    w = float(g.wxyz)
+    err = 0.0
+    for a, b in ((0, int(lv.npsi[0])), (int(lv.npsi[0]), int(lv.npsi[0]) + int(lv.npsi[1]))):
+        p = lv.psi[a:b].reshape(b - a, -1).to(torch.complex128)
+        S = (p.conj() @ p.transpose(0, 1).contiguous()) * w
+        err = max(err, (S - torch.eye(b - a, dtype=S.dtype, device=S.device)).abs().max().item())
+    return err
+
+
+@pytest.fixture(scope="module")
+def pb208(pkg, dev):
+    cfg = dict(pkg.driver.CONFIGS["Pb208"])
+    out = {}
+    for prec in ("f64", "mixed"):
+        run = pkg.driver.HFRun.from_config(precision=prec, **cfg)
+        run.prelude()
+        hist = [run.energies_host()]
+        ortho = [_overlap_error(run)] if prec == "f64" else []
+        for i in range(1, 13):
+            run.iteration(i)
+            hist.append(run.energies_host())
+            if prec == "f64" and i % 4 == 0:
+                ortho.append(_overlap_error(run))
+        print("Pb208", prec, [round(h["ehf"], 4) for h in hist], flush=True)
+        out[prec] = (run, hist, ortho)
+    return out
+
+
+def test_pb208_states_stay_orthonormal(pb208):
+    _, _, ortho = pb208["f64"]
+    print("Pb208 max |psi^H psi - 1|:", ortho)
+    assert max(ortho) < 1e-10
+
+
+def test_pb208_particle_numbers(pb208):
+    run, _, _ = pb208["f64"]
+    g = run.grids
+    n = (run.densities.rho.sum(dim=(1, 2, 3)) * float(g.wxyz)).tolist()
+    assert abs(n[0] - 126.0) < 1e-8 and abs(n[1] - 82.0) < 1e-8, n
+
+
+def test_pb208_energy_falls(pb208):
+    _, hist, _ = pb208["f64"]
+    for h in hist:
+        for k in ("ehf", "ehfint", "tke", "ehfc", "ecorc"):
+            assert np.isfinite(h[k]), (k, h[k])
+    e = [h["ehf"] for h in hist]
+    # a closed-shell start: the energy drops steadily over the first iterations
+    assert e[-1] < e[1] < e[0], e
+    assert -2500.0 < e[-1] < -1000.0, e[-1]
+
+
+def test_pb208_mixed_follows_f64(pb208):
+    _, h64, _ = pb208["f64"]
+    _, hmx, _ = pb208["mixed"]
+    d = abs(h64[-1]["ehf"] - hmx[-1]["ehf"])
+    print("Pb208 |E_mixed - E_f64| after 12 iterations [MeV]:", d)
+    assert d < 0.05, d
