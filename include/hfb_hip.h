@@ -201,10 +201,17 @@ int hfb_diag_small(int nst0, int nst1, int nmax, const double* const* mats, int 
 
 /* out1 = psi_2d U1 and out2 = psi_2d U2 over the nst states of one block
  * (static.py:316 psi_2d @ unitary; static.py:557 lagrange = psi_new @
- * lambda+), psi (nst, K).  out1 may alias psi (in place, nst <= 64);
- * U2/out2 may be null. */
+ * lambda+), psi (nst, K), U1/U2 (nst, nst) complex128.  out1 may alias psi
+ * (in place); U2/out2 may be null.  A workgroup stages 64 elements of
+ * every state in LDS: HFB_E_SIZE above 300 states (FP32) or 150 (FP64). */
 int hfb_rotate(int prec, const void* psi, const double* U1, const double* U2, void* out1, void* out2, int nst,
                long long K, void* stream);
+
+/* hfb_rotate for both isospin blocks in one launch (the two rotations of
+ * static.py:316 and static.py:557 for neutrons and protons): arrays of two
+ * entries indexed by isospin; a block with nst[iq] == 0 is skipped. */
+int hfb_rotate2(int prec, const void* const* psi, const double* const* U1, const double* const* U2,
+                void* const* out1, void* const* out2, const int* nst, long long K, void* stream);
 
 /* ---------------------------------------------------------------------
  * Densities and single-particle properties (densities.py:177-337,

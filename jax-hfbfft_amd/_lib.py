@@ -45,6 +45,8 @@ _SIGS = {
                           c_double, POINTER(c_void_p), POINTER(c_void_p), c_void_p]),
     "hfb_rotate": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, ctypes.c_longlong,
                            c_void_p]),
+    "hfb_rotate2": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_longlong,
+                            c_void_p]),
     "hfb_diag_small": (c_int, [c_int, c_int, c_int, POINTER(c_void_p), c_int, c_int, c_int, c_int, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_void_p),

@@ -254,17 +254,27 @@ struct LaneFFT {
   // frequency index of element k1 after fwd
   __device__ __forceinline__ int freq(int k1) const { return k1 + N1 * brev; }
 
+  // Cross-lane radix-2 butterflies.  The twiddle of stage h is
+  // W_{2h}^{j mod h} on the bottom lanes: 1 for h = 1, 1 or -i for h = 2
+  // (applied exactly, as a swap and a sign), a table value for h = 4.  Top
+  // and bottom outputs share one form, s v + u with s = -1 on the bottom.
   template <int ST>
   __device__ __forceinline__ void fwd_stages(cx<T>* v) const {
     if constexpr (ST >= 0) {
       constexpr int H = 1 << ST;
       const bool bottom = (j & H) != 0;
+      const T sg = bottom ? T(-1) : T(1);
+      const bool rot = bottom && (j & 1);  // times -i (stage h = 2)
 #pragma unroll
       for (int k1 = 0; k1 < N1; ++k1) {
         const cx<T> u = lane_xor<H>(v[k1]);
-        const cx<T> top = cadd(v[k1], u);
-        const cx<T> bot = cmul(csub(u, v[k1]), tws[ST]);
-        v[k1] = bottom ? bot : top;
+        cx<T> d = {fma(sg, v[k1].x, u.x), fma(sg, v[k1].y, u.y)};
+        if constexpr (H == 2) {
+          d = {rot ? d.y : d.x, rot ? -d.x : d.y};
+        } else if constexpr (H >= 4) {
+          if (bottom) d = cmul(d, tws[ST]);
+        }
+        v[k1] = d;
       }
       fwd_stages<ST - 1>(v);
     }
@@ -275,12 +285,19 @@ struct LaneFFT {
     if constexpr (ST < NS) {
       constexpr int H = 1 << ST;
       const bool bottom = (j & H) != 0;
+      const T sg = bottom ? T(-1) : T(1);
+      const bool rot = bottom && (j & 1);  // times +i (stage h = 2)
       const cx<T> wc = cconj(tws[ST]);
 #pragma unroll
       for (int k1 = 0; k1 < N1; ++k1) {
-        const cx<T> sv = bottom ? cmul(v[k1], wc) : v[k1];
+        cx<T> sv = v[k1];
+        if constexpr (H == 2) {
+          sv = {rot ? -sv.y : sv.x, rot ? sv.x : sv.y};
+        } else if constexpr (H >= 4) {
+          if (bottom) sv = cmul(sv, wc);
+        }
         const cx<T> r = lane_xor<H>(sv);
-        v[k1] = bottom ? csub(r, sv) : cadd(v[k1], r);
+        v[k1] = {fma(sg, sv.x, r.x), fma(sg, sv.y, r.y)};
       }
       inv_stages<ST + 1>(v);
     }

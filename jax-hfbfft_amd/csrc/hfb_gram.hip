@@ -122,13 +122,13 @@ __global__ void __launch_bounds__(256) gram_vec_kernel(GramVecArgs ga) {
   for (int a = 0; a < GV_T; ++a)
 #pragma unroll
     for (int b = 0; b < GV_T; ++b) cr[a][b] = ci[a][b] = 0.0;
-  const bool sym = B == A && ti == tj;
+#pragma unroll 2
   for (int64_t k = k0 + lane; k < k1; k += 64) {
     cx<double> av[GV_T], bv[GV_T];
 #pragma unroll
     for (int u = 0; u < GV_T; ++u) {
       av[u] = ld64(A + (int64_t)ri[u] * K + k);
-      bv[u] = sym ? av[u] : ld64(B + (int64_t)rj[u] * K + k);
+      bv[u] = ld64(B + (int64_t)rj[u] * K + k);
     }
 #pragma unroll
     for (int a = 0; a < GV_T; ++a)
@@ -205,15 +205,36 @@ __global__ void __launch_bounds__(256) gram_vec_reduce_kernel(const double* __re
 // cache into scalar registers (no LDS bandwidth spent on broadcasts).  A
 // workgroup reads all of its columns of psi before its first store, so out1
 // may alias psi.
+struct RotIso {
+  const void* psi;
+  const double* U1;
+  const double* U2;  // may be null (then out2 is not written)
+  void* out1;
+  void* out2;
+  int nst;
+};
+
+struct RotArgs {
+  RotIso iso[2];
+  int64_t K;
+};
+
 template <typename T, int RJ, int KPL>
-__global__ void __launch_bounds__(256) rotate_kernel(const cx<T>* psi, const double* __restrict__ U1,
-                                                     const double* __restrict__ U2, cx<T>* out1, cx<T>* out2,
-                                                     int nst, int64_t K) {
+__global__ void __launch_bounds__(256) rotate_kernel(RotArgs ra) {
+  const RotIso& ri = ra.iso[blockIdx.y];
+  const int nst = ri.nst;
+  const int64_t K = ra.K;
+  const cx<T>* psi = (const cx<T>*)ri.psi;
+  const double* __restrict__ U1 = ri.U1;
+  const double* __restrict__ U2 = ri.U2;
+  cx<T>* out1 = (cx<T>*)ri.out1;
+  cx<T>* out2 = (cx<T>*)ri.out2;
   constexpr int KT = 64 * KPL;  // k per workgroup
   extern __shared__ unsigned char rot_lds[];
   cx<T>* sp = (cx<T>*)rot_lds;  // [nst][KT]
   const bool two = U2 != nullptr;
   const int64_t k0 = (int64_t)blockIdx.x * KT;
+  if (nst <= 0 || k0 >= K) return;  // uniform over the workgroup
   for (int e = threadIdx.x; e < nst * KT; e += blockDim.x) {
     const int i = e / KT, kk = e % KT;
     const int64_t k = k0 + kk;
@@ -222,7 +243,11 @@ __global__ void __launch_bounds__(256) rotate_kernel(const cx<T>* psi, const dou
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (int j0 = wave * RJ; j0 < nst; j0 += 4 * RJ) {
+  for (int jg = wave * RJ; jg < nst; jg += 4 * RJ) {
+    // the last group is moved down to end at nst (its first outputs are then
+    // also produced, with identical values, by the previous group), so the
+    // RJ coefficients of a row are contiguous and load as one scalar block
+    const int j0 = min(jg, nst - RJ);
     double r1[KPL][RJ], i1[KPL][RJ], r2[KPL][RJ], i2[KPL][RJ];
 #pragma unroll
     for (int c = 0; c < KPL; ++c)
@@ -236,19 +261,24 @@ __global__ void __launch_bounds__(256) rotate_kernel(const cx<T>* psi, const dou
         vx[c] = double(pv.x);
         vy[c] = double(pv.y);
       }
-      const double* u1 = U1 + 2 * ((int64_t)i * nst);
-      const double* u2 = two ? U2 + 2 * ((int64_t)i * nst) : u1;
+      const double* u1 = U1 + 2 * ((int64_t)i * nst + j0);
+      const double* u2 = two ? U2 + 2 * ((int64_t)i * nst + j0) : u1;
+      double ax[RJ], ay[RJ], bx[RJ], by[RJ];
 #pragma unroll
       for (int t = 0; t < RJ; ++t) {
-        const int j = min(j0 + t, nst - 1);
-        const double ax = u1[2 * j], ay = u1[2 * j + 1];
-        const double bx = u2[2 * j], by = u2[2 * j + 1];
+        ax[t] = u1[2 * t];
+        ay[t] = u1[2 * t + 1];
+        bx[t] = u2[2 * t];
+        by[t] = u2[2 * t + 1];
+      }
+#pragma unroll
+      for (int t = 0; t < RJ; ++t) {
 #pragma unroll
         for (int c = 0; c < KPL; ++c) {
-          r1[c][t] = fma(vx[c], ax, fma(-vy[c], ay, r1[c][t]));
-          i1[c][t] = fma(vx[c], ay, fma(vy[c], ax, i1[c][t]));
-          r2[c][t] = fma(vx[c], bx, fma(-vy[c], by, r2[c][t]));
-          i2[c][t] = fma(vx[c], by, fma(vy[c], bx, i2[c][t]));
+          r1[c][t] = fma(vx[c], ax[t], fma(-vy[c], ay[t], r1[c][t]));
+          i1[c][t] = fma(vx[c], ay[t], fma(vy[c], ax[t], i1[c][t]));
+          r2[c][t] = fma(vx[c], bx[t], fma(-vy[c], by[t], r2[c][t]));
+          i2[c][t] = fma(vx[c], by[t], fma(vy[c], bx[t], i2[c][t]));
         }
       }
     }
@@ -259,10 +289,8 @@ __global__ void __launch_bounds__(256) rotate_kernel(const cx<T>* psi, const dou
 #pragma unroll
       for (int t = 0; t < RJ; ++t) {
         const int j = j0 + t;
-        if (j < nst) {
-          out1[(int64_t)j * K + k] = cx<T>{T(r1[c][t]), T(i1[c][t])};
-          if (two) out2[(int64_t)j * K + k] = cx<T>{T(r2[c][t]), T(i2[c][t])};
-        }
+        out1[(int64_t)j * K + k] = cx<T>{T(r1[c][t]), T(i1[c][t])};
+        if (two) out2[(int64_t)j * K + k] = cx<T>{T(r2[c][t]), T(i2[c][t])};
       }
     }
   }
@@ -354,37 +382,46 @@ extern "C" int hfb_gram(int prec, const void* A, const void* const* B, int nm, i
   return hfb_gram2(prec, As, Bs, nm, ns, K, scale, parts, outs, stream);
 }
 
-extern "C" int hfb_rotate(int prec, const void* psi, const double* U1, const double* U2, void* out1, void* out2,
-                          int nst, long long K, void* stream) {
-  if (!psi || !U1 || !out1 || nst <= 0 || K <= 0) return HFB_E_ARG;
-  if (U2 && !out2) return HFB_E_ARG;
+namespace {
+
+int launch_rotate(int prec, RotArgs ra, int niso, hipStream_t st) {
   if (prec != HFB_F64 && prec != HFB_F32) return HFB_E_ARG;
+  int nmax = 0, nmin = 1 << 30;
+  for (int iq = 0; iq < niso; ++iq) {
+    const int n = ra.iso[iq].nst;
+    if (n <= 0) continue;
+    nmax = n > nmax ? n : nmax;
+    nmin = n < nmin ? n : nmin;
+  }
+  if (nmax == 0) return HFB_OK;
   const size_t csz = prec == HFB_F64 ? sizeof(cx<double>) : sizeof(cx<float>);
   // two k per lane while the staged tile fits in LDS, else one
-  const int kpl = csz * (size_t)nst * 128 <= 150 * 1024 ? 2 : 1;
-  const size_t lds = csz * (size_t)nst * 64 * kpl;
+  const int kpl = csz * (size_t)nmax * 128 <= 150 * 1024 ? 2 : 1;
+  const size_t lds = csz * (size_t)nmax * 64 * kpl;
   if (lds > 150 * 1024) return HFB_E_SIZE;
-  const int nb = (int)((K + 64 * kpl - 1) / (64 * kpl));
-  hipStream_t st = (hipStream_t)stream;
-  // outputs per wavefront pass: the four wavefronts cover nst in one pass
-  // up to 32 states, and loop over groups of 8 beyond
-  const int rj = nst <= 12 ? 3 : nst <= 20 ? 5 : 8;
-#define HFB_ROT(TT, RJV, KP)                                                                                   \
-  {                                                                                                            \
-    auto kern = rotate_kernel<TT, RJV, KP>;                                                                    \
-    if (lds > 32 * 1024) allow_full_lds((const void*)kern);                                                    \
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), lds, st, (const cx<TT>*)psi, U1, U2, (cx<TT>*)out1,           \
-                       (cx<TT>*)out2, nst, (int64_t)K);                                                        \
+  const int nb = (int)((ra.K + 64 * kpl - 1) / (64 * kpl));
+  // outputs per wavefront pass (rj <= every block's nst: a group never
+  // reaches past the last state); the four wavefronts cover a block in one
+  // pass up to 32 states and loop over groups of 8 beyond
+  const int rj = nmin < 3 ? 1 : nmin <= 12 ? 3 : nmin <= 20 ? 5 : 8;
+  const dim3 grid(nb, niso);
+#define HFB_ROT(TT, RJV, KP)                                                     \
+  {                                                                              \
+    auto kern = rotate_kernel<TT, RJV, KP>;                                      \
+    if (lds > 32 * 1024) allow_full_lds((const void*)kern);                      \
+    hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, ra);                      \
   }
 #define HFB_ROTK(TT, RJV) \
   if (kpl == 2) HFB_ROT(TT, RJV, 2) else HFB_ROT(TT, RJV, 1)
-#define HFB_ROTP(TT)                  \
-  if (rj == 3) {                      \
-    HFB_ROTK(TT, 3)                   \
-  } else if (rj == 5) {               \
-    HFB_ROTK(TT, 5)                   \
-  } else {                            \
-    HFB_ROTK(TT, 8)                   \
+#define HFB_ROTP(TT)      \
+  if (rj == 1) {          \
+    HFB_ROTK(TT, 1)       \
+  } else if (rj == 3) {   \
+    HFB_ROTK(TT, 3)       \
+  } else if (rj == 5) {   \
+    HFB_ROTK(TT, 5)       \
+  } else {                \
+    HFB_ROTK(TT, 8)       \
   }
   if (prec == HFB_F64) {
     HFB_ROTP(double)
@@ -395,4 +432,30 @@ extern "C" int hfb_rotate(int prec, const void* psi, const double* U1, const dou
 #undef HFB_ROTK
 #undef HFB_ROT
   return launch_status();
+}
+
+}  // namespace
+
+extern "C" int hfb_rotate(int prec, const void* psi, const double* U1, const double* U2, void* out1, void* out2,
+                          int nst, long long K, void* stream) {
+  if (!psi || !U1 || !out1 || nst <= 0 || K <= 0) return HFB_E_ARG;
+  if (U2 && !out2) return HFB_E_ARG;
+  RotArgs ra{};
+  ra.iso[0] = RotIso{psi, U1, U2, out1, out2, nst};
+  ra.K = K;
+  return launch_rotate(prec, ra, 1, (hipStream_t)stream);
+}
+
+extern "C" int hfb_rotate2(int prec, const void* const* psi, const double* const* U1, const double* const* U2,
+                           void* const* out1, void* const* out2, const int* nst, long long K, void* stream) {
+  if (!psi || !U1 || !U2 || !out1 || !out2 || !nst || K <= 0) return HFB_E_ARG;
+  RotArgs ra{};
+  for (int iq = 0; iq < 2; ++iq) {
+    if (nst[iq] < 0) return HFB_E_ARG;
+    if (nst[iq] == 0) continue;
+    if (!psi[iq] || !U1[iq] || !out1[iq] || (U2[iq] && !out2[iq])) return HFB_E_ARG;
+    ra.iso[iq] = RotIso{psi[iq], U1[iq], U2[iq], out1[iq], out2[iq], nst[iq]};
+  }
+  ra.K = K;
+  return launch_rotate(prec, ra, 2, (hipStream_t)stream);
 }

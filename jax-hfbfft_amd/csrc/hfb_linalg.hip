@@ -164,13 +164,18 @@ struct JacobiShared {
   double ev[NMAXJ];
 };
 
-// Eigen-decomposition of the Hermitian A (LDS, destroyed); V (LDS) gets the
+// Eigen-decomposition of the Hermitian A (destroyed); V gets the
 // eigenvectors as columns, js.ev the eigenvalues; sorted ascending when
 // `sort` (T is scratch for the column permutation).  Parallel cyclic Jacobi:
-// in each round the n/2 disjoint pairs of a round-robin ordering are rotated
-// together (columns of A and V, then rows of A).
+// in each round the n/2 disjoint pairs (p_k, q_k) of a round-robin ordering
+// are rotated together, A <- R^H A R with R block-diagonal over the pairs.
+// A round computes the rotations (one thread per pair), then every 2x2
+// block (pair a, pair b) of the new matrix, R_a^H A[a,b] R_b, into the
+// second buffer A2 and V <- V R in place; the buffers then swap.  For odd n
+// one index per round has no partner and is carried unrotated.
 template <int NP, int NT>
-__device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* V, int n, JacobiShared& js, bool sort, cx<double>* T) {
+__device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* A2, cx<double>* V, int n, JacobiShared& js,
+                                           bool sort, cx<double>* T) {
   const int tid = threadIdx.x;
   tile_for<NP, NT>(n, [&](int i, int j) { V[i * n + j] = cx<double>{i == j ? 1.0 : 0.0, 0.0}; });
   const int m = n + (n & 1);
@@ -194,7 +199,7 @@ __device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* V, int n, 
         const int kb = m - 1 - k;
         const int b0 = ((kb - 1 + t) % (m - 1)) + 1;
         int p = a0 < b0 ? a0 : b0, q = a0 < b0 ? b0 : a0;
-        double c = 1.0, s = 0.0;
+        double c = 1.0, sn = 0.0;
         cx<double> ea = {1.0, 0.0};
         if (q < n) {
           const double app = A[p * n + p].x, aqq = A[q * n + q].x;
@@ -205,56 +210,68 @@ __device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* V, int n, 
             const double th = (aqq - app) / (2.0 * r);
             const double tt = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
             c = 1.0 / sqrt(1.0 + tt * tt);
-            s = tt * c;
+            sn = tt * c;
           }
         } else {
-          p = q = -1;
+          q = -1;  // p has no partner this round (odd n)
         }
         js.p[k] = p;
         js.q[k] = q;
         js.c[k] = c;
-        js.s[k] = s;
+        js.s[k] = sn;
         js.e[k] = ea;
       }
       __syncthreads();
-      // columns of A and V: X <- X R for each rotated pair
-      for (int it = tid; it < NP * NP; it += NT) {
-        const int r = it % NP;
-        const int rest = it / NP;
-        const int k = rest % (NP / 2);
-        const int mat = rest / (NP / 2);
-        if (r >= n || k >= npair) continue;
-        const int p = js.p[k], q = js.q[k];
-        if (p < 0 || js.s[k] == 0.0) continue;
-        cx<double>* X = mat ? V : A;
-        const cx<double> xp = X[r * n + p], xq = X[r * n + q];
-        const cx<double> ec = cscale(js.e[k], js.c[k]), es = cscale(js.e[k], js.s[k]);
-        X[r * n + p] = csub(cmul(xp, ec), cscale(xq, js.s[k]));
-        X[r * n + q] = cadd(cmul(xp, es), cscale(xq, js.c[k]));
-      }
-      __syncthreads();
-      // rows of A: A <- R^H A; the rotated pair's off-diagonal entries are
-      // set to zero and its diagonal to real
-      for (int it = tid; it < (NP / 2) * NP; it += NT) {
-        const int k = it / NP, r = it % NP;
-        if (r >= n || k >= npair) continue;
-        const int p = js.p[k], q = js.q[k];
-        if (p < 0 || js.s[k] == 0.0) continue;
-        const cx<double> xp = A[p * n + r], xq = A[q * n + r];
-        const cx<double> ecc = cscale(cconj(js.e[k]), js.c[k]), esc = cscale(cconj(js.e[k]), js.s[k]);
-        cx<double> np_ = csub(cmul(ecc, xp), cscale(xq, js.s[k]));
-        cx<double> nq_ = cadd(cmul(esc, xp), cscale(xq, js.c[k]));
-        if (r == q) {
-          np_ = {0.0, 0.0};
-          nq_.y = 0.0;
-        } else if (r == p) {
-          nq_ = {0.0, 0.0};
-          np_.y = 0.0;
+      const int nblk = npair * npair;
+      for (int it = tid; it < nblk + n * npair; it += NT) {
+        if (it < nblk) {  // block (a, b) of R^H A R into A2
+          const int ka = it / npair, kb = it % npair;
+          const int pa = js.p[ka], qa = js.q[ka], pb = js.p[kb], qb = js.q[kb];
+          const double ca = js.c[ka], sa = js.s[ka], cb = js.c[kb], sb = js.s[kb];
+          const cx<double> eca = cscale(js.e[ka], ca), esa = cscale(js.e[ka], sa);
+          const cx<double> ecb = cscale(js.e[kb], cb), esb = cscale(js.e[kb], sb);
+          const cx<double> z = {0.0, 0.0};
+          // rows pa, qa of A, columns pb, qb (absent partners read as zero)
+          const cx<double> x00 = A[pa * n + pb];
+          const cx<double> x01 = qb >= 0 ? A[pa * n + qb] : z;
+          const cx<double> x10 = qa >= 0 ? A[qa * n + pb] : z;
+          const cx<double> x11 = qa >= 0 && qb >= 0 ? A[qa * n + qb] : z;
+          // columns: [xp xq] R_b
+          const cx<double> y00 = csub(cmul(x00, ecb), cscale(x01, sb));
+          const cx<double> y01 = cadd(cmul(x00, esb), cscale(x01, cb));
+          const cx<double> y10 = csub(cmul(x10, ecb), cscale(x11, sb));
+          const cx<double> y11 = cadd(cmul(x10, esb), cscale(x11, cb));
+          // rows: R_a^H
+          cx<double> n00 = csub(cmul(cconj(eca), y00), cscale(y10, sa));
+          cx<double> n01 = csub(cmul(cconj(eca), y01), cscale(y11, sa));
+          cx<double> n10 = cadd(cmul(cconj(esa), y00), cscale(y10, ca));
+          cx<double> n11 = cadd(cmul(cconj(esa), y01), cscale(y11, ca));
+          if (ka == kb && qa >= 0 && sa != 0.0) {  // the rotated pair: diagonal 2x2 block
+            n01 = z;
+            n10 = z;
+            n00.y = 0.0;
+            n11.y = 0.0;
+          }
+          A2[pa * n + pb] = n00;
+          if (qb >= 0) A2[pa * n + qb] = n01;
+          if (qa >= 0) A2[qa * n + pb] = n10;
+          if (qa >= 0 && qb >= 0) A2[qa * n + qb] = n11;
+        } else {  // V <- V R, row r, pair k
+          const int e2 = it - nblk;
+          const int r = e2 / npair, k = e2 % npair;
+          const int p = js.p[k], q = js.q[k];
+          if (q >= 0 && js.s[k] != 0.0) {
+            const cx<double> xp = V[r * n + p], xq = V[r * n + q];
+            const cx<double> ec = cscale(js.e[k], js.c[k]), es = cscale(js.e[k], js.s[k]);
+            V[r * n + p] = csub(cmul(xp, ec), cscale(xq, js.s[k]));
+            V[r * n + q] = cadd(cmul(xp, es), cscale(xq, js.c[k]));
+          }
         }
-        A[p * n + r] = np_;
-        A[q * n + r] = nq_;
       }
       __syncthreads();
+      cx<double>* sw = A;
+      A = A2;
+      A2 = sw;
     }
   }
   for (int i = tid; i < n; i += NT) js.ev[i] = A[i * n + i].x;
@@ -369,7 +386,7 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
     __syncthreads();
   } else {
     load_lower_herm<NP, NT>(B0, d.S, n);
-    jacobi_eigh<NP, NT>(B0, B1, n, js, false, B2);
+    jacobi_eigh<NP, NT>(B0, B3, B1, n, js, false, B2);
     __shared__ double fac[NMAXJ];
     if (tid == 0) {
       double wmax = js.ev[0];
@@ -395,7 +412,7 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
   cx<double>* U = Urho;
   if (a.tdiag && d.Lam) {
     load_lower_herm<NP, NT>(B0, d.Lam, n);
-    const int sw = jacobi_eigh<NP, NT>(B0, B1, n, js, true, B2);
+    const int sw = jacobi_eigh<NP, NT>(B0, B3, B1, n, js, true, B2);
     if (a.stats && tid == 0) a.stats[iq] = sw;
     mm<NP, NT, false>(B3, Urho, B1, n);
     U = B3;

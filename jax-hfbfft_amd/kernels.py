@@ -232,6 +232,30 @@ def rotate(A, U1, U2, out1, out2):
     check(lib().hfb_rotate(prec, ptr(A), ptr(U1), ptr(U2), ptr(out1), ptr(out2), nst, K, stream_handle()), "rotate")
 
 
+def rotate2(blocks, K):
+    """rotate for both isospin blocks in one launch; blocks[iq] = (A, U1, U2,
+    out1, out2) or None for an empty block.  K elements per state."""
+    arrs = [(c_void_p * 2)() for _ in range(5)]
+    ns = (ctypes.c_int * 2)()
+    prec = None
+    for iq, blk in enumerate(blocks):
+        if blk is None:
+            continue
+        A = blk[0]
+        p = _lib.prec_of(A.dtype)
+        if prec is not None and p != prec:
+            raise ValueError("both isospin blocks need the same precision")
+        prec = p
+        if A[0].numel() != K:
+            raise ValueError("block length differs from K")
+        ns[iq] = A.shape[0]
+        for a, t in zip(arrs, blk):
+            a[iq] = t.data_ptr() if t is not None else None
+    if prec is None:
+        return
+    check(lib().hfb_rotate2(prec, *arrs, ns, K, stream_handle()), "rotate2")
+
+
 def diag_small(nst0, nst1, nmax, mats, tdiag, ipair, tbcs, construct, levels, static, E, uo, work=None):
     """Loewdin, optional diagonalisation and HFB matrices for both isospins."""
     m = (c_void_p * 8)(*[(t.data_ptr() if t is not None else None) for t in mats])

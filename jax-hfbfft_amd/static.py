@@ -220,9 +220,11 @@ def diagstep(energies, forces, grids, levels, static, diagonalize=False, constru
                        energies.vec, uo, work)
     # psi <- psi U and lagrange <- psi M2 in place (every workgroup reads its
     # columns of psi before it writes them)
+    rot = []
     for iq, (a, b) in enumerate(((0, n0), (n0, n0 + n1))):
-        if b > a:
-            kernels.rotate(levels.psi[a:b], uo[2 * iq], uo[2 * iq + 1], levels.psi[a:b], levels.lagrange[a:b])
+        rot.append((levels.psi[a:b], uo[2 * iq], uo[2 * iq + 1], levels.psi[a:b], levels.lagrange[a:b])
+                   if b > a else None)
+    kernels.rotate2(rot, levels.psi[0].numel())
     return energies, levels, static
 
 

@@ -170,3 +170,34 @@ def test_rotate_matches_matmul(pkg, dev, nst, dtype):
         r = (p64.T @ U).T
         tol = 1e-12 if dtype == torch.complex128 else 1e-6
         assert np.abs(got.cpu().numpy() - r).max() <= tol * np.abs(r).max()
+
+
+@pytest.mark.parametrize("nsts", [(20, 20), (36, 30), (126, 82), (12, 0)])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_rotate2_matches_matmul(pkg, dev, nsts, dtype):
+    """Both isospin blocks in one launch, in place, against numpy."""
+    rng = np.random.default_rng(sum(nsts))
+    K = 2 * 24 ** 3
+    blocks, refs = [], []
+    for nst in nsts:
+        if nst == 0:
+            blocks.append(None)
+            refs.append(None)
+            continue
+        psi = rng.standard_normal((nst, K)) + 1j * rng.standard_normal((nst, K))
+        U1 = rng.standard_normal((nst, nst)) + 1j * rng.standard_normal((nst, nst))
+        U2 = rng.standard_normal((nst, nst)) + 1j * rng.standard_normal((nst, nst))
+        tp = torch.tensor(psi, dtype=dtype, device=dev)
+        out2 = torch.empty_like(tp)
+        blocks.append((tp, torch.tensor(U1, dtype=torch.complex128, device=dev),
+                       torch.tensor(U2, dtype=torch.complex128, device=dev), tp, out2))
+        p64 = torch.tensor(psi, dtype=dtype).numpy().astype(np.complex128)
+        refs.append(((p64.T @ U1).T, (p64.T @ U2).T))
+    pkg.kernels.rotate2(blocks, K)
+    torch.cuda.synchronize()
+    tol = 1e-12 if dtype == torch.complex128 else 1e-6
+    for blk, ref in zip(blocks, refs):
+        if blk is None:
+            continue
+        for got, r in ((blk[3], ref[0]), (blk[4], ref[1])):
+            assert np.abs(got.cpu().numpy() - r).max() <= tol * np.abs(r).max()
