@@ -69,6 +69,19 @@ def hpsi_bytes(S, nvox, c, r):
     return S * 2 * nvox * c * 3 + 2 * 12 * nvox * r
 
 
+def measured_traffic(workload):
+    """HBM bytes of one hfb_hpsi call (FP32) measured with rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE and scaled by a plain copy, as recorded in
+    profiles/hpsi_<workload>_traffic.md (line `hpsi_traffic_bytes = ...`);
+    None when that workload has no such record."""
+    import re
+    f = ROOT / "profiles" / f"hpsi_{workload.lower()}_traffic.md"
+    if not f.exists():
+        return None
+    m = re.search(r"hpsi_traffic_bytes\s*=\s*([0-9]+)", f.read_text())
+    return int(m.group(1)) if m else None
+
+
 def cpu_baseline(cfg, iters):
     sys.path.insert(0, str(ROOT))
     from oracle import hfb_oracle as O
@@ -156,7 +169,7 @@ def main():
     hp_avg = sum(hp) / max(1, len(hp))
     ach = hpsi_bytes(S, nvox, 8, 4) / (hp_avg * 1e-3) / 1e9 if hp else None
     roof = {"bound": "hbm", "kernel": "hpsi (hpsi_lane_kernel x3, FP32)", "achieved": ach, "peak": 8000.0,
-            "unit": "GB/s", "frac": (ach / 8000.0) if ach else None, "traffic": None,
+            "unit": "GB/s", "frac": (ach / 8000.0) if ach else None, "traffic": measured_traffic(args.workload),
             "avg_launch_ms": hp_avg, "launches": len(hp),
             "algorithmic_bytes": hpsi_bytes(S, nvox, 8, 4)}
     cpu = None

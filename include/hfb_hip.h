@@ -247,18 +247,24 @@ int hfb_sp_finish(hfb_ctx* ctx, const double* spart, int nst0, int nst1, double 
  * Potentials and energies (meanfield.py:163-391, energies.py:90-371)
  * --------------------------------------------------------------------- */
 
+/* Doubles of the workspace of hfb_density_derivs and hfb_meanfield_derivs
+ * (the terms of the multi-term fields, summed in term order afterwards). */
+long long hfb_deriv_workspace(hfb_ctx* ctx);
+
 /* Derived density fields shared by skyrme and integ_energy, each per
  * isospin: divj = div sodens (meanfield.py:195-200), lap = Laplacian rho
  * (meanfield.py:227-231), grad = grad rho (meanfield.py:262-268), curls =
  * curl sdens (meanfield.py:279-289), curlj = curl current
- * (meanfield.py:304-314), with the sder/sder2 matrices of grids.py:36-63. */
+ * (meanfield.py:304-314), with the sder/sder2 matrices of grids.py:36-63.
+ * work: hfb_deriv_workspace doubles. */
 int hfb_density_derivs(hfb_ctx* ctx, const double* rho, const double* sdens, const double* current,
                        const double* sodens, double* divj, double* lap, double* grad, double* curls,
-                       double* curlj, void* stream);
+                       double* curlj, double* work, void* stream);
 
-/* divaq = div aq and dbmass = grad bmass (meanfield.py:329-344). */
+/* divaq = div aq and dbmass = grad bmass (meanfield.py:329-344); work as
+ * for hfb_density_derivs. */
 int hfb_meanfield_derivs(hfb_ctx* ctx, const double* aq, const double* bmass, double* divaq, double* dbmass,
-                         void* stream);
+                         double* work, void* stream);
 
 /* Doubles of the epart scratch of hfb_skyrme (x20 for hfb_integ_energy). */
 int hfb_skyrme_nblocks(hfb_ctx* ctx);

@@ -58,8 +58,9 @@ _SIGS = {
                                    c_int, c_double, c_void_p]),
     "hfb_sp_finish": (c_int, [c_void_p, c_void_p, c_int, c_int, c_double, c_double, c_void_p, c_void_p, c_void_p,
                               c_void_p, c_void_p]),
-    "hfb_density_derivs": (c_int, [c_void_p] + [c_void_p] * 9 + [c_void_p]),
-    "hfb_meanfield_derivs": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "hfb_deriv_workspace": (ctypes.c_longlong, [c_void_p]),
+    "hfb_density_derivs": (c_int, [c_void_p] + [c_void_p] * 9 + [c_void_p, c_void_p]),
+    "hfb_meanfield_derivs": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hfb_skyrme_nblocks": (c_int, [c_void_p]),
     "hfb_skyrme": (c_int, [c_void_p, c_void_p] + [c_void_p] * 11 + [c_void_p] * 6 + [c_void_p] * 6 +
                    [c_void_p, c_void_p, c_void_p]),

@@ -395,8 +395,9 @@ int launch_rotate(int prec, RotArgs ra, int niso, hipStream_t st) {
   }
   if (nmax == 0) return HFB_OK;
   const size_t csz = prec == HFB_F64 ? sizeof(cx<double>) : sizeof(cx<float>);
-  // two k per lane while the staged tile fits in LDS, else one
-  const int kpl = csz * (size_t)nmax * 128 <= 150 * 1024 ? 2 : 1;
+  // two k per lane while the staged tile stays within 32 KB (several
+  // workgroups per CU), else one
+  const int kpl = csz * (size_t)nmax * 128 <= 32 * 1024 ? 2 : 1;
   const size_t lds = csz * (size_t)nmax * 64 * kpl;
   if (lds > 150 * 1024) return HFB_E_SIZE;
   const int nb = (int)((ra.K + 64 * kpl - 1) / (64 * kpl));

@@ -134,8 +134,28 @@ __device__ __forceinline__ void mm(cx<double>* __restrict__ C, const cx<double>*
       ic[t] = i < n ? i : n - 1;
       acc[t] = {0.0, 0.0};
     }
-#pragma unroll 2
-    for (int k = 0; k < n; ++k) {
+    // operands of KC consecutive k are loaded before their FMAs, so the
+    // loads of a chunk are in flight together
+    constexpr int KC = NP <= 64 ? 4 : 1;
+    int k = 0;
+    for (; k + KC <= n; k += KC) {
+      cx<double> b[KC], a[KC][TC];
+#pragma unroll
+      for (int c = 0; c < KC; ++c) {
+        b[c] = B[(k + c) * n + jc];
+#pragma unroll
+        for (int t = 0; t < TC; ++t)
+          a[c][t] = CONJ_A ? cconj(A[(k + c) * n + ic[t]]) : A[ic[t] * n + k + c];
+      }
+#pragma unroll
+      for (int c = 0; c < KC; ++c)
+#pragma unroll
+        for (int t = 0; t < TC; ++t) {
+          acc[t].x = fma(a[c][t].x, b[c].x, fma(-a[c][t].y, b[c].y, acc[t].x));
+          acc[t].y = fma(a[c][t].x, b[c].y, fma(a[c][t].y, b[c].x, acc[t].y));
+        }
+    }
+    for (; k < n; ++k) {
       const cx<double> b = B[k * n + jc];
 #pragma unroll
       for (int t = 0; t < TC; ++t) {

@@ -26,9 +26,15 @@ struct DerivJob {
 };
 
 constexpr int MAXJOB = 24;
+constexpr int MAXTERM = 3 * MAXJOB;
 struct DerivJobs {
   DerivJob j[MAXJOB];
   int njob;
+  // every term of every job, in job order: job index, term index, and the
+  // slot of the term in the workspace (-1: single-term job, written to dst)
+  int tjob[MAXTERM], tidx[MAXTERM], tslot[MAXTERM];
+  int nterm_all;
+  int slot0[MAXJOB];  // workspace slot of term 0 of a multi-term job
 };
 
 __global__ void __launch_bounds__(256) deriv_kernel(DerivJobs jobs, int nx, int ny, int nz, const double* __restrict__ d1x,
@@ -95,16 +101,22 @@ __device__ __forceinline__ int64_t pencil_off(int ax, int k, int gp) {
   return (int64_t)gp * N + k;
 }
 
+// One launch for every term of every job: block (tile, term) computes
+// coef * D src along the term's axis for 64 pencils and writes it to its
+// workspace slot (or to dst for a single-term job); deriv_sum_kernel then
+// adds the terms of each job in term order.
 template <int N>
-__global__ void __launch_bounds__(256) deriv_tile_kernel(DerivJobs jobs, int term, DerivMats dm) {
+__global__ void __launch_bounds__(256) deriv_tile_kernel(DerivJobs jobs, DerivMats dm, double* __restrict__ work) {
   __shared__ double sS[N * DT_LD];
-  const DerivJob& jb = jobs.j[blockIdx.y];
-  if (term >= jb.nterm) return;
+  const int tg = blockIdx.y;
+  const DerivJob& jb = jobs.j[jobs.tjob[tg]];
+  const int term = jobs.tidx[tg];
+  const int slot = jobs.tslot[tg];
   const int ax = jb.axis[term];
   const double* __restrict__ src = jb.src[term];
   const double* __restrict__ D = jb.order[term] == 1 ? dm.d1[ax] : dm.d2[ax];
   const double coef = jb.coef[term];
-  double* __restrict__ dst = jb.dst;
+  double* __restrict__ dst = slot < 0 ? jb.dst : work + (int64_t)slot * N * N * N;
   const int p0 = blockIdx.x * DT_P;
   // element (k, p) of the tile; consecutive threads walk the contiguous direction
   auto kp = [&](int e, int& k, int& p) {
@@ -141,9 +153,20 @@ __global__ void __launch_bounds__(256) deriv_tile_kernel(DerivJobs jobs, int ter
   for (int e = threadIdx.x; e < N * DT_P; e += 256) {
     int k, pp;
     kp(e, k, pp);
-    const int64_t off = pencil_off<N>(ax, k, p0 + pp);
-    const double v = coef * sS[k * DT_LD + pp];
-    dst[off] = term == 0 ? v : dst[off] + v;
+    dst[pencil_off<N>(ax, k, p0 + pp)] = coef * sS[k * DT_LD + pp];
+  }
+}
+
+// dst = term 0 + term 1 (+ term 2) for every multi-term job, in that order.
+__global__ void __launch_bounds__(256) deriv_sum_kernel(DerivJobs jobs, const double* __restrict__ work,
+                                                        int64_t nvox) {
+  const DerivJob& jb = jobs.j[blockIdx.y];
+  if (jb.nterm < 2) return;
+  const double* w0 = work + (int64_t)jobs.slot0[blockIdx.y] * nvox;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nvox; e += (int64_t)gridDim.x * blockDim.x) {
+    double v = w0[e] + w0[nvox + e];
+    if (jb.nterm == 3) v += w0[2 * nvox + e];
+    jb.dst[e] = v;
   }
 }
 
@@ -394,20 +417,41 @@ void push(DerivJobs& J, double* dst, int nterm, const double* s0, int a0, int o0
   j.src[2] = s2; j.axis[2] = a2; j.order[2] = o2; j.coef[2] = c2;
 }
 
-int run_jobs(const hfb_ctx* c, const DerivJobs& J, hipStream_t st) {
+// Workspace slots used by a job list (terms of its multi-term jobs).
+int deriv_slots(const DerivJobs& J) {
+  int n = 0;
+  for (int j = 0; j < J.njob; ++j)
+    if (J.j[j].nterm > 1) n += J.j[j].nterm;
+  return n;
+}
+
+int run_jobs(const hfb_ctx* c, DerivJobs& J, double* work, hipStream_t st) {
   if (c->nx == c->ny && c->ny == c->nz) {
+    if (!work && deriv_slots(J) > 0) return HFB_E_ARG;
     DerivMats dm{{c->der1[0], c->der1[1], c->der1[2]}, {c->der2[0], c->der2[1], c->der2[2]}};
-    int nterm = 0;
-    for (int j = 0; j < J.njob; ++j) nterm = J.j[j].nterm > nterm ? J.j[j].nterm : nterm;
-    const int n = c->nx;
-    for (int t = 0; t < nterm; ++t) {
-      HFB_DISPATCH_N(n, {
-        hipLaunchKernelGGL(deriv_tile_kernel<N_>, dim3(N_ * N_ / DT_P, J.njob), dim3(256), 0, st, J, t, dm);
-      })
-      const int rc = launch_status();
-      if (rc != HFB_OK) return rc;
+    int nt = 0, slot = 0;
+    for (int j = 0; j < J.njob; ++j) {
+      const bool multi = J.j[j].nterm > 1;
+      J.slot0[j] = multi ? slot : -1;
+      for (int t = 0; t < J.j[j].nterm; ++t) {
+        J.tjob[nt] = j;
+        J.tidx[nt] = t;
+        J.tslot[nt] = multi ? slot++ : -1;
+        ++nt;
+      }
     }
-    return HFB_OK;
+    J.nterm_all = nt;
+    const int n = c->nx;
+    const int64_t nvox = (int64_t)n * n * n;
+    HFB_DISPATCH_N(n, {
+      hipLaunchKernelGGL(deriv_tile_kernel<N_>, dim3(N_ * N_ / DT_P, nt), dim3(256), 0, st, J, dm, work);
+    })
+    int rc = launch_status();
+    if (rc != HFB_OK || slot == 0) return rc;
+    int gx = (int)((nvox + 255) / 256);
+    if (gx > 256) gx = 256;
+    hipLaunchKernelGGL(deriv_sum_kernel, dim3(gx, J.njob), dim3(256), 0, st, J, (const double*)work, nvox);
+    return launch_status();
   }
   const int64_t nvox = (int64_t)c->nx * c->ny * c->nz;
   const int gx = (int)((nvox + 255) / 256);
@@ -424,9 +468,14 @@ int run_jobs(const hfb_ctx* c, const DerivJobs& J, hipStream_t st) {
 //   grad (2,3,n3) = grad rho   (meanfield.py:262-268)
 //   curls(2,3,n3) = curl sdens (meanfield.py:279-289)
 //   curlj(2,3,n3) = curl current (meanfield.py:304-314, energies.py:196-209)
+extern "C" long long hfb_deriv_workspace(hfb_ctx* c) {
+  if (!c) return HFB_E_NOCTX;
+  return 36LL * c->nx * c->ny * c->nz;  // the 36 terms of the multi-term jobs of hfb_density_derivs
+}
+
 extern "C" int hfb_density_derivs(hfb_ctx* c, const double* rho, const double* sdens, const double* current,
                                   const double* sodens, double* divj, double* lap, double* grad, double* curls,
-                                  double* curlj, void* stream) {
+                                  double* curlj, double* work, void* stream) {
   if (!c) return HFB_E_NOCTX;
   if (!rho || !sdens || !current || !sodens || !divj || !lap || !grad || !curls || !curlj) return HFB_E_ARG;
   const int64_t n = (int64_t)c->nx * c->ny * c->nz;
@@ -447,13 +496,13 @@ extern "C" int hfb_density_derivs(hfb_ctx* c, const double* rho, const double* s
     push(J, curlj + (iq * 3 + 1) * n, 2, cu + 0 * n, 2, 1, 1.0, cu + 2 * n, 0, 1, -1.0);
     push(J, curlj + (iq * 3 + 2) * n, 2, cu + 1 * n, 0, 1, 1.0, cu + 0 * n, 1, 1, -1.0);
   }
-  return run_jobs(c, J, (hipStream_t)stream);
+  return run_jobs(c, J, work, (hipStream_t)stream);
 }
 
 // divaq = div aq, dbmass = grad bmass (meanfield.py:329-344); computed for
 // completeness of the Meanfield state (hpsi01 does not read them).
 extern "C" int hfb_meanfield_derivs(hfb_ctx* c, const double* aq, const double* bmass, double* divaq, double* dbmass,
-                                    void* stream) {
+                                    double* work, void* stream) {
   if (!c) return HFB_E_NOCTX;
   if (!aq || !bmass || !divaq || !dbmass) return HFB_E_ARG;
   const int64_t n = (int64_t)c->nx * c->ny * c->nz;
@@ -463,7 +512,7 @@ extern "C" int hfb_meanfield_derivs(hfb_ctx* c, const double* aq, const double* 
     push(J, divaq + iq * n, 3, A, 0, 1, 1.0, A + n, 1, 1, 1.0, A + 2 * n, 2, 1, 1.0);
     for (int a = 0; a < 3; ++a) push(J, dbmass + (iq * 3 + a) * n, 1, bmass + iq * n, a, 1, 1.0);
   }
-  return run_jobs(c, J, (hipStream_t)stream);
+  return run_jobs(c, J, work, (hipStream_t)stream);
 }
 
 // Potentials of meanfield.py:163-391 from densities + derived fields +

@@ -313,15 +313,22 @@ EN = {"EHFT": 0, "EHF0": 1, "EHF1": 2, "EHF2": 3, "EHF3": 4, "EHFLS": 5, "EHFLSO
 EN_LEN = 48
 
 
+def _deriv_work(ctx):
+    n = lib().hfb_deriv_workspace(ctx.handle)
+    if n < 0:
+        check(int(n), "deriv_workspace")
+    return ctx.work(("deriv_work",), (n,), torch.float64)
+
+
 def density_derivs(ctx, dens, der):
     check(lib().hfb_density_derivs(ctx.handle, ptr(dens.rho), ptr(dens.sdens), ptr(dens.current), ptr(dens.sodens),
                                    ptr(der["divj"]), ptr(der["lap"]), ptr(der["grad"]), ptr(der["curls"]),
-                                   ptr(der["curlj"]), stream_handle()), "density_derivs")
+                                   ptr(der["curlj"]), ptr(_deriv_work(ctx)), stream_handle()), "density_derivs")
 
 
 def meanfield_derivs(ctx, mf):
     check(lib().hfb_meanfield_derivs(ctx.handle, ptr(mf.aq), ptr(mf.bmass), ptr(mf.divaq), ptr(mf.dbmass),
-                                     stream_handle()), "meanfield_derivs")
+                                     ptr(_deriv_work(ctx)), stream_handle()), "meanfield_derivs")
 
 
 def skyrme(ctx, fc, dens, der, wcoul, mf, E, f32=None):

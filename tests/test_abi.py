@@ -19,7 +19,7 @@ HEADER = ROOT / "include" / "hfb_hip.h"
 
 def header_entries():
     text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int)\s+(hfb_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|long\s+long|int)\s+(hfb_\w+)\s*\(", text, flags=re.M)))
 
 
 def header_enum(name):
