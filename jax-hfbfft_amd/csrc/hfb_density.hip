@@ -11,14 +11,15 @@
 // (Nyquist zeroed, cderv*00's d1).  Density contributions accumulate in
 // registers over the wavefront's states and are combined over the four
 // wavefronts in LDS; the chunks are summed by hfb_density_finish, so no
-// per-state field is ever written to HBM.  The x pass also adds the
-// pointwise rho, chi, spin density and the per-state spin and parity sums.
+// per-state field is ever written to HBM.  The pointwise fields are spread
+// over the passes to even out their register load: the y pass adds rho and
+// chi, the z pass the spin density, the per-state spin sums and the parity.
 //
 // Density partials (double) go per chunk and axis into slots of
 // `dout` = [DENS_G chunk][2 iso][17 slot][nvox]:
-//   0 rho 1 chi 2 sx 3 sy 4 sz 5 tau_x 6 j_x 7 Jy(x) 8 Jz(x)     (x pass)
-//   9 tau_y 10 j_y 11 Jx(y) 12 Jz(y)                              (y pass)
-//   13 tau_z 14 j_z 15 Jx(z) 16 Jy(z)                             (z pass)
+//   5 tau_x 6 j_x 7 Jy(x) 8 Jz(x)                              (x pass)
+//   9 tau_y 10 j_y 11 Jx(y) 12 Jz(y)  0 rho 1 chi                (y pass)
+//   13 tau_z 14 j_z 15 Jx(z) 16 Jy(z)  2 sx 3 sy 4 sz            (z pass)
 // and combined by hfb_density_finish (with the linear mixing of
 // static.py:1817-1823 when requested).  Per-state sums go to
 // spart = [3 axis][S][ntile_a][8]: kin, Lx, Ly, Lz, sx, sy, sz, parity.
@@ -67,7 +68,7 @@ __host__ __device__ __forceinline__ int pencils_axis(int ax, int nx, int ny, int
 template <typename T, int N, int AXIS>
 __global__ void __launch_bounds__(256) dens_lane_kernel(DensArgs a, const cx<T>* __restrict__ tw) {
   constexpr int L = DENS_L, N1 = N / L, PW = DENS_PW;
-  constexpr int NACC = AXIS == 0 ? 9 : 4;
+  constexpr int NACC = AXIS == 0 ? 4 : (AXIS == 1 ? 6 : 7);
   __shared__ double comb[4][PW * N];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = lane % L, q = lane / L;
@@ -111,6 +112,7 @@ __global__ void __launch_bounds__(256) dens_lane_kernel(DensArgs a, const cx<T>*
 #pragma unroll
     for (int k = 0; k < NACC; ++k) acc[i][k] = 0.0;
 
+#pragma unroll 1
   for (int sl = cs + wave; sl < ce; sl += 4) {
     const int s = start + sl;
     const cx<T>* p0 = psi + (int64_t)(2 * s) * nvox;
@@ -163,20 +165,32 @@ __global__ void __launch_bounds__(256) dens_lane_kernel(DensArgs a, const cx<T>*
       const double ia = im_ab(cx0, cy0, ax, ay) + im_ab(cx1, cy1, bx, by);  // Im(d psi conj psi)
       acc[m][0] += w * tau;
       acc[m][1] += w * ia;
+      const double n0 = ax * ax + ay * ay, n1 = bx * bx + by * by;
       if constexpr (AXIS == 0) {
         // densities.py:270-277
         acc[m][2] += -w * (im_ab(cx0, cy0, ax, ay) - im_ab(cx1, cy1, bx, by));   // J_y
         acc[m][3] += -w * (re_ab(ax, ay, cx1, cy1) - re_ab(bx, by, cx0, cy0));   // J_z
-        const double n0 = ax * ax + ay * ay, n1 = bx * bx + by * by;
-        const double rsx = ax * bx + ay * by;  // Re(conj psi0 psi1)
-        const double rsy = ax * by - ay * bx;  // Im(conj psi0 psi1)
-        acc[m][4] += w * (n0 + n1);
-        acc[m][5] += 0.5 * wuv * (n0 + n1);
-        acc[m][6] += 2.0 * w * rsx;
-        acc[m][7] += 2.0 * w * rsy;
-        acc[m][8] += w * (n0 - n1);
         sp[2] += zc * ia;
         sp[3] -= yc * ia;
+      } else if constexpr (AXIS == 1) {
+        // densities.py:289-296
+        acc[m][2] += w * (im_ab(cx0, cy0, ax, ay) - im_ab(cx1, cy1, bx, by));    // J_x
+        acc[m][3] += -w * (im_ab(cx1, cy1, ax, ay) + im_ab(cx0, cy0, bx, by));   // J_z
+        acc[m][4] += w * (n0 + n1);                                                // rho
+        acc[m][5] += 0.5 * wuv * (n0 + n1);                                        // chi
+        sp[1] -= zc * ia;
+        sp[3] += xc * ia;
+      } else {
+        // densities.py:308-315
+        acc[m][2] += w * (re_ab(cx1, cy1, ax, ay) - re_ab(cx0, cy0, bx, by));    // J_x
+        acc[m][3] += w * (im_ab(cx1, cy1, ax, ay) + im_ab(cx0, cy0, bx, by));    // J_y
+        const double rsx = ax * bx + ay * by;  // Re(conj psi0 psi1)
+        const double rsy = ax * by - ay * bx;  // Im(conj psi0 psi1)
+        acc[m][4] += 2.0 * w * rsx;
+        acc[m][5] += 2.0 * w * rsy;
+        acc[m][6] += w * (n0 - n1);
+        sp[1] += yc * ia;
+        sp[2] -= xc * ia;
         sp[4] += 2.0 * rsx;
         sp[5] += 2.0 * rsy;
         sp[6] += n0 - n1;
@@ -184,18 +198,6 @@ __global__ void __launch_bounds__(256) dens_lane_kernel(DensArgs a, const cx<T>*
         const int64_t mo = ((int64_t)(a.nx - 1 - x) * a.ny + (a.ny - 1 - y)) * a.nz + (a.nz - 1 - z);
         const cx<T> f0 = p0[mo], f1 = p1[mo];
         sp[7] += ax * double(f0.x) + ay * double(f0.y) + bx * double(f1.x) + by * double(f1.y);
-      } else if constexpr (AXIS == 1) {
-        // densities.py:289-296
-        acc[m][2] += w * (im_ab(cx0, cy0, ax, ay) - im_ab(cx1, cy1, bx, by));    // J_x
-        acc[m][3] += -w * (im_ab(cx1, cy1, ax, ay) + im_ab(cx0, cy0, bx, by));   // J_z
-        sp[1] -= zc * ia;
-        sp[3] += xc * ia;
-      } else {
-        // densities.py:308-315
-        acc[m][2] += w * (re_ab(cx1, cy1, ax, ay) - re_ab(cx0, cy0, bx, by));    // J_x
-        acc[m][3] += w * (im_ab(cx1, cy1, ax, ay) + im_ab(cx0, cy0, bx, by));    // J_y
-        sp[1] += yc * ia;
-        sp[2] -= xc * ia;
       }
     }
     if (a.want_sp) {  // the wavefront covers this tile's pencils for state s
@@ -216,7 +218,6 @@ __global__ void __launch_bounds__(256) dens_lane_kernel(DensArgs a, const cx<T>*
   }
   if (!a.want_dens) return;
   // combine the four wavefronts' accumulators in LDS, slot by slot
-  const int slot0 = AXIS == 0 ? 0 : (AXIS == 1 ? 9 : 13);
 #pragma unroll
   for (int k = 0; k < NACC; ++k) {
     __syncthreads();
@@ -225,9 +226,11 @@ __global__ void __launch_bounds__(256) dens_lane_kernel(DensArgs a, const cx<T>*
     __syncthreads();
     int slot;
     if constexpr (AXIS == 0)
-      slot = k < 4 ? 5 + k : k - 4;
+      slot = 5 + k;
+    else if constexpr (AXIS == 1)
+      slot = k < 4 ? 9 + k : k - 4;  // rho 0, chi 1
     else
-      slot = slot0 + k;
+      slot = k < 4 ? 13 + k : k - 2;  // sx 2, sy 3, sz 4
     double* dst = a.dout + (((int64_t)chunk * 2 + iq) * NSLOT + slot) * nvox;
     for (int e = threadIdx.x; e < PW * N; e += 256) {
       int qq, kk;

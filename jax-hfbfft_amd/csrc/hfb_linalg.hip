@@ -57,6 +57,20 @@ struct DiagArgs {
 #define HFB_TILE_LOOP(NP, NT, n, i, j)                               \
   for (int j = threadIdx.x % (NP), i = threadIdx.x / (NP); j < (n) && i < (n); i += (NT) / (NP))
 
+// 1/sqrt(x) and 1/x for x > 0: the hardware estimates refined by two
+// Newton steps to double precision (a few ulp), far shorter chains than
+// the correctly rounded library sqrt and division.
+__device__ __forceinline__ double rsq_step(double x, double y) { return y * fma(-0.5 * x * y, y, 1.5); }
+__device__ __forceinline__ double rsq_full(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  return rsq_step(x, rsq_step(x, y));
+}
+__device__ __forceinline__ double rcp_full(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = y * fma(-x, y, 2.0);
+  return y * fma(-x, y, 2.0);
+}
+
 template <int NP, int NT, typename F>
 __device__ __forceinline__ void tile_for(int n, F&& f) {
   constexpr int RG = NT / NP;
@@ -136,7 +150,7 @@ __device__ __forceinline__ void mm(cx<double>* __restrict__ C, const cx<double>*
     }
     // operands of KC consecutive k are loaded before their FMAs, so the
     // loads of a chunk are in flight together
-    constexpr int KC = NP <= 64 ? 4 : 1;
+    constexpr int KC = NP > 64 ? 1 : (TC >= 8 ? 2 : 4);
     int k = 0;
     for (; k + KC <= n; k += KC) {
       cx<double> b[KC], a[KC][TC];
@@ -214,22 +228,30 @@ __device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* A2, cx<dou
     if (off <= 1e-30 * dia || off == 0.0) break;
     for (int t = 0; t < m - 1; ++t) {
       if (tid < npair) {
+        // round-robin pairs: index 0 fixed, the others rotate by t
         const int k = tid;
-        const int a0 = (k == 0) ? 0 : ((k - 1 + t) % (m - 1)) + 1;
-        const int kb = m - 1 - k;
-        const int b0 = ((kb - 1 + t) % (m - 1)) + 1;
+        const int m1 = m - 1;
+        int xa = k - 1 + t;  // < 2 (m - 1)
+        xa = xa >= m1 ? xa - m1 : xa;
+        int xb = m1 - k - 1 + t;
+        xb = xb >= m1 ? xb - m1 : xb;
+        const int a0 = (k == 0) ? 0 : xa + 1;
+        const int b0 = xb + 1;
         int p = a0 < b0 ? a0 : b0, q = a0 < b0 ? b0 : a0;
         double c = 1.0, sn = 0.0;
         cx<double> ea = {1.0, 0.0};
         if (q < n) {
           const double app = A[p * n + p].x, aqq = A[q * n + q].x;
           const cx<double> apq = A[p * n + q];
-          const double r = sqrt(cabs2(apq));
-          if (r > 1e-300 && r > 1e-18 * (fabs(app) + fabs(aqq))) {
-            ea = {apq.x / r, apq.y / r};
-            const double th = (aqq - app) / (2.0 * r);
-            const double tt = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
-            c = 1.0 / sqrt(1.0 + tt * tt);
+          const double r2 = cabs2(apq), den = fabs(app) + fabs(aqq);
+          if (r2 > 0.0 && r2 > 1e-36 * den * den) {
+            const double ri = rsq_full(r2);  // 1 / |a_pq|
+            ea = {apq.x * ri, apq.y * ri};
+            const double th = 0.5 * (aqq - app) * ri;
+            const double h = 1.0 + th * th;
+            const double rt = h * rsq_full(h);  // sqrt(1 + th^2)
+            const double tt = (th >= 0.0 ? 1.0 : -1.0) * rcp_full(fabs(th) + rt);
+            c = rsq_full(1.0 + tt * tt);
             sn = tt * c;
           }
         } else {
@@ -580,9 +602,9 @@ extern "C" int hfb_diag_small(int nst0, int nst1, int nmax, const double* const*
     if (nbig <= 16)
       HFB_DIAG(16, 256, lds)
     else if (nbig <= 32)
-      HFB_DIAG(32, 256, lds)
+      HFB_DIAG(32, 512, lds)
     else
-      HFB_DIAG(64, 256, lds)
+      HFB_DIAG(64, 512, lds)
   } else {
     HFB_DIAG(128, 1024, 0)
   }

@@ -123,7 +123,7 @@ def test_poisson_matches_oracle(pkg, O, dev, n):
     assert relerr(out.cpu().numpy(), ref) < 1e-12
 
 
-@pytest.mark.parametrize("nsts", [(8, 8), (20, 20), (36, 30), (64, 50), (126, 82)])
+@pytest.mark.parametrize("nsts", [(8, 8), (20, 20), (21, 7), (36, 30), (40, 0), (64, 50), (126, 82)])
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
 def test_gram2_matches_matmul(pkg, dev, nsts, dtype):
     """scale * A^H B_m for both isospin blocks, with B_0 = A (Hermitian, half
@@ -138,11 +138,14 @@ def test_gram2_matches_matmul(pkg, dev, nsts, dtype):
         ts = [torch.tensor(a, dtype=dtype, device=dev) for a in arrs]
         blocks.append((ts[0], [ts[0], ts[1], ts[2]]))
         outs.append(torch.empty((3, nst, nst), dtype=torch.complex128, device=dev))
-        A2 = ts[0].cpu().numpy().astype(np.complex128).reshape(nst, -1)
-        ref.append([0.7 * A2.conj() @ t.cpu().numpy().astype(np.complex128).reshape(nst, -1).T for t in ts])
+        A2 = ts[0].cpu().numpy().astype(np.complex128).reshape(nst, 2 * 24 ** 3)
+        ref.append([0.7 * A2.conj() @ t.cpu().numpy().astype(np.complex128).reshape(nst, 2 * 24 ** 3).T
+                    for t in ts])
     pkg.kernels.gram2(ctx, blocks, 0.7, outs)
     torch.cuda.synchronize()
     for iq in range(2):
+        if nsts[iq] == 0:
+            continue
         for m in range(3):
             r = ref[iq][m]
             got = outs[iq][m].cpu().numpy()
