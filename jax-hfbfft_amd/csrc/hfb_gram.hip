@@ -94,7 +94,7 @@ __device__ __forceinline__ void gv_tile(int t, int nt, int nm, int bsame, int& m
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) gram_vec_kernel(GramVecArgs ga) {
+__global__ void __launch_bounds__(256, 3) gram_vec_kernel(GramVecArgs ga) {
   const int iq = blockIdx.y;
   const GramIso& g = ga.iso[iq];
   const int nst = g.nst;
@@ -163,119 +163,6 @@ __global__ void __launch_bounds__(256) gram_vec_kernel(GramVecArgs ga) {
 // Sum the chunks of each output element (one wavefront per element, fixed
 // order); entries below the diagonal of a B == A matrix are the conjugates
 // of the computed upper ones.
-// ---------------------------------------------------------------------------
-// Gram for blocks up to GL_NMAX states, staged through LDS: a workgroup owns
-// a K-chunk of one isospin and stages GL_KC columns of A and of every
-// distinct B_m at a time, transposed to [k][state] so that a thread's two
-// rows are one contiguous read.  Every thread owns up to TPT 2x2 complex
-// tiles (upper tiles only for B_m == A) and accumulates them in FP64 over
-// the whole chunk; the chunk partials go to the same layout as the
-// register-tiled path and are combined by gram_vec_reduce_kernel.
-// ---------------------------------------------------------------------------
-constexpr int GL_KC = 64;     // columns per LDS stage
-constexpr int GL_NMAX = 40;   // largest block on this path
-
-// columns per chunk: whole stages, about 256 chunks per isospin
-int64_t gram_lds_kc(int64_t K) {
-  int64_t st = (K + GL_KC - 1) / GL_KC;
-  int64_t per = (st + 255) / 256;
-  return per * GL_KC;
-}
-int gram_lds_nchunk(int64_t K) { return (int)((K + gram_lds_kc(K) - 1) / gram_lds_kc(K)); }
-
-struct GramLdsArgs {
-  GramIso iso[2];
-  int nm, bsame;
-  int nslot;        // staged matrices: A plus every distinct B_m
-  int slot[4];      // LDS slot of B_m
-  const void* src[2][4];  // per isospin, the matrix staged in each slot
-  int64_t K, kc;    // columns, columns per chunk (multiple of GL_KC)
-  int nchunk;
-  int ntile[2];     // 2x2 tiles per isospin over all m
-  int npad;         // rows per LDS stage (max nst rounded up to even)
-};
-
-template <typename T, int TPT>
-__global__ void __launch_bounds__(256) gram_lds_kernel(GramLdsArgs ga) {
-  extern __shared__ unsigned char gl_raw[];
-  cx<T>* sm = (cx<T>*)gl_raw;  // [slot][GL_KC][npad]
-  const int iq = blockIdx.y;
-  const GramIso& g = ga.iso[iq];
-  const int nst = g.nst;
-  if (nst == 0) return;  // uniform over the workgroup
-  const int npad = ga.npad;
-  const int nt2 = (nst + 1) / 2;
-  const int64_t K = ga.K;
-  const int64_t k0 = (int64_t)blockIdx.x * ga.kc;
-  const int64_t k1 = min(K, k0 + ga.kc);
-  const int ntile = ga.ntile[iq];
-  int tm[TPT], ta[TPT], tb[TPT];
-  bool on[TPT];
-#pragma unroll
-  for (int c = 0; c < TPT; ++c) {
-    const int t = threadIdx.x + 256 * c;
-    on[c] = t < ntile;
-    int m = 0, a = 0, b = 0;
-    if (on[c]) gv_tile(t, nt2, ga.nm, ga.bsame, m, a, b);
-    tm[c] = ga.slot[m];
-    ta[c] = a;
-    tb[c] = b;
-  }
-  double acc[TPT][8];
-#pragma unroll
-  for (int c = 0; c < TPT; ++c)
-#pragma unroll
-    for (int u = 0; u < 8; ++u) acc[c][u] = 0.0;
-  const int per = GL_KC * npad;
-  for (int64_t kb = k0; kb < k1; kb += GL_KC) {
-    __syncthreads();  // the previous stage has been read
-    for (int e = threadIdx.x; e < ga.nslot * GL_KC * npad; e += 256) {
-      const int sl = e / per, r = e % per;
-      const int i = r / GL_KC, kk = r % GL_KC;  // consecutive threads walk k (coalesced rows)
-      const int64_t k = kb + kk;
-      const cx<T>* M = (const cx<T>*)ga.src[iq][sl];
-      sm[sl * per + kk * npad + i] = (i < nst && k < k1) ? M[(int64_t)i * K + k] : cx<T>{T(0), T(0)};
-    }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < TPT; ++c) {
-      if (!on[c]) continue;
-      const cx<T>* xa = sm + 2 * ta[c];
-      const cx<T>* xb = sm + tm[c] * per + 2 * tb[c];
-#pragma unroll 4
-      for (int kk = 0; kk < GL_KC; ++kk) {
-        const cx<T> a0 = xa[kk * npad], a1 = xa[kk * npad + 1];
-        const cx<T> b0 = xb[kk * npad], b1 = xb[kk * npad + 1];
-        const double ar[2] = {double(a0.x), double(a1.x)}, ai[2] = {double(a0.y), double(a1.y)};
-        const double br[2] = {double(b0.x), double(b1.x)}, bi[2] = {double(b0.y), double(b1.y)};
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-#pragma unroll
-          for (int v = 0; v < 2; ++v) {  // conj(a_u) b_v
-            acc[c][4 * u + 2 * v] = fma(ar[u], br[v], fma(ai[u], bi[v], acc[c][4 * u + 2 * v]));
-            acc[c][4 * u + 2 * v + 1] = fma(ar[u], bi[v], fma(-ai[u], br[v], acc[c][4 * u + 2 * v + 1]));
-          }
-      }
-    }
-  }
-  const int np = (nst + GV_T - 1) / GV_T * GV_T;
-#pragma unroll
-  for (int c = 0; c < TPT; ++c) {
-    if (!on[c]) continue;
-    int m = 0, a = 0, b = 0;
-    gv_tile(threadIdx.x + 256 * c, nt2, ga.nm, ga.bsame, m, a, b);
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-      for (int v = 0; v < 2; ++v) {
-        const int row = 2 * a + u, col = 2 * b + v;
-        double* dst = g.part + 2 * ((((int64_t)m * np + row) * np + col) * ga.nchunk + blockIdx.x);
-        dst[0] = acc[c][4 * u + 2 * v];
-        dst[1] = acc[c][4 * u + 2 * v + 1];
-      }
-  }
-}
-
 __global__ void __launch_bounds__(256) gram_vec_reduce_kernel(const double* __restrict__ part, int nchunk, int nm,
                                                               int np, int nst, int bsame, double scale,
                                                               double* __restrict__ out) {
@@ -413,12 +300,7 @@ __global__ void __launch_bounds__(256) rotate_kernel(RotArgs ra) {
 
 extern "C" int hfb_gram_workspace(int nst, long long K) {
   const int np4 = (nst + GV_T - 1) / GV_T * GV_T;
-  int nc = gram_vec_nchunk(nst, K);
-  if (nst <= GL_NMAX) {
-    const int nl = gram_lds_nchunk(K);
-    nc = nl > nc ? nl : nc;
-  }
-  return nc * 4 * np4 * np4 * 2;  // doubles for up to four matrices
+  return gram_vec_nchunk(nst, K) * 4 * np4 * np4 * 2;  // doubles for up to four matrices
 }
 
 // Both isospin blocks in one launch: A[iq], B[iq*4 + m], out[iq] (nm, nst_iq,
@@ -454,67 +336,6 @@ extern "C" int hfb_gram2(int prec, const void* const* A, const void* const* B, i
   if (npmax == 0) return HFB_OK;
   if (prec != HFB_F64 && prec != HFB_F32) return HFB_E_ARG;
   hipStream_t st = (hipStream_t)stream;
-  int nmaxst = nst[0] > nst[1] ? nst[0] : nst[1];
-  const size_t csz = prec == HFB_F64 ? sizeof(cx<double>) : sizeof(cx<float>);
-  GramLdsArgs L{};
-  L.nslot = 1;
-  for (int m = 0; m < nm; ++m) L.slot[m] = ((v.bsame >> m) & 1) ? 0 : L.nslot++;
-  L.npad = (nmaxst + 1) / 2 * 2;
-  const size_t lds = csz * (size_t)L.nslot * GL_KC * L.npad;
-  if (nmaxst <= GL_NMAX && lds <= 150 * 1024) {  // LDS-staged path
-    L.iso[0] = v.iso[0];
-    L.iso[1] = v.iso[1];
-    L.nm = nm;
-    L.bsame = v.bsame;
-    L.K = K;
-    L.kc = gram_lds_kc(K);
-    L.nchunk = gram_lds_nchunk(K);
-    int tmax = 0;
-    for (int iq = 0; iq < 2; ++iq) {
-      const int nt2 = (nst[iq] + 1) / 2;
-      int cnt = 0;
-      for (int m = 0; m < nm; ++m) cnt += ((v.bsame >> m) & 1) ? nt2 * (nt2 + 1) / 2 : nt2 * nt2;
-      L.ntile[iq] = nst[iq] > 0 ? cnt : 0;
-      tmax = L.ntile[iq] > tmax ? L.ntile[iq] : tmax;
-      if (nst[iq] > 0) {
-        L.src[iq][0] = A[iq];
-        for (int m = 0; m < nm; ++m)
-          if (L.slot[m] > 0) L.src[iq][L.slot[m]] = B[iq * 4 + m];
-      }
-    }
-    const int tpt = (tmax + 255) / 256;
-    const dim3 grid(L.nchunk, 2);
-#define HFB_GL(TT, TP)                                                      \
-  {                                                                         \
-    auto kern = gram_lds_kernel<TT, TP>;                                    \
-    if (lds > 32 * 1024) allow_full_lds((const void*)kern);                 \
-    hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, L);                  \
-  }
-#define HFB_GLP(TT)                          \
-  if (tpt <= 1) HFB_GL(TT, 1)                \
-  else if (tpt <= 2) HFB_GL(TT, 2)           \
-  else if (tpt <= 4) HFB_GL(TT, 4)           \
-  else HFB_GL(TT, 8)
-    if (prec == HFB_F64) {
-      HFB_GLP(double)
-    } else {
-      HFB_GLP(float)
-    }
-#undef HFB_GLP
-#undef HFB_GL
-    int rc = launch_status();
-    for (int iq = 0; iq < 2 && rc == HFB_OK; ++iq) {
-      if (nst[iq] <= 0) continue;
-      const int np4 = (nst[iq] + GV_T - 1) / GV_T * GV_T;
-      const int64_t tot = (int64_t)nm * nst[iq] * nst[iq];
-      int nb = (int)((tot * 64 + 255) / 256);
-      if (nb > 4096) nb = 4096;
-      hipLaunchKernelGGL(gram_vec_reduce_kernel, dim3(nb < 1 ? 1 : nb), dim3(256), 0, st, part[iq], L.nchunk, nm,
-                         np4, nst[iq], v.bsame, scale, out[iq]);
-      rc = launch_status();
-    }
-    return rc;
-  }
   {  // register-tiled path (gram_vec_kernel)
     v.nm = nm;
     v.K = K;

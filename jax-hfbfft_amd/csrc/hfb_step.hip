@@ -1,15 +1,18 @@
 // Damped gradient step with the FFT-diagonal preconditioner
 // (static.py:93-146 e0dmp_gt_zero, levels.py:397-437 laplace).
 //
-// The 3-D transform of the residual is five pencil passes over each spin
-// field: z (residual r formed from ps1 and lagrange or h_exp*psi, forward),
-// y (forward), x (forward, divide by the preconditioner, inverse), y
-// (inverse), z (inverse, psi <- psi - x0dmp * P r).  A scratch array of
-// the spinor block's size carries the transform between passes; along an
-// axis that has been transformed, element k of a pencil holds frequency k.
-// Each pencil sits in a group of L lanes (LaneFFT layout): a forward pass
-// stores element k1 of lane j at frequency position k1 + N1 bitrev(j), an
-// inverse pass reads from there.
+// The 3-D transform of the residual takes three passes over each spin field
+// when ny == nz: a y-z plane pass (residual r formed from ps1 and lagrange
+// or h_exp*psi, forward transforms along z then y with the plane held in
+// LDS between them), an x pencil pass (forward, divide by the
+// preconditioner, inverse), and a y-z plane pass (inverse along y then z,
+// psi <- psi - x0dmp * P r).  Other grids take five pencil passes: z (in),
+// y (forward), x (divide), y (inverse), z (out).  A scratch array of the
+// spinor block's size carries the transform between passes; along an axis
+// that has been transformed, element k of a pencil holds frequency k.  Each
+// pencil sits in a group of L lanes (LaneFFT layout): a forward transform
+// leaves element k1 of lane j at frequency position k1 + N1 bitrev(j), an
+// inverse one reads from there.
 #include <math.h>
 
 #include "hfb_ctx.h"
@@ -138,6 +141,86 @@ __global__ void __launch_bounds__(256) step_lane(StepArgs a, const cx<T>* __rest
   }
 }
 
+// One y-z plane (x fixed) of one spin field per workgroup, N pencils of L =
+// 8 lanes.  FWD: residual along z pencils (as SM_IN), forward z transform,
+// the plane through LDS, forward y transform, work[x][ky][kz].  INV: the
+// reverse, ending with the update of psi.
+constexpr int PLANE_L = 8;
+
+template <typename T, int N, bool FWD>
+__global__ void __launch_bounds__(N * PLANE_L) step_plane(StepArgs a, const cx<T>* __restrict__ tw) {
+  constexpr int L = PLANE_L, N1 = N / L, NT = N * L, P = N + 1;  // P: LDS row pitch
+  __shared__ cx<T> pl[N * P];
+  __shared__ double red[(NT / 64) * 2];
+  const int j = threadIdx.x % L, q = threadIdx.x / L;  // lane in the group, pencil
+  const int x = blockIdx.x, f = blockIdx.y, s = f >> 1;
+  const int64_t nvox = (int64_t)a.nx * N * N;
+  const int64_t pb = (int64_t)f * nvox + (int64_t)x * N * N;  // plane base
+  LaneFFT<T, N, L> F;
+  F.init(j, tw);
+  cx<T>* psi = (cx<T>*)a.psi;
+  cx<T>* R = (cx<T>*)a.work;
+  cx<T> v[N1];
+  if constexpr (FWD) {
+    double acc[2] = {0.0, 0.0};
+    const T hexp = T(a.wxyz * a.hsum[2 * s + 1]);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {  // z pencil q = y
+      const int64_t off = pb + (int64_t)q * N + j + L * m;
+      const cx<T> p1 = ((const cx<T>*)a.ps1)[off];
+      const cx<T> ps = psi[off];
+      const cx<T> lg = ((const cx<T>*)a.lag)[off];
+      const cx<T> hp = cscale(ps, hexp);
+      v[m] = a.use_lag ? csub(p1, lg) : csub(p1, hp);
+      const cx<T> e2 = csub(lg, hp);
+      acc[0] += double(v[m].x) * double(v[m].x) + double(v[m].y) * double(v[m].y);
+      acc[1] += double(e2.x) * double(e2.x) + double(e2.y) * double(e2.y);
+    }
+    if (a.fpart) {
+      block_sum<2>(acc, red);
+      if (threadIdx.x == 0) {
+        a.fpart[((int64_t)f * gridDim.x + x) * 2 + 0] = acc[0];
+        a.fpart[((int64_t)f * gridDim.x + x) * 2 + 1] = acc[1];
+      }
+    }
+    F.fwd(v, tw);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) pl[q * P + F.freq(m)] = v[m];  // [y][kz]
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < N1; ++m) v[m] = pl[(j + L * m) * P + q];  // y pencil q = kz
+    F.fwd(v, tw);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) R[pb + (int64_t)F.freq(m) * N + q] = v[m];  // work[x][ky][kz]
+  } else {
+#pragma unroll
+    for (int m = 0; m < N1; ++m) v[m] = R[pb + (int64_t)F.freq(m) * N + q];  // y pencil q = kz
+    F.inv(v, tw);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) pl[(j + L * m) * P + q] = v[m];  // [y][kz]
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < N1; ++m) v[m] = pl[q * P + F.freq(m)];  // z pencil q = y
+    F.inv(v, tw);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {
+      const int64_t off = pb + (int64_t)q * N + j + L * m;
+      psi[off] = csub(psi[off], cscale(v[m], T(a.x0)));
+    }
+  }
+}
+
+template <typename T, bool FWD>
+int launch_plane(const hfb_ctx* c, const StepArgs& a, int nfields, hipStream_t st) {
+  HFB_DISPATCH_N(c->ny, {
+    hipLaunchKernelGGL((step_plane<T, N_, FWD>), dim3(c->nx, nfields), dim3(N_ * PLANE_L), 0, st, a,
+                       twiddles<T>(c, N_));
+  })
+  return launch_status();
+}
+
+inline bool plane_path(const hfb_ctx* c) { return c->ny == c->nz; }
+
 template <typename T, int AXIS, int MODE>
 int launch_step(const hfb_ctx* c, const StepArgs& a, int nfields, hipStream_t st) {
   const int n = axis_len(c, AXIS);
@@ -152,6 +235,12 @@ int launch_step(const hfb_ctx* c, const StepArgs& a, int nfields, hipStream_t st
 
 template <typename T>
 int step_all(const hfb_ctx* c, const StepArgs& a, int ns, hipStream_t st) {
+  if (plane_path(c)) {
+    int rc = launch_plane<T, true>(c, a, 2 * ns, st);
+    if (rc == HFB_OK) rc = launch_step<T, 0, SM_DIV>(c, a, 2 * ns, st);
+    if (rc == HFB_OK) rc = launch_plane<T, false>(c, a, 2 * ns, st);
+    return rc;
+  }
   int rc = launch_step<T, 2, SM_IN>(c, a, 2 * ns, st);
   if (rc == HFB_OK) rc = launch_step<T, 1, SM_FWD>(c, a, 2 * ns, st);
   if (rc == HFB_OK) rc = launch_step<T, 0, SM_DIV>(c, a, 2 * ns, st);
@@ -232,7 +321,7 @@ __global__ void grstep_finish_kernel(int ns, const double* __restrict__ hsum, do
 extern "C" int hfb_step_ntiles(hfb_ctx* c, int prec) {
   if (!c) return HFB_E_NOCTX;
   (void)prec;
-  return step_pencils(c, 2) / STEP_PPB;
+  return plane_path(c) ? c->nx : step_pencils(c, 2) / STEP_PPB;
 }
 
 extern "C" int hfb_grstep_weights(int nstates, int npsi0, int tbcs, const double* wocc, const double* wstates,
