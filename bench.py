@@ -168,7 +168,8 @@ def main():
     hp = sorted(hpsi_ms)
     hp_avg = sum(hp) / max(1, len(hp))
     ach = hpsi_bytes(S, nvox, 8, 4) / (hp_avg * 1e-3) / 1e9 if hp else None
-    roof = {"bound": "hbm", "kernel": "hpsi (hpsi_lane_kernel x3, FP32)", "achieved": ach, "peak": 8000.0,
+    roof = {"bound": "hbm", "kernel": "hpsi (hpsi_lane_kernel x pass + hpsi_plane_kernel y-z, FP32)", "achieved": ach,
+            "peak": 8000.0,
             "unit": "GB/s", "frac": (ach / 8000.0) if ach else None, "traffic": measured_traffic(args.workload),
             "avg_launch_ms": hp_avg, "launches": len(hp),
             "algorithmic_bytes": hpsi_bytes(S, nvox, 8, 4)}

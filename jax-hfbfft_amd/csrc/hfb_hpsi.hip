@@ -142,6 +142,98 @@ struct MF {
   const T* V;  // (2, nvox)
 };
 
+// The contribution of axis AXIS to h psi on one pencil (meanfield.py:61-145,
+// levels.py:283-395), in the LaneFFT layout of L lanes: a, b the two spin
+// components, B, A the axis' effective mass and current potential, wa, wb
+// the two spin-orbit components the axis uses (W1, W2 for x; W0, W2 for y;
+// W0, W1 for z), d the grid spacing.  o0, o1 receive
+//   d(f) - d(B d1) + (the axis' spin-orbit and current terms on d1),
+// where d1 = IFFT(i k FFT psi) and the Nyquist entry of d(B d1) is
+// -(pi/d)^2 mean(B) psi_N / N.
+template <typename T, int N, int L, int AXIS>
+__device__ __forceinline__ void axis_terms(const LaneFFT<T, N, L>& F, const cx<T>* __restrict__ tw,
+                                           const cx<T> (&a)[N / L], const cx<T> (&b)[N / L], const T (&Bv)[N / L],
+                                           const T (&Av)[N / L], const T (&wa)[N / L], const T (&wb)[N / L], T d,
+                                           cx<T> (&o0)[N / L], cx<T> (&o1)[N / L]) {
+  constexpr int N1 = N / L;
+  cx<T> fa[N1], fb[N1], ga[N1], gb[N1];
+  T bsum = T(0);
+#pragma unroll
+  for (int m = 0; m < N1; ++m) {
+    bsum += Bv[m];
+    fa[m] = a[m];
+    fb[m] = b[m];
+  }
+  const T mb = group_sum<L>(bsum) / T(N);
+  const T kfac = T(2.0 * M_PI) / (d * T(N));
+  // d1 = d/da psi
+  F.fwd(fa, tw);
+  F.fwd(fb, tw);
+  const cx<T> nya = fa[0], nyb = fb[0];  // psi_N on the lane that holds k = N/2 at k1 = 0
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) {
+    const T f = d1_factor<T, N>(F.freq(k1), kfac);
+    fa[k1] = cmul_is(fa[k1], f);
+    fb[k1] = cmul_is(fb[k1], f);
+  }
+  F.inv(fa, tw);
+  F.inv(fb, tw);
+  // f - B d1 with the axis' combination f (meanfield.py:75-83, 104-112, 133-141)
+#pragma unroll
+  for (int m = 0; m < N1; ++m) {
+    const cx<T> av = a[m], bv = b[m];
+    const T A = Av[m];
+    cx<T> q0, q1;
+    if constexpr (AXIS == 0) {  // wa = W1, wb = W2
+      q0 = csub(cmul_is(av, T(-0.5) * (A - wa[m])), cscale(bv, T(0.5) * wb[m]));
+      q1 = cadd(cmul_is(bv, T(-0.5) * (A + wa[m])), cscale(av, T(0.5) * wb[m]));
+    } else if constexpr (AXIS == 1) {  // wa = W0, wb = W2
+      q0 = cadd(cmul_is(av, T(-0.5) * (A + wa[m])), cmul_is(bv, T(0.5) * wb[m]));
+      q1 = cadd(cmul_is(bv, T(-0.5) * (A - wa[m])), cmul_is(av, T(0.5) * wb[m]));
+    } else {  // wa = W0, wb = W1
+      q0 = cadd(cmul_is(av, T(-0.5) * A), cadd(cscale(bv, T(0.5) * wa[m]), cmul_is(bv, T(-0.5) * wb[m])));
+      q1 = cadd(cmul_is(bv, T(-0.5) * A), cadd(cscale(av, T(-0.5) * wa[m]), cmul_is(av, T(-0.5) * wb[m])));
+    }
+    ga[m] = csub(q0, cscale(fa[m], Bv[m]));
+    gb[m] = csub(q1, cscale(fb[m], Bv[m]));
+  }
+  F.fwd(ga, tw);
+  F.fwd(gb, tw);
+  const T nq = -T(M_PI / (double)d) * T(M_PI / (double)d) / T(N);
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) {
+    const int kf = F.freq(k1);
+    if (kf == N / 2) {  // only d(B d1) has a Nyquist entry (levels.py:309-315)
+      ga[k1] = cscale(nya, -nq * mb);
+      gb[k1] = cscale(nyb, -nq * mb);
+    } else {
+      const T f = d1_factor<T, N>(kf, kfac);
+      ga[k1] = cmul_is(ga[k1], f);
+      gb[k1] = cmul_is(gb[k1], f);
+    }
+  }
+  F.inv(ga, tw);
+  F.inv(gb, tw);
+#pragma unroll
+  for (int m = 0; m < N1; ++m) {
+    const cx<T> da = fa[m], db = fb[m];
+    const T A = Av[m];
+    cx<T> r0 = ga[m], r1 = gb[m];
+    if constexpr (AXIS == 0) {  // meanfield.py:63-73
+      r0 = cadd(r0, csub(cmul_is(da, -(T(0.5) * A - T(0.5) * wa[m])), cscale(db, T(0.5) * wb[m])));
+      r1 = cadd(r1, cadd(cmul_is(db, -(T(0.5) * A + T(0.5) * wa[m])), cscale(da, T(0.5) * wb[m])));
+    } else if constexpr (AXIS == 1) {  // meanfield.py:92-102
+      r0 = cadd(r0, cadd(cmul_is(da, -(T(0.5) * A + T(0.5) * wa[m])), cmul_is(db, T(0.5) * wb[m])));
+      r1 = cadd(r1, cadd(cmul_is(db, -(T(0.5) * A - T(0.5) * wa[m])), cmul_is(da, T(0.5) * wb[m])));
+    } else {  // meanfield.py:121-131
+      r0 = cadd(r0, cadd(cmul_is(da, -T(0.5) * A), cadd(cscale(db, T(0.5) * wa[m]), cmul_is(db, T(-0.5) * wb[m]))));
+      r1 = cadd(r1, cadd(cmul_is(db, -T(0.5) * A), cadd(cscale(da, T(-0.5) * wa[m]), cmul_is(da, T(-0.5) * wb[m]))));
+    }
+    o0[m] = r0;
+    o1[m] = r1;
+  }
+}
+
 // One axis of h psi for the pencils of one state, in registers: a group of L
 // lanes holds one pencil of both spin components (LaneFFT layout), the
 // block's four wavefronts hold 4 * 64 / L pencils.  Per pencil:
@@ -191,9 +283,8 @@ __global__ void __launch_bounds__(256) hpsi_lane_kernel(const cx<T>* __restrict_
   const T* Wa = mf.W + (iq * 3 + WA) * nvox;
   const T* Wb = mf.W + (iq * 3 + WB) * nvox;
 
-  cx<T> a[N1], b[N1], fa[N1], fb[N1], ga[N1], gb[N1];
+  cx<T> a[N1], b[N1], o0v[N1], o1v[N1];
   T Bv[N1], Av[N1], wa[N1], wb[N1];
-  T bsum = T(0);
 #pragma unroll
   for (int m = 0; m < N1; ++m) {
     const int64_t off = base + (int64_t)(j + L * m) * stride;
@@ -203,87 +294,22 @@ __global__ void __launch_bounds__(256) hpsi_lane_kernel(const cx<T>* __restrict_
     Av[m] = Aa[off];
     wa[m] = Wa[off];
     wb[m] = Wb[off];
-    bsum += Bv[m];
-    fa[m] = a[m];
-    fb[m] = b[m];
   }
-  const T mb = group_sum<L>(bsum) / T(N);
-  const T kfac = T(2.0 * M_PI) / (d * T(N));
-
-  // d1 = d/da psi
-  F.fwd(fa, tw);
-  F.fwd(fb, tw);
-  const cx<T> nya = fa[0], nyb = fb[0];  // psi_N on the lane that holds k = N/2 at k1 = 0
-#pragma unroll
-  for (int k1 = 0; k1 < N1; ++k1) {
-    const T f = d1_factor<T, N>(F.freq(k1), kfac);
-    fa[k1] = cmul_is(fa[k1], f);
-    fb[k1] = cmul_is(fb[k1], f);
-  }
-  F.inv(fa, tw);
-  F.inv(fb, tw);
-
-  // f - B d1 with the axis' combination f (meanfield.py:75-83, 104-112, 133-141)
-#pragma unroll
-  for (int m = 0; m < N1; ++m) {
-    const cx<T> av = a[m], bv = b[m];
-    const T A = Av[m];
-    cx<T> q0, q1;
-    if constexpr (AXIS == 0) {  // wa = W1, wb = W2
-      q0 = csub(cmul_is(av, T(-0.5) * (A - wa[m])), cscale(bv, T(0.5) * wb[m]));
-      q1 = cadd(cmul_is(bv, T(-0.5) * (A + wa[m])), cscale(av, T(0.5) * wb[m]));
-    } else if constexpr (AXIS == 1) {  // wa = W0, wb = W2
-      q0 = cadd(cmul_is(av, T(-0.5) * (A + wa[m])), cmul_is(bv, T(0.5) * wb[m]));
-      q1 = cadd(cmul_is(bv, T(-0.5) * (A - wa[m])), cmul_is(av, T(0.5) * wb[m]));
-    } else {  // wa = W0, wb = W1
-      q0 = cadd(cmul_is(av, T(-0.5) * A), cadd(cscale(bv, T(0.5) * wa[m]), cmul_is(bv, T(-0.5) * wb[m])));
-      q1 = cadd(cmul_is(bv, T(-0.5) * A), cadd(cscale(av, T(-0.5) * wa[m]), cmul_is(av, T(-0.5) * wb[m])));
-    }
-    ga[m] = csub(q0, cscale(fa[m], Bv[m]));
-    gb[m] = csub(q1, cscale(fb[m], Bv[m]));
-  }
-  F.fwd(ga, tw);
-  F.fwd(gb, tw);
-  const T nq = -T(M_PI / (double)d) * T(M_PI / (double)d) / T(N);
-#pragma unroll
-  for (int k1 = 0; k1 < N1; ++k1) {
-    const int kf = F.freq(k1);
-    if (kf == N / 2) {  // only d(B d1) has a Nyquist entry (levels.py:309-315)
-      ga[k1] = cscale(nya, -nq * mb);
-      gb[k1] = cscale(nyb, -nq * mb);
-    } else {
-      const T f = d1_factor<T, N>(kf, kfac);
-      ga[k1] = cmul_is(ga[k1], f);
-      gb[k1] = cmul_is(gb[k1], f);
-    }
-  }
-  F.inv(ga, tw);
-  F.inv(gb, tw);
+  axis_terms<T, N, L, AXIS>(F, tw, a, b, Bv, Av, wa, wb, d, o0v, o1v);
 
   double acc[2] = {0.0, 0.0};
 #pragma unroll
   for (int m = 0; m < N1; ++m) {
     const int64_t off = base + (int64_t)(j + L * m) * stride;
     const cx<T> av = a[m], bv = b[m];
-    const cx<T> da = fa[m], db = fb[m];
-    const T A = Av[m];
-    cx<T> o0 = ga[m], o1 = gb[m];
-    if constexpr (AXIS == 0) {  // meanfield.py:63-73
-      o0 = cadd(o0, csub(cmul_is(da, -(T(0.5) * A - T(0.5) * wa[m])), cscale(db, T(0.5) * wb[m])));
-      o1 = cadd(o1, cadd(cmul_is(db, -(T(0.5) * A + T(0.5) * wa[m])), cscale(da, T(0.5) * wb[m])));
-      // meanfield.py:47-58: U psi and the spin-current coupling
+    cx<T> o0 = o0v[m], o1 = o1v[m];
+    if constexpr (AXIS == 0) {  // meanfield.py:47-58: U psi and the spin-current coupling
       const T u = mf.U[iq * nvox + off];
       const T s0 = mf.S[(iq * 3 + 0) * nvox + off];
       const T s1 = mf.S[(iq * 3 + 1) * nvox + off];
       const T s2 = mf.S[(iq * 3 + 2) * nvox + off];
       o0 = cadd(o0, cadd(cscale(av, u + s2), cadd(cscale(bv, s0), cmul_is(bv, -s1))));
       o1 = cadd(o1, cadd(cscale(bv, u - s2), cadd(cscale(av, s0), cmul_is(av, s1))));
-    } else if constexpr (AXIS == 1) {  // meanfield.py:92-102
-      o0 = cadd(o0, cadd(cmul_is(da, -(T(0.5) * A + T(0.5) * wa[m])), cmul_is(db, T(0.5) * wb[m])));
-      o1 = cadd(o1, cadd(cmul_is(db, -(T(0.5) * A - T(0.5) * wa[m])), cmul_is(da, T(0.5) * wb[m])));
-    } else {  // meanfield.py:121-131
-      o0 = cadd(o0, cadd(cmul_is(da, -T(0.5) * A), cadd(cscale(db, T(0.5) * wa[m]), cmul_is(db, T(-0.5) * wb[m]))));
-      o1 = cadd(o1, cadd(cmul_is(db, -T(0.5) * A), cadd(cscale(da, T(-0.5) * wa[m]), cmul_is(da, T(-0.5) * wb[m]))));
     }
     if constexpr (AXIS != 0) {
       o0 = cadd(o0, pmf[f0 + off]);
@@ -322,10 +348,133 @@ __global__ void __launch_bounds__(256) hpsi_lane_kernel(const cx<T>* __restrict_
   }
 }
 
+// The y and z axes of h psi for one y-z plane (x fixed) of one state, when
+// ny == nz: the plane of both spin components is read once along z pencils
+// and kept in LDS; the y pencils take it from there, compute the y terms
+// and leave them in LDS; the z pencils add the z terms, the y terms and the
+// x pass's accumulator, then write h psi, the weighted result and the
+// overlap partial sums (one pair per plane).  N pencils of HP_L lanes.
+constexpr int HP_L = 8;
+
+template <typename T, int N>
+__global__ void __launch_bounds__(N * HP_L) hpsi_plane_kernel(const cx<T>* __restrict__ psi, cx<T>* __restrict__ pmf,
+                                                              cx<T>* __restrict__ pout, cx<T>* __restrict__ pdel,
+                                                              MF<T> mf, const double* __restrict__ w,
+                                                              const double* __restrict__ wuv, int npsi0, int nx, T d,
+                                                              const cx<T>* __restrict__ tw,
+                                                              double* __restrict__ partial) {
+  constexpr int L = HP_L, N1 = N / L, P = N + 1;  // P: LDS row pitch
+  extern __shared__ unsigned char hp_lds[];
+  cx<T>* s0 = (cx<T>*)hp_lds;  // [y][z], spin up
+  cx<T>* s1 = s0 + N * P;      // spin down
+  __shared__ double red[(N * L / 64) * 2];
+  const int j = threadIdx.x % L, q = threadIdx.x / L;
+  const int x = blockIdx.x, s = blockIdx.y;
+  const int iq = s >= npsi0 ? 1 : 0;
+  const int64_t nvox = (int64_t)nx * N * N;
+  const int64_t f0 = (int64_t)(2 * s) * nvox;
+  const int64_t po = (int64_t)x * N * N;
+  const cx<T>* ps0 = psi + f0;
+  const cx<T>* ps1 = ps0 + nvox;
+  const T* Bq = mf.B + iq * nvox;
+  const T* W0 = mf.W + (iq * 3 + 0) * nvox;
+  const T* W1 = mf.W + (iq * 3 + 1) * nvox;
+  const T* W2 = mf.W + (iq * 3 + 2) * nvox;
+  LaneFFT<T, N, L> F;
+  F.init(j, tw);
+  // z pencils (pencil q = y): the spinor plane
+  cx<T> a[N1], b[N1];
+#pragma unroll
+  for (int m = 0; m < N1; ++m) {
+    const int z = j + L * m;
+    const int64_t off = po + (int64_t)q * N + z;
+    a[m] = ps0[off];
+    b[m] = ps1[off];
+    s0[q * P + z] = a[m];
+    s1[q * P + z] = b[m];
+  }
+  __syncthreads();
+  {  // y pencils (pencil q = z)
+    cx<T> ya[N1], yb[N1], o0[N1], o1[N1];
+    T Bv[N1], Av[N1], wa[N1], wb[N1];
+    const T* A1 = mf.A + (iq * 3 + 1) * nvox;
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {
+      const int y = j + L * m;
+      const int64_t off = po + (int64_t)y * N + q;
+      ya[m] = s0[y * P + q];
+      yb[m] = s1[y * P + q];
+      Bv[m] = Bq[off];
+      Av[m] = A1[off];
+      wa[m] = W0[off];
+      wb[m] = W2[off];
+    }
+    axis_terms<T, N, L, 1>(F, tw, ya, yb, Bv, Av, wa, wb, d, o0, o1);
+    __syncthreads();  // every y pencil has read the plane
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {
+      const int y = j + L * m;
+      s0[y * P + q] = o0[m];
+      s1[y * P + q] = o1[m];
+    }
+  }
+  __syncthreads();
+  double acc[2] = {0.0, 0.0};
+  {  // z pencils (pencil q = y)
+    cx<T> o0[N1], o1[N1];
+    T Bv[N1], Av[N1], wa[N1], wb[N1];
+    const T* A2 = mf.A + (iq * 3 + 2) * nvox;
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {
+      const int64_t off = po + (int64_t)q * N + j + L * m;
+      Bv[m] = Bq[off];
+      Av[m] = A2[off];
+      wa[m] = W0[off];
+      wb[m] = W1[off];
+    }
+    axis_terms<T, N, L, 2>(F, tw, a, b, Bv, Av, wa, wb, d, o0, o1);
+    const T ws = T(w[s]), wu = T(wuv[s]);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {
+      const int z = j + L * m;
+      const int64_t off = po + (int64_t)q * N + z;
+      const cx<T> av = a[m], bv = b[m];
+      const cx<T> h0 = cadd(cadd(o0[m], s0[q * P + z]), pmf[f0 + off]);
+      const cx<T> h1 = cadd(cadd(o1[m], s1[q * P + z]), pmf[f0 + nvox + off]);
+      pmf[f0 + off] = h0;
+      pmf[f0 + nvox + off] = h1;
+      // meanfield.py:150-157
+      const T v = mf.V[iq * nvox + off];
+      const cx<T> e0 = cscale(av, v), e1 = cscale(bv, v);
+      if (pdel) {
+        pdel[f0 + off] = e0;
+        pdel[f0 + nvox + off] = e1;
+      }
+      const cx<T> r0 = csub(cscale(h0, ws), cscale(e0, wu));
+      const cx<T> r1 = csub(cscale(h1, ws), cscale(e1, wu));
+      if (pout) {
+        pout[f0 + off] = r0;
+        pout[f0 + nvox + off] = r1;
+      }
+      acc[0] += double(av.x) * double(h0.x) + double(av.y) * double(h0.y) + double(bv.x) * double(h1.x) +
+                double(bv.y) * double(h1.y);
+      acc[1] += double(av.x) * double(r0.x) + double(av.y) * double(r0.y) + double(bv.x) * double(r1.x) +
+                double(bv.y) * double(r1.y);
+    }
+  }
+  if (partial) {
+    block_sum<2>(acc, red);
+    if (threadIdx.x == 0) {
+      partial[((int64_t)s * gridDim.x + x) * 2 + 0] = acc[0];
+      partial[((int64_t)s * gridDim.x + x) * 2 + 1] = acc[1];
+    }
+  }
+}
+
 // Lanes per pencil: FP32 pencils split over 4 lanes, FP64 over 8, which
 // keeps the six per-lane register arrays of the kernel at N/L elements.
 template <typename T>
-constexpr int lanes_per_pencil() { return sizeof(T) == 4 ? 4 : 8; }
+constexpr int lanes_per_pencil() { return 8; }
 
 template <typename T>
 int hpsi_pencils_per_block() { return 4 * (64 / lanes_per_pencil<T>()); }
@@ -350,10 +499,36 @@ int launch_hpsi_axis(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T>
   return launch_status();
 }
 
+// the plane kernel for square y-z planes; FP64 planes above 32 points per
+// side would spill registers and take the pencil passes instead
+inline bool hpsi_plane_path(const hfb_ctx* c, int prec) {
+  return c->ny == c->nz && (prec == HFB_F32 || c->ny <= 32);
+}
+
+template <typename T>
+int launch_hpsi_plane(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T> mf, const double* w,
+                      const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st) {
+  HFB_DISPATCH_N(c->ny, {
+    constexpr int NP_ = (sizeof(T) == 8 && N_ > 32) ? 32 : N_;  // never taken for FP64 above 32
+    if (NP_ != N_) return HFB_E_SIZE;
+    const size_t lds = 2 * sizeof(cx<T>) * (size_t)NP_ * (NP_ + 1);
+    auto kern = hpsi_plane_kernel<T, NP_>;
+    if (lds > 32 * 1024) allow_full_lds((const void*)kern);
+    hipLaunchKernelGGL(kern, dim3(c->nx, ns), dim3(NP_ * HP_L), lds, st, (const cx<T>*)psi, (cx<T>*)pmf,
+                       (cx<T>*)pout, (cx<T>*)pdel, mf, w, wuv, npsi0, c->nx, T(axis_d(c, 1)), twiddles<T>(c, NP_),
+                       partial);
+  })
+  return launch_status();
+}
+
 template <typename T>
 int hpsi_all(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T> mf, const double* w, const double* wuv,
              void* pout, void* pmf, void* pdel, double* partial, hipStream_t st) {
   int rc = launch_hpsi_axis<T, 0>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
+  if (hpsi_plane_path(c, sizeof(T) == 4 ? HFB_F32 : HFB_F64)) {
+    if (rc == HFB_OK) rc = launch_hpsi_plane<T>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
+    return rc;
+  }
   if (rc == HFB_OK) rc = launch_hpsi_axis<T, 1>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
   if (rc == HFB_OK) rc = launch_hpsi_axis<T, 2>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
   return rc;
@@ -374,6 +549,7 @@ extern "C" int hfb_cderv(hfb_ctx* c, int prec, int axis, int kind, const void* i
 
 extern "C" int hfb_hpsi_ntiles(hfb_ctx* c, int prec) {
   if (!c) return HFB_E_NOCTX;
+  if (hpsi_plane_path(c, prec)) return c->nx;  // one partial per y-z plane
   const int ppb = prec == HFB_F32 ? hpsi_pencils_per_block<float>() : hpsi_pencils_per_block<double>();
   return pencils_of_axis(c, 2) / ppb;
 }
