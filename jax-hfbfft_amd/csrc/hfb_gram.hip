@@ -163,9 +163,19 @@ __global__ void __launch_bounds__(256, 3) gram_vec_kernel(GramVecArgs ga) {
 // Sum the chunks of each output element (one wavefront per element, fixed
 // order); entries below the diagonal of a B == A matrix are the conjugates
 // of the computed upper ones.
-__global__ void __launch_bounds__(256) gram_vec_reduce_kernel(const double* __restrict__ part, int nchunk, int nm,
-                                                              int np, int nst, int bsame, double scale,
-                                                              double* __restrict__ out) {
+struct GramRedIso {
+  const double* part;
+  double* out;
+  int nst, np;
+};
+
+// Chunk sums of both isospins (blockIdx.y), one wavefront per entry.
+__global__ void __launch_bounds__(256) gram_vec_reduce_kernel(GramRedIso r0, GramRedIso r1, int nchunk, int nm,
+                                                              int bsame, double scale) {
+  const GramRedIso& r = blockIdx.y == 0 ? r0 : r1;
+  const double* __restrict__ part = r.part;
+  double* __restrict__ out = r.out;
+  const int nst = r.nst, np = r.np;
   const int64_t total = (int64_t)nm * nst * nst;
   const int lane = threadIdx.x & 63;
   const int64_t wid = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
@@ -356,16 +366,23 @@ extern "C" int hfb_gram2(int prec, const void* const* A, const void* const* B, i
     else
       return HFB_E_ARG;
     int rc = launch_status();
-    for (int iq = 0; iq < 2 && rc == HFB_OK; ++iq) {
+    if (rc != HFB_OK) return rc;
+    GramRedIso rr[2] = {};
+    int64_t tot = 0;
+    for (int iq = 0; iq < 2; ++iq) {
+      rr[iq].nst = nst[iq];
       if (nst[iq] <= 0) continue;
-      const int np4 = (nst[iq] + GV_T - 1) / GV_T * GV_T;
-      const int64_t tot = (int64_t)nm * nst[iq] * nst[iq];
-      int nb = (int)((tot * 64 + 255) / 256);
-      if (nb > 4096) nb = 4096;
-      hipLaunchKernelGGL(gram_vec_reduce_kernel, dim3(nb < 1 ? 1 : nb), dim3(256), 0, st, part[iq], v.nchunk, nm,
-                         np4, nst[iq], v.bsame, scale, out[iq]);
-      rc = launch_status();
+      rr[iq].part = part[iq];
+      rr[iq].out = out[iq];
+      rr[iq].np = (nst[iq] + GV_T - 1) / GV_T * GV_T;
+      const int64_t t = (int64_t)nm * nst[iq] * nst[iq];
+      tot = t > tot ? t : tot;
     }
+    int nb = (int)((tot * 64 + 255) / 256);
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(gram_vec_reduce_kernel, dim3(nb < 1 ? 1 : nb, 2), dim3(256), 0, st, rr[0], rr[1], v.nchunk, nm,
+                       v.bsame, scale);
+    rc = launch_status();
     return rc;
   }
 }

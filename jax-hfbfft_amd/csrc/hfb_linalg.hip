@@ -128,62 +128,59 @@ __device__ void load_lower_herm(cx<double>* A, const cx<double>* M, int n) {
 }
 
 // C = op(A) B for n x n matrices (C distinct from A and B); op(A) = A or A^H.
-// Each thread accumulates a column strip of NP*NP/NT entries in
-// registers; the loads of A are broadcasts within a wavefront.
+// Each thread owns a 2x2 block of C and keeps it in registers; per k it
+// reads two entries of op(A) and two of B (four complex loads for four
+// complex multiply-adds, which keeps the LDS data path from being the
+// limit), operands of MMK consecutive k at a time.  Odd n: the last block
+// row/column reads row/column n-1 twice and stores only what exists.
+__device__ __forceinline__ void mac(cx<double>& c, const cx<double> a, const cx<double> b) {
+  c.x = fma(a.x, b.x, fma(-a.y, b.y, c.x));
+  c.y = fma(a.x, b.y, fma(a.y, b.x, c.y));
+}
+template <bool CONJ_A>
+__device__ __forceinline__ cx<double> mm_opa(const cx<double>* A, int n, int i, int k) {
+  return CONJ_A ? cconj(A[k * n + i]) : A[i * n + k];
+}
+
 template <int NP, int NT, bool CONJ_A>
 __device__ __forceinline__ void mm(cx<double>* __restrict__ C, const cx<double>* A, const cx<double>* B, int n) {
-  constexpr int RG = NT / NP;
-  constexpr int TI = NP / RG;
-  constexpr int TC = TI < 8 ? TI : 8;  // rows accumulated per pass
-  const int j = threadIdx.x % NP;
-  const int r0 = threadIdx.x / NP;
-  const int jc = j < n ? j : n - 1;  // idle lanes read in bounds and store nothing
-#pragma unroll 1
-  for (int t0 = 0; t0 < TI; t0 += TC) {
-    int ic[TC];
-    cx<double> acc[TC];
-#pragma unroll
-    for (int t = 0; t < TC; ++t) {
-      const int i = r0 + RG * (t0 + t);
-      ic[t] = i < n ? i : n - 1;
-      acc[t] = {0.0, 0.0};
-    }
-    // operands of KC consecutive k are loaded before their FMAs, so the
-    // loads of a chunk are in flight together
-    constexpr int KC = NP > 64 ? 1 : (TC >= 8 ? 2 : 4);
+  constexpr int MMK = NT >= 1024 ? 1 : 4;  // (1024-thread blocks have 128 registers)
+  const int nh = (n + 1) >> 1;
+  for (int t = threadIdx.x; t < nh * nh; t += NT) {
+    const int i0 = 2 * (t / nh), j0 = 2 * (t % nh);
+    const int i1 = min(i0 + 1, n - 1), j1 = min(j0 + 1, n - 1);
+    cx<double> c00 = {0.0, 0.0}, c01 = {0.0, 0.0}, c10 = {0.0, 0.0}, c11 = {0.0, 0.0};
     int k = 0;
-    for (; k + KC <= n; k += KC) {
-      cx<double> b[KC], a[KC][TC];
+    for (; k + MMK <= n; k += MMK) {
+      cx<double> a0[MMK], a1[MMK], b0[MMK], b1[MMK];
 #pragma unroll
-      for (int c = 0; c < KC; ++c) {
-        b[c] = B[(k + c) * n + jc];
-#pragma unroll
-        for (int t = 0; t < TC; ++t)
-          a[c][t] = CONJ_A ? cconj(A[(k + c) * n + ic[t]]) : A[ic[t] * n + k + c];
+      for (int c = 0; c < MMK; ++c) {
+        a0[c] = mm_opa<CONJ_A>(A, n, i0, k + c);
+        a1[c] = mm_opa<CONJ_A>(A, n, i1, k + c);
+        b0[c] = B[(k + c) * n + j0];
+        b1[c] = B[(k + c) * n + j1];
       }
 #pragma unroll
-      for (int c = 0; c < KC; ++c)
-#pragma unroll
-        for (int t = 0; t < TC; ++t) {
-          acc[t].x = fma(a[c][t].x, b[c].x, fma(-a[c][t].y, b[c].y, acc[t].x));
-          acc[t].y = fma(a[c][t].x, b[c].y, fma(a[c][t].y, b[c].x, acc[t].y));
-        }
+      for (int c = 0; c < MMK; ++c) {
+        mac(c00, a0[c], b0[c]);
+        mac(c01, a0[c], b1[c]);
+        mac(c10, a1[c], b0[c]);
+        mac(c11, a1[c], b1[c]);
+      }
     }
     for (; k < n; ++k) {
-      const cx<double> b = B[k * n + jc];
-#pragma unroll
-      for (int t = 0; t < TC; ++t) {
-        const cx<double> a = CONJ_A ? cconj(A[k * n + ic[t]]) : A[ic[t] * n + k];
-        acc[t].x = fma(a.x, b.x, fma(-a.y, b.y, acc[t].x));
-        acc[t].y = fma(a.x, b.y, fma(a.y, b.x, acc[t].y));
-      }
+      const cx<double> a0 = mm_opa<CONJ_A>(A, n, i0, k), a1 = mm_opa<CONJ_A>(A, n, i1, k);
+      const cx<double> b0 = B[k * n + j0], b1 = B[k * n + j1];
+      mac(c00, a0, b0);
+      mac(c01, a0, b1);
+      mac(c10, a1, b0);
+      mac(c11, a1, b1);
     }
-    if (j < n) {
-#pragma unroll
-      for (int t = 0; t < TC; ++t) {
-        const int i = r0 + RG * (t0 + t);
-        if (i < n) C[i * n + j] = acc[t];
-      }
+    C[i0 * n + j0] = c00;
+    if (j0 + 1 < n) C[i0 * n + j0 + 1] = c01;
+    if (i0 + 1 < n) {
+      C[(i0 + 1) * n + j0] = c10;
+      if (j0 + 1 < n) C[(i0 + 1) * n + j0 + 1] = c11;
     }
   }
   __syncthreads();

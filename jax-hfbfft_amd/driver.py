@@ -122,8 +122,14 @@ class HFRun:
                 psi0 = add_noise(psi0, noise, seed)
         levels.psi.copy_(torch.from_numpy(np.ascontiguousarray(psi0)).to(dtype))
         meanfield = init_meanfield(grids, dev, f32=(precision == "mixed"))
+        energies = init_energies(dev)
+        pairs = Pairs()
+        # the pairing energies live in their slots of the energy vector, so the
+        # energy kernels read them without a copy
+        from .kernels import EN
+        pairs.epair = energies.vec[EN["EPAIR"]:EN["EPAIR"] + 2]
         return cls(params, forces, grids, levels, meanfield, init_densities(grids, dev), coulomb, static,
-                   init_energies(dev), Pairs(), init_moment(torch.zeros(3, dtype=torch.float64, device=dev)))
+                   energies, pairs, init_moment(torch.zeros(3, dtype=torch.float64, device=dev)))
 
     @property
     def state(self):

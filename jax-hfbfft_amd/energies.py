@@ -75,6 +75,14 @@ def init_energies(device=None):
     return Energies(device)
 
 
+def _set_slots(E, lo, src):
+    """E[lo:lo+len(src)] = src, skipped when src already is that slice (the
+    driver keeps pairs.epair and meanfield.ecorrp_dev as views of E)."""
+    dst = E[lo:lo + src.numel()]
+    if src.data_ptr() != dst.data_ptr():
+        dst.copy_(src.reshape(dst.shape))
+
+
 def integ_energy(coulomb, densities, energies, forces, grids, levels, params, pairs, derivs=None):
     """energies.py:90-306.  ``derivs``: DensityDerivs of these densities, if
     already computed for skyrme.  Pairing energies come from pairs.epair."""
@@ -82,7 +90,7 @@ def integ_energy(coulomb, densities, energies, forces, grids, levels, params, pa
     if derivs is None:
         derivs = density_derivs(grids, densities)
     E = energies.vec
-    E[EN["EPAIR"]:EN["EPAIR"] + 2] = pairs.epair
+    _set_slots(E, EN["EPAIR"], pairs.epair)
     fc = forces.to_c(params, levels.mass_number)
     kernels.integ_energy(grids.ctx, fc, densities, derivs, coulomb.wcoul, E)
     return energies
@@ -91,7 +99,7 @@ def integ_energy(coulomb, densities, energies, forces, grids, levels, params, pa
 def sum_energy(energies, levels, meanfield, pairs):
     """energies.py:309-371."""
     E = energies.vec
-    E[EN["EPAIR"]:EN["EPAIR"] + 2] = pairs.epair
-    E[EN["ECORRP"]] = meanfield.ecorrp_dev
+    _set_slots(E, EN["EPAIR"], pairs.epair)
+    _set_slots(E, EN["ECORRP"], meanfield.ecorrp_dev)
     kernels.sum_energy(levels, E)
     return energies

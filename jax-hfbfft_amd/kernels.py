@@ -70,7 +70,7 @@ def cderv(ctx, axis: int, kind: int, x: torch.Tensor, bmass: torch.Tensor | None
 
 
 def hpsi(ctx, psi, npsi0, upot, bmass, aq, spot, wlspot, v_pair, w, wuv,
-         want_pout=True, want_del=True, want_partial=True, out=None, timer=None):
+         want_pout=True, want_del=True, want_partial=True, out=None, timer=None, reduce_sums=True):
     """Batched hpsi01 (meanfield.py:40-159) on psi (S, 2, nx, ny, nz).
 
     Returns (pout, pout_mf, pout_del, sums) where sums (S, 2) double holds
@@ -107,7 +107,7 @@ def hpsi(ctx, psi, npsi0, upot, bmass, aq, spot, wlspot, v_pair, w, wuv,
                              ptr(wlspot), ptr(v_pair), ptr(w), ptr(wuv), ptr(pout), ptr(pmf), ptr(pdel), ptr(part),
                              ctypes.byref(nt), stream_handle()), "hpsi")
     sums = None
-    if want_partial:
+    if want_partial and reduce_sums:
         sums = ctx.work(("hpsi_sums", S), (S, 2), torch.float64)
         check(lib().hfb_reduce_partials(ptr(part), S, nt.value, 2, 1.0, ptr(sums), stream_handle()),
               "reduce_partials")

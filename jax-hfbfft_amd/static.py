@@ -161,7 +161,7 @@ def grstep(forces, grids, levels, meanfield, params, static):
     fields = meanfield.fields(levels.psi.dtype)
     # h psi on the old states: ps1 (weighted) and psi_mf -> hampsi (static.py:152-173)
     _, _, _, hsum = kernels.hpsi(ctx, levels.psi, n0, *fields, ws["w"], ws["wuv"], want_del=False,
-                                 out={"pout": ws["pout"], "pout_mf": levels.hampsi}, timer="hpsi")
+                                 out={"pout": ws["pout"], "pout_mf": levels.hampsi})
     want_fl = params.mprint > 0 and params.iteration % params.mprint == 0
     if static.e0dmp > 0.0:
         use_lag = (params.iteration > 1) and (not forces.tbcs)

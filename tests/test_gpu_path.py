@@ -61,12 +61,14 @@ _FAULTED = pytest.mark.skip(reason="this case faulted the device with a since-re
                                    "in hfb_diag_small and the cause is not found; it stays off the device")
 
 
-@pytest.mark.parametrize("npsi", [[12, 10], [36, 30], pytest.param([40, 33], marks=_FAULTED),
-                                  pytest.param([64, 50], marks=_FAULTED), [90, 60], [126, 82]])
+@pytest.mark.parametrize("npsi", [[12, 10], [21, 15], [36, 30], pytest.param([40, 33], marks=_FAULTED),
+                                  pytest.param([64, 50], marks=_FAULTED), [90, 60], [91, 61], [126, 82]])
 def test_diagstep_matches_oracle(pkg, O, dev, npsi):
     """Loewdin + subspace diagonalisation + HFB matrices on the same states;
     block sizes of O16-BCS and Cr48 (work matrices in LDS) and of the
-    global-memory path up to Pb208's 126/82."""
+    global-memory path up to Pb208's 126/82; odd sizes for the last block
+    row and column of the matrix products and the unpaired index of the
+    Jacobi rounds."""
     ref, run = small_case(O, pkg, ipair=5, tbcs=True, npsi=npsi)
     rng = np.random.default_rng(5)
     shp = ref.levels.psi.shape
