@@ -416,17 +416,30 @@ __device__ __forceinline__ T k2_factor(int k, T kfac) {
 }
 
 // Block-wide sum of `nv` values per thread into out[0..nv) (thread 0 holds
-// the result).  `scratch` needs blockDim.x/64 * nv doubles of LDS.
+// the result).  `scratch` needs blockDim.x/64 * nv doubles of LDS.  The
+// wavefront sums run level by level over all values (xor butterflies: DPP
+// within quads, swizzles within 32 lanes, a permute across the halves), so
+// the nv chains overlap; every lane ends with its wavefront's sum.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* scratch) {
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
   const int nw = (blockDim.x + 63) >> 6;
 #pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    double s = v[j];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-    if (lane == 0) scratch[wid * NV + j] = s;
+  for (int j = 0; j < NV; ++j) v[j] += lane_xor<1>(v[j]);
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] += lane_xor<2>(v[j]);
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] += lane_xor<4>(v[j]);
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] += lane_xor<8>(v[j]);
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] += lane_xor<16>(v[j]);
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] += __shfl_xor(v[j], 32, 64);
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) scratch[wid * NV + j] = v[j];
   }
   __syncthreads();
   if (threadIdx.x == 0) {

@@ -201,12 +201,19 @@ __global__ void __launch_bounds__(256) dens_lane_kernel(DensArgs a, const cx<T>*
       }
     }
     if (a.want_sp) {  // the wavefront covers this tile's pencils for state s
+      // lane sums level by level over the eight values
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        double v = sp[k];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        sp[k] = v;
-      }
+      for (int k = 0; k < 8; ++k) sp[k] += lane_xor<1>(sp[k]);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sp[k] += lane_xor<2>(sp[k]);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sp[k] += lane_xor<4>(sp[k]);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sp[k] += lane_xor<8>(sp[k]);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sp[k] += lane_xor<16>(sp[k]);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sp[k] += __shfl_xor(sp[k], 32, 64);
       if (lane < 8) {
         double v = sp[0];
 #pragma unroll

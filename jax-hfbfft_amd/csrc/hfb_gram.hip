@@ -138,14 +138,26 @@ __global__ void __launch_bounds__(256, 3) gram_vec_kernel(GramVecArgs ga) {
         ci[a][b] = fma(av[a].x, bv[b].y, fma(-av[a].y, bv[b].x, ci[a][b]));
       }
   }
+  // lane sums, level by level over all 32 values so the chains overlap
+#define HFB_GV_LEVEL(XOR)                                \
+  _Pragma("unroll") for (int a = 0; a < GV_T; ++a)       \
+  _Pragma("unroll") for (int b = 0; b < GV_T; ++b) {     \
+    cr[a][b] += XOR(cr[a][b]);                           \
+    ci[a][b] += XOR(ci[a][b]);                           \
+  }
+  HFB_GV_LEVEL(lane_xor<1>)
+  HFB_GV_LEVEL(lane_xor<2>)
+  HFB_GV_LEVEL(lane_xor<4>)
+  HFB_GV_LEVEL(lane_xor<8>)
+  HFB_GV_LEVEL(lane_xor<16>)
+#undef HFB_GV_LEVEL
 #pragma unroll
   for (int a = 0; a < GV_T; ++a)
 #pragma unroll
-    for (int b = 0; b < GV_T; ++b)
-      for (int o = 32; o > 0; o >>= 1) {
-        cr[a][b] += __shfl_xor(cr[a][b], o, 64);
-        ci[a][b] += __shfl_xor(ci[a][b], o, 64);
-      }
+    for (int b = 0; b < GV_T; ++b) {
+      cr[a][b] += __shfl_xor(cr[a][b], 32, 64);
+      ci[a][b] += __shfl_xor(ci[a][b], 32, 64);
+    }
   if (lane == 0) {
     const int np = nt * GV_T;
 #pragma unroll

@@ -82,10 +82,18 @@ __device__ __forceinline__ void tile_for(int n, F&& f) {
 // Sum of two values over the block, returned to every thread.
 template <int NT>
 __device__ __forceinline__ void block_sum2_all(double& a, double& b, double* red) {
-  for (int o = 32; o > 0; o >>= 1) {
-    a += __shfl_xor(a, o, 64);
-    b += __shfl_xor(b, o, 64);
-  }
+  a += lane_xor<1>(a);
+  b += lane_xor<1>(b);
+  a += lane_xor<2>(a);
+  b += lane_xor<2>(b);
+  a += lane_xor<4>(a);
+  b += lane_xor<4>(b);
+  a += lane_xor<8>(a);
+  b += lane_xor<8>(b);
+  a += lane_xor<16>(a);
+  b += lane_xor<16>(b);
+  a += __shfl_xor(a, 32, 64);
+  b += __shfl_xor(b, 32, 64);
   const int wid = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     red[2 * wid] = a;
@@ -102,7 +110,12 @@ __device__ __forceinline__ void block_sum2_all(double& a, double& b, double* red
 
 template <int NT>
 __device__ __forceinline__ double block_max_all(double v, double* red) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  v = fmax(v, lane_xor<1>(v));
+  v = fmax(v, lane_xor<2>(v));
+  v = fmax(v, lane_xor<4>(v));
+  v = fmax(v, lane_xor<8>(v));
+  v = fmax(v, lane_xor<16>(v));
+  v = fmax(v, __shfl_xor(v, 32, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   v = red[0];

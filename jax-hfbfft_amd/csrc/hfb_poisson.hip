@@ -2,7 +2,11 @@
 // (coulomb.py:14-75).  A 3-D transform is three pencil passes; the
 // Poisson solve fuses zero-padding into the first pass, the kernel product
 // into the x pass (forward, multiply, inverse) and the corner extraction
-// into the last pass.
+// into the last pass.  On a cubic doubled grid the solve takes three
+// launches that skip the known zeros: y-z planes of the nx nonzero x
+// planes (forward along z then y, the plane in LDS), x pencils (forward
+// with the upper half implicitly zero, kernel product, inverse, lower half
+// kept), y-z planes again (inverse along y then z, corner extracted).
 #include <math.h>
 
 #include "hfb_ctx.h"
@@ -110,6 +114,148 @@ __global__ void __launch_bounds__(128) pois_pass(cx<double>* __restrict__ buf, P
   }
 }
 
+// ----- cubic doubled grid, three launches -------------------------------
+constexpr int PP_L = 8;  // lanes per pencil
+
+// Forward y-z plane of x plane blockIdx.x (< nx): z transforms of the rows
+// y < ny (zero-padded beyond nz), rows y >= ny zero, then y transforms;
+// out[x][ky][kz].
+template <int N>
+__global__ void __launch_bounds__(N * PP_L) pois_plane_fwd(cx<double>* __restrict__ out, const double* __restrict__ rho,
+                                                           int nx, int ny, int nz, const cx<double>* __restrict__ tw) {
+  constexpr int L = PP_L, N1 = N / L, P = N + 1;
+  extern __shared__ unsigned char pp_lds[];
+  cx<double>* pl = (cx<double>*)pp_lds;  // [y][kz]
+  const int j = threadIdx.x % L, q = threadIdx.x / L;
+  const int x = blockIdx.x;
+  LaneFFT<double, N, L> F;
+  F.init(j, tw);
+  cx<double> v[N1];
+  if (q < ny) {  // z pencil q = y
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {
+      const int z = j + L * m;
+      v[m] = {z < nz ? rho[((int64_t)x * ny + q) * nz + z] : 0.0, 0.0};
+    }
+    F.fwd(v, tw);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) pl[q * P + F.freq(m)] = v[m];
+  } else {
+#pragma unroll
+    for (int m = 0; m < N1; ++m) pl[q * P + j + L * m] = {0.0, 0.0};
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < N1; ++m) v[m] = pl[(j + L * m) * P + q];  // y pencil q = kz
+  F.fwd(v, tw);
+#pragma unroll
+  for (int m = 0; m < N1; ++m) out[((int64_t)x * N + F.freq(m)) * N + q] = v[m];
+}
+
+// x pencils (ky, kz): forward with x >= nx zero, times scale * qhat,
+// inverse, rows x < nx written back.  PP_PW pencils per 64-lane wavefront.
+constexpr int PP_PW = 64 / PP_L;
+
+template <int N>
+__global__ void __launch_bounds__(64) pois_pencil_mul(cx<double>* __restrict__ buf, const cx<double>* __restrict__ qhat,
+                                                      double scale, int nx, const cx<double>* __restrict__ tw) {
+  constexpr int L = PP_L, N1 = N / L;
+  const int j = threadIdx.x % L;
+  const int g = blockIdx.x * PP_PW + threadIdx.x / L;  // pencil = ky * N + kz
+  const int64_t plane = (int64_t)N * N;
+  LaneFFT<double, N, L> F;
+  F.init(j, tw);
+  cx<double> v[N1];
+#pragma unroll
+  for (int m = 0; m < N1; ++m) {
+    const int x = j + L * m;
+    v[m] = x < nx ? buf[x * plane + g] : cx<double>{0.0, 0.0};
+  }
+  F.fwd(v, tw);
+#pragma unroll
+  for (int m = 0; m < N1; ++m) v[m] = cmul(v[m], cscale(qhat[F.freq(m) * plane + g], scale));
+  F.inv(v, tw);
+#pragma unroll
+  for (int m = 0; m < N1; ++m) {
+    const int x = j + L * m;
+    if (x < nx) buf[x * plane + g] = v[m];
+  }
+}
+
+// Inverse y-z plane of x plane blockIdx.x (< nx): y transforms of every kz,
+// rows y < ny kept in LDS, z transforms of those rows, wcoul[x][y][z] for
+// z < nz (real part, times scale = 1 / N^3).
+template <int N>
+__global__ void __launch_bounds__(N * PP_L) pois_plane_inv(const cx<double>* __restrict__ in, double* __restrict__ wcoul,
+                                                           double scale, int ny, int nz,
+                                                           const cx<double>* __restrict__ tw) {
+  constexpr int L = PP_L, N1 = N / L, P = N + 1;
+  extern __shared__ unsigned char pp_lds[];
+  cx<double>* pl = (cx<double>*)pp_lds;  // [y][kz]
+  const int j = threadIdx.x % L, q = threadIdx.x / L;
+  const int x = blockIdx.x;
+  LaneFFT<double, N, L> F;
+  F.init(j, tw);
+  cx<double> v[N1];
+#pragma unroll
+  for (int m = 0; m < N1; ++m) v[m] = in[((int64_t)x * N + F.freq(m)) * N + q];  // y pencil q = kz
+  F.inv(v, tw);
+#pragma unroll
+  for (int m = 0; m < N1; ++m) {
+    const int y = j + L * m;
+    if (y < ny) pl[y * P + q] = v[m];
+  }
+  __syncthreads();
+  if (q < ny) {  // z pencil q = y
+#pragma unroll
+    for (int m = 0; m < N1; ++m) v[m] = pl[q * P + F.freq(m)];
+    F.inv(v, tw);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {
+      const int z = j + L * m;
+      if (z < nz) wcoul[((int64_t)x * ny + q) * nz + z] = v[m].x * scale;
+    }
+  }
+}
+
+int poisson_planes(const hfb_ctx* c, const double* rho_p, double* wcoul, double e2, cx<double>* buf,
+                   hipStream_t st) {
+  const int n2 = c->nx2;
+  const cx<double>* tw = twiddles<double>(c, n2);
+  if (!tw) return HFB_E_SIZE;
+  const double sc = e2 * c->dx * c->dy * c->dz;
+  const double inv = 1.0 / ((double)n2 * n2 * n2);
+#define HFB_PPL(NN)                                                                                         \
+  case NN: {                                                                                                \
+    const size_t lds = sizeof(cx<double>) * (size_t)NN * (NN + 1);                                          \
+    allow_full_lds((const void*)pois_plane_fwd<NN>);                                                        \
+    allow_full_lds((const void*)pois_plane_inv<NN>);                                                        \
+    hipLaunchKernelGGL(pois_plane_fwd<NN>, dim3(c->nx), dim3(NN * PP_L), lds, st, buf, rho_p, c->nx, c->ny,   \
+                       c->nz, tw);                                                                          \
+    int rc = launch_status();                                                                               \
+    if (rc != HFB_OK) return rc;                                                                            \
+    hipLaunchKernelGGL(pois_pencil_mul<NN>, dim3(NN * NN / PP_PW), dim3(64), 0, st, buf, c->qhat, sc, c->nx,  \
+                       tw);                                                                                 \
+    rc = launch_status();                                                                                   \
+    if (rc != HFB_OK) return rc;                                                                            \
+    hipLaunchKernelGGL(pois_plane_inv<NN>, dim3(c->nx), dim3(NN * PP_L), lds, st, buf, wcoul, inv, c->ny,     \
+                       c->nz, tw);                                                                          \
+    return launch_status();                                                                                 \
+  }
+  switch (n2) {
+    HFB_PPL(48)
+    HFB_PPL(64)
+    HFB_PPL(96)
+    default:
+      return HFB_E_SIZE;
+  }
+#undef HFB_PPL
+}
+
+bool planes_apply(const hfb_ctx* c) {
+  return !c->periodic && c->nx2 == c->ny2 && c->ny2 == c->nz2 && (c->nx2 == 48 || c->nx2 == 64 || c->nx2 == 96);
+}
+
 template <int AXIS, int MODE>
 int launch_pass(const hfb_ctx* c, cx<double>* buf, const PoisArgs& a, hipStream_t st) {
   constexpr int P = 8;
@@ -170,8 +316,9 @@ extern "C" int hfb_poisson(hfb_ctx* c, const double* rho, double* wcoul, double 
   if (!rho || !wcoul || !work) return HFB_E_ARG;
   hipStream_t st = (hipStream_t)stream;
   cx<double>* buf = (cx<double>*)work;
-  PoisArgs a = base_args(c);
   const int64_t nvox = (int64_t)c->nx * c->ny * c->nz;
+  if (planes_apply(c)) return poisson_planes(c, rho + nvox, wcoul, e2, buf, st);
+  PoisArgs a = base_args(c);
   a.rho = rho + nvox;  // proton density rho[1] (coulomb.py:66)
   a.wcoul = wcoul;
   int rc = launch_pass<2, PM_PAD>(c, buf, a, st);
