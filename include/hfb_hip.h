@@ -59,7 +59,8 @@ enum {
   HFB_EN_JTOT = HFB_EN_SPIN + 3, HFB_EN_EFL1 = HFB_EN_JTOT + 3, HFB_EN_EFL2, HFB_EN_EFL1Q /* 2 */,
   HFB_EN_EFL2Q = HFB_EN_EFL1Q + 2, HFB_EN_ECORRP = HFB_EN_EFL2Q + 2, HFB_EN_SUMFLU, HFB_EN_DELESUM,
   HFB_EN_EPAIR /* 2 */, HFB_EN_JACOBI = HFB_EN_EPAIR + 2 /* 2 */, HFB_EN_NSITER = HFB_EN_JACOBI + 2 /* 2 */,
-  HFB_EN_LEN = 48
+  HFB_EN_VMAX = HFB_EN_NSITER + 2 /* 2: max of v_pair per isospin, from hfb_skyrme */,
+  HFB_EN_LEN = 50
 };
 
 const char* hfb_strerror(int code);
@@ -125,10 +126,12 @@ int hfb_poisson(hfb_ctx* ctx, const double* rho, double* wcoul, double e2, void*
  * wuv = wstates*wguv*pairwg (w=1, wuv=0 when tbcs), and the preconditioner
  * parameters of e0dmp_gt_zero (static.py:131-143): pw = weight of the
  * (E0 - h2ma k^2) term, pshift = 1/2 max(wuv,.1) max_r v_pair[iq].
- * v_pair: (2, nvox) double. */
+ * vpair_max (2 doubles, e.g. E + HFB_EN_VMAX as written by hfb_skyrme)
+ * gives max_r v_pair[iq]; when it is null, v_pair (2, nvox) double is
+ * scanned instead. */
 int hfb_grstep_weights(int nstates, int npsi0, int tbcs, const double* wocc, const double* wstates,
                        const double* wguv, const double* pairwg, const double* v_pair, long long nvox,
-                       double* w, double* wuv, double* pw, double* pshift, void* stream);
+                       const double* vpair_max, double* w, double* wuv, double* pw, double* pshift, void* stream);
 
 /* Number of partial-sum tiles per field written by hfb_precond_step. */
 int hfb_step_ntiles(hfb_ctx* ctx, int prec);
@@ -136,6 +139,8 @@ int hfb_step_ntiles(hfb_ctx* ctx, int prec);
 /* In place psi <- psi - x0dmp * laplace(r) for nstates two-spinors
  * (e0dmp_gt_zero, static.py:93-146, with laplace levels.py:397-437):
  *   h_exp = wxyz * hsum[2s+1]   (Re<psi|ps1>, from hfb_hpsi's partials)
+ * hsum: per-state sums (S, 2) when hsum_ntile == 0, else hfb_hpsi's
+ * partials (S, hsum_ntile, 2), added in tile order inside the kernels.
  *   r = ps1 - lagrange          if use_lagrange   (iteration > 1 and !tbcs)
  *   r = ps1 - h_exp psi         otherwise
  *   laplace: FFT3 r / (pw (e0dmp - h2ma k^2) + pshift), Nyquist kept.
@@ -143,17 +148,19 @@ int hfb_step_ntiles(hfb_ctx* ctx, int prec);
  * psi|^2) partial sums for sp_efluct1/2 (static.py:120-129); reduce with
  * hfb_reduce_partials.  work: one spinor block of prec's complex type. */
 int hfb_precond_step(hfb_ctx* ctx, int prec, void* psi, const void* ps1, const void* lagrange, int nstates,
-                     const double* hsum, double wxyz, int use_lagrange, const double* pw, const double* pshift,
+                     const double* hsum, int hsum_ntile, double wxyz, int use_lagrange, const double* pw, const double* pshift,
                      double e0dmp, double h2ma, double x0dmp, double* fpart, void* work, void* stream);
 
 /* Scalar tail of grstep (static.py:204-213, 250-257): sp_energy <- wxyz *
  * hsum[2s]; denerg = (spe_old - spe_new)/|spe_new|; sp_efluct1/2 <-
  * sqrt(fsum) when fsum is given (mprint iterations); acc[0] += sum
  * sp_efluct1 (static.sumflu), acc[1] += sum wocc*wstates*denerg
- * (static.delesum). */
-int hfb_grstep_finish(int nstates, const double* hsum, double wxyz, double* sp_energy, const double* wocc,
-                      const double* wstates, const double* fsum, double* sp_efluct1, double* sp_efluct2,
-                      double* acc, void* stream);
+ * (static.delesum).  hsum as in hfb_precond_step; fsum: (S, 2) totals
+ * (fsum_ntile == 0) or hfb_precond_step's fpart with fsum_ntile = twice its
+ * tile count (both spin fields of a state), scaled by wxyz here. */
+int hfb_grstep_finish(int nstates, const double* hsum, int hsum_ntile, double wxyz, double* sp_energy,
+                      const double* wocc, const double* wstates, const double* fsum, int fsum_ntile,
+                      double* sp_efluct1, double* sp_efluct2, double* acc, void* stream);
 
 /* ---------------------------------------------------------------------
  * diagstep (static.py:261-577): Gram matrices on MFMA, small dense
@@ -191,8 +198,10 @@ int hfb_gram2(int prec, const void* const* A, const void* const* B, int nm, cons
  * have at most 36 states (the work matrices then live in LDS), else
  * 10 * nmax^2 complex doubles.  stats (optional, 4
  * doubles): Jacobi sweeps of the lambda_lin eigensolve per isospin, then
- * Newton-Schulz iterations per isospin (-1: fell back to the eigensolve). */
-int hfb_diag_small(int nst0, int nst1, int nmax, const double* const* mats, int tdiag, int ipair, int tbcs,
+ * Newton-Schulz iterations per isospin (-1: fell back to the eigensolve).
+ * efluct1/2 are combined over the isospins inside the same launch (an
+ * arrival counter of the context). */
+int hfb_diag_small(hfb_ctx* c, int nst0, int nst1, int nmax, const double* const* mats, int tdiag, int ipair, int tbcs,
                    int construct, const double* wocc, const double* wstates, const double* wguv,
                    const double* pairwg, double* sp_norm, double* sp_energy, double* deltaf, double* hmatrix,
                    double* gapmatrix, double* symcond, double* lambda_save, double* efluct1q, double* efluct2q,
@@ -266,14 +275,16 @@ int hfb_density_derivs(hfb_ctx* ctx, const double* rho, const double* sdens, con
 int hfb_meanfield_derivs(hfb_ctx* ctx, const double* aq, const double* bmass, double* divaq, double* dbmass,
                          double* work, void* stream);
 
-/* Doubles of the epart scratch of hfb_skyrme (x20 for hfb_integ_energy). */
+/* Workgroups of the skyrme and energy kernels: hfb_skyrme's epart scratch
+ * holds 3x, hfb_integ_energy's 20x that many doubles. */
 int hfb_skyrme_nblocks(hfb_ctx* ctx);
 
 /* Mean-field potentials upot, bmass, v_pair (2,n3), wlspot, aq, spot
  * (2,3,n3) from the densities, the derived fields and wcoul (skyrme,
  * meanfield.py:163-391, incl. the DDDI rearrangement and ecorrp into
- * E[HFB_EN_ECORRP]).  The *32 outputs are float copies for the FP32
- * spinor kernels and may be null. */
+ * E[HFB_EN_ECORRP]).  The maximum of v_pair per isospin goes to
+ * E[HFB_EN_VMAX..+1] (for hfb_grstep_weights).  The *32 outputs are
+ * float copies for the FP32 spinor kernels and may be null. */
 int hfb_skyrme(hfb_ctx* ctx, const hfb_force* f, const double* rho, const double* tau, const double* chi,
                const double* current, const double* sdens, const double* divj, const double* lap,
                const double* grad, const double* curls, const double* curlj, const double* wcoul, double* upot,

@@ -33,11 +33,12 @@ _SIGS = {
     "hfb_selftest_mfma_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "hfb_step_ntiles": (c_int, [c_void_p, c_int]),
     "hfb_grstep_weights": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                   ctypes.c_longlong, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
-    "hfb_precond_step": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_double, c_int,
-                                 c_void_p, c_void_p, c_double, c_double, c_double, c_void_p, c_void_p, c_void_p]),
-    "hfb_grstep_finish": (c_int, [c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                  c_void_p, c_void_p, c_void_p]),
+                                   ctypes.c_longlong, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "hfb_precond_step": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_double,
+                                 c_int, c_void_p, c_void_p, c_double, c_double, c_double, c_void_p, c_void_p,
+                                 c_void_p]),
+    "hfb_grstep_finish": (c_int, [c_int, c_void_p, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                  c_void_p, c_void_p, c_void_p, c_void_p]),
     "hfb_gram_workspace": (c_int, [c_int, ctypes.c_longlong]),
     "hfb_gram": (c_int, [c_int, c_void_p, POINTER(c_void_p), c_int, c_int, ctypes.c_longlong, c_double, c_void_p,
                          c_void_p, c_void_p]),
@@ -47,7 +48,7 @@ _SIGS = {
                            c_void_p]),
     "hfb_rotate2": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_longlong,
                             c_void_p]),
-    "hfb_diag_small": (c_int, [c_int, c_int, c_int, POINTER(c_void_p), c_int, c_int, c_int, c_int, c_void_p,
+    "hfb_diag_small": (c_int, [c_void_p, c_int, c_int, c_int, POINTER(c_void_p), c_int, c_int, c_int, c_int, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_void_p),
                                c_void_p, c_void_p, c_void_p]),

@@ -453,4 +453,28 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* scratch) {
   __syncthreads();
 }
 
+// Single-pass reductions across workgroups.  A producer
+// that has written its share calls *_arrive on a counter with the number of
+// producers; the call returns true in exactly one of them, the last to
+// arrive, which may then read every share written before the other
+// arrivals.  The last arrival resets the counter to zero, so every launch
+// (and every replay of a captured launch) starts from a clean counter.
+// Release: each thread's device-scope fence before the count; acquire: the
+// fence in the last producer after it.
+__device__ __forceinline__ bool block_arrive(unsigned int* counter, unsigned int expected) {
+  __shared__ unsigned int arrive_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int prev = atomicAdd(counter, 1u);
+    const bool last = prev + 1u == expected;
+    if (last) atomicExch(counter, 0u);
+    arrive_last = last ? 1u : 0u;
+  }
+  __syncthreads();
+  const bool last = arrive_last != 0u;
+  if (last) __threadfence();
+  return last;
+}
+
 }  // namespace hfb

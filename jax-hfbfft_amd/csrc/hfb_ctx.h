@@ -20,6 +20,17 @@ struct hfb_ctx {
   double* der1[3];  // dense first/second derivative matrices (n x n), FP64
   double* der2[3];
   hfb::cx<double>* qhat;  // FFT of the Coulomb kernel on the doubled grid
+  // arrival counters of the single-pass reductions (zero between launches;
+  // regions below), so calls on one context must not overlap in time
+  unsigned int* sync;
+};
+
+// Counter slots of hfb_ctx::sync.  A device-scope fence writes back and
+// invalidates an XCD's L2, so arrival counting only pays off for launches
+// of a few workgroups (it costs more than a separate launch for wide ones).
+enum : int {
+  HFB_SYNC_DIAG = 0,      // diag_kernel: one counter for its two isospin blocks
+  HFB_SYNC_COUNT = 64
 };
 
 namespace hfb {

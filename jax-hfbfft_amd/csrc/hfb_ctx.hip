@@ -113,6 +113,12 @@ extern "C" int hfb_ctx_create(hfb_ctx** out, int device, int nx, int ny, int nz,
       rc = HFB_E_HIP;
   }
   if (rc == HFB_OK) rc = hfb_coulomb_setup(c);
+  if (rc == HFB_OK) {
+    const size_t bytes = sizeof(unsigned int) * HFB_SYNC_COUNT;
+    if (hipMalloc(&c->sync, bytes) != hipSuccess || hipMemset(c->sync, 0, bytes) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess)
+      rc = HFB_E_HIP;
+  }
   if (rc != HFB_OK) {
     hfb_ctx_destroy(c);
     return rc;
@@ -132,6 +138,7 @@ extern "C" int hfb_ctx_destroy(hfb_ctx* c) {
     if (c->der2[a]) hipFree(c->der2[a]);
   }
   if (c->qhat) hipFree(c->qhat);
+  if (c->sync) hipFree(c->sync);
   free(c);
   return HFB_OK;
 }

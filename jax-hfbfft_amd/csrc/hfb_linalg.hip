@@ -46,6 +46,9 @@ struct DiagArgs {
   double* deltaf;
   double* efl1q;
   double* efl2q;
+  double* efl1;
+  double* efl2;
+  unsigned int* sync;  // arrival counter of the two isospin blocks
   double* stats;  // optional: [Jacobi sweeps (2), Newton-Schulz iterations (2, -1 = fell back)]
   int tdiag, ipair, tbcs, construct;
 };
@@ -393,6 +396,15 @@ __device__ __forceinline__ bool newton_schulz(cx<double>*& Y, cx<double>*& Z, cx
   return false;
 }
 
+// efluct1 = max over the isospins, efluct2 = their mean (static.py:571-576),
+// by the second of the two isospin blocks to finish.
+__device__ __forceinline__ void diag_finish_arrive(const DiagArgs& a) {
+  if (block_arrive(a.sync, gridDim.x) && threadIdx.x == 0) {
+    a.efl1[0] = fmax(a.efl1q[0], a.efl1q[1]);
+    a.efl2[0] = 0.5 * (a.efl2q[0] + a.efl2q[1]);
+  }
+}
+
 template <int NP, int NT>
 __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
   extern __shared__ double lds_raw[];
@@ -401,7 +413,10 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
   const int iq = blockIdx.x;
   const DiagIso d = a.iso[iq];
   const int n = d.nst;
-  if (n <= 0) return;
+  if (n <= 0) {
+    if (a.construct) diag_finish_arrive(a);
+    return;
+  }
   const int tid = threadIdx.x;
   const int nm = a.nmax;
   // five work matrices, in LDS for small blocks and in global memory (L2
@@ -540,23 +555,20 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
   // M2 = U lambda+ (static.py:557-570)
   mm<NP, NT, false>(B0, U, B1, n);
   tile_for<NP, NT>(n, [&](int i, int j) { d.M2[i * n + j] = B0[i * n + j]; });
-}
-
-__global__ void diag_finish_kernel(const double* efl1q, const double* efl2q, double* efl1, double* efl2) {
-  efl1[0] = fmax(efl1q[0], efl1q[1]);
-  efl2[0] = 0.5 * (efl2q[0] + efl2q[1]);
+  if (a.construct) diag_finish_arrive(a);
 }
 
 }  // namespace
 
 // mats[iq*4 + m]: m=0 S, 1 H_old, 2 lambda_lin (or null), 3 G_old (or null),
 // each nst_iq x nst_iq complex; uo[iq*2]: U, uo[iq*2+1]: M2 outputs.
-extern "C" int hfb_diag_small(int nst0, int nst1, int nmax, const double* const* mats, int tdiag, int ipair,
+extern "C" int hfb_diag_small(hfb_ctx* c, int nst0, int nst1, int nmax, const double* const* mats, int tdiag, int ipair,
                               int tbcs, int construct, const double* wocc, const double* wstates, const double* wguv,
                               const double* pairwg, double* sp_norm, double* sp_energy, double* deltaf,
                               double* hmatrix, double* gapmatrix, double* symcond, double* lambda_save,
                               double* efluct1q, double* efluct2q, double* efluct1, double* efluct2,
                               double* const* uo, double* work, double* stats, void* stream) {
+  if (!c) return HFB_E_NOCTX;
   if (!mats || !uo || !wocc || !wstates || !wguv || !pairwg || !sp_norm || !sp_energy || !deltaf ||
       !hmatrix || !gapmatrix || !symcond || !lambda_save || !efluct1q || !efluct2q || !efluct1 || !efluct2)
     return HFB_E_ARG;
@@ -595,6 +607,9 @@ extern "C" int hfb_diag_small(int nst0, int nst1, int nmax, const double* const*
   a.deltaf = deltaf;
   a.efl1q = efluct1q;
   a.efl2q = efluct2q;
+  a.efl1 = efluct1;
+  a.efl2 = efluct2;
+  a.sync = c->sync + HFB_SYNC_DIAG;
   a.tdiag = tdiag;
   a.ipair = ipair;
   a.tbcs = tbcs;
@@ -619,11 +634,5 @@ extern "C" int hfb_diag_small(int nst0, int nst1, int nmax, const double* const*
     HFB_DIAG(128, 1024, 0)
   }
 #undef HFB_DIAG
-  int rc = launch_status();
-  if (rc != HFB_OK) return rc;
-  if (construct) {
-    hipLaunchKernelGGL(diag_finish_kernel, dim3(1), dim3(1), 0, st, efluct1q, efluct2q, efluct1, efluct2);
-    rc = launch_status();
-  }
-  return rc;
+  return launch_status();
 }

@@ -178,14 +178,16 @@ struct SkyArgs {
   const double* wcoul;
   double *upot, *bmass, *wlspot, *aq, *spot, *vpair;  // FP64 out
   float *upot32, *bmass32, *wlspot32, *aq32, *spot32, *vpair32;  // optional FP32 copies
-  double* epart;  // per-block partial of sum rho_tot * rearr (DDDI)
+  double* epart;  // per-block partials: [0, nb) sum rho_tot * rearr (DDDI), [nb, 3 nb) max v_pair per isospin
 };
 
 __global__ void __launch_bounds__(256) skyrme_combine_kernel(SkyArgs a) {
   const hfb_force& f = a.f;
   const int64_t n = a.nvox;
   __shared__ double red[4];
+  __shared__ double vred[4][2];
   double ep[1] = {0.0};
+  double vm[2] = {-INFINITY, -INFINITY};
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
     const double rn = a.rho[v], rp = a.rho[n + v];
     const double rt = rn + rp;
@@ -219,6 +221,7 @@ __global__ void __launch_bounds__(256) skyrme_combine_kernel(SkyArgs a) {
       a.upot[iq * n + v] = u;
       a.bmass[iq * n + v] = b;
       a.vpair[iq * n + v] = vp;
+      vm[iq] = fmax(vm[iq], vp);
       if (a.upot32) {
         a.upot32[iq * n + v] = float(u);
         a.bmass32[iq * n + v] = float(b);
@@ -241,9 +244,23 @@ __global__ void __launch_bounds__(256) skyrme_combine_kernel(SkyArgs a) {
       }
     }
   }
-  if (a.epart) {
-    block_sum<1>(ep, red);
-    if (threadIdx.x == 0) a.epart[blockIdx.x] = ep[0];
+  block_sum<1>(ep, red);
+  if (threadIdx.x == 0) a.epart[blockIdx.x] = ep[0];
+  // per-block maxima of v_pair
+  for (int o = 32; o > 0; o >>= 1) {
+    vm[0] = fmax(vm[0], __shfl_xor(vm[0], o, 64));
+    vm[1] = fmax(vm[1], __shfl_xor(vm[1], o, 64));
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    vred[wid][0] = vm[0];
+    vred[wid][1] = vm[1];
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double m = vred[0][threadIdx.x];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, vred[w][threadIdx.x]);
+    a.epart[(1 + threadIdx.x) * gridDim.x + blockIdx.x] = m;
   }
 }
 
@@ -372,12 +389,35 @@ __global__ void energy_finish_kernel(EnArgs a, int nblk, double wxyz, const doub
   E[HFB_EN_EHFINT] = ehft + ehf0 + ehf1 + ehf2 + ehf3 + ehfls + ehfc - epair - ezpe;
 }
 
+// ecorrp from the block partials (meanfield.py:360-363) and the maxima of
+// v_pair per isospin into E[HFB_EN_VMAX..+1].
 __global__ void ecorrp_finish_kernel(const double* part, int nblk, double wxyz, int ipair, double* E) {
   __shared__ double red[4];
+  __shared__ double vred[4][2];
   double s[1] = {0.0};
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s[0] += part[b];
+  double vm[2] = {-INFINITY, -INFINITY};
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    s[0] += part[b];
+    vm[0] = fmax(vm[0], part[nblk + b]);
+    vm[1] = fmax(vm[1], part[2 * nblk + b]);
+  }
   block_sum<1>(s, red);
   if (threadIdx.x == 0) E[HFB_EN_ECORRP] = ipair == 6 ? -s[0] * wxyz / 2.0 : 0.0;
+  for (int o = 32; o > 0; o >>= 1) {
+    vm[0] = fmax(vm[0], __shfl_xor(vm[0], o, 64));
+    vm[1] = fmax(vm[1], __shfl_xor(vm[1], o, 64));
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    vred[wid][0] = vm[0];
+    vred[wid][1] = vm[1];
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double m = vred[0][threadIdx.x];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, vred[w][threadIdx.x]);
+    E[HFB_EN_VMAX + threadIdx.x] = m;
+  }
 }
 
 // sum_energy (energies.py:309-371)

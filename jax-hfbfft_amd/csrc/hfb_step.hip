@@ -29,7 +29,8 @@ struct StepArgs {
   const void* lag;       // Lagrange field
   void* psi;             // in/out
   void* work;            // transform scratch, same shape as psi
-  const double* hsum;    // (S,2): (sum Re psi* h psi, sum Re psi* ps1)
+  const double* hsum;    // (S,2): (sum Re psi* h psi, sum Re psi* ps1), or (S, hsum_nt, 2) partials
+  int hsum_nt;           // 0: hsum holds totals
   double wxyz;
   int use_lag;           // (iteration > 1) && !tbcs  (static.py:99-104)
   const double* pw;      // max(weight, 0.1) per state
@@ -38,6 +39,16 @@ struct StepArgs {
   double kfx, kfy, kfz;  // 2 pi / (d n) per axis
   double* fpart;         // (2S fields, ntile, 2) partials of |r|^2, |lag - h_exp psi|^2
 };
+
+// Total j of state s from per-state sums (nt == 0) or from nt tile
+// partials per state, added in tile order.
+__device__ __forceinline__ double state_sum(const double* __restrict__ p, int nt, int s, int j) {
+  if (nt <= 0) return p[2 * s + j];
+  const double* q = p + (int64_t)s * nt * 2 + j;
+  double v = 0.0;
+  for (int t = 0; t < nt; ++t) v += q[2 * t];
+  return v;
+}
 
 constexpr int STEP_L = 4;                 // lanes per pencil
 constexpr int STEP_PPB = 4 * (64 / STEP_L);  // pencils per block
@@ -82,12 +93,16 @@ __global__ void __launch_bounds__(256) step_lane(StepArgs a, const cx<T>* __rest
   constexpr bool IN_SPECTRAL = MODE == SM_INV || MODE == SM_OUT;  // input pencil holds frequencies
   cx<T> v[N1];
   double acc[2] = {0.0, 0.0};
+  // h_exp only enters the residual without the Lagrange field and the
+  // fluctuation sums, so the partials are not read otherwise
+  T hexp = T(0);
+  if constexpr (MODE == SM_IN)
+    if (!a.use_lag || a.fpart) hexp = T(a.wxyz * state_sum(a.hsum, a.hsum_nt, s, 1));
 #pragma unroll
   for (int m = 0; m < N1; ++m) {
     const int kk = IN_SPECTRAL ? F.freq(m) : j + L * m;
     const int64_t off = fo + base + (int64_t)kk * stride;
     if constexpr (MODE == SM_IN) {
-      const T hexp = T(a.wxyz * a.hsum[2 * s + 1]);
       const cx<T> p1 = ((const cx<T>*)a.ps1)[off];
       const cx<T> ps = psi[off];
       const cx<T> lg = ((const cx<T>*)a.lag)[off];
@@ -163,7 +178,9 @@ __global__ void __launch_bounds__(N * PLANE_L) step_plane(StepArgs a, const cx<T
   cx<T> v[N1];
   if constexpr (FWD) {
     double acc[2] = {0.0, 0.0};
-    const T hexp = T(a.wxyz * a.hsum[2 * s + 1]);
+    // h_exp only enters the residual without the Lagrange field and the
+    // fluctuation sums, so the partials are not read otherwise
+    const T hexp = (!a.use_lag || a.fpart) ? T(a.wxyz * state_sum(a.hsum, a.hsum_nt, s, 1)) : T(0);
 #pragma unroll
     for (int m = 0; m < N1; ++m) {  // z pencil q = y
       const int64_t off = pb + (int64_t)q * N + j + L * m;
@@ -254,14 +271,38 @@ int step_all(const hfb_ctx* c, const StepArgs& a, int ns, hipStream_t st) {
 __global__ void grstep_weights_kernel(int ns, int npsi0, int tbcs, const double* __restrict__ wocc,
                                       const double* __restrict__ wstates, const double* __restrict__ wguv,
                                       const double* __restrict__ pairwg, const double* __restrict__ vpair,
-                                      int64_t nvox, double* __restrict__ w, double* __restrict__ wuv,
-                                      double* __restrict__ pw, double* __restrict__ pshift) {
+                                      int64_t nvox, const double* __restrict__ vpair_max, double* __restrict__ w,
+                                      double* __restrict__ wuv, double* __restrict__ pw,
+                                      double* __restrict__ pshift) {
   __shared__ double vmax_s[2];
   __shared__ double red[16];
   for (int iq = 0; iq < 2; ++iq) {
     double m = -INFINITY;
-    if (!tbcs)
-      for (int64_t i = threadIdx.x; i < nvox; i += blockDim.x) m = fmax(m, vpair[iq * nvox + i]);
+    if (!tbcs && vpair_max) {
+      if (threadIdx.x == 0) m = vpair_max[iq];
+    } else if (!tbcs) {
+      // eight loads in flight per thread (four independent max chains); the
+      // field offset iq * nvox keeps 16-byte alignment when nvox is even
+      const double* v = vpair + iq * nvox;
+      const int64_t bd = blockDim.x;
+      int64_t i = threadIdx.x;
+      if ((nvox & 1) == 0) {
+        const double2* v2 = reinterpret_cast<const double2*>(v);
+        const int64_t n2 = nvox / 2;
+        double m0 = -INFINITY, m1 = -INFINITY, m2 = -INFINITY, m3 = -INFINITY;
+        for (; i + 3 * bd < n2; i += 4 * bd) {
+          const double2 a = v2[i], b = v2[i + bd], c = v2[i + 2 * bd], d = v2[i + 3 * bd];
+          m0 = fmax(m0, fmax(a.x, a.y));
+          m1 = fmax(m1, fmax(b.x, b.y));
+          m2 = fmax(m2, fmax(c.x, c.y));
+          m3 = fmax(m3, fmax(d.x, d.y));
+        }
+        for (; i < n2; i += bd) m0 = fmax(m0, fmax(v2[i].x, v2[i].y));
+        m = fmax(fmax(m0, m1), fmax(m2, m3));
+      } else {
+        for (; i < nvox; i += bd) m = fmax(m, v[i]);
+      }
+    }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
     if (lane == 0) red[wid] = m;
@@ -290,24 +331,59 @@ __global__ void grstep_weights_kernel(int ns, int npsi0, int tbcs, const double*
 }
 
 // Bookkeeping after the step (static.py:173, 211, 247-257).
-__global__ void grstep_finish_kernel(int ns, const double* __restrict__ hsum, double wxyz,
+// hsum / fsum: per-state totals (nt == 0) or tile partials (nt > 0; fsum's
+// partials then still carry no wxyz factor).
+__global__ void grstep_finish_kernel(int ns, const double* __restrict__ hsum, int hsum_nt, double wxyz,
                                      double* __restrict__ sp_energy, const double* __restrict__ wocc,
-                                     const double* __restrict__ wstates, const double* __restrict__ fsum,
+                                     const double* __restrict__ wstates, const double* __restrict__ fsum, int fsum_nt,
                                      double* __restrict__ efl1, double* __restrict__ efl2, double* __restrict__ acc) {
   __shared__ double red[16 * 2];
-  double v[2] = {0.0, 0.0};
-  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
-    const double spe_old = sp_energy[s];
-    const double spe_new = wxyz * hsum[2 * s];
-    const double denerg = (spe_old - spe_new) / fabs(spe_new);
-    if (fsum) {
-      efl1[s] = sqrt(fsum[2 * s]);
-      efl2[s] = sqrt(fsum[2 * s + 1]);
+  const double fsc = fsum_nt > 0 ? wxyz : 1.0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  double dsum = 0.0;  // lane 0: sum of wocc*wstates*denerg over this wavefront's states
+  // one wavefront per state: the lanes take the tile partials, a butterfly
+  // adds them (the same total in every lane)
+  for (int s = wave; s < ns; s += nwave) {
+    double t[3] = {0.0, 0.0, 0.0};
+    if (hsum_nt > 0) {
+      for (int k = lane; k < hsum_nt; k += 64) t[0] += hsum[((int64_t)s * hsum_nt + k) * 2];
+    } else if (lane == 0) {
+      t[0] = hsum[2 * s];
     }
-    v[1] += wocc[s] * wstates[s] * denerg;
-    sp_energy[s] = spe_new;
+    if (fsum) {
+      if (fsum_nt > 0) {
+        for (int k = lane; k < fsum_nt; k += 64) {
+          t[1] += fsum[((int64_t)s * fsum_nt + k) * 2];
+          t[2] += fsum[((int64_t)s * fsum_nt + k) * 2 + 1];
+        }
+      } else if (lane == 0) {
+        t[1] = fsum[2 * s];
+        t[2] = fsum[2 * s + 1];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      t[j] += lane_xor<1>(t[j]);
+      t[j] += lane_xor<2>(t[j]);
+      t[j] += lane_xor<4>(t[j]);
+      t[j] += lane_xor<8>(t[j]);
+      t[j] += lane_xor<16>(t[j]);
+      t[j] += __shfl_xor(t[j], 32, 64);
+    }
+    if (lane == 0) {
+      const double spe_old = sp_energy[s];
+      const double spe_new = wxyz * t[0];
+      dsum += wocc[s] * wstates[s] * ((spe_old - spe_new) / fabs(spe_new));
+      if (fsum) {
+        efl1[s] = sqrt(fsc * t[1]);
+        efl2[s] = sqrt(fsc * t[2]);
+      }
+      sp_energy[s] = spe_new;
+    }
   }
   __syncthreads();
+  // sums over the states (static.py:211, 255-257)
+  double v[2] = {0.0, dsum};
   for (int s = threadIdx.x; s < ns; s += blockDim.x) v[0] += efl1[s];
   block_sum<2>(v, red);
   if (threadIdx.x == 0) {
@@ -326,20 +402,23 @@ extern "C" int hfb_step_ntiles(hfb_ctx* c, int prec) {
 
 extern "C" int hfb_grstep_weights(int nstates, int npsi0, int tbcs, const double* wocc, const double* wstates,
                                   const double* wguv, const double* pairwg, const double* v_pair, long long nvox,
-                                  double* w, double* wuv, double* pw, double* pshift, void* stream) {
-  if (nstates <= 0 || !wocc || !wstates || !wguv || !pairwg || !v_pair || !w || !wuv || !pw || !pshift)
+                                  const double* vpair_max, double* w, double* wuv, double* pw, double* pshift,
+                                  void* stream) {
+  if (nstates <= 0 || !wocc || !wstates || !wguv || !pairwg || (!v_pair && !vpair_max) || !w || !wuv || !pw ||
+      !pshift)
     return HFB_E_ARG;
   hipLaunchKernelGGL(grstep_weights_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, nstates, npsi0, tbcs,
-                     wocc, wstates, wguv, pairwg, v_pair, (int64_t)nvox, w, wuv, pw, pshift);
+                     wocc, wstates, wguv, pairwg, v_pair, (int64_t)nvox, vpair_max, w, wuv, pw, pshift);
   return launch_status();
 }
 
 extern "C" int hfb_precond_step(hfb_ctx* c, int prec, void* psi, const void* ps1, const void* lagrange,
-                                int nstates, const double* hsum, double wxyz, int use_lagrange, const double* pw,
+                                int nstates, const double* hsum, int hsum_ntile, double wxyz, int use_lagrange, const double* pw,
                                 const double* pshift, double e0dmp, double h2ma, double x0dmp, double* fpart,
                                 void* work, void* stream) {
   if (!c) return HFB_E_NOCTX;
-  if (!psi || !ps1 || !lagrange || !hsum || !pw || !pshift || !work || nstates <= 0) return HFB_E_ARG;
+  if (!psi || !ps1 || !lagrange || !hsum || !pw || !pshift || !work || nstates <= 0 || hsum_ntile < 0)
+    return HFB_E_ARG;
   StepArgs a{};
   a.nx = c->nx;
   a.ny = c->ny;
@@ -349,6 +428,7 @@ extern "C" int hfb_precond_step(hfb_ctx* c, int prec, void* psi, const void* ps1
   a.psi = psi;
   a.work = work;
   a.hsum = hsum;
+  a.hsum_nt = hsum_ntile;
   a.wxyz = wxyz;
   a.use_lag = use_lagrange;
   a.pw = pw;
@@ -366,12 +446,13 @@ extern "C" int hfb_precond_step(hfb_ctx* c, int prec, void* psi, const void* ps1
   return HFB_E_ARG;
 }
 
-extern "C" int hfb_grstep_finish(int nstates, const double* hsum, double wxyz, double* sp_energy,
-                                 const double* wocc, const double* wstates, const double* fsum, double* sp_efluct1,
-                                 double* sp_efluct2, double* acc, void* stream) {
-  if (nstates <= 0 || !hsum || !sp_energy || !wocc || !wstates || !sp_efluct1 || !sp_efluct2 || !acc)
+extern "C" int hfb_grstep_finish(int nstates, const double* hsum, int hsum_ntile, double wxyz, double* sp_energy,
+                                 const double* wocc, const double* wstates, const double* fsum, int fsum_ntile,
+                                 double* sp_efluct1, double* sp_efluct2, double* acc, void* stream) {
+  if (nstates <= 0 || !hsum || !sp_energy || !wocc || !wstates || !sp_efluct1 || !sp_efluct2 || !acc ||
+      hsum_ntile < 0 || fsum_ntile < 0)
     return HFB_E_ARG;
-  hipLaunchKernelGGL(grstep_finish_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, nstates, hsum, wxyz,
-                     sp_energy, wocc, wstates, fsum, sp_efluct1, sp_efluct2, acc);
+  hipLaunchKernelGGL(grstep_finish_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, nstates, hsum, hsum_ntile,
+                     wxyz, sp_energy, wocc, wstates, fsum, fsum_ntile, sp_efluct1, sp_efluct2, acc);
   return launch_status();
 }

@@ -74,7 +74,9 @@ def hpsi(ctx, psi, npsi0, upot, bmass, aq, spot, wlspot, v_pair, w, wuv,
     """Batched hpsi01 (meanfield.py:40-159) on psi (S, 2, nx, ny, nz).
 
     Returns (pout, pout_mf, pout_del, sums) where sums (S, 2) double holds
-    (Re<psi|h psi>, Re<psi|pout>) without the wxyz factor, or None."""
+    (Re<psi|h psi>, Re<psi|pout>) without the wxyz factor, or None.  With
+    reduce_sums=False, sums is an HSum of the unreduced tile partials, which
+    precond_step and grstep_finish add up themselves."""
     prec = _lib.prec_of(psi.dtype)
     rt = _lib.real_of(prec)
     S = psi.shape[0]
@@ -107,11 +109,34 @@ def hpsi(ctx, psi, npsi0, upot, bmass, aq, spot, wlspot, v_pair, w, wuv,
                              ptr(wlspot), ptr(v_pair), ptr(w), ptr(wuv), ptr(pout), ptr(pmf), ptr(pdel), ptr(part),
                              ctypes.byref(nt), stream_handle()), "hpsi")
     sums = None
-    if want_partial and reduce_sums:
-        sums = ctx.work(("hpsi_sums", S), (S, 2), torch.float64)
-        check(lib().hfb_reduce_partials(ptr(part), S, nt.value, 2, 1.0, ptr(sums), stream_handle()),
-              "reduce_partials")
+    if want_partial:
+        sums = HSum(part, nt.value)
+        if reduce_sums:
+            sums = reduce_hsum(ctx, sums)
     return pout, pmf, pdel, sums
+
+
+class HSum:
+    """Per-state (Re<psi|h psi>, Re<psi|pout>) as tile partials (S, ntile, 2)
+    not yet added up."""
+
+    def __init__(self, part, ntile):
+        self.part, self.ntile = part, int(ntile)
+
+
+def reduce_hsum(ctx, hs):
+    """(S, 2) totals of an HSum (a tensor is returned unchanged)."""
+    if not isinstance(hs, HSum):
+        return hs
+    S = hs.part.shape[0]
+    sums = ctx.work(("hpsi_sums", S), (S, 2), torch.float64)
+    check(lib().hfb_reduce_partials(ptr(hs.part), S, hs.ntile, 2, 1.0, ptr(sums), stream_handle()),
+          "reduce_partials")
+    return sums
+
+
+def _hsum_args(hs):
+    return (hs.part, hs.ntile) if isinstance(hs, HSum) else (hs, 0)
 
 
 def poisson(ctx, rho: torch.Tensor, e2: float, wcoul: torch.Tensor | None = None):
@@ -135,19 +160,24 @@ def selftest_mfma_f64(a: torch.Tensor, b: torch.Tensor):
 # ---------------------------------------------------------------------------
 # Gradient step (static.py:93-259)
 # ---------------------------------------------------------------------------
-def grstep_weights(psi_states, npsi0, tbcs, wocc, wstates, wguv, pairwg, v_pair, nvox, out):
-    """Per-state hpsi weights and preconditioner parameters into out['w','wuv','pw','pshift']."""
+def grstep_weights(psi_states, npsi0, tbcs, wocc, wstates, wguv, pairwg, v_pair, nvox, out, vpair_max=None):
+    """Per-state hpsi weights and preconditioner parameters into
+    out['w','wuv','pw','pshift']; vpair_max (2,) device maxima of v_pair
+    per isospin when known (else v_pair is scanned)."""
     S = int(psi_states)
     check(lib().hfb_grstep_weights(S, int(npsi0), int(bool(tbcs)), ptr(wocc), ptr(wstates), ptr(wguv), ptr(pairwg),
-                                   ptr(v_pair), int(nvox), ptr(out["w"]), ptr(out["wuv"]), ptr(out["pw"]),
-                                   ptr(out["pshift"]), stream_handle()), "grstep_weights")
+                                   ptr(v_pair), int(nvox), ptr(vpair_max), ptr(out["w"]), ptr(out["wuv"]),
+                                   ptr(out["pw"]), ptr(out["pshift"]), stream_handle()), "grstep_weights")
 
 
 def precond_step(ctx, psi, ps1, lagrange, hsum, wxyz, use_lagrange, pw, pshift, e0dmp, h2ma, x0dmp,
                  fluct=None):
     """psi <- psi - x0dmp * laplace(residual) in place (static.py:93-146).
 
-    If ``fluct`` (S,2) float64 is given it receives (sum|r|^2, sum|lag - h_exp psi|^2)*wxyz."""
+    hsum: (S, 2) totals or an HSum.  If ``fluct`` is given it receives the
+    fluctuation sums: a (S,2) float64 tensor gets (sum|r|^2, sum|lag -
+    h_exp psi|^2)*wxyz; a dict gets {"part": tile partials, "ntile": n}
+    for grstep_finish to add up."""
     prec = _lib.prec_of(psi.dtype)
     S = psi.shape[0]
     need(psi, _lib.cplx_of(prec), (S, 2, ctx.nx, ctx.ny, ctx.nz), "psi")
@@ -158,19 +188,30 @@ def precond_step(ctx, psi, ps1, lagrange, hsum, wxyz, use_lagrange, pw, pshift, 
     nt = lib().hfb_step_ntiles(ctx.handle, prec)
     if fluct is not None:
         fpart = ctx.work(("step_part", prec, S), (2 * S, nt, 2), torch.float64)
-    check(lib().hfb_precond_step(ctx.handle, prec, ptr(psi), ptr(ps1), ptr(lagrange), S, ptr(hsum), float(wxyz),
+    hp, hnt = _hsum_args(hsum)
+    check(lib().hfb_precond_step(ctx.handle, prec, ptr(psi), ptr(ps1), ptr(lagrange), S, ptr(hp), hnt, float(wxyz),
                                  int(bool(use_lagrange)), ptr(pw), ptr(pshift), float(e0dmp), float(h2ma),
                                  float(x0dmp), ptr(fpart), ptr(work), stream_handle()), "precond_step")
-    if fluct is not None:
+    if isinstance(fluct, dict):
+        fluct["part"], fluct["ntile"] = fpart, 2 * nt
+    elif fluct is not None:
         check(lib().hfb_reduce_partials(ptr(fpart), S, 2 * nt, 2, float(wxyz), ptr(fluct), stream_handle()),
               "reduce_partials")
     return psi
 
 
 def grstep_finish(hsum, wxyz, sp_energy, wocc, wstates, fluct, sp_efluct1, sp_efluct2, acc):
+    """hsum: (S, 2) totals or an HSum; fluct: None, (S, 2) totals or the dict
+    filled by precond_step."""
     S = sp_energy.shape[0]
-    check(lib().hfb_grstep_finish(S, ptr(hsum), float(wxyz), ptr(sp_energy), ptr(wocc), ptr(wstates), ptr(fluct),
-                                  ptr(sp_efluct1), ptr(sp_efluct2), ptr(acc), stream_handle()), "grstep_finish")
+    hp, hnt = _hsum_args(hsum)
+    if isinstance(fluct, dict):
+        fp, fnt = fluct["part"], fluct["ntile"]
+    else:
+        fp, fnt = fluct, 0
+    check(lib().hfb_grstep_finish(S, ptr(hp), hnt, float(wxyz), ptr(sp_energy), ptr(wocc), ptr(wstates), ptr(fp),
+                                  fnt, ptr(sp_efluct1), ptr(sp_efluct2), ptr(acc), stream_handle()),
+          "grstep_finish")
 
 
 # ---------------------------------------------------------------------------
@@ -256,13 +297,13 @@ def rotate2(blocks, K):
     check(lib().hfb_rotate2(prec, *arrs, ns, K, stream_handle()), "rotate2")
 
 
-def diag_small(nst0, nst1, nmax, mats, tdiag, ipair, tbcs, construct, levels, static, E, uo, work=None):
+def diag_small(ctx, nst0, nst1, nmax, mats, tdiag, ipair, tbcs, construct, levels, static, E, uo, work=None):
     """Loewdin, optional diagonalisation and HFB matrices for both isospins."""
     m = (c_void_p * 8)(*[(t.data_ptr() if t is not None else None) for t in mats])
     u = (c_void_p * 4)(*[t.data_ptr() for t in uo])
     Ep = E.data_ptr()
     sz = 8
-    check(lib().hfb_diag_small(int(nst0), int(nst1), int(nmax), m, int(bool(tdiag)), int(bool(ipair)),
+    check(lib().hfb_diag_small(ctx.handle, int(nst0), int(nst1), int(nmax), m, int(bool(tdiag)), int(bool(ipair)),
                                int(bool(tbcs)), int(bool(construct)), ptr(levels.wocc), ptr(levels.wstates),
                                ptr(levels.wguv), ptr(levels.pairwg), ptr(levels.sp_norm), ptr(levels.sp_energy),
                                ptr(levels.deltaf), ptr(static.hmatrix), ptr(static.gapmatrix), ptr(static.symcond),
@@ -309,8 +350,8 @@ EN = {"EHFT": 0, "EHF0": 1, "EHF1": 2, "EHF2": 3, "EHF3": 4, "EHFLS": 5, "EHFLSO
       "EHFINT": 9, "E3CORR": 10, "EZPE": 11, "CRHO0": 12, "CRHO1": 13, "CDRHO0": 14, "CDRHO1": 15, "CTAU0": 16,
       "CTAU1": 17, "CDJ0": 18, "CDJ1": 19, "CJ0": 20, "CJ1": 21, "EHF": 22, "TKE": 23, "ORB": 24, "SPIN": 27,
       "JTOT": 30, "EFL1": 33, "EFL2": 34, "EFL1Q": 35, "EFL2Q": 37, "ECORRP": 39, "SUMFLU": 40, "DELESUM": 41,
-      "EPAIR": 42, "JACOBI": 44, "NSITER": 46}
-EN_LEN = 48
+      "EPAIR": 42, "JACOBI": 44, "NSITER": 46, "VMAX": 48}
+EN_LEN = 50
 
 
 def _deriv_work(ctx):
@@ -327,13 +368,13 @@ def density_derivs(ctx, dens, der):
 
 
 def meanfield_derivs(ctx, mf):
-    check(lib().hfb_meanfield_derivs(ctx.handle, ptr(mf.aq), ptr(mf.bmass), ptr(mf.divaq), ptr(mf.dbmass),
+    check(lib().hfb_meanfield_derivs(ctx.handle, ptr(mf.aq), ptr(mf.bmass), ptr(mf.divaq_buf), ptr(mf.dbmass_buf),
                                      ptr(_deriv_work(ctx)), stream_handle()), "meanfield_derivs")
 
 
 def skyrme(ctx, fc, dens, der, wcoul, mf, E, f32=None):
     nb = lib().hfb_skyrme_nblocks(ctx.handle)
-    epart = ctx.work("sky_part", (nb,), torch.float64)
+    epart = ctx.work("sky_part", (3 * nb,), torch.float64)
     g = f32 or {}
     check(lib().hfb_skyrme(ctx.handle, ctypes.byref(fc), ptr(dens.rho), ptr(dens.tau), ptr(dens.chi),
                            ptr(dens.current), ptr(dens.sdens), ptr(der["divj"]), ptr(der["lap"]), ptr(der["grad"]),

@@ -17,18 +17,39 @@ from . import _lib, kernels
 class Meanfield:
     upot: torch.Tensor
     bmass: torch.Tensor
-    divaq: torch.Tensor
+    divaq_buf: torch.Tensor
     v_pair: torch.Tensor
     aq: torch.Tensor
     spot: torch.Tensor
     wlspot: torch.Tensor
-    dbmass: torch.Tensor
+    dbmass_buf: torch.Tensor
     ecorrp_dev: torch.Tensor            # device scalar view (E vector slot or own storage)
     f32: dict = field(default_factory=dict)
+    derivs_ctx: object = None           # set by skyrme: divaq/dbmass are due for this context
+    vpair_max: torch.Tensor = None      # (2,) device maxima of v_pair per isospin, written by skyrme
 
     @property
     def ecorrp(self) -> float:
         return float(self.ecorrp_dev.item())
+
+    # divaq = div aq and dbmass = grad bmass (meanfield.py:329-344) are part of
+    # the reference's Meanfield, but hpsi01 reads neither; they are computed
+    # from the current aq and bmass when first read after a skyrme call, so
+    # the SCF loop does not pay for them.
+    def _refresh_derivs(self):
+        if self.derivs_ctx is not None:
+            ctx, self.derivs_ctx = self.derivs_ctx, None
+            kernels.meanfield_derivs(ctx, self)
+
+    @property
+    def divaq(self) -> torch.Tensor:
+        self._refresh_derivs()
+        return self.divaq_buf
+
+    @property
+    def dbmass(self) -> torch.Tensor:
+        self._refresh_derivs()
+        return self.dbmass_buf
 
     def fields(self, dtype):
         """(upot, bmass, aq, spot, wlspot, v_pair) in the precision of a spinor dtype."""
@@ -95,12 +116,13 @@ def density_derivs(grids, densities, out=None) -> DensityDerivs:
 
 
 def skyrme(coulomb, densities, forces, grids, meanfield, params, static, constraints=None, derivs=None, E=None,
-           mass_number=None):
+           mass_number=None, wcoul_ready=False):
     """Potentials from the densities (meanfield.py:163-391); returns (meanfield, coulomb).
 
     ``derivs`` may carry precomputed DensityDerivs of these densities; ``E``
     is a device energy vector whose ecorrp slot is written (a private one is
-    used otherwise)."""
+    used otherwise); ``wcoul_ready``: coulomb.wcoul already holds the
+    Coulomb potential of these densities."""
     if constraints is not None and getattr(constraints, "tconstraint", False):
         raise NotImplementedError("constraint potentials are not part of the static path")
     if static is not None and getattr(static, "outertype", "N") != "N":
@@ -109,13 +131,14 @@ def skyrme(coulomb, densities, forces, grids, meanfield, params, static, constra
     if derivs is None:
         derivs = density_derivs(grids, densities)
     from .coulomb import poisson
-    if params.tcoul:
+    if params.tcoul and not wcoul_ready:
         coulomb.wcoul = poisson(grids, params, coulomb, densities.rho, out=coulomb.wcoul)
     if E is None:
         E = torch.zeros(kernels.EN_LEN, dtype=torch.float64, device=densities.rho.device)
     A = mass_number if mass_number is not None else 0.0
     fc = forces.to_c(params, A)
     kernels.skyrme(ctx, fc, densities, derivs, coulomb.wcoul, meanfield, E, meanfield.f32 or None)
-    kernels.meanfield_derivs(ctx, meanfield)
+    meanfield.derivs_ctx = ctx
     meanfield.ecorrp_dev = E[kernels.EN["ECORRP"]]
+    meanfield.vpair_max = E[kernels.EN["VMAX"]:kernels.EN["VMAX"] + 2]
     return meanfield, coulomb

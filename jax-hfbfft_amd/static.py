@@ -147,7 +147,7 @@ def _ws(ctx, levels):
     f64 = torch.float64
     return dict(w=ctx.work(("w", S), (S,), f64), wuv=ctx.work(("wuv", S), (S,), f64),
                 pw=ctx.work(("pw", S), (S,), f64), pshift=ctx.work(("pshift", S), (S,), f64),
-                fluct=ctx.work(("fluct", S), (S, 2), f64), pout=ctx.work(("pout", dt, S), shp, dt))
+                pout=ctx.work(("pout", dt, S), shp, dt))
 
 
 def grstep(forces, grids, levels, meanfield, params, static):
@@ -157,18 +157,21 @@ def grstep(forces, grids, levels, meanfield, params, static):
     n0 = int(levels.npsi[0])
     ws = _ws(ctx, levels)
     kernels.grstep_weights(S, n0, forces.tbcs, levels.wocc, levels.wstates, levels.wguv, levels.pairwg,
-                           meanfield.v_pair, grids.nx * grids.ny * grids.nz, ws)
+                           meanfield.v_pair, grids.nx * grids.ny * grids.nz, ws, vpair_max=meanfield.vpair_max)
     fields = meanfield.fields(levels.psi.dtype)
     # h psi on the old states: ps1 (weighted) and psi_mf -> hampsi (static.py:152-173)
+    # the per-state sums stay as tile partials, added up by the step and the
+    # finishing kernel themselves
     _, _, _, hsum = kernels.hpsi(ctx, levels.psi, n0, *fields, ws["w"], ws["wuv"], want_del=False,
-                                 out={"pout": ws["pout"], "pout_mf": levels.hampsi})
+                                 out={"pout": ws["pout"], "pout_mf": levels.hampsi}, reduce_sums=False)
     want_fl = params.mprint > 0 and params.iteration % params.mprint == 0
+    fluct = {} if want_fl else None
     if static.e0dmp > 0.0:
         use_lag = (params.iteration > 1) and (not forces.tbcs)
         kernels.precond_step(ctx, levels.psi, ws["pout"], levels.lagrange, hsum, grids.wxyz, use_lag, ws["pw"],
-                             ws["pshift"], static.e0dmp, forces.h2ma, static.x0dmp,
-                             fluct=ws["fluct"] if want_fl else None)
+                             ws["pshift"], static.e0dmp, forces.h2ma, static.x0dmp, fluct=fluct)
     else:  # static.py:181-185
+        hsum = kernels.reduce_hsum(ctx, hsum)
         spe_new = grids.wxyz * hsum[:, 0]
         fac = (1.0 + static.x0dmp * (spe_new - levels.sp_energy)).to(levels.psi.real.dtype)
         levels.psi.mul_(fac.view(-1, 1, 1, 1, 1)).sub_(static.x0dmp * ws["pout"])
@@ -181,7 +184,7 @@ def grstep(forces, grids, levels, meanfield, params, static):
     kernels.hpsi(ctx, levels.psi, n0, *fields, ws["w"], ws["wuv"], want_pout=False, want_del=forces.ipair != 0,
                  want_partial=False, out=out)
     kernels.grstep_finish(hsum, grids.wxyz, levels.sp_energy, levels.wocc, levels.wstates,
-                          ws["fluct"] if want_fl else None, levels.sp_efluct1, levels.sp_efluct2, static.acc)
+                          fluct if want_fl else None, levels.sp_efluct1, levels.sp_efluct2, static.acc)
     return levels, static
 
 
@@ -216,7 +219,7 @@ def diagstep(energies, forces, grids, levels, static, diagonalize=False, constru
     kernels.gram2(ctx, blocks, grids.wxyz, outs)
     # blocks above 36 states keep the subspace work matrices in global memory
     work = ctx.work(("diag_work", nmax), (10 * nmax * nmax,), torch.complex128) if max(n0, n1) > 36 else None
-    kernels.diag_small(n0, n1, nmax, mats, diagonalize, forces.ipair != 0, forces.tbcs, construct, levels, static,
+    kernels.diag_small(ctx, n0, n1, nmax, mats, diagonalize, forces.ipair != 0, forces.tbcs, construct, levels, static,
                        energies.vec, uo, work)
     # psi <- psi U and lagrange <- psi M2 in place (every workgroup reads its
     # columns of psi before it writes them)

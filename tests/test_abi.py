@@ -102,6 +102,7 @@ def test_energy_slots_match_header(pkg):
     assert pkg.kernels.EN["SPIN"] == 27 and pkg.kernels.EN["JTOT"] == 30 and pkg.kernels.EN["EFL1"] == 33
     assert pkg.kernels.EN["EFL1Q"] == 35 and pkg.kernels.EN["EFL2Q"] == 37 and pkg.kernels.EN["ECORRP"] == 39
     assert pkg.kernels.EN["EPAIR"] == 42 and pkg.kernels.EN["JACOBI"] == 44 and pkg.kernels.EN["NSITER"] == 46
+    assert pkg.kernels.EN["VMAX"] == 48
 
 
 def test_forces_match_oracle(pkg, O):
