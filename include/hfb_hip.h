@@ -107,6 +107,12 @@ int hfb_reduce_partials(const double* partial, int nstates, int ntile, int nv, d
  * a (16x4) and b (4x16) row-major doubles in, d = a*b (16x16) out. */
 int hfb_selftest_mfma_f64(const double* a, const double* b, double* d, void* stream);
 
+/* Lane layout check of the FP32 16x16x4 matrix-core product the FP32 h.psi
+ * kernels are built on (no counterpart in the reference): a (16, 4), b
+ * (4, 16) float; d (64 lanes x 4 registers) float receives the raw result
+ * registers, lane-major. */
+int hfb_selftest_mfma_f32(const float* a, const float* b, float* d, void* stream);
+
 
 /* ---------------------------------------------------------------------
  * Coulomb (coulomb.py:54-75, poisson)

@@ -31,6 +31,7 @@ _SIGS = {
     "hfb_reduce_partials": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_void_p, c_void_p]),
     "hfb_poisson": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
     "hfb_selftest_mfma_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "hfb_selftest_mfma_f32": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "hfb_step_ntiles": (c_int, [c_void_p, c_int]),
     "hfb_grstep_weights": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                    ctypes.c_longlong, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

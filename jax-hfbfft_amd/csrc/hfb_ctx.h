@@ -19,6 +19,7 @@ struct hfb_ctx {
   int ntw;
   double* der1[3];  // dense first/second derivative matrices (n x n), FP64
   double* der2[3];
+  float* der1f[3];  // der1 in FP32 (the matrix-core derivatives of the FP32 spinor kernels)
   hfb::cx<double>* qhat;  // FFT of the Coulomb kernel on the doubled grid
   // arrival counters of the single-pass reductions (zero between launches;
   // regions below), so calls on one context must not overlap in time

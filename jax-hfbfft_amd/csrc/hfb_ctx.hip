@@ -111,6 +111,11 @@ extern "C" int hfb_ctx_create(hfb_ctx** out, int device, int nx, int ny, int nz,
         hipMemcpy(c->der1[a], m1.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->der2[a], m2.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
       rc = HFB_E_HIP;
+    std::vector<float> f1(m1.begin(), m1.end());
+    if (rc == HFB_OK && (hipMalloc(&c->der1f[a], sizeof(float) * f1.size()) != hipSuccess ||
+                         hipMemcpy(c->der1f[a], f1.data(), sizeof(float) * f1.size(), hipMemcpyHostToDevice) !=
+                             hipSuccess))
+      rc = HFB_E_HIP;
   }
   if (rc == HFB_OK) rc = hfb_coulomb_setup(c);
   if (rc == HFB_OK) {
@@ -136,6 +141,7 @@ extern "C" int hfb_ctx_destroy(hfb_ctx* c) {
   for (int a = 0; a < 3; ++a) {
     if (c->der1[a]) hipFree(c->der1[a]);
     if (c->der2[a]) hipFree(c->der2[a]);
+    if (c->der1f[a]) hipFree(c->der1f[a]);
   }
   if (c->qhat) hipFree(c->qhat);
   if (c->sync) hipFree(c->sync);
@@ -174,6 +180,7 @@ extern "C" int hfb_reduce_partials(const double* partial, int nstates, int ntile
 // d[0..256) is written with that layout, d[256..512) lane-major raw.
 // ---------------------------------------------------------------------------
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
 __global__ void mfma_probe_kernel(const double* __restrict__ a, const double* __restrict__ b,
                                   double* __restrict__ d) {
@@ -191,5 +198,28 @@ __global__ void mfma_probe_kernel(const double* __restrict__ a, const double* __
 extern "C" int hfb_selftest_mfma_f64(const double* a, const double* b, double* d, void* stream) {
   if (!a || !b || !d) return HFB_E_ARG;
   hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, a, b, d);
+  return launch_status();
+}
+
+// ---------------------------------------------------------------------------
+// MFMA f32 16x16x4 lane layout self-test (one wavefront): the A and B
+// operands as above (a[row=l%16][k=l/16], b[k=l/16][col=l%16]); the four
+// result registers of every lane are written lane-major, d[l*4 + r], so a
+// test can check the D layout the h.psi kernels assume
+// (d[row=4*(l/16)+r][col=l%16]).
+// ---------------------------------------------------------------------------
+__global__ void mfma_f32_probe_kernel(const float* __restrict__ a, const float* __restrict__ b,
+                                      float* __restrict__ d) {
+  const int l = threadIdx.x;
+  const float av = a[(l % 16) * 4 + (l / 16)];
+  const float bv = b[(l / 16) * 16 + (l % 16)];
+  v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) d[l * 4 + r] = acc[r];
+}
+
+extern "C" int hfb_selftest_mfma_f32(const float* a, const float* b, float* d, void* stream) {
+  if (!a || !b || !d) return HFB_E_ARG;
+  hipLaunchKernelGGL(mfma_f32_probe_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, a, b, d);
   return launch_status();
 }

@@ -348,6 +348,353 @@ __global__ void __launch_bounds__(256) hpsi_lane_kernel(const cx<T>* __restrict_
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP32 x pass on the matrix cores (32-point x axis).
+//
+// Along an axis the spectral derivative with the Nyquist mode zeroed is the
+// dense real matrix der1 (grids.py:36-50 sder; cderv00's operator,
+// levels.py:131-189), so both derivatives of the x terms of hpsi01 are
+// 32 x 32 matrix products over the x index, with the columns (y, z) of the
+// spinor block as the other dimension:
+//   d1 = D psi,  dd = D (f - B d1) + (-1)^x (pi/dx)^2 mean_x(B) psi_N / N,
+// the second term being cderv02's Nyquist entry (levels.py:309-315) as a
+// rank-one correction, psi_N = sum_x (-1)^x psi.  A wavefront owns 32 x
+// values by 16 z values at one y of one state and multiplies the four real
+// components (both spins, real and imaginary parts) with
+// v_mfma_f32_16x16x4f32: the B operand of k-step s in lane l is the
+// component at x = 4 s + l/16, z = l%16.  The rows of D are permuted so that
+// result register r of block b in lane l holds x = 16 b + 4 r + l/16, the
+// same x as operand register 4 b + r: products feed the pointwise terms
+// and the second product without any data exchange.
+// ---------------------------------------------------------------------------
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Element at a 32-bit unsigned byte offset from a wavefront-uniform base:
+// the address stays base (scalar) + offset (one vector register).
+template <typename T>
+__device__ __forceinline__ T ldb(const T* base, unsigned byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ void stb(T* base, unsigned byte_off, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
+// result register r of lane group q (= l/16) of a 16x16x4 product is row
+// 4 q + r (hfb_selftest_mfma_f32 checks this); the position a result row holds:
+__device__ __forceinline__ int mfx_row_x(int blk, int row) { return 16 * blk + 4 * (row % 4) + row / 4; }
+
+// The axis' spin-orbit / current combination f of a two-spinor (a, b)
+// (meanfield.py:75-83 x, 104-112 y, 133-141 z); the same linear form gives
+// the axis terms on d1 (meanfield.py:63-73, 92-102, 121-131).  A is the
+// axis' component of aq, wa/wb the two wlspot components the axis uses
+// (x: W1, W2; y: W0, W2; z: W0, W1).
+template <int AXIS>
+__device__ __forceinline__ void fcomb(float ar, float ai, float br, float bi, float A, float wa, float wb,
+                                      float& o0r, float& o0i, float& o1r, float& o1i) {
+  if constexpr (AXIS == 0) {  // i p a - hw b,  i pm b + hw a
+    const float p = -0.5f * (A - wa), pm = -0.5f * (A + wa), hw = 0.5f * wb;
+    o0r = -ai * p - hw * br;
+    o0i = ar * p - hw * bi;
+    o1r = -bi * pm + hw * ar;
+    o1i = br * pm + hw * ai;
+  } else if constexpr (AXIS == 1) {  // i p a + i hw b,  i pm b + i hw a
+    const float p = -0.5f * (A + wa), pm = -0.5f * (A - wa), hw = 0.5f * wb;
+    o0r = -ai * p - bi * hw;
+    o0i = ar * p + br * hw;
+    o1r = -bi * pm - ai * hw;
+    o1i = br * pm + ar * hw;
+  } else {  // i p a + hw b + i hv b,  i p b - hw a + i hv a
+    const float p = -0.5f * A, hw = 0.5f * wa, hv = -0.5f * wb;
+    o0r = -ai * p + hw * br - hv * bi;
+    o0i = ar * p + hw * bi + hv * br;
+    o1r = -bi * p - hw * ar - hv * ai;
+    o1i = br * p - hw * ai + hv * ar;
+  }
+}
+
+// One wavefront's share of one axis of h psi on the matrix cores: 32
+// positions along the axis by 16 columns.  Operand register m of lane l
+// holds position 4 m + l/16 of column l % 16: the spinor components
+// ar..bi, the effective mass Bv, the axis' aq component Av and wlspot pair
+// wa, wb.  Af: the D operands (rows in result order).  extra(m, e[4]) adds
+// further pointwise terms at operand position m.  Returns, in result
+// register (b, r) = position 16 b + 4 r + l/16 (the operand position of
+// m = 4 b + r), the axis' contribution to h psi:
+//   D (f - B d1) + f(d1) + (-1)^i (pi/d)^2 mean(B) psi_N / N + extra,
+// with d1 = D psi and psi_N = sum_i (-1)^i psi (levels.py:283-395).
+template <int AXIS, typename Extra>
+__device__ __forceinline__ void mfma_axis(const float (&Af)[2][8], const float (&ar)[8], const float (&ai)[8],
+                                          const float (&br)[8], const float (&bi)[8], const float (&Bv)[8],
+                                          const float (&Av)[8], const float (&wa)[8], const float (&wb)[8], float d,
+                                          int q, Extra extra, v4f (&h)[4][2]) {
+  constexpr int N = 32;
+  v4f d1[4][2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    v4f t0 = {0.f, 0.f, 0.f, 0.f}, t1 = t0, t2 = t0, t3 = t0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      t0 = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[b][k], ar[k], t0, 0, 0, 0);
+      t1 = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[b][k], ai[k], t1, 0, 0, 0);
+      t2 = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[b][k], br[k], t2, 0, 0, 0);
+      t3 = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[b][k], bi[k], t3, 0, 0, 0);
+    }
+    d1[0][b] = t0;
+    d1[1][b] = t1;
+    d1[2][b] = t2;
+    d1[3][b] = t3;
+  }
+  // mean(B) and psi_N over the column (its four lane groups)
+  float bs = 0.f, nar = 0.f, nai = 0.f, nbr = 0.f, nbi = 0.f;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    bs += Bv[m];
+    nar += ar[m];
+    nai += ai[m];
+    nbr += br[m];
+    nbi += bi[m];
+  }
+  const float sg = (q & 1) ? -1.f : 1.f;  // (-1)^i, i = 4 m + q
+  nar *= sg;
+  nai *= sg;
+  nbr *= sg;
+  nbi *= sg;
+#define HFB_XSUM(v) v += __shfl_xor(v, 16, 64); v += __shfl_xor(v, 32, 64);
+  HFB_XSUM(bs)
+  HFB_XSUM(nar)
+  HFB_XSUM(nai)
+  HFB_XSUM(nbr)
+  HFB_XSUM(nbi)
+#undef HFB_XSUM
+  // cderv02's Nyquist entry (levels.py:309-315), times (-1)^i
+  const float nyf = sg * float(M_PI / (double)d) * float(M_PI / (double)d) / float(N) * (bs / float(N));
+  float gar[8], gai[8], gbr[8], gbi[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int b = m / 4, r = m % 4;
+    const float da_r = d1[0][b][r], da_i = d1[1][b][r], db_r = d1[2][b][r], db_i = d1[3][b][r];
+    float f0r, f0i, f1r, f1i;
+    fcomb<AXIS>(ar[m], ai[m], br[m], bi[m], Av[m], wa[m], wb[m], f0r, f0i, f1r, f1i);
+    gar[m] = f0r - Bv[m] * da_r;
+    gai[m] = f0i - Bv[m] * da_i;
+    gbr[m] = f1r - Bv[m] * db_r;
+    gbi[m] = f1i - Bv[m] * db_i;
+    float e[4];
+    fcomb<AXIS>(da_r, da_i, db_r, db_i, Av[m], wa[m], wb[m], e[0], e[1], e[2], e[3]);
+    e[0] += nyf * nar;
+    e[1] += nyf * nai;
+    e[2] += nyf * nbr;
+    e[3] += nyf * nbi;
+    extra(m, e);
+    h[0][b][r] = e[0];
+    h[1][b][r] = e[1];
+    h[2][b][r] = e[2];
+    h[3][b][r] = e[3];
+  }
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      h[0][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[b][k], gar[k], h[0][b], 0, 0, 0);
+      h[1][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[b][k], gai[k], h[1][b], 0, 0, 0);
+      h[2][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[b][k], gbr[k], h[2][b], 0, 0, 0);
+      h[3][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[b][k], gbi[k], h[3][b], 0, 0, 0);
+    }
+  }
+}
+
+// The D operands of lane (q, c): D rows in result order, k = 4 kstep + q.
+__device__ __forceinline__ void load_dfrag(const float* __restrict__ D1, int q, int c, float (&Af)[2][8]) {
+#pragma unroll
+  for (int b = 0; b < 2; ++b)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) Af[b][k] = D1[mfx_row_x(b, c) * 32 + 4 * k + q];
+}
+
+// x pass: a wavefront owns 32 x by the 16 z of one slice at one y; it adds
+// U psi and the spin-current coupling (meanfield.py:47-58) and writes the
+// partial h psi.
+__global__ void __launch_bounds__(256, 3) hpsi_mfma_x_kernel(const cx<float>* __restrict__ psi,
+                                                             cx<float>* __restrict__ pmf, MF<float> mf, int npsi0,
+                                                             int ny, int nz, float d, const float* __restrict__ D1) {
+  constexpr int N = 32;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = lane >> 4, c = lane & 15;
+  const int nzh = nz >> 4;
+  const int tile = blockIdx.x * 4 + wave;  // (y, 16-column z slice)
+  const int y = tile / nzh, z = (tile % nzh) * 16 + c;
+  const int s = blockIdx.y;
+  const int iq = s >= npsi0 ? 1 : 0;
+  const int sx = ny * nz;  // offsets below: unsigned 32-bit from a uniform field base
+  const int64_t nvox = (int64_t)N * sx;
+  const cx<float>* pa = psi + (int64_t)(2 * s) * nvox;
+  const cx<float>* pb = pa + nvox;
+  const int col = y * nz + z;
+  float Af[2][8];
+  load_dfrag(D1, q, c, Af);
+  float ar[8], ai[8], br[8], bi[8], Bv[8], Av[8], w1[8], w2[8];
+  const float* Bq = mf.B + iq * nvox;
+  const float* Ax = mf.A + (iq * 3 + 0) * nvox;
+  const float* W1 = mf.W + (iq * 3 + 1) * nvox;
+  const float* W2 = mf.W + (iq * 3 + 2) * nvox;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const unsigned off = (unsigned)((4 * m + q) * sx + col);
+    const cx<float> va = ldb(pa, 8u * off), vb = ldb(pb, 8u * off);
+    ar[m] = va.x;
+    ai[m] = va.y;
+    br[m] = vb.x;
+    bi[m] = vb.y;
+    Bv[m] = ldb(Bq, 4u * off);
+    Av[m] = ldb(Ax, 4u * off);
+    w1[m] = ldb(W1, 4u * off);
+    w2[m] = ldb(W2, 4u * off);
+  }
+  const float* U = mf.U + iq * nvox;
+  const float* S0 = mf.S + (iq * 3 + 0) * nvox;
+  const float* S1 = mf.S + (iq * 3 + 1) * nvox;
+  const float* S2 = mf.S + (iq * 3 + 2) * nvox;
+  v4f h[4][2];
+  mfma_axis<0>(Af, ar, ai, br, bi, Bv, Av, w1, w2, d, q, [&](int m, float (&e)[4]) {
+    const unsigned off = (unsigned)((4 * m + q) * sx + col);
+    const float u = ldb(U, 4u * off), s0 = ldb(S0, 4u * off), s1 = ldb(S1, 4u * off), s2 = ldb(S2, 4u * off);
+    e[0] += ar[m] * (u + s2) + br[m] * s0 + bi[m] * s1;
+    e[1] += ai[m] * (u + s2) + bi[m] * s0 - br[m] * s1;
+    e[2] += br[m] * (u - s2) + ar[m] * s0 - ai[m] * s1;
+    e[3] += bi[m] * (u - s2) + ai[m] * s0 + ar[m] * s1;
+  }, h);
+  cx<float>* oa = pmf + (int64_t)(2 * s) * nvox;
+  cx<float>* ob = oa + nvox;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int b = m / 4, r = m % 4;
+    const unsigned off = (unsigned)((4 * m + q) * sx + col);
+    stb(oa, 8u * off, cx<float>{h[0][b][r], h[1][b][r]});
+    stb(ob, 8u * off, cx<float>{h[2][b][r], h[3][b][r]});
+  }
+}
+
+// y-z plane pass for 32 x 32 planes (x fixed, one state per workgroup):
+// the spinor plane goes to LDS; wavefronts 0, 1 compute the y terms of the
+// 16-column z slices, wavefronts 2, 3 the z terms of the 16-row y slices,
+// both into LDS; then every thread finishes its points: h psi = x pass +
+// y terms + z terms, v_pair psi, pout = w h psi - wuv v_pair psi
+// (meanfield.py:150-157) and the overlap partial sums of the plane.
+constexpr int MP = 33;  // LDS row pitch of a 32 x 32 plane (complex)
+
+__global__ void __launch_bounds__(256, 3) hpsi_mfma_plane_kernel(const cx<float>* __restrict__ psi,
+                                                                 cx<float>* __restrict__ pmf,
+                                                                 cx<float>* __restrict__ pout,
+                                                                 cx<float>* __restrict__ pdel, MF<float> mf,
+                                                                 const double* __restrict__ w,
+                                                                 const double* __restrict__ wuv, int npsi0, int nx,
+                                                                 float dy, float dz, const float* __restrict__ Dy,
+                                                                 const float* __restrict__ Dz,
+                                                                 double* __restrict__ partial) {
+  constexpr int N = 32, NP = N * MP;
+  extern __shared__ unsigned char mp_lds[];
+  cx<float>* P = (cx<float>*)mp_lds;  // [spin][y][z]: spinor plane
+  cx<float>* H = P + 2 * NP;          // [axis y,z][spin][y][z]: axis terms
+  __shared__ double red[4 * 2];
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int q = lane >> 4, c = lane & 15;
+  const int x = blockIdx.x, s = blockIdx.y;
+  const int iq = s >= npsi0 ? 1 : 0;
+  const int64_t nvox = (int64_t)nx * N * N;
+  const cx<float>* pa = psi + (int64_t)(2 * s) * nvox + (int64_t)x * N * N;  // plane of spin up
+  const cx<float>* pb = pa + nvox;
+  const int64_t fo = (int64_t)iq * nvox + (int64_t)x * N * N;  // plane in an isospin field
+  for (int e = threadIdx.x; e < 2 * N * N; e += 256) {
+    const int sp = e / (N * N), yz = e % (N * N);
+    P[sp * NP + (yz / N) * MP + yz % N] = (sp ? pb : pa)[yz];
+  }
+  __syncthreads();
+  {
+    const int axis = wave < 2 ? 1 : 2;  // wavefront-uniform
+    const int sl = (wave & 1) * 16;     // z slice (y pass) or y slice (z pass)
+    float Af[2][8];
+    load_dfrag(axis == 1 ? Dy : Dz, q, c, Af);
+    float ar[8], ai[8], br[8], bi[8], Bv[8], Av[8], wa[8], wb[8];
+    const float* Bq = mf.B + fo;
+    const float* Aa = mf.A + fo + (int64_t)(iq * 2 + axis) * nvox;  // aq[iq][axis]
+    const float* W0 = mf.W + fo + (int64_t)(iq * 2) * nvox;         // wlspot[iq][0]
+    const float* Wb = mf.W + fo + (int64_t)(iq * 2 + (axis == 1 ? 2 : 1)) * nvox;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int i = 4 * m + q;  // position along the axis
+      const int yy = axis == 1 ? i : sl + c, zz = axis == 1 ? sl + c : i;
+      const cx<float> va = P[yy * MP + zz], vb = P[NP + yy * MP + zz];
+      ar[m] = va.x;
+      ai[m] = va.y;
+      br[m] = vb.x;
+      bi[m] = vb.y;
+      const unsigned off = (unsigned)(yy * N + zz);
+      Bv[m] = ldb(Bq, 4u * off);
+      Av[m] = ldb(Aa, 4u * off);
+      wa[m] = ldb(W0, 4u * off);
+      wb[m] = ldb(Wb, 4u * off);
+    }
+    v4f h[4][2];
+    const float d = axis == 1 ? dy : dz;
+    auto none = [](int, float (&)[4]) {};
+    if (axis == 1)
+      mfma_axis<1>(Af, ar, ai, br, bi, Bv, Av, wa, wb, d, q, none, h);
+    else
+      mfma_axis<2>(Af, ar, ai, br, bi, Bv, Av, wa, wb, d, q, none, h);
+    cx<float>* Ha = H + (axis - 1) * 2 * NP;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int b = m / 4, r = m % 4;
+      const int i = 4 * m + q;
+      const int yy = axis == 1 ? i : sl + c, zz = axis == 1 ? sl + c : i;
+      Ha[yy * MP + zz] = cx<float>{h[0][b][r], h[1][b][r]};
+      Ha[NP + yy * MP + zz] = cx<float>{h[2][b][r], h[3][b][r]};
+    }
+  }
+  __syncthreads();
+  // finish: 4 points of both spins per thread, consecutive z across lanes
+  const double ws = w[s], wu = wuv[s];
+  const float* V = mf.V + fo;
+  cx<float>* ma = pmf + (int64_t)(2 * s) * nvox + (int64_t)x * N * N;
+  cx<float>* mb = ma + nvox;
+  double acc[2] = {0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int yz = threadIdx.x + 256 * j;
+    const int li = (yz / N) * MP + yz % N;
+    const cx<float> av = P[li], bv = P[NP + li];
+    const cx<float> xa = ma[yz], xb = mb[yz];
+    const cx<float> h0 = cadd(cadd(xa, H[li]), H[2 * NP + li]);
+    const cx<float> h1 = cadd(cadd(xb, H[NP + li]), H[3 * NP + li]);
+    ma[yz] = h0;
+    mb[yz] = h1;
+    const float v = V[yz];
+    const cx<float> e0 = cscale(av, v), e1 = cscale(bv, v);
+    if (pdel) {
+      pdel[(int64_t)(2 * s) * nvox + (int64_t)x * N * N + yz] = e0;
+      pdel[(int64_t)(2 * s + 1) * nvox + (int64_t)x * N * N + yz] = e1;
+    }
+    const cx<float> r0 = csub(cscale(h0, float(ws)), cscale(e0, float(wu)));
+    const cx<float> r1 = csub(cscale(h1, float(ws)), cscale(e1, float(wu)));
+    if (pout) {
+      pout[(int64_t)(2 * s) * nvox + (int64_t)x * N * N + yz] = r0;
+      pout[(int64_t)(2 * s + 1) * nvox + (int64_t)x * N * N + yz] = r1;
+    }
+    acc[0] += double(av.x) * double(h0.x) + double(av.y) * double(h0.y) + double(bv.x) * double(h1.x) +
+              double(bv.y) * double(h1.y);
+    acc[1] += double(av.x) * double(r0.x) + double(av.y) * double(r0.y) + double(bv.x) * double(r1.x) +
+              double(bv.y) * double(r1.y);
+  }
+  if (partial) {
+    block_sum<2>(acc, red);
+    if (threadIdx.x == 0) {
+      partial[((int64_t)s * nx + x) * 2 + 0] = acc[0];
+      partial[((int64_t)s * nx + x) * 2 + 1] = acc[1];
+    }
+  }
+}
+
 // The y and z axes of h psi for one y-z plane (x fixed) of one state, when
 // ny == nz: the plane of both spin components is read once along z pencils
 // and kept in LDS; the y pencils take it from there, compute the y terms
@@ -521,9 +868,41 @@ int launch_hpsi_plane(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T
   return launch_status();
 }
 
+// FP32 spinors on a 32^3 grid: the x pass on the matrix cores, then the
+// y-z plane pass (LaneFFT, or the matrix-core plane kernel)
+inline bool hpsi_mfma_path(const hfb_ctx* c, int prec) {
+  return prec == HFB_F32 && c->nx == 32 && c->ny == 32 && c->nz == 32;
+}
+
+#ifndef HFB_HPSI_MFMA_PLANE
+#define HFB_HPSI_MFMA_PLANE 0
+#endif
+
+template <typename T>
+int launch_hpsi_plane(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T> mf, const double* w,
+                      const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st);
+
+int hpsi_mfma(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<float> mf, const double* w,
+              const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st) {
+  hipLaunchKernelGGL(hpsi_mfma_x_kernel, dim3(c->ny * (c->nz / 16) / 4, ns), dim3(256), 0, st,
+                     (const cx<float>*)psi, (cx<float>*)pmf, mf, npsi0, c->ny, c->nz, float(c->dx), c->der1f[0]);
+  int rc = launch_status();
+  if (rc != HFB_OK) return rc;
+  if (!HFB_HPSI_MFMA_PLANE) return launch_hpsi_plane<float>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
+  const size_t lds = 6 * sizeof(cx<float>) * 32 * MP;
+  allow_full_lds((const void*)hpsi_mfma_plane_kernel);
+  hipLaunchKernelGGL(hpsi_mfma_plane_kernel, dim3(c->nx, ns), dim3(256), lds, st, (const cx<float>*)psi,
+                     (cx<float>*)pmf, (cx<float>*)pout, (cx<float>*)pdel, mf, w, wuv, npsi0, c->nx, float(c->dy),
+                     float(c->dz), c->der1f[1], c->der1f[2], partial);
+  return launch_status();
+}
+
 template <typename T>
 int hpsi_all(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T> mf, const double* w, const double* wuv,
              void* pout, void* pmf, void* pdel, double* partial, hipStream_t st) {
+  if constexpr (sizeof(T) == 4) {
+    if (hpsi_mfma_path(c, HFB_F32)) return hpsi_mfma(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
+  }
   int rc = launch_hpsi_axis<T, 0>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
   if (hpsi_plane_path(c, sizeof(T) == 4 ? HFB_F32 : HFB_F64)) {
     if (rc == HFB_OK) rc = launch_hpsi_plane<T>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
@@ -549,7 +928,7 @@ extern "C" int hfb_cderv(hfb_ctx* c, int prec, int axis, int kind, const void* i
 
 extern "C" int hfb_hpsi_ntiles(hfb_ctx* c, int prec) {
   if (!c) return HFB_E_NOCTX;
-  if (hpsi_plane_path(c, prec)) return c->nx;  // one partial per y-z plane
+  if (hpsi_mfma_path(c, prec) || hpsi_plane_path(c, prec)) return c->nx;  // one partial per y-z plane
   const int ppb = prec == HFB_F32 ? hpsi_pencils_per_block<float>() : hpsi_pencils_per_block<double>();
   return pencils_of_axis(c, 2) / ppb;
 }

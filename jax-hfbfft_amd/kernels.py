@@ -149,6 +149,15 @@ def poisson(ctx, rho: torch.Tensor, e2: float, wcoul: torch.Tensor | None = None
     return wcoul
 
 
+def selftest_mfma_f32(a: torch.Tensor, b: torch.Tensor):
+    """Raw (64 lanes, 4 registers) result of one 16x16x4 FP32 matrix-core product."""
+    need(a, torch.float32, (16, 4), "a")
+    need(b, torch.float32, (4, 16), "b")
+    d = torch.empty((64, 4), dtype=torch.float32, device=a.device)
+    check(lib().hfb_selftest_mfma_f32(ptr(a), ptr(b), ptr(d), stream_handle()), "selftest_mfma_f32")
+    return d
+
+
 def selftest_mfma_f64(a: torch.Tensor, b: torch.Tensor):
     need(a, torch.float64, (16, 4), "a")
     need(b, torch.float64, (4, 16), "b")

@@ -39,6 +39,22 @@ def test_mfma_f64_layout(pkg, dev):
     assert err < 1e-12
 
 
+def test_mfma_f32_layout(pkg, dev):
+    """The FP32 h.psi kernels read register r of lane l of a 16x16x4 product
+    as row 4*(l//16) + r, column l % 16."""
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((16, 4)).astype(np.float32)
+    b = rng.standard_normal((4, 16)).astype(np.float32)
+    raw = pkg.kernels.selftest_mfma_f32(torch.tensor(a, device=dev), torch.tensor(b, device=dev)).cpu().numpy()
+    ref = a.astype(np.float64) @ b.astype(np.float64)
+    want = np.array([[ref[4 * (l // 16) + r, l % 16] for r in range(4)] for l in range(64)])
+    err = relerr(raw, want)
+    if err > 1e-6:
+        print("raw lane layout:\n", raw)
+        print("reference:\n", ref)
+    assert err < 1e-6
+
+
 @pytest.mark.parametrize("n", [24, 32, 48])
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
 def test_cderv_all_kinds(pkg, O, dev, n, dtype):
