@@ -290,20 +290,36 @@ int hfb_skyrme_nblocks(hfb_ctx* ctx);
  * meanfield.py:163-391, incl. the DDDI rearrangement and ecorrp into
  * E[HFB_EN_ECORRP]).  The maximum of v_pair per isospin goes to
  * E[HFB_EN_VMAX..+1] (for hfb_grstep_weights).  The *32 outputs are
- * float copies for the FP32 spinor kernels and may be null. */
+ * float copies for the FP32 spinor kernels and may be null.  finish == 0
+ * leaves ecorrp and the v_pair maxima as partials in epart for
+ * hfb_energy_tail. */
 int hfb_skyrme(hfb_ctx* ctx, const hfb_force* f, const double* rho, const double* tau, const double* chi,
                const double* current, const double* sdens, const double* divj, const double* lap,
                const double* grad, const double* curls, const double* curlj, const double* wcoul, double* upot,
                double* bmass, double* wlspot, double* aq, double* spot, double* v_pair, float* upot32,
                float* bmass32, float* wlspot32, float* aq32, float* spot32, float* v_pair32, double* epart,
-               double* E, void* stream);
+               double* E, int finish, void* stream);
 
 /* Energy-functional integrals of integ_energy (energies.py:90-306) into the
  * slots of E, incl. the C-coupling split and ehfint; pairing energies are
- * read from E[HFB_EN_EPAIR..+1]. */
+ * read from E[HFB_EN_EPAIR..+1].  finish == 0 leaves the block partials
+ * in epart for hfb_energy_tail. */
 int hfb_integ_energy(hfb_ctx* ctx, const hfb_force* f, const double* rho, const double* tau,
                      const double* current, const double* sdens, const double* divj, const double* lap,
-                     const double* curlj, const double* wcoul, double* epart, double* E, void* stream);
+                     const double* curlj, const double* wcoul, double* epart, double* E, int finish,
+                     void* stream);
+
+/* The scalar tail of the SCF loop body (static.py:1825-1833) in one
+ * launch: ecorrp and the v_pair maxima from hfb_skyrme's partials
+ * (sky_part, finish == 0), the integrals of integ_energy from
+ * hfb_integ_energy's partials (en_part, finish == 0; current as given
+ * there), then sum_energy (energies.py:309-371) on the single-particle
+ * properties.  No single reference function corresponds: it finishes
+ * skyrme, integ_energy and sum_energy together. */
+int hfb_energy_tail(hfb_ctx* ctx, const hfb_force* f, const double* sky_part, const double* en_part,
+                    const double* current, int nstates, const double* wocc, const double* wstates,
+                    const double* sp_kinetic, const double* sp_energy, const double* sp_orbital,
+                    const double* sp_spin, double* E, void* stream);
 
 /* ehf, tke, orbital/spin/total angular momentum of sum_energy
  * (energies.py:309-371) into E. */

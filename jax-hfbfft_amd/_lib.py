@@ -65,8 +65,10 @@ _SIGS = {
     "hfb_meanfield_derivs": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hfb_skyrme_nblocks": (c_int, [c_void_p]),
     "hfb_skyrme": (c_int, [c_void_p, c_void_p] + [c_void_p] * 11 + [c_void_p] * 6 + [c_void_p] * 6 +
-                   [c_void_p, c_void_p, c_void_p]),
-    "hfb_integ_energy": (c_int, [c_void_p, c_void_p] + [c_void_p] * 8 + [c_void_p, c_void_p, c_void_p]),
+                   [c_void_p, c_void_p, c_int, c_void_p]),
+    "hfb_integ_energy": (c_int, [c_void_p, c_void_p] + [c_void_p] * 8 + [c_void_p, c_void_p, c_int, c_void_p]),
+    "hfb_energy_tail": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hfb_sum_energy": (c_int, [c_int] + [c_void_p] * 7 + [c_void_p]),
     "hfb_energy_bwd_nblocks": (c_int, [c_void_p]),
     "hfb_energy_bwd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int] + [c_void_p] * 11 + [c_void_p] * 6 +

@@ -331,8 +331,10 @@ __global__ void __launch_bounds__(256) energy_kernel(EnArgs a) {
     for (int j = 0; j < NES; ++j) a.part[(int64_t)blockIdx.x * NES + j] = s[j];
 }
 
-__global__ void energy_finish_kernel(EnArgs a, int nblk, double wxyz, const double* __restrict__ cur, int nx,
-                                     int ny, int nz, double* __restrict__ E) {
+// The integrals from energy_kernel's block partials (fixed order) into E
+// (energies.py:90-306); one workgroup.
+__device__ void energy_finish_body(const EnArgs& a, int nblk, double wxyz, const double* __restrict__ cur, int nx,
+                                   int ny, int nz, double* __restrict__ E) {
   __shared__ double red[4 * NES];
   double s[NES];
   for (int j = 0; j < NES; ++j) s[j] = 0.0;
@@ -389,11 +391,16 @@ __global__ void energy_finish_kernel(EnArgs a, int nblk, double wxyz, const doub
   E[HFB_EN_EHFINT] = ehft + ehf0 + ehf1 + ehf2 + ehf3 + ehfls + ehfc - epair - ezpe;
 }
 
+__global__ void energy_finish_kernel(EnArgs a, int nblk, double wxyz, const double* __restrict__ cur, int nx,
+                                     int ny, int nz, double* __restrict__ E) {
+  energy_finish_body(a, nblk, wxyz, cur, nx, ny, nz, E);
+}
+
 // ecorrp from the block partials (meanfield.py:360-363) and the maxima of
 // v_pair per isospin into E[HFB_EN_VMAX..+1].
-__global__ void ecorrp_finish_kernel(const double* part, int nblk, double wxyz, int ipair, double* E) {
+__device__ void ecorrp_finish_body(const double* part, int nblk, double wxyz, int ipair, double* E) {
   __shared__ double red[4];
-  __shared__ double vred[4][2];
+  __shared__ double vred[16][2];
   double s[1] = {0.0};
   double vm[2] = {-INFINITY, -INFINITY};
   for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
@@ -420,10 +427,14 @@ __global__ void ecorrp_finish_kernel(const double* part, int nblk, double wxyz, 
   }
 }
 
+__global__ void ecorrp_finish_kernel(const double* part, int nblk, double wxyz, int ipair, double* E) {
+  ecorrp_finish_body(part, nblk, wxyz, ipair, E);
+}
+
 // sum_energy (energies.py:309-371)
-__global__ void sum_energy_kernel(int ns, const double* wocc, const double* wstates, const double* kin,
-                                  const double* spe, const double* orb, const double* spin, double* E) {
-  __shared__ double red[4 * 8];
+__device__ void sum_energy_body(int ns, const double* wocc, const double* wstates, const double* kin,
+                                const double* spe, const double* orb, const double* spin, double* E) {
+  __shared__ double red[16 * 8];
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int s = threadIdx.x; s < ns; s += blockDim.x) {
     const double w = wocc[s] * wstates[s];
@@ -445,6 +456,25 @@ __global__ void sum_energy_kernel(int ns, const double* wocc, const double* wsta
       E[HFB_EN_JTOT + c] = v[2 + c] + v[5 + c];
     }
   }
+}
+
+__global__ void sum_energy_kernel(int ns, const double* wocc, const double* wstates, const double* kin,
+                                  const double* spe, const double* orb, const double* spin, double* E) {
+  sum_energy_body(ns, wocc, wstates, kin, spe, orb, spin, E);
+}
+
+// The scalar tail of an SCF iteration in one workgroup: ecorrp and the
+// v_pair maxima from the skyrme partials, the energy integrals from the
+// energy partials, then sum_energy on the single-particle properties.
+__global__ void __launch_bounds__(256) energy_tail_kernel(const double* sky_part, int nblk, int ipair, EnArgs a,
+                                                          double wxyz, const double* cur, int nx, int ny, int nz,
+                                                          int ns, const double* wocc, const double* wstates,
+                                                          const double* kin, const double* spe, const double* orb,
+                                                          const double* spin, double* E) {
+  ecorrp_finish_body(sky_part, nblk, wxyz, ipair, E);
+  energy_finish_body(a, nblk, wxyz, cur, nx, ny, nz, E);
+  __syncthreads();
+  sum_energy_body(ns, wocc, wstates, kin, spe, orb, spin, E);
 }
 
 void push(DerivJobs& J, double* dst, int nterm, const double* s0, int a0, int o0, double c0, const double* s1 = nullptr,
@@ -571,7 +601,7 @@ extern "C" int hfb_skyrme(hfb_ctx* c, const hfb_force* f, const double* rho, con
                           const double* grad, const double* curls, const double* curlj, const double* wcoul,
                           double* upot, double* bmass, double* wlspot, double* aq, double* spot, double* v_pair,
                           float* upot32, float* bmass32, float* wlspot32, float* aq32, float* spot32, float* v_pair32,
-                          double* epart, double* E, void* stream) {
+                          double* epart, double* E, int finish, void* stream) {
   if (!c) return HFB_E_NOCTX;
   if (!f || !rho || !tau || !chi || !current || !sdens || !divj || !lap || !grad || !curls || !curlj || !upot ||
       !bmass || !wlspot || !aq || !spot || !v_pair || !epart || !E)
@@ -609,7 +639,7 @@ extern "C" int hfb_skyrme(hfb_ctx* c, const hfb_force* f, const double* rho, con
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(skyrme_combine_kernel, dim3(nb), dim3(256), 0, st, a);
   int rc = launch_status();
-  if (rc != HFB_OK) return rc;
+  if (rc != HFB_OK || !finish) return rc;
   hipLaunchKernelGGL(ecorrp_finish_kernel, dim3(1), dim3(256), 0, st, epart, nb, c->dx * c->dy * c->dz, f->ipair, E);
   return launch_status();
 }
@@ -619,7 +649,8 @@ extern "C" int hfb_skyrme(hfb_ctx* c, const hfb_force* f, const double* rho, con
 // E[HFB_EN_EPAIR..+1].
 extern "C" int hfb_integ_energy(hfb_ctx* c, const hfb_force* f, const double* rho, const double* tau,
                                 const double* current, const double* sdens, const double* divj, const double* lap,
-                                const double* curlj, const double* wcoul, double* epart, double* E, void* stream) {
+                                const double* curlj, const double* wcoul, double* epart, double* E, int finish,
+                                void* stream) {
   if (!c) return HFB_E_NOCTX;
   if (!f || !rho || !tau || !current || !sdens || !divj || !lap || !curlj || !epart || !E) return HFB_E_ARG;
   if (f->tcoul && !wcoul) return HFB_E_ARG;
@@ -638,9 +669,28 @@ extern "C" int hfb_integ_energy(hfb_ctx* c, const hfb_force* f, const double* rh
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(energy_kernel, dim3(nb), dim3(256), 0, st, a);
   int rc = launch_status();
-  if (rc != HFB_OK) return rc;
+  if (rc != HFB_OK || !finish) return rc;
   hipLaunchKernelGGL(energy_finish_kernel, dim3(1), dim3(256), 0, st, a, nb, c->dx * c->dy * c->dz, current, c->nx,
                      c->ny, c->nz, E);
+  return launch_status();
+}
+
+extern "C" int hfb_energy_tail(hfb_ctx* c, const hfb_force* f, const double* sky_part, const double* en_part,
+                               const double* current, int nstates, const double* wocc, const double* wstates,
+                               const double* sp_kinetic, const double* sp_energy, const double* sp_orbital,
+                               const double* sp_spin, double* E, void* stream) {
+  if (!c) return HFB_E_NOCTX;
+  if (!f || !sky_part || !en_part || !current || nstates <= 0 || !wocc || !wstates || !sp_kinetic || !sp_energy ||
+      !sp_orbital || !sp_spin || !E)
+    return HFB_E_ARG;
+  EnArgs a{};
+  a.f = *f;
+  a.nvox = (int64_t)c->nx * c->ny * c->nz;
+  a.part = (double*)en_part;
+  const int nb = hfb_skyrme_nblocks(c);
+  hipLaunchKernelGGL(energy_tail_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, sky_part, nb, f->ipair, a,
+                     c->dx * c->dy * c->dz, current, c->nx, c->ny, c->nz, nstates, wocc, wstates, sp_kinetic,
+                     sp_energy, sp_orbital, sp_spin, E);
   return launch_status();
 }
 

@@ -83,16 +83,17 @@ def _set_slots(E, lo, src):
         dst.copy_(src.reshape(dst.shape))
 
 
-def integ_energy(coulomb, densities, energies, forces, grids, levels, params, pairs, derivs=None):
+def integ_energy(coulomb, densities, energies, forces, grids, levels, params, pairs, derivs=None, finish=True):
     """energies.py:90-306.  ``derivs``: DensityDerivs of these densities, if
-    already computed for skyrme.  Pairing energies come from pairs.epair."""
+    already computed for skyrme.  Pairing energies come from pairs.epair.
+    ``finish=False`` leaves the integrals to a later kernels.energy_tail."""
     from .meanfield import density_derivs
     if derivs is None:
         derivs = density_derivs(grids, densities)
     E = energies.vec
     _set_slots(E, EN["EPAIR"], pairs.epair)
     fc = forces.to_c(params, levels.mass_number)
-    kernels.integ_energy(grids.ctx, fc, densities, derivs, coulomb.wcoul, E)
+    kernels.integ_energy(grids.ctx, fc, densities, derivs, coulomb.wcoul, E, finish=finish)
     return energies
 
 

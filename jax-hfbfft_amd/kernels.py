@@ -381,7 +381,9 @@ def meanfield_derivs(ctx, mf):
                                      ptr(_deriv_work(ctx)), stream_handle()), "meanfield_derivs")
 
 
-def skyrme(ctx, fc, dens, der, wcoul, mf, E, f32=None):
+def skyrme(ctx, fc, dens, der, wcoul, mf, E, f32=None, finish=True):
+    """finish=False leaves ecorrp and the v_pair maxima as partials (the
+    "sky_part" workspace) for energy_tail."""
     nb = lib().hfb_skyrme_nblocks(ctx.handle)
     epart = ctx.work("sky_part", (3 * nb,), torch.float64)
     g = f32 or {}
@@ -390,15 +392,30 @@ def skyrme(ctx, fc, dens, der, wcoul, mf, E, f32=None):
                            ptr(der["curls"]), ptr(der["curlj"]), ptr(wcoul), ptr(mf.upot), ptr(mf.bmass),
                            ptr(mf.wlspot), ptr(mf.aq), ptr(mf.spot), ptr(mf.v_pair), ptr(g.get("upot")),
                            ptr(g.get("bmass")), ptr(g.get("wlspot")), ptr(g.get("aq")), ptr(g.get("spot")),
-                           ptr(g.get("v_pair")), ptr(epart), ptr(E), stream_handle()), "skyrme")
+                           ptr(g.get("v_pair")), ptr(epart), ptr(E), int(bool(finish)), stream_handle()), "skyrme")
 
 
-def integ_energy(ctx, fc, dens, der, wcoul, E):
+def integ_energy(ctx, fc, dens, der, wcoul, E, finish=True):
+    """finish=False leaves the block partials (the "en_part" workspace) for
+    energy_tail."""
     nb = lib().hfb_skyrme_nblocks(ctx.handle)
     epart = ctx.work("en_part", (nb * 20,), torch.float64)
     check(lib().hfb_integ_energy(ctx.handle, ctypes.byref(fc), ptr(dens.rho), ptr(dens.tau), ptr(dens.current),
                                  ptr(dens.sdens), ptr(der["divj"]), ptr(der["lap"]), ptr(der["curlj"]), ptr(wcoul),
-                                 ptr(epart), ptr(E), stream_handle()), "integ_energy")
+                                 ptr(epart), ptr(E), int(bool(finish)),
+                                 stream_handle()), "integ_energy")
+
+
+def energy_tail(ctx, fc, dens, levels, E):
+    """ecorrp, the integrals and sum_energy from the partials that
+    skyrme(..., finish=False) and integ_energy(..., finish=False) left."""
+    nb = lib().hfb_skyrme_nblocks(ctx.handle)
+    sky = ctx.work("sky_part", (3 * nb,), torch.float64)
+    en = ctx.work("en_part", (nb * 20,), torch.float64)
+    check(lib().hfb_energy_tail(ctx.handle, ctypes.byref(fc), ptr(sky), ptr(en), ptr(dens.current),
+                                int(levels.nstmax), ptr(levels.wocc), ptr(levels.wstates), ptr(levels.sp_kinetic),
+                                ptr(levels.sp_energy), ptr(levels.sp_orbital), ptr(levels.sp_spin), ptr(E),
+                                stream_handle()), "energy_tail")
 
 
 def sum_energy(levels, E):

@@ -279,13 +279,18 @@ def iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, m
     # properties from the same transforms (static.py:1800-1833)
     _, spart = add_density_sweep(densities, grids, levels, mix=True, addnew=0.2, moment=moment, want_sp=True)
     derivs = density_derivs(grids, densities, out=derivs)
+    # skyrme, sp_properties, integ_energy and sum_energy (static.py:1810-1833)
+    # with the scalar finishing steps of the first, third and fourth in one
+    # launch at the end
     skyrme(coulomb, densities, forces, grids, meanfield, params, static, derivs=derivs, E=energies.vec,
-           mass_number=levels.mass_number)
+           mass_number=levels.mass_number, finish=False)
     if static.tsort:
         raise NotImplementedError("sort_states is outside the static HF+BCS path")
     sp_properties(forces, grids, levels, moment, spart=spart)
-    integ_energy(coulomb, densities, energies, forces, grids, levels, params, pairs, derivs=derivs)
-    sum_energy(energies, levels, meanfield, pairs)
+    integ_energy(coulomb, densities, energies, forces, grids, levels, params, pairs, derivs=derivs, finish=False)
+    # (integ_energy has put pairs.epair into the energy vector, skyrme writes
+    # ecorrp there: everything sum_energy reads is in place)
+    kernels.energy_tail(grids.ctx, forces.to_c(params, levels.mass_number), densities, levels, energies.vec)
     converged = False
     if check or static.tvaryx_0:
         e = energies.host()
