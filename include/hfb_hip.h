@@ -253,6 +253,14 @@ int hfb_density(hfb_ctx* ctx, int prec, const void* psi, int nst0, int nst1, con
 int hfb_density_finish(hfb_ctx* ctx, const double* dout, double* rho, double* chi, double* tau, double* current,
                        double* sdens, double* sodens, int mix, double addnew, void* stream);
 
+/* hfb_density_finish and hfb_sp_finish in one launch (the loop's density
+ * update and single-particle properties, static.py:1800-1823 with
+ * inout.py:64-149); arguments as there. */
+int hfb_density_sp_finish(hfb_ctx* ctx, const double* dout, double* rho, double* chi, double* tau,
+                          double* current, double* sdens, double* sodens, int mix, double addnew,
+                          const double* spart, int nst0, int nst1, double h2m0, double h2m1, double* sp_kinetic,
+                          double* sp_orbital, double* sp_spin, double* sp_parity, void* stream);
+
 /* spart -> sp_kinetic (S), sp_orbital (S,3), sp_spin (S,3), sp_parity (S)
  * with the wxyz and h2m_q factors of inout.py:140-148. */
 int hfb_sp_finish(hfb_ctx* ctx, const double* spart, int nst0, int nst1, double h2m0, double h2m1,
@@ -308,6 +316,18 @@ int hfb_integ_energy(hfb_ctx* ctx, const hfb_force* f, const double* rho, const 
                      const double* current, const double* sdens, const double* divj, const double* lap,
                      const double* curlj, const double* wcoul, double* epart, double* E, int finish,
                      void* stream);
+
+/* hfb_skyrme and hfb_integ_energy with finish == 0 in one launch (the
+ * loop body's skyrme and integ_energy, meanfield.py:163-391 and
+ * energies.py:90-306): the potentials, and the partials of ecorrp, the
+ * v_pair maxima (sky_part, 3x hfb_skyrme_nblocks doubles) and of the
+ * integrals (en_part, 20x) for hfb_energy_tail. */
+int hfb_skyrme_energy(hfb_ctx* ctx, const hfb_force* f, const double* rho, const double* tau, const double* chi,
+                      const double* current, const double* sdens, const double* divj, const double* lap,
+                      const double* grad, const double* curls, const double* curlj, const double* wcoul,
+                      double* upot, double* bmass, double* wlspot, double* aq, double* spot, double* v_pair,
+                      float* upot32, float* bmass32, float* wlspot32, float* aq32, float* spot32, float* v_pair32,
+                      double* sky_part, double* en_part, void* stream);
 
 /* The scalar tail of the SCF loop body (static.py:1825-1833) in one
  * launch: ecorrp and the v_pair maxima from hfb_skyrme's partials

@@ -58,6 +58,8 @@ _SIGS = {
                             c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "hfb_density_finish": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                    c_int, c_double, c_void_p]),
+    "hfb_density_sp_finish": (c_int, [c_void_p] * 8 + [c_int, c_double, c_void_p, c_int, c_int, c_double, c_double]
+                              + [c_void_p] * 5),
     "hfb_sp_finish": (c_int, [c_void_p, c_void_p, c_int, c_int, c_double, c_double, c_void_p, c_void_p, c_void_p,
                               c_void_p, c_void_p]),
     "hfb_deriv_workspace": (ctypes.c_longlong, [c_void_p]),
@@ -67,6 +69,8 @@ _SIGS = {
     "hfb_skyrme": (c_int, [c_void_p, c_void_p] + [c_void_p] * 11 + [c_void_p] * 6 + [c_void_p] * 6 +
                    [c_void_p, c_void_p, c_int, c_void_p]),
     "hfb_integ_energy": (c_int, [c_void_p, c_void_p] + [c_void_p] * 8 + [c_void_p, c_void_p, c_int, c_void_p]),
+    "hfb_skyrme_energy": (c_int, [c_void_p, c_void_p] + [c_void_p] * 11 + [c_void_p] * 6 + [c_void_p] * 6 +
+                          [c_void_p, c_void_p, c_void_p]),
     "hfb_energy_tail": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hfb_sum_energy": (c_int, [c_int] + [c_void_p] * 7 + [c_void_p]),

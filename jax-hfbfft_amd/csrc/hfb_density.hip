@@ -65,17 +65,17 @@ __host__ __device__ __forceinline__ int pencils_axis(int ax, int nx, int ny, int
   return ax == 0 ? ny * nz : (ax == 1 ? nx * nz : nx * ny);
 }
 
+// One workgroup of the axis pass AXIS: pencil tile `tile` of isospin iq,
+// state chunk `chunk`.
 template <typename T, int N, int AXIS>
-__global__ void __launch_bounds__(256) dens_lane_kernel(DensArgs a, const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void dens_body(const DensArgs& a, const cx<T>* __restrict__ tw, int tile, int iq,
+                                          int chunk) {
   constexpr int L = DENS_L, N1 = N / L, PW = DENS_PW;
   constexpr int NACC = AXIS == 0 ? 4 : (AXIS == 1 ? 6 : 7);
   __shared__ double comb[4][PW * N];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = lane % L, q = lane / L;
-  const int tile = blockIdx.x;
   const int g = tile * PW + q;  // pencil
-  const int iq = blockIdx.y;
-  const int chunk = blockIdx.z;
   const int nst = iq == 0 ? a.nst0 : a.nst1;
   const int start = iq == 0 ? 0 : a.nst0;
   const int per = (nst + DENS_G - 1) / DENS_G;
@@ -263,6 +263,41 @@ __global__ void __launch_bounds__(256) dens_lane_kernel(DensArgs a, const cx<T>*
   }
 }
 
+template <typename T, int N, int AXIS>
+__global__ void __launch_bounds__(256) dens_lane_kernel(DensArgs a, const cx<T>* __restrict__ tw) {
+  dens_body<T, N, AXIS>(a, tw, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// All three axis passes of a cubic grid in one launch: the workgroups of
+// the x, y and z passes follow each other along blockIdx.x (the passes are
+// independent: each reads the states and writes its own slots).
+template <typename T, int N>
+__global__ void __launch_bounds__(256) dens_all_kernel(DensArgs a, const cx<T>* __restrict__ tw, int ntile,
+                                                       double* sp0, double* sp1, double* sp2) {
+  const int t = blockIdx.x;
+  a.ntile = ntile;
+  if (t < ntile) {
+    a.spart = sp0;
+    dens_body<T, N, 0>(a, tw, t, blockIdx.y, blockIdx.z);
+  } else if (t < 2 * ntile) {
+    a.spart = sp1;
+    dens_body<T, N, 1>(a, tw, t - ntile, blockIdx.y, blockIdx.z);
+  } else {
+    a.spart = sp2;
+    dens_body<T, N, 2>(a, tw, t - 2 * ntile, blockIdx.y, blockIdx.z);
+  }
+}
+
+template <typename T>
+int launch_dens_all(const hfb_ctx* c, DensArgs a, double* sp0, double* sp1, double* sp2, hipStream_t st) {
+  const int ntile = pencils_axis(0, c->nx, c->ny, c->nz) / DENS_PW;
+  HFB_DISPATCH_N(c->nx, {
+    hipLaunchKernelGGL((dens_all_kernel<T, N_>), dim3(3 * ntile, 2, DENS_G), dim3(256), 0, st, a,
+                       twiddles<T>(c, N_), ntile, sp0, sp1, sp2);
+  })
+  return launch_status();
+}
+
 template <typename T, int AXIS>
 int launch_dens(const hfb_ctx* c, DensArgs a, hipStream_t st) {
   const int n = axis_len(c, AXIS);
@@ -278,10 +313,12 @@ int launch_dens(const hfb_ctx* c, DensArgs a, hipStream_t st) {
 
 // Sum the axis slots, optional mixing new*addnew + old*(1-addnew) of rho,
 // tau, chi (static.py:1817-1823).
-__global__ void dens_finish_kernel(const double* __restrict__ d, int64_t nvox, double* rho, double* chi,
-                                   double* tau, double* cur, double* sdens, double* sodens, int mix, double addnew) {
+// (workgroup vb of vg workgroups)
+__device__ __forceinline__ void dens_finish_body(const double* __restrict__ d, int64_t nvox, double* rho,
+                                                 double* chi, double* tau, double* cur, double* sdens,
+                                                 double* sodens, int mix, double addnew, int vb, int vg) {
   const int64_t tot = 2 * nvox;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t e = vb * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)vg * blockDim.x) {
     const int iq = e / nvox;
     const int64_t v = e % nvox;
     double sl[NSLOT];
@@ -316,11 +353,15 @@ __global__ void dens_finish_kernel(const double* __restrict__ d, int64_t nvox, d
   }
 }
 
+__global__ void dens_finish_kernel(const double* __restrict__ d, int64_t nvox, double* rho, double* chi,
+                                   double* tau, double* cur, double* sdens, double* sodens, int mix, double addnew) {
+  dens_finish_body(d, nvox, rho, chi, tau, cur, sdens, sodens, mix, addnew, blockIdx.x, gridDim.x);
+}
+
 // Per-state properties from the axis partials (inout.py:98-148).
-__global__ void sp_finish_kernel(const double* __restrict__ spart, int ns, int nst0, int nt0, int nt1, int nt2,
-                                 double wxyz, double h2m0, double h2m1, double* kin, double* orb, double* spin,
-                                 double* par) {
-  const int s = blockIdx.x;
+__device__ __forceinline__ void sp_finish_body(const double* __restrict__ spart, int ns, int nst0, int nt0, int nt1,
+                                               int nt2, double wxyz, double h2m0, double h2m1, double* kin, double* orb,
+                                               double* spin, double* par, int s) {
   __shared__ double red[4 * 8];
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int nts[3] = {nt0, nt1, nt2};
@@ -345,6 +386,32 @@ __global__ void sp_finish_kernel(const double* __restrict__ spart, int ns, int n
     spin[3 * s + 2] = wxyz * 0.5 * v[6];
     par[s] = wxyz * v[7];
   }
+}
+
+__global__ void sp_finish_kernel(const double* __restrict__ spart, int ns, int nst0, int nt0, int nt1, int nt2,
+                                 double wxyz, double h2m0, double h2m1, double* kin, double* orb, double* spin,
+                                 double* par) {
+  sp_finish_body(spart, ns, nst0, nt0, nt1, nt2, wxyz, h2m0, h2m1, kin, orb, spin, par, blockIdx.x);
+}
+
+// density finish (workgroups [0, nfin)) and sp_finish (one workgroup per
+// state after them) in one launch: both only read the axis passes' output
+struct SpFin {
+  const double* spart;
+  int ns, nst0, nt0, nt1, nt2;
+  double wxyz, h2m0, h2m1;
+  double *kin, *orb, *spin, *par;
+};
+
+__global__ void __launch_bounds__(256) dens_sp_finish_kernel(const double* __restrict__ d, int64_t nvox, double* rho,
+                                                             double* chi, double* tau, double* cur, double* sdens,
+                                                             double* sodens, int mix, double addnew, int nfin,
+                                                             SpFin f) {
+  if ((int)blockIdx.x < nfin)
+    dens_finish_body(d, nvox, rho, chi, tau, cur, sdens, sodens, mix, addnew, blockIdx.x, nfin);
+  else
+    sp_finish_body(f.spart, f.ns, f.nst0, f.nt0, f.nt1, f.nt2, f.wxyz, f.h2m0, f.h2m1, f.kin, f.orb, f.spin, f.par,
+                   blockIdx.x - nfin);
 }
 
 }  // namespace
@@ -392,6 +459,14 @@ extern "C" int hfb_density(hfb_ctx* c, int prec, const void* psi, int nst0, int 
   const int nt1 = pencils_axis(1, c->nx, c->ny, c->nz) / DENS_PW;
   hipStream_t st = (hipStream_t)stream;
   int rc = HFB_OK;
+  if (c->nx == c->ny && c->ny == c->nz && pencils_axis(0, c->nx, c->ny, c->nz) % DENS_PW == 0) {
+    double* sp0 = spart;
+    double* sp1 = spart ? spart + (int64_t)ns * nt0 * 8 : nullptr;
+    double* sp2 = spart ? spart + (int64_t)ns * (nt0 + nt1) * 8 : nullptr;
+    if (prec == HFB_F64) return launch_dens_all<double>(c, a, sp0, sp1, sp2, st);
+    if (prec == HFB_F32) return launch_dens_all<float>(c, a, sp0, sp1, sp2, st);
+    return HFB_E_ARG;
+  }
   for (int ax = 0; ax < 3 && rc == HFB_OK; ++ax) {
     a.spart = spart ? spart + (ax == 0 ? 0 : (ax == 1 ? (int64_t)ns * nt0 * 8 : (int64_t)ns * (nt0 + nt1) * 8)) : nullptr;
     if (prec == HFB_F64) {
@@ -418,6 +493,26 @@ extern "C" int hfb_density_finish(hfb_ctx* c, const double* dout, double* rho, d
   const int nb = (int)((2 * nvox + 255) / 256);
   hipLaunchKernelGGL(dens_finish_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, dout, nvox, rho, chi, tau,
                      current, sdens, sodens, mix, addnew);
+  return launch_status();
+}
+
+extern "C" int hfb_density_sp_finish(hfb_ctx* c, const double* dout, double* rho, double* chi, double* tau,
+                                     double* current, double* sdens, double* sodens, int mix, double addnew,
+                                     const double* spart, int nst0, int nst1, double h2m0, double h2m1,
+                                     double* sp_kinetic, double* sp_orbital, double* sp_spin, double* sp_parity,
+                                     void* stream) {
+  if (!c) return HFB_E_NOCTX;
+  if (!dout || !rho || !chi || !tau || !current || !sdens || !sodens) return HFB_E_ARG;
+  if (!spart || !sp_kinetic || !sp_orbital || !sp_spin || !sp_parity) return HFB_E_ARG;
+  const int ns = nst0 + nst1;
+  if (ns <= 0) return HFB_E_ARG;
+  const int64_t nvox = (int64_t)c->nx * c->ny * c->nz;
+  const int nfin = (int)((2 * nvox + 255) / 256);
+  SpFin f{spart, ns, nst0, pencils_axis(0, c->nx, c->ny, c->nz) / DENS_PW,
+          pencils_axis(1, c->nx, c->ny, c->nz) / DENS_PW, pencils_axis(2, c->nx, c->ny, c->nz) / DENS_PW,
+          c->dx * c->dy * c->dz, h2m0, h2m1, sp_kinetic, sp_orbital, sp_spin, sp_parity};
+  hipLaunchKernelGGL(dens_sp_finish_kernel, dim3(nfin + ns), dim3(256), 0, (hipStream_t)stream, dout, nvox, rho,
+                     chi, tau, current, sdens, sodens, mix, addnew, nfin, f);
   return launch_status();
 }
 

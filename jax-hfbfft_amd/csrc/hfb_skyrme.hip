@@ -181,14 +181,15 @@ struct SkyArgs {
   double* epart;  // per-block partials: [0, nb) sum rho_tot * rearr (DDDI), [nb, 3 nb) max v_pair per isospin
 };
 
-__global__ void __launch_bounds__(256) skyrme_combine_kernel(SkyArgs a) {
+// (workgroup vb of vg)
+__device__ __forceinline__ void skyrme_body(const SkyArgs& a, int vb, int vg) {
   const hfb_force& f = a.f;
   const int64_t n = a.nvox;
   __shared__ double red[4];
   __shared__ double vred[4][2];
   double ep[1] = {0.0};
   double vm[2] = {-INFINITY, -INFINITY};
-  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t v = vb * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)vg * blockDim.x) {
     const double rn = a.rho[v], rp = a.rho[n + v];
     const double rt = rn + rp;
     const double rtp = pow(rt, f.power);
@@ -245,7 +246,7 @@ __global__ void __launch_bounds__(256) skyrme_combine_kernel(SkyArgs a) {
     }
   }
   block_sum<1>(ep, red);
-  if (threadIdx.x == 0) a.epart[blockIdx.x] = ep[0];
+  if (threadIdx.x == 0) a.epart[vb] = ep[0];
   // per-block maxima of v_pair
   for (int o = 32; o > 0; o >>= 1) {
     vm[0] = fmax(vm[0], __shfl_xor(vm[0], o, 64));
@@ -260,9 +261,11 @@ __global__ void __launch_bounds__(256) skyrme_combine_kernel(SkyArgs a) {
   if (threadIdx.x < 2) {
     double m = vred[0][threadIdx.x];
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, vred[w][threadIdx.x]);
-    a.epart[(1 + threadIdx.x) * gridDim.x + blockIdx.x] = m;
+    a.epart[(1 + threadIdx.x) * vg + vb] = m;
   }
 }
+
+__global__ void __launch_bounds__(256) skyrme_combine_kernel(SkyArgs a) { skyrme_body(a, blockIdx.x, gridDim.x); }
 
 // ---------------------------------------------------------------------------
 // Energy integrals (energies.py:90-306): per-block partial sums.
@@ -282,7 +285,8 @@ struct EnArgs {
   double* part;  // [block][NES]
 };
 
-__global__ void __launch_bounds__(256) energy_kernel(EnArgs a) {
+// (workgroup vb of vg)
+__device__ __forceinline__ void energy_body(const EnArgs& a, int vb, int vg) {
   const hfb_force& f = a.f;
   const int64_t n = a.nvox;
   __shared__ double red[4 * NES];
@@ -290,7 +294,7 @@ __global__ void __launch_bounds__(256) energy_kernel(EnArgs a) {
 #pragma unroll
   for (int j = 0; j < NES; ++j) s[j] = 0.0;
   const double sc = f.ex != 0 ? -0.75 * f.slate : 0.0;
-  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t v = vb * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)vg * blockDim.x) {
     const double rn = a.rho[v], rp = a.rho[n + v];
     const double rt = rn + rp, r1 = -rn + rp;
     const double ln = a.lap[v], lp = a.lap[n + v];
@@ -328,7 +332,18 @@ __global__ void __launch_bounds__(256) energy_kernel(EnArgs a) {
   }
   block_sum<NES>(s, red);
   if (threadIdx.x == 0)
-    for (int j = 0; j < NES; ++j) a.part[(int64_t)blockIdx.x * NES + j] = s[j];
+    for (int j = 0; j < NES; ++j) a.part[(int64_t)vb * NES + j] = s[j];
+}
+
+__global__ void __launch_bounds__(256) energy_kernel(EnArgs a) { energy_body(a, blockIdx.x, gridDim.x); }
+
+// skyrme_combine and the energy partials in one launch (both read only the
+// densities, the derived fields and wcoul): workgroups [0, nb) and [nb, 2 nb)
+__global__ void __launch_bounds__(256) skyrme_energy_kernel(SkyArgs sa, EnArgs ea, int nb) {
+  if ((int)blockIdx.x < nb)
+    skyrme_body(sa, blockIdx.x, nb);
+  else
+    energy_body(ea, blockIdx.x - nb, nb);
 }
 
 // The integrals from energy_kernel's block partials (fixed order) into E
@@ -672,6 +687,62 @@ extern "C" int hfb_integ_energy(hfb_ctx* c, const hfb_force* f, const double* rh
   if (rc != HFB_OK || !finish) return rc;
   hipLaunchKernelGGL(energy_finish_kernel, dim3(1), dim3(256), 0, st, a, nb, c->dx * c->dy * c->dz, current, c->nx,
                      c->ny, c->nz, E);
+  return launch_status();
+}
+
+extern "C" int hfb_skyrme_energy(hfb_ctx* c, const hfb_force* f, const double* rho, const double* tau,
+                                 const double* chi, const double* current, const double* sdens, const double* divj,
+                                 const double* lap, const double* grad, const double* curls, const double* curlj,
+                                 const double* wcoul, double* upot, double* bmass, double* wlspot, double* aq,
+                                 double* spot, double* v_pair, float* upot32, float* bmass32, float* wlspot32,
+                                 float* aq32, float* spot32, float* v_pair32, double* sky_part, double* en_part,
+                                 void* stream) {
+  if (!c) return HFB_E_NOCTX;
+  if (!f || !rho || !tau || !chi || !current || !sdens || !divj || !lap || !grad || !curls || !curlj || !upot ||
+      !bmass || !wlspot || !aq || !spot || !v_pair || !sky_part || !en_part)
+    return HFB_E_ARG;
+  if (f->tcoul && !wcoul) return HFB_E_ARG;
+  if (upot32 && (!bmass32 || !wlspot32 || !aq32 || !spot32 || !v_pair32)) return HFB_E_ARG;
+  SkyArgs a{};
+  a.f = *f;
+  a.nvox = (int64_t)c->nx * c->ny * c->nz;
+  a.rho = rho;
+  a.tau = tau;
+  a.chi = chi;
+  a.cur = current;
+  a.sdens = sdens;
+  a.divj = divj;
+  a.lap = lap;
+  a.grad = grad;
+  a.curls = curls;
+  a.curlj = curlj;
+  a.wcoul = wcoul;
+  a.upot = upot;
+  a.bmass = bmass;
+  a.wlspot = wlspot;
+  a.aq = aq;
+  a.spot = spot;
+  a.vpair = v_pair;
+  a.upot32 = upot32;
+  a.bmass32 = bmass32;
+  a.wlspot32 = wlspot32;
+  a.aq32 = aq32;
+  a.spot32 = spot32;
+  a.vpair32 = v_pair32;
+  a.epart = sky_part;
+  EnArgs e{};
+  e.f = *f;
+  e.nvox = a.nvox;
+  e.rho = rho;
+  e.tau = tau;
+  e.sdens = sdens;
+  e.divj = divj;
+  e.lap = lap;
+  e.curlj = curlj;
+  e.wcoul = wcoul;
+  e.part = en_part;
+  const int nb = hfb_skyrme_nblocks(c);
+  hipLaunchKernelGGL(skyrme_energy_kernel, dim3(2 * nb), dim3(256), 0, (hipStream_t)stream, a, e, nb);
   return launch_status();
 }
 

@@ -34,11 +34,13 @@ def add_density(densities, grids, levels):
     return add_density_sweep(densities, grids, levels)[0]
 
 
-def add_density_sweep(densities, grids, levels, mix=False, addnew=0.2, moment=None, want_sp=False):
+def add_density_sweep(densities, grids, levels, mix=False, addnew=0.2, moment=None, want_sp=False, sp_h2m=None):
     """add_density in one sweep over the states, optionally fused with the
     loop's linear mixing of rho, tau, chi (``mix``, static.py:1817-1823) and
     with the per-state kinetic energy, <L>, <S> and parity of sp_properties
-    (``want_sp``, inout.py:64-149) from the same transforms.  Returns
+    (``want_sp``, inout.py:64-149) from the same transforms.  With
+    ``sp_h2m`` (forces.h2m) the single-particle properties are also
+    finished into levels (what sp_properties(..., spart) does).  Returns
     (densities, per-state partial sums or None)."""
     ctx = grids.ctx
     n0, n1 = int(levels.npsi[0]), int(levels.npsi[1])
@@ -46,5 +48,8 @@ def add_density_sweep(densities, grids, levels, mix=False, addnew=0.2, moment=No
         torch.zeros(3, dtype=torch.float64, device=levels.psi.device)
     dout, spart = kernels.density(ctx, levels.psi, n0, n1, levels.wocc, levels.wstates, levels.wguv, levels.pairwg,
                                   cm, True, want_sp)
-    kernels.density_finish(ctx, dout, densities, mix, addnew)
+    if want_sp and sp_h2m is not None:
+        kernels.density_sp_finish(ctx, dout, densities, mix, addnew, spart, n0, n1, sp_h2m, levels)
+    else:
+        kernels.density_finish(ctx, dout, densities, mix, addnew)
     return densities, spart

@@ -346,6 +346,15 @@ def density_finish(ctx, dout, dens, mix, addnew=0.2):
                                    float(addnew), stream_handle()), "density_finish")
 
 
+def density_sp_finish(ctx, dout, dens, mix, addnew, spart, nst0, nst1, h2m, levels):
+    """density_finish and sp_finish in one launch."""
+    check(lib().hfb_density_sp_finish(ctx.handle, ptr(dout), ptr(dens.rho), ptr(dens.chi), ptr(dens.tau),
+                                      ptr(dens.current), ptr(dens.sdens), ptr(dens.sodens), int(bool(mix)),
+                                      float(addnew), ptr(spart), int(nst0), int(nst1), float(h2m[0]), float(h2m[1]),
+                                      ptr(levels.sp_kinetic), ptr(levels.sp_orbital), ptr(levels.sp_spin),
+                                      ptr(levels.sp_parity), stream_handle()), "density_sp_finish")
+
+
 def sp_finish(ctx, spart, nst0, nst1, h2m, levels):
     check(lib().hfb_sp_finish(ctx.handle, ptr(spart), int(nst0), int(nst1), float(h2m[0]), float(h2m[1]),
                               ptr(levels.sp_kinetic), ptr(levels.sp_orbital), ptr(levels.sp_spin),
@@ -404,6 +413,22 @@ def integ_energy(ctx, fc, dens, der, wcoul, E, finish=True):
                                  ptr(dens.sdens), ptr(der["divj"]), ptr(der["lap"]), ptr(der["curlj"]), ptr(wcoul),
                                  ptr(epart), ptr(E), int(bool(finish)),
                                  stream_handle()), "integ_energy")
+
+
+def skyrme_energy(ctx, fc, dens, der, wcoul, mf, f32=None):
+    """skyrme(..., finish=False) and integ_energy(..., finish=False) in one
+    launch; energy_tail finishes both."""
+    nb = lib().hfb_skyrme_nblocks(ctx.handle)
+    sky = ctx.work("sky_part", (3 * nb,), torch.float64)
+    en = ctx.work("en_part", (nb * 20,), torch.float64)
+    g = f32 or {}
+    check(lib().hfb_skyrme_energy(ctx.handle, ctypes.byref(fc), ptr(dens.rho), ptr(dens.tau), ptr(dens.chi),
+                                  ptr(dens.current), ptr(dens.sdens), ptr(der["divj"]), ptr(der["lap"]),
+                                  ptr(der["grad"]), ptr(der["curls"]), ptr(der["curlj"]), ptr(wcoul), ptr(mf.upot),
+                                  ptr(mf.bmass), ptr(mf.wlspot), ptr(mf.aq), ptr(mf.spot), ptr(mf.v_pair),
+                                  ptr(g.get("upot")), ptr(g.get("bmass")), ptr(g.get("wlspot")), ptr(g.get("aq")),
+                                  ptr(g.get("spot")), ptr(g.get("v_pair")), ptr(sky), ptr(en), stream_handle()),
+          "skyrme_energy")
 
 
 def energy_tail(ctx, fc, dens, levels, E):

@@ -116,14 +116,16 @@ def density_derivs(grids, densities, out=None) -> DensityDerivs:
 
 
 def skyrme(coulomb, densities, forces, grids, meanfield, params, static, constraints=None, derivs=None, E=None,
-           mass_number=None, wcoul_ready=False, finish=True):
+           mass_number=None, wcoul_ready=False, finish=True, with_energy=False):
     """Potentials from the densities (meanfield.py:163-391); returns (meanfield, coulomb).
 
     ``derivs`` may carry precomputed DensityDerivs of these densities; ``E``
     is a device energy vector whose ecorrp slot is written (a private one is
     used otherwise); ``wcoul_ready``: coulomb.wcoul already holds the
     Coulomb potential of these densities; ``finish=False`` leaves ecorrp
-    and the v_pair maxima to a later kernels.energy_tail."""
+    and the v_pair maxima to a later kernels.energy_tail; ``with_energy``
+    (with finish=False) also leaves the partials of integ_energy for it,
+    from the same launch."""
     if constraints is not None and getattr(constraints, "tconstraint", False):
         raise NotImplementedError("constraint potentials are not part of the static path")
     if static is not None and getattr(static, "outertype", "N") != "N":
@@ -138,7 +140,12 @@ def skyrme(coulomb, densities, forces, grids, meanfield, params, static, constra
         E = torch.zeros(kernels.EN_LEN, dtype=torch.float64, device=densities.rho.device)
     A = mass_number if mass_number is not None else 0.0
     fc = forces.to_c(params, A)
-    kernels.skyrme(ctx, fc, densities, derivs, coulomb.wcoul, meanfield, E, meanfield.f32 or None, finish=finish)
+    if with_energy:
+        if finish:
+            raise ValueError("with_energy leaves the finishing step to kernels.energy_tail (finish=False)")
+        kernels.skyrme_energy(ctx, fc, densities, derivs, coulomb.wcoul, meanfield, meanfield.f32 or None)
+    else:
+        kernels.skyrme(ctx, fc, densities, derivs, coulomb.wcoul, meanfield, E, meanfield.f32 or None, finish=finish)
     meanfield.derivs_ctx = ctx
     meanfield.ecorrp_dev = E[kernels.EN["ECORRP"]]
     meanfield.vpair_max = E[kernels.EN["VMAX"]:kernels.EN["VMAX"] + 2]

@@ -16,7 +16,7 @@ import torch
 
 from . import kernels
 from .densities import add_density, add_density_sweep
-from .energies import integ_energy, sum_energy
+from .energies import _set_slots, integ_energy, sum_energy
 from .inout import sp_properties
 from .meanfield import density_derivs, skyrme
 from .pairs import pair
@@ -277,19 +277,20 @@ def iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, m
         raise NotImplementedError("hfb_natural sub-iterations are outside the static HF+BCS path")
     # density with the 0.2/0.8 mixing of rho, tau, chi and the per-state
     # properties from the same transforms (static.py:1800-1833)
-    _, spart = add_density_sweep(densities, grids, levels, mix=True, addnew=0.2, moment=moment, want_sp=True)
+    # (sp_properties, static.py:1831, is finished in the same launch)
+    _, spart = add_density_sweep(densities, grids, levels, mix=True, addnew=0.2, moment=moment, want_sp=True,
+                                 sp_h2m=forces.h2m)
     derivs = density_derivs(grids, densities, out=derivs)
     # skyrme, sp_properties, integ_energy and sum_energy (static.py:1810-1833)
     # with the scalar finishing steps of the first, third and fourth in one
     # launch at the end
     skyrme(coulomb, densities, forces, grids, meanfield, params, static, derivs=derivs, E=energies.vec,
-           mass_number=levels.mass_number, finish=False)
+           mass_number=levels.mass_number, finish=False, with_energy=True)
     if static.tsort:
         raise NotImplementedError("sort_states is outside the static HF+BCS path")
-    sp_properties(forces, grids, levels, moment, spart=spart)
-    integ_energy(coulomb, densities, energies, forces, grids, levels, params, pairs, derivs=derivs, finish=False)
-    # (integ_energy has put pairs.epair into the energy vector, skyrme writes
-    # ecorrp there: everything sum_energy reads is in place)
+    # integ_energy's integrals come from the same launch as the potentials;
+    # its pairing energies go to the energy vector as integ_energy does
+    _set_slots(energies.vec, kernels.EN["EPAIR"], pairs.epair)
     kernels.energy_tail(grids.ctx, forces.to_c(params, levels.mass_number), densities, levels, energies.vec)
     converged = False
     if check or static.tvaryx_0:
