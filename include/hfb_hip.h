@@ -50,6 +50,28 @@ typedef struct hfb_force {
   int ipair, ex, zpe, tcoul;
 } hfb_force;
 
+/* Arguments of hfb_grstep_weights as one record, for the launches that run
+ * it in an extra workgroup (hfb_hpsi_ex). */
+typedef struct hfb_weights_job {
+  int nstates, npsi0, tbcs;
+  const double *wocc, *wstates, *wguv, *pairwg, *v_pair;
+  long long nvox;
+  const double* vpair_max;
+  double *w, *wuv, *pw, *pshift;
+} hfb_weights_job;
+
+/* Arguments of hfb_grstep_finish as one record (hfb_gram2_ex). */
+typedef struct hfb_finish_job {
+  int nstates;
+  const double* hsum;
+  int hsum_ntile;
+  double wxyz;
+  double* sp_energy;
+  const double *wocc, *wstates, *fsum;
+  int fsum_ntile;
+  double *sp_efluct1, *sp_efluct2, *acc;
+} hfb_finish_job;
+
 /* Slots of the device energy vector (double[HFB_EN_LEN]), energies.py:6-43. */
 enum {
   HFB_EN_EHFT = 0, HFB_EN_EHF0, HFB_EN_EHF1, HFB_EN_EHF2, HFB_EN_EHF3, HFB_EN_EHFLS, HFB_EN_EHFLSODD,
@@ -95,6 +117,15 @@ int hfb_hpsi(hfb_ctx* ctx, int prec, const void* psi, int nstates, int npsi0, co
              const void* bmass, const void* aq, const void* spot, const void* wlspot,
              const void* v_pair, const double* w, const double* wuv, void* pout, void* pout_mf,
              void* pout_del, double* partial, int* ntile_out, void* stream);
+
+/* hfb_hpsi with hfb_grstep_weights (job, may be null) in one extra
+ * workgroup of the first launch: the gradient step's weights
+ * (static.py:152-170, 125-136) are written while the x pass runs, which
+ * does not read them; the later passes and hfb_precond_step do. */
+int hfb_hpsi_ex(hfb_ctx* ctx, int prec, const void* psi, int nstates, int npsi0, const void* upot,
+                const void* bmass, const void* aq, const void* spot, const void* wlspot, const void* v_pair,
+                const double* w, const double* wuv, void* pout, void* pout_mf, void* pout_del, double* partial,
+                int* ntile_out, const hfb_weights_job* job, void* stream);
 
 /* Number of partial-sum tiles per state written by hfb_hpsi. */
 int hfb_hpsi_ntiles(hfb_ctx* ctx, int prec);
@@ -189,6 +220,13 @@ int hfb_gram(int prec, const void* A, const void* const* B, int nm, int nst, lon
  * nst[iq], part[iq], out[iq]; a block with nst[iq] == 0 is skipped. */
 int hfb_gram2(int prec, const void* const* A, const void* const* B, int nm, const int* nst, long long K,
               double scale, double* const* part, double* const* out, void* stream);
+
+/* hfb_gram2 with hfb_grstep_finish (job, may be null) in one extra
+ * workgroup of the same launch: the gradient step's scalar tail
+ * (static.py:204-213, 250-257) next to the Gram sums of diagstep, which
+ * read none of what it writes. */
+int hfb_gram2_ex(int prec, const void* const* A, const void* const* B, int nm, const int* nst, long long K,
+                 double scale, double* const* part, double* const* out, const hfb_finish_job* job, void* stream);
 
 /* Loewdin orthonormalisation, optional subspace diagonalisation and the HFB
  * matrices for both isospins (static.py:463-482 loewdin_orthonormalize with

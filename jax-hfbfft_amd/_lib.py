@@ -18,6 +18,21 @@ HFB_F32 = 1
 
 _lib = None
 
+
+class WeightsJob(ctypes.Structure):
+    """hfb_weights_job (include/hfb_hip.h)."""
+    _fields_ = [("nstates", c_int), ("npsi0", c_int), ("tbcs", c_int), ("wocc", c_void_p), ("wstates", c_void_p),
+                ("wguv", c_void_p), ("pairwg", c_void_p), ("v_pair", c_void_p), ("nvox", ctypes.c_longlong),
+                ("vpair_max", c_void_p), ("w", c_void_p), ("wuv", c_void_p), ("pw", c_void_p),
+                ("pshift", c_void_p)]
+
+
+class FinishJob(ctypes.Structure):
+    """hfb_finish_job (include/hfb_hip.h)."""
+    _fields_ = [("nstates", c_int), ("hsum", c_void_p), ("hsum_ntile", c_int), ("wxyz", c_double),
+                ("sp_energy", c_void_p), ("wocc", c_void_p), ("wstates", c_void_p), ("fsum", c_void_p),
+                ("fsum_ntile", c_int), ("sp_efluct1", c_void_p), ("sp_efluct2", c_void_p), ("acc", c_void_p)]
+
 # name -> (restype, argtypes); every entry returns int status
 _SIGS = {
     "hfb_strerror": (ctypes.c_char_p, [c_int]),
@@ -27,6 +42,9 @@ _SIGS = {
     "hfb_hpsi": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                          POINTER(c_int), c_void_p]),
+    "hfb_hpsi_ex": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                            POINTER(c_int), c_void_p, c_void_p]),
     "hfb_hpsi_ntiles": (c_int, [c_void_p, c_int]),
     "hfb_reduce_partials": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_void_p, c_void_p]),
     "hfb_poisson": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
@@ -45,6 +63,8 @@ _SIGS = {
                          c_void_p, c_void_p]),
     "hfb_gram2": (c_int, [c_int, POINTER(c_void_p), POINTER(c_void_p), c_int, POINTER(c_int), ctypes.c_longlong,
                           c_double, POINTER(c_void_p), POINTER(c_void_p), c_void_p]),
+    "hfb_gram2_ex": (c_int, [c_int, POINTER(c_void_p), POINTER(c_void_p), c_int, POINTER(c_int), ctypes.c_longlong,
+                             c_double, POINTER(c_void_p), POINTER(c_void_p), c_void_p, c_void_p]),
     "hfb_rotate": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, ctypes.c_longlong,
                            c_void_p]),
     "hfb_rotate2": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_longlong,

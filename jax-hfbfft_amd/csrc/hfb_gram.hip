@@ -14,6 +14,7 @@
 #include <math.h>
 
 #include "hfb_ctx.h"
+#include "hfb_jobs.h"
 
 using namespace hfb;
 
@@ -94,7 +95,11 @@ __device__ __forceinline__ void gv_tile(int t, int nt, int nm, int bsame, int& m
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256, 3) gram_vec_kernel(GramVecArgs ga) {
+__global__ void __launch_bounds__(256, 3) gram_vec_kernel(GramVecArgs ga, hfb_finish_job fjob) {
+  if (fjob.nstates > 0 && blockIdx.x == gridDim.x - 1) {  // the extra column: the step's scalar tail
+    if (blockIdx.y == 0) grstep_finish_body(fjob);
+    return;
+  }
   const int iq = blockIdx.y;
   const GramIso& g = ga.iso[iq];
   const int nst = g.nst;
@@ -328,9 +333,12 @@ extern "C" int hfb_gram_workspace(int nst, long long K) {
 // Both isospin blocks in one launch: A[iq], B[iq*4 + m], out[iq] (nm, nst_iq,
 // nst_iq) complex, part[iq] workspaces of hfb_gram_workspace(nst_iq, K).
 // A block with nst 0 is skipped.
-extern "C" int hfb_gram2(int prec, const void* const* A, const void* const* B, int nm, const int* nst, long long K,
-                         double scale, double* const* part, double* const* out, void* stream) {
+extern "C" int hfb_gram2_ex(int prec, const void* const* A, const void* const* B, int nm, const int* nst,
+                            long long K, double scale, double* const* part, double* const* out,
+                            const hfb_finish_job* job, void* stream) {
   if (!A || !B || !part || !out || !nst || nm < 1 || nm > 4 || K <= 0) return HFB_E_ARG;
+  const hfb_finish_job fj = job ? *job : hfb_finish_job{};
+  if (job && !finish_job_ok(fj)) return HFB_E_ARG;
   GramVecArgs v{};
   v.bsame = 0;
   for (int m = 0; m < nm; ++m) {
@@ -355,7 +363,11 @@ extern "C" int hfb_gram2(int prec, const void* const* A, const void* const* B, i
     }
     npmax = gi.np > npmax ? gi.np : npmax;
   }
-  if (npmax == 0) return HFB_OK;
+  if (npmax == 0) {
+    if (!job) return HFB_OK;
+    hipLaunchKernelGGL(gram_vec_kernel<double>, dim3(1, 2), dim3(256), 0, (hipStream_t)stream, v, fj);
+    return launch_status();
+  }
   if (prec != HFB_F64 && prec != HFB_F32) return HFB_E_ARG;
   hipStream_t st = (hipStream_t)stream;
   {  // register-tiled path (gram_vec_kernel)
@@ -371,10 +383,11 @@ extern "C" int hfb_gram2(int prec, const void* const* A, const void* const* B, i
       v.ntile[iq] = nst[iq] > 0 ? cnt : 0;
       items = v.ntile[iq] * v.nchunk > items ? v.ntile[iq] * v.nchunk : items;
     }
+    const int gx = (items + 3) / 4 + (job ? 1 : 0);
     if (prec == HFB_F64)
-      hipLaunchKernelGGL(gram_vec_kernel<double>, dim3((items + 3) / 4, 2), dim3(256), 0, st, v);
+      hipLaunchKernelGGL(gram_vec_kernel<double>, dim3(gx, 2), dim3(256), 0, st, v, fj);
     else if (prec == HFB_F32)
-      hipLaunchKernelGGL(gram_vec_kernel<float>, dim3((items + 3) / 4, 2), dim3(256), 0, st, v);
+      hipLaunchKernelGGL(gram_vec_kernel<float>, dim3(gx, 2), dim3(256), 0, st, v, fj);
     else
       return HFB_E_ARG;
     int rc = launch_status();
@@ -397,6 +410,11 @@ extern "C" int hfb_gram2(int prec, const void* const* A, const void* const* B, i
     rc = launch_status();
     return rc;
   }
+}
+
+extern "C" int hfb_gram2(int prec, const void* const* A, const void* const* B, int nm, const int* nst, long long K,
+                         double scale, double* const* part, double* const* out, void* stream) {
+  return hfb_gram2_ex(prec, A, B, nm, nst, K, scale, part, out, nullptr, stream);
 }
 
 // One isospin block (kept for callers of a single block).

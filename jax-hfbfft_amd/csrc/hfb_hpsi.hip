@@ -10,6 +10,7 @@
 #include <math.h>
 
 #include "hfb_ctx.h"
+#include "hfb_jobs.h"
 
 using namespace hfb;
 
@@ -249,7 +250,11 @@ __global__ void __launch_bounds__(256) hpsi_lane_kernel(const cx<T>* __restrict_
                                                         MF<T> mf, const double* __restrict__ w,
                                                         const double* __restrict__ wuv, int npsi0, int nx, int ny,
                                                         int nz, T d, const cx<T>* __restrict__ tw,
-                                                        double* __restrict__ partial) {
+                                                        double* __restrict__ partial, hfb_weights_job job) {
+  if (job.nstates > 0 && blockIdx.y == gridDim.y - 1) {  // the extra row: the step's weights
+    if (blockIdx.x == 0) grstep_weights_body(job);
+    return;
+  }
   constexpr int N1 = N / L;
   constexpr int PW = 64 / L;  // pencils per wavefront
   __shared__ double red[4 * 2];
@@ -517,7 +522,12 @@ __device__ __forceinline__ void load_dfrag(const float* __restrict__ D1, int q, 
 // partial h psi.
 __global__ void __launch_bounds__(256, 3) hpsi_mfma_x_kernel(const cx<float>* __restrict__ psi,
                                                              cx<float>* __restrict__ pmf, MF<float> mf, int npsi0,
-                                                             int ny, int nz, float d, const float* __restrict__ D1) {
+                                                             int ny, int nz, float d, const float* __restrict__ D1,
+                                                             hfb_weights_job job) {
+  if (job.nstates > 0 && blockIdx.y == gridDim.y - 1) {  // the extra row: the step's weights
+    if (blockIdx.x == 0) grstep_weights_body(job);
+    return;
+  }
   constexpr int N = 32;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int q = lane >> 4, c = lane & 15;
@@ -832,16 +842,18 @@ inline int pencils_of_axis(const hfb_ctx* c, int axis) {
 
 template <typename T, int AXIS>
 int launch_hpsi_axis(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T> mf, const double* w,
-                     const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st) {
+                     const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st,
+                     const hfb_weights_job& job = hfb_weights_job{}) {
   constexpr int L = lanes_per_pencil<T>();
   const int ppb = hpsi_pencils_per_block<T>();
   const int np = pencils_of_axis(c, AXIS);
   if (np % ppb != 0) return HFB_E_SIZE;
   const int n = axis_len(c, AXIS);
+  const int rows = ns + (job.nstates > 0 ? 1 : 0);
   HFB_DISPATCH_N(n, {
-    hipLaunchKernelGGL((hpsi_lane_kernel<T, N_, AXIS, L>), dim3(np / ppb, ns), dim3(256), 0, st, (const cx<T>*)psi,
-                       (cx<T>*)pmf, (cx<T>*)pout, (cx<T>*)pdel, mf, w, wuv, npsi0, c->nx, c->ny, c->nz,
-                       T(axis_d(c, AXIS)), twiddles<T>(c, N_), partial);
+    hipLaunchKernelGGL((hpsi_lane_kernel<T, N_, AXIS, L>), dim3(np / ppb, rows), dim3(256), 0, st,
+                       (const cx<T>*)psi, (cx<T>*)pmf, (cx<T>*)pout, (cx<T>*)pdel, mf, w, wuv, npsi0, c->nx, c->ny,
+                       c->nz, T(axis_d(c, AXIS)), twiddles<T>(c, N_), partial, job);
   })
   return launch_status();
 }
@@ -883,9 +895,12 @@ int launch_hpsi_plane(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T
                       const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st);
 
 int hpsi_mfma(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<float> mf, const double* w,
-              const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st) {
-  hipLaunchKernelGGL(hpsi_mfma_x_kernel, dim3(c->ny * (c->nz / 16) / 4, ns), dim3(256), 0, st,
-                     (const cx<float>*)psi, (cx<float>*)pmf, mf, npsi0, c->ny, c->nz, float(c->dx), c->der1f[0]);
+              const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st,
+              const hfb_weights_job& job) {
+  const int rows = ns + (job.nstates > 0 ? 1 : 0);
+  hipLaunchKernelGGL(hpsi_mfma_x_kernel, dim3(c->ny * (c->nz / 16) / 4, rows), dim3(256), 0, st,
+                     (const cx<float>*)psi, (cx<float>*)pmf, mf, npsi0, c->ny, c->nz, float(c->dx), c->der1f[0],
+                     job);
   int rc = launch_status();
   if (rc != HFB_OK) return rc;
   if (!HFB_HPSI_MFMA_PLANE) return launch_hpsi_plane<float>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
@@ -899,11 +914,12 @@ int hpsi_mfma(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<float> mf
 
 template <typename T>
 int hpsi_all(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T> mf, const double* w, const double* wuv,
-             void* pout, void* pmf, void* pdel, double* partial, hipStream_t st) {
+             void* pout, void* pmf, void* pdel, double* partial, hipStream_t st, const hfb_weights_job& job) {
   if constexpr (sizeof(T) == 4) {
-    if (hpsi_mfma_path(c, HFB_F32)) return hpsi_mfma(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
+    if (hpsi_mfma_path(c, HFB_F32))
+      return hpsi_mfma(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st, job);
   }
-  int rc = launch_hpsi_axis<T, 0>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
+  int rc = launch_hpsi_axis<T, 0>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st, job);
   if (hpsi_plane_path(c, sizeof(T) == 4 ? HFB_F32 : HFB_F64)) {
     if (rc == HFB_OK) rc = launch_hpsi_plane<T>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
     return rc;
@@ -933,24 +949,35 @@ extern "C" int hfb_hpsi_ntiles(hfb_ctx* c, int prec) {
   return pencils_of_axis(c, 2) / ppb;
 }
 
-extern "C" int hfb_hpsi(hfb_ctx* c, int prec, const void* psi, int nstates, int npsi0, const void* upot,
+extern "C" int hfb_hpsi_ex(hfb_ctx* c, int prec, const void* psi, int nstates, int npsi0, const void* upot,
                         const void* bmass, const void* aq, const void* spot, const void* wlspot, const void* v_pair,
                         const double* w, const double* wuv, void* pout, void* pout_mf, void* pout_del,
-                        double* partial, int* ntile_out, void* stream) {
+                        double* partial, int* ntile_out, const hfb_weights_job* job,
+                           void* stream) {
   if (!c) return HFB_E_NOCTX;
   if (!psi || !pout_mf || !upot || !bmass || !aq || !spot || !wlspot || !v_pair || !w || !wuv || nstates <= 0)
     return HFB_E_ARG;
+  const hfb_weights_job jb = job ? *job : hfb_weights_job{};
+  if (job && !weights_job_ok(jb)) return HFB_E_ARG;
   hipStream_t st = (hipStream_t)stream;
   if (ntile_out) *ntile_out = hfb_hpsi_ntiles(c, prec);
   if (prec == HFB_F64) {
     MF<double> mf{(const double*)upot, (const double*)bmass, (const double*)aq,
                   (const double*)spot, (const double*)wlspot, (const double*)v_pair};
-    return hpsi_all<double>(c, psi, nstates, npsi0, mf, w, wuv, pout, pout_mf, pout_del, partial, st);
+    return hpsi_all<double>(c, psi, nstates, npsi0, mf, w, wuv, pout, pout_mf, pout_del, partial, st, jb);
   }
   if (prec == HFB_F32) {
     MF<float> mf{(const float*)upot, (const float*)bmass, (const float*)aq,
                  (const float*)spot, (const float*)wlspot, (const float*)v_pair};
-    return hpsi_all<float>(c, psi, nstates, npsi0, mf, w, wuv, pout, pout_mf, pout_del, partial, st);
+    return hpsi_all<float>(c, psi, nstates, npsi0, mf, w, wuv, pout, pout_mf, pout_del, partial, st, jb);
   }
   return HFB_E_ARG;
+}
+
+extern "C" int hfb_hpsi(hfb_ctx* c, int prec, const void* psi, int nstates, int npsi0, const void* upot,
+                        const void* bmass, const void* aq, const void* spot, const void* wlspot, const void* v_pair,
+                        const double* w, const double* wuv, void* pout, void* pout_mf, void* pout_del,
+                        double* partial, int* ntile_out, void* stream) {
+  return hfb_hpsi_ex(c, prec, psi, nstates, npsi0, upot, bmass, aq, spot, wlspot, v_pair, w, wuv, pout, pout_mf,
+                     pout_del, partial, ntile_out, nullptr, stream);
 }

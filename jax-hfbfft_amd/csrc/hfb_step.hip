@@ -16,6 +16,7 @@
 #include <math.h>
 
 #include "hfb_ctx.h"
+#include "hfb_jobs.h"
 
 using namespace hfb;
 
@@ -266,131 +267,9 @@ int step_all(const hfb_ctx* c, const StepArgs& a, int ns, hipStream_t st) {
   return rc;
 }
 
-// Per-state weights of the gradient step (static.py:152-170, 125-136):
-// hpsi weights (w, wuv) and the preconditioner's (max(w,.1), shift).
-__global__ void grstep_weights_kernel(int ns, int npsi0, int tbcs, const double* __restrict__ wocc,
-                                      const double* __restrict__ wstates, const double* __restrict__ wguv,
-                                      const double* __restrict__ pairwg, const double* __restrict__ vpair,
-                                      int64_t nvox, const double* __restrict__ vpair_max, double* __restrict__ w,
-                                      double* __restrict__ wuv, double* __restrict__ pw,
-                                      double* __restrict__ pshift) {
-  __shared__ double vmax_s[2];
-  __shared__ double red[16];
-  for (int iq = 0; iq < 2; ++iq) {
-    double m = -INFINITY;
-    if (!tbcs && vpair_max) {
-      if (threadIdx.x == 0) m = vpair_max[iq];
-    } else if (!tbcs) {
-      // eight loads in flight per thread (four independent max chains); the
-      // field offset iq * nvox keeps 16-byte alignment when nvox is even
-      const double* v = vpair + iq * nvox;
-      const int64_t bd = blockDim.x;
-      int64_t i = threadIdx.x;
-      if ((nvox & 1) == 0) {
-        const double2* v2 = reinterpret_cast<const double2*>(v);
-        const int64_t n2 = nvox / 2;
-        double m0 = -INFINITY, m1 = -INFINITY, m2 = -INFINITY, m3 = -INFINITY;
-        for (; i + 3 * bd < n2; i += 4 * bd) {
-          const double2 a = v2[i], b = v2[i + bd], c = v2[i + 2 * bd], d = v2[i + 3 * bd];
-          m0 = fmax(m0, fmax(a.x, a.y));
-          m1 = fmax(m1, fmax(b.x, b.y));
-          m2 = fmax(m2, fmax(c.x, c.y));
-          m3 = fmax(m3, fmax(d.x, d.y));
-        }
-        for (; i < n2; i += bd) m0 = fmax(m0, fmax(v2[i].x, v2[i].y));
-        m = fmax(fmax(m0, m1), fmax(m2, m3));
-      } else {
-        for (; i < nvox; i += bd) m = fmax(m, v[i]);
-      }
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
-    if (lane == 0) red[wid] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double mm = -INFINITY;
-      for (int j = 0; j < (int)(blockDim.x + 63) / 64; ++j) mm = fmax(mm, red[j]);
-      vmax_s[iq] = mm;
-    }
-    __syncthreads();
-  }
-  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
-    const int iq = s >= npsi0 ? 1 : 0;
-    if (tbcs) {
-      w[s] = 1.0;
-      wuv[s] = 0.0;
-      pw[s] = 1.0;
-      pshift[s] = 0.0;  // laplace(..., 1.0, 0.0, ps1, 0.0, e0dmp)
-    } else {
-      w[s] = wstates[s] * wocc[s];
-      wuv[s] = wstates[s] * wguv[s] * pairwg[s];
-      pw[s] = fmax(wocc[s], 0.1);
-      pshift[s] = 0.5 * fmax(wguv[s] * pairwg[s], 0.1) * vmax_s[iq];
-    }
-  }
-}
+__global__ void grstep_weights_kernel(hfb_weights_job job) { grstep_weights_body(job); }
 
-// Bookkeeping after the step (static.py:173, 211, 247-257).
-// hsum / fsum: per-state totals (nt == 0) or tile partials (nt > 0; fsum's
-// partials then still carry no wxyz factor).
-__global__ void grstep_finish_kernel(int ns, const double* __restrict__ hsum, int hsum_nt, double wxyz,
-                                     double* __restrict__ sp_energy, const double* __restrict__ wocc,
-                                     const double* __restrict__ wstates, const double* __restrict__ fsum, int fsum_nt,
-                                     double* __restrict__ efl1, double* __restrict__ efl2, double* __restrict__ acc) {
-  __shared__ double red[16 * 2];
-  const double fsc = fsum_nt > 0 ? wxyz : 1.0;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  double dsum = 0.0;  // lane 0: sum of wocc*wstates*denerg over this wavefront's states
-  // one wavefront per state: the lanes take the tile partials, a butterfly
-  // adds them (the same total in every lane)
-  for (int s = wave; s < ns; s += nwave) {
-    double t[3] = {0.0, 0.0, 0.0};
-    if (hsum_nt > 0) {
-      for (int k = lane; k < hsum_nt; k += 64) t[0] += hsum[((int64_t)s * hsum_nt + k) * 2];
-    } else if (lane == 0) {
-      t[0] = hsum[2 * s];
-    }
-    if (fsum) {
-      if (fsum_nt > 0) {
-        for (int k = lane; k < fsum_nt; k += 64) {
-          t[1] += fsum[((int64_t)s * fsum_nt + k) * 2];
-          t[2] += fsum[((int64_t)s * fsum_nt + k) * 2 + 1];
-        }
-      } else if (lane == 0) {
-        t[1] = fsum[2 * s];
-        t[2] = fsum[2 * s + 1];
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      t[j] += lane_xor<1>(t[j]);
-      t[j] += lane_xor<2>(t[j]);
-      t[j] += lane_xor<4>(t[j]);
-      t[j] += lane_xor<8>(t[j]);
-      t[j] += lane_xor<16>(t[j]);
-      t[j] += __shfl_xor(t[j], 32, 64);
-    }
-    if (lane == 0) {
-      const double spe_old = sp_energy[s];
-      const double spe_new = wxyz * t[0];
-      dsum += wocc[s] * wstates[s] * ((spe_old - spe_new) / fabs(spe_new));
-      if (fsum) {
-        efl1[s] = sqrt(fsc * t[1]);
-        efl2[s] = sqrt(fsc * t[2]);
-      }
-      sp_energy[s] = spe_new;
-    }
-  }
-  __syncthreads();
-  // sums over the states (static.py:211, 255-257)
-  double v[2] = {0.0, dsum};
-  for (int s = threadIdx.x; s < ns; s += blockDim.x) v[0] += efl1[s];
-  block_sum<2>(v, red);
-  if (threadIdx.x == 0) {
-    acc[0] += v[0];
-    acc[1] += v[1];
-  }
-}
+__global__ void grstep_finish_kernel(hfb_finish_job job) { grstep_finish_body(job); }
 
 }  // namespace
 
@@ -404,11 +283,10 @@ extern "C" int hfb_grstep_weights(int nstates, int npsi0, int tbcs, const double
                                   const double* wguv, const double* pairwg, const double* v_pair, long long nvox,
                                   const double* vpair_max, double* w, double* wuv, double* pw, double* pshift,
                                   void* stream) {
-  if (nstates <= 0 || !wocc || !wstates || !wguv || !pairwg || (!v_pair && !vpair_max) || !w || !wuv || !pw ||
-      !pshift)
-    return HFB_E_ARG;
-  hipLaunchKernelGGL(grstep_weights_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, nstates, npsi0, tbcs,
-                     wocc, wstates, wguv, pairwg, v_pair, (int64_t)nvox, vpair_max, w, wuv, pw, pshift);
+  const hfb_weights_job job{nstates, npsi0, tbcs, wocc, wstates, wguv, pairwg, v_pair, nvox, vpair_max, w, wuv, pw,
+                            pshift};
+  if (!weights_job_ok(job)) return HFB_E_ARG;
+  hipLaunchKernelGGL(grstep_weights_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, job);
   return launch_status();
 }
 
@@ -449,10 +327,9 @@ extern "C" int hfb_precond_step(hfb_ctx* c, int prec, void* psi, const void* ps1
 extern "C" int hfb_grstep_finish(int nstates, const double* hsum, int hsum_ntile, double wxyz, double* sp_energy,
                                  const double* wocc, const double* wstates, const double* fsum, int fsum_ntile,
                                  double* sp_efluct1, double* sp_efluct2, double* acc, void* stream) {
-  if (nstates <= 0 || !hsum || !sp_energy || !wocc || !wstates || !sp_efluct1 || !sp_efluct2 || !acc ||
-      hsum_ntile < 0 || fsum_ntile < 0)
-    return HFB_E_ARG;
-  hipLaunchKernelGGL(grstep_finish_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, nstates, hsum, hsum_ntile,
-                     wxyz, sp_energy, wocc, wstates, fsum, fsum_ntile, sp_efluct1, sp_efluct2, acc);
+  const hfb_finish_job job{nstates, hsum, hsum_ntile, wxyz, sp_energy, wocc, wstates, fsum, fsum_ntile, sp_efluct1,
+                           sp_efluct2, acc};
+  if (!finish_job_ok(job)) return HFB_E_ARG;
+  hipLaunchKernelGGL(grstep_finish_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, job);
   return launch_status();
 }

@@ -70,13 +70,15 @@ def cderv(ctx, axis: int, kind: int, x: torch.Tensor, bmass: torch.Tensor | None
 
 
 def hpsi(ctx, psi, npsi0, upot, bmass, aq, spot, wlspot, v_pair, w, wuv,
-         want_pout=True, want_del=True, want_partial=True, out=None, timer=None, reduce_sums=True):
+         want_pout=True, want_del=True, want_partial=True, out=None, timer=None, reduce_sums=True, weights_job=None):
     """Batched hpsi01 (meanfield.py:40-159) on psi (S, 2, nx, ny, nz).
 
     Returns (pout, pout_mf, pout_del, sums) where sums (S, 2) double holds
     (Re<psi|h psi>, Re<psi|pout>) without the wxyz factor, or None.  With
     reduce_sums=False, sums is an HSum of the unreduced tile partials, which
-    precond_step and grstep_finish add up themselves."""
+    precond_step and grstep_finish add up themselves.  ``weights_job``
+    (from weights_job()) runs grstep_weights in the same launch; it must
+    write the w, wuv given here."""
     prec = _lib.prec_of(psi.dtype)
     rt = _lib.real_of(prec)
     S = psi.shape[0]
@@ -105,9 +107,11 @@ def hpsi(ctx, psi, npsi0, upot, bmass, aq, spot, wlspot, v_pair, w, wuv,
     part = ctx.work(("hpsi_part", prec, S), (S, ntile, 2), torch.float64) if want_partial else None
     nt = ctypes.c_int()
     with _timed(timer):
-        check(lib().hfb_hpsi(ctx.handle, prec, ptr(psi), S, int(npsi0), ptr(upot), ptr(bmass), ptr(aq), ptr(spot),
-                             ptr(wlspot), ptr(v_pair), ptr(w), ptr(wuv), ptr(pout), ptr(pmf), ptr(pdel), ptr(part),
-                             ctypes.byref(nt), stream_handle()), "hpsi")
+        check(lib().hfb_hpsi_ex(ctx.handle, prec, ptr(psi), S, int(npsi0), ptr(upot), ptr(bmass), ptr(aq), ptr(spot),
+                                ptr(wlspot), ptr(v_pair), ptr(w), ptr(wuv), ptr(pout), ptr(pmf), ptr(pdel),
+                                ptr(part), ctypes.byref(nt),
+                                ctypes.byref(weights_job) if weights_job is not None else None, stream_handle()),
+              "hpsi")
     sums = None
     if want_partial:
         sums = HSum(part, nt.value)
@@ -169,6 +173,13 @@ def selftest_mfma_f64(a: torch.Tensor, b: torch.Tensor):
 # ---------------------------------------------------------------------------
 # Gradient step (static.py:93-259)
 # ---------------------------------------------------------------------------
+def weights_job(psi_states, npsi0, tbcs, wocc, wstates, wguv, pairwg, v_pair, nvox, out, vpair_max=None):
+    """The arguments of grstep_weights as a record for hpsi(..., weights_job=)."""
+    return _lib.WeightsJob(int(psi_states), int(npsi0), int(bool(tbcs)), ptr(wocc), ptr(wstates), ptr(wguv),
+                           ptr(pairwg), ptr(v_pair), int(nvox), ptr(vpair_max), ptr(out["w"]), ptr(out["wuv"]),
+                           ptr(out["pw"]), ptr(out["pshift"]))
+
+
 def grstep_weights(psi_states, npsi0, tbcs, wocc, wstates, wguv, pairwg, v_pair, nvox, out, vpair_max=None):
     """Per-state hpsi weights and preconditioner parameters into
     out['w','wuv','pw','pshift']; vpair_max (2,) device maxima of v_pair
@@ -209,18 +220,27 @@ def precond_step(ctx, psi, ps1, lagrange, hsum, wxyz, use_lagrange, pw, pshift, 
     return psi
 
 
-def grstep_finish(hsum, wxyz, sp_energy, wocc, wstates, fluct, sp_efluct1, sp_efluct2, acc):
-    """hsum: (S, 2) totals or an HSum; fluct: None, (S, 2) totals or the dict
-    filled by precond_step."""
+def finish_job(hsum, wxyz, sp_energy, wocc, wstates, fluct, sp_efluct1, sp_efluct2, acc):
+    """The arguments of grstep_finish as a record (gram2(..., finish_job=));
+    the referenced tensors are kept alive on the record."""
     S = sp_energy.shape[0]
     hp, hnt = _hsum_args(hsum)
     if isinstance(fluct, dict):
         fp, fnt = fluct["part"], fluct["ntile"]
     else:
         fp, fnt = fluct, 0
-    check(lib().hfb_grstep_finish(S, ptr(hp), hnt, float(wxyz), ptr(sp_energy), ptr(wocc), ptr(wstates), ptr(fp),
-                                  fnt, ptr(sp_efluct1), ptr(sp_efluct2), ptr(acc), stream_handle()),
-          "grstep_finish")
+    job = _lib.FinishJob(int(S), ptr(hp), int(hnt), float(wxyz), ptr(sp_energy), ptr(wocc), ptr(wstates), ptr(fp),
+                         int(fnt), ptr(sp_efluct1), ptr(sp_efluct2), ptr(acc))
+    job._keep = (hp, fp)
+    return job
+
+
+def grstep_finish(hsum, wxyz, sp_energy, wocc, wstates, fluct, sp_efluct1, sp_efluct2, acc):
+    """hsum: (S, 2) totals or an HSum; fluct: None, (S, 2) totals or the dict
+    filled by precond_step."""
+    j = finish_job(hsum, wxyz, sp_energy, wocc, wstates, fluct, sp_efluct1, sp_efluct2, acc)
+    check(lib().hfb_grstep_finish(j.nstates, j.hsum, j.hsum_ntile, j.wxyz, j.sp_energy, j.wocc, j.wstates, j.fsum,
+                                  j.fsum_ntile, j.sp_efluct1, j.sp_efluct2, j.acc, stream_handle()), "grstep_finish")
 
 
 # ---------------------------------------------------------------------------
@@ -242,9 +262,10 @@ def gram(ctx, A, Bs, scale, out):
     return out
 
 
-def gram2(ctx, blocks, scale, outs):
+def gram2(ctx, blocks, scale, outs, finish_job=None):
     """Both isospin blocks in one launch: blocks[iq] = (A, [B_m]) and
-    outs[iq] (nm, nst, nst) complex128 receives scale * A^H B_m."""
+    outs[iq] (nm, nst, nst) complex128 receives scale * A^H B_m.
+    ``finish_job`` (from finish_job()) runs grstep_finish in the same launch."""
     A0 = blocks[0][0]
     prec = _lib.prec_of(A0.dtype)
     K = A0[0].numel()
@@ -270,7 +291,8 @@ def gram2(ctx, blocks, scale, outs):
         part = ctx.work(("gram_part", iq, nst), (lib().hfb_gram_workspace(nst, K),), torch.float64)
         parts[iq] = part.data_ptr()
         op[iq] = outs[iq].data_ptr()
-    check(lib().hfb_gram2(prec, As, Bs, nm, ns, K, float(scale), parts, op, stream_handle()), "gram2")
+    check(lib().hfb_gram2_ex(prec, As, Bs, nm, ns, K, float(scale), parts, op,
+                             ctypes.byref(finish_job) if finish_job is not None else None, stream_handle()), "gram2")
     return outs
 
 

@@ -150,20 +150,26 @@ def _ws(ctx, levels):
                 pout=ctx.work(("pout", dt, S), shp, dt))
 
 
-def grstep(forces, grids, levels, meanfield, params, static):
-    """Damped gradient step on all states (static.py:215-259)."""
+def grstep(forces, grids, levels, meanfield, params, static, defer_finish=False):
+    """Damped gradient step on all states (static.py:215-259).  With
+    ``defer_finish`` the scalar tail (sp_energy, fluctuations, sumflu,
+    delesum) is returned as a job for diagstep(..., finish_job=) to run in
+    its first launch: (levels, static, job)."""
     ctx = grids.ctx
     S = levels.nstmax
     n0 = int(levels.npsi[0])
     ws = _ws(ctx, levels)
-    kernels.grstep_weights(S, n0, forces.tbcs, levels.wocc, levels.wstates, levels.wguv, levels.pairwg,
-                           meanfield.v_pair, grids.nx * grids.ny * grids.nz, ws, vpair_max=meanfield.vpair_max)
+    # the step's weights are computed by an extra workgroup of the first h psi
+    # launch (its x pass does not read them)
+    wjob = kernels.weights_job(S, n0, forces.tbcs, levels.wocc, levels.wstates, levels.wguv, levels.pairwg,
+                               meanfield.v_pair, grids.nx * grids.ny * grids.nz, ws, vpair_max=meanfield.vpair_max)
     fields = meanfield.fields(levels.psi.dtype)
     # h psi on the old states: ps1 (weighted) and psi_mf -> hampsi (static.py:152-173)
     # the per-state sums stay as tile partials, added up by the step and the
     # finishing kernel themselves
     _, _, _, hsum = kernels.hpsi(ctx, levels.psi, n0, *fields, ws["w"], ws["wuv"], want_del=False,
-                                 out={"pout": ws["pout"], "pout_mf": levels.hampsi}, reduce_sums=False)
+                                 out={"pout": ws["pout"], "pout_mf": levels.hampsi}, reduce_sums=False,
+                                 weights_job=wjob)
     want_fl = params.mprint > 0 and params.iteration % params.mprint == 0
     fluct = {} if want_fl else None
     if static.e0dmp > 0.0:
@@ -183,8 +189,11 @@ def grstep(forces, grids, levels, meanfield, params, static):
         out["pout_del"] = levels.delpsi
     kernels.hpsi(ctx, levels.psi, n0, *fields, ws["w"], ws["wuv"], want_pout=False, want_del=forces.ipair != 0,
                  want_partial=False, out=out)
-    kernels.grstep_finish(hsum, grids.wxyz, levels.sp_energy, levels.wocc, levels.wstates,
-                          fluct if want_fl else None, levels.sp_efluct1, levels.sp_efluct2, static.acc)
+    fin = (hsum, grids.wxyz, levels.sp_energy, levels.wocc, levels.wstates, fluct if want_fl else None,
+           levels.sp_efluct1, levels.sp_efluct2, static.acc)
+    if defer_finish:
+        return levels, static, kernels.finish_job(*fin)
+    kernels.grstep_finish(*fin)
     return levels, static
 
 
@@ -192,9 +201,10 @@ def grstep(forces, grids, levels, meanfield, params, static):
 # Orthonormalisation / subspace (static.py:261-577)
 # ---------------------------------------------------------------------------
 def diagstep(energies, forces, grids, levels, static, diagonalize=False, construct=True, npsi_neutron=None,
-             npsi_proton=None):
+             npsi_proton=None, finish_job=None):
     """Loewdin orthonormalisation, optional subspace diagonalisation and the
-    HFB matrices / Lagrange field (static.py:261-461)."""
+    HFB matrices / Lagrange field (static.py:261-461).  ``finish_job``: a
+    gradient step's deferred tail, run by the Gram launch."""
     ctx = grids.ctx
     n0 = int(levels.npsi[0]) if npsi_neutron is None else int(npsi_neutron)
     n1 = int(levels.npsi[1]) if npsi_proton is None else int(npsi_proton)
@@ -216,7 +226,7 @@ def diagstep(energies, forces, grids, levels, static, diagonalize=False, constru
         uo += [ctx.work(("U", iq, nst), (nst, nst), torch.complex128), ctx.work(("M2", iq, nst), (nst, nst),
                                                                               torch.complex128)]
     # overlap, H_old, lambda_lin and G_old of both isospins in one pass over the states
-    kernels.gram2(ctx, blocks, grids.wxyz, outs)
+    kernels.gram2(ctx, blocks, grids.wxyz, outs, finish_job=finish_job)
     # blocks above 36 states keep the subspace work matrices in global memory
     work = ctx.work(("diag_work", nmax), (10 * nmax * nmax,), torch.complex128) if max(n0, n1) > 36 else None
     kernels.diag_small(ctx, n0, n1, nmax, mats, diagonalize, forces.ipair != 0, forces.tbcs, construct, levels, static,
@@ -267,10 +277,10 @@ def iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, m
     params.iteration = i
     forces.tbcs = True if i <= static.inibcs else tbcssav
     static.tdiag = not (i > static.inidiag)
-    grstep(forces, grids, levels, meanfield, params, static)
+    _, _, fin = grstep(forces, grids, levels, meanfield, params, static, defer_finish=True)
     if forces.tbcs:
         static.tdiag = True
-    diagstep(energies, forces, grids, levels, static, static.tdiag, True)
+    diagstep(energies, forces, grids, levels, static, static.tdiag, True, finish_job=fin)
     if forces.ipair != 0:
         pair(levels, meanfield, forces, params, grids, pairs, E=energies.vec, check_status=False)
     if forces.ipair != 0 and static.iternat > 0 and i > static.iternat_start and not forces.tbcs:
