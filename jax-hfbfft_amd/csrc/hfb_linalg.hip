@@ -435,16 +435,57 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
   // Loewdin S^{-1/2} (static.py:463-482)
   for (int i = tid; i < n; i += NT) a.sp_norm[d.start + i] = d.S[i * n + i].x;
   load_lower_herm<NP, NT>(B0, d.S, n);
-  double rowsum = 0.0;  // Gershgorin bound on the largest eigenvalue
-  if (tid < n)
-    for (int j = 0; j < n; ++j) rowsum += sqrt(cabs2(B0[tid * n + j]));
-  double cs = block_max_all<NT>(rowsum, red);
-  if (!(cs > 0.0)) cs = 1.0;
-  tile_for<NP, NT>(n, [&](int i, int j) { B0[i * n + j] = cscale(B0[i * n + j], 1.0 / cs); });
-  __syncthreads();
   cx<double>*Y = B0, *Z = B1, *Y2 = B3, *Z2 = B4;
   int ns_iters = 0;
-  const bool ok = newton_schulz<NP, NT>(Y, Z, B2, Y2, Z2, n, red, ns_iters);
+  double cs = 1.0;
+  // Near convergence the overlap is S = I + E with a small E: the binomial
+  // series (I + E)^{-1/2} = sum_k c_k E^k, c_k = (-1)^k C(2k,k)/4^k, is then
+  // exact to round-off in fewer products than the Newton-Schulz iteration
+  // (three per step): to degree 4 for |E|_F < 1e-3 (first omitted term below
+  // 0.25e-15) in 3 products, to degree 7 for |E|_F < 1e-2 (below 0.2e-16) in
+  // 6.  Horner: T = c_{d-1} I + c_d E, then T <- c_k I + E T.  The series
+  // works on S itself; Newton-Schulz on S scaled by a Gershgorin bound.
+  bool ok;
+  {
+    double fr = 0.0, dummy = 0.0;
+    HFB_TILE_LOOP(NP, NT, n, i, j) {
+      const cx<double> v = B0[i * n + j];
+      const double e = i == j ? v.x - 1.0 : v.x;
+      fr += e * e + v.y * v.y;
+    }
+    block_sum2_all<NT>(fr, dummy, red);
+    const int deg = fr < 1e-6 ? 4 : (fr < 1e-4 ? 7 : 0);
+    if (deg > 0) {
+      constexpr double ck[8] = {1.0, -0.5, 0.375, -0.3125, 0.2734375, -0.24609375, 0.2255859375, -0.20947265625};
+      // E in B0, Horner terms in B2 (products in B3), result Z in B1
+      tile_for<NP, NT>(n, [&](int i, int j) {
+        if (i == j) B0[i * n + j].x -= 1.0;
+        const cx<double> e = B0[i * n + j];
+        B2[i * n + j] = cx<double>{(i == j ? ck[deg - 1] : 0.0) + ck[deg] * e.x, ck[deg] * e.y};
+      });
+      __syncthreads();
+      for (int k = deg - 2; k >= 0; --k) {
+        mm<NP, NT, false>(B3, B0, B2, n);
+        cx<double>* dst = k == 0 ? B1 : B2;
+        tile_for<NP, NT>(n, [&](int i, int j) {
+          const cx<double> v = B3[i * n + j];
+          dst[i * n + j] = cx<double>{v.x + (i == j ? ck[k] : 0.0), v.y};
+        });
+        __syncthreads();
+      }
+      Z = B1;
+      ok = true;
+    } else {
+      double rowsum = 0.0;  // Gershgorin bound on the largest eigenvalue
+      if (tid < n)
+        for (int j = 0; j < n; ++j) rowsum += sqrt(cabs2(B0[tid * n + j]));
+      cs = block_max_all<NT>(rowsum, red);
+      if (!(cs > 0.0)) cs = 1.0;
+      tile_for<NP, NT>(n, [&](int i, int j) { B0[i * n + j] = cscale(B0[i * n + j], 1.0 / cs); });
+      __syncthreads();
+      ok = newton_schulz<NP, NT>(Y, Z, B2, Y2, Z2, n, red, ns_iters);
+    }
+  }
   cx<double>* Urho = B4;
   if (ok) {
     const double f = 1.0 / sqrt(cs);

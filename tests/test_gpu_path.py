@@ -113,6 +113,40 @@ def test_diagstep_matches_oracle(pkg, O, dev, npsi):
         lv.psi, lv.lagrange = saved["psi"], saved["lagrange"]
 
 
+@pytest.mark.parametrize("eps,path", [(2e-5, "series4"), (1e-3, "series7"), (2e-2, "newton")])
+def test_loewdin_near_orthonormal(pkg, O, dev, eps, path):
+    """Loewdin of nearly orthonormal states, psi_i + eps sum_j A_ij psi_j
+    with A Hermitian and |A|_F = 1 (so S = I + E, |E|_F ~ 2 eps): the
+    binomial series of degree 4 (|E|_F < 1e-3) and 7 (< 1e-2) and the
+    Newton-Schulz iteration give the oracle's eigh-based S^{-1/2} states
+    (static.py:463-482) to 1e-11; the Newton-Schulz iteration count tells
+    which path ran."""
+    ref, run = small_case(O, pkg, npsi=[12, 10])
+    # orthonormal states first (one Loewdin on the oracle side), then mixed
+    O.diagstep(ref.energies, ref.forces, ref.grids, ref.levels, ref.static, False, True)
+    rng = np.random.default_rng(11)
+    psi = ref.levels.psi.copy()
+    for iq in range(2):
+        sl = ref.levels.block(iq)
+        m = sl.stop - sl.start
+        a = rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))
+        a = a + a.conj().T
+        a /= np.linalg.norm(a)
+        psi[sl] = psi[sl] + eps * np.einsum("ij,j...->i...", a, ref.levels.psi[sl])
+    ref.levels.psi = psi
+    run.levels.psi.copy_(T(ref.levels.psi))
+    O.diagstep(ref.energies, ref.forces, ref.grids, ref.levels, ref.static, False, True)
+    pkg.static.diagstep(run.energies, run.forces, run.grids, run.levels, run.static, False, True)
+    torch.cuda.synchronize()
+    its = run.energies.host()["ns_iterations"]
+    if path == "newton":
+        assert (its > 0).all(), its
+    else:
+        assert (its == 0).all(), its
+    assert rel(run.levels.psi.cpu().numpy(), ref.levels.psi) < 1e-11
+    assert rel(run.levels.sp_norm.cpu().numpy(), ref.levels.sp_norm) < 1e-12
+
+
 def test_density_sp_skyrme_energy(pkg, O, dev):
     ref, run = small_case(O, pkg, ipair=6, npsi=[10, 9])
     rng = np.random.default_rng(9)
