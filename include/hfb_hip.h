@@ -50,14 +50,28 @@ typedef struct hfb_force {
   int ipair, ex, zpe, tcoul;
 } hfb_force;
 
+/* Arguments of hfb_energy_tail as one record (hfb_weights_job.tail). */
+typedef struct hfb_tail_job {
+  const hfb_force* force;
+  const double *sky_part, *en_part, *current;
+  int nstates;
+  const double *wocc, *wstates, *sp_kinetic, *sp_energy, *sp_orbital, *sp_spin;
+  double* E;
+} hfb_tail_job;
+
 /* Arguments of hfb_grstep_weights as one record, for the launches that run
- * it in an extra workgroup (hfb_hpsi_ex). */
+ * it in an extra workgroup (hfb_hpsi_ex).  tail (host memory, read at the
+ * call; NULL for none): an hfb_energy_tail that the same h.psi call runs in
+ * an extra workgroup of one of its launches; the weights then take the
+ * v_pair maxima from the tail's skyrme partials (the values it writes to
+ * E[HFB_EN_VMAX]) instead of vpair_max. */
 typedef struct hfb_weights_job {
   int nstates, npsi0, tbcs;
   const double *wocc, *wstates, *wguv, *pairwg, *v_pair;
   long long nvox;
   const double* vpair_max;
   double *w, *wuv, *pw, *pshift;
+  const struct hfb_tail_job* tail;
 } hfb_weights_job;
 
 /* Arguments of hfb_grstep_finish as one record (hfb_gram2_ex). */
@@ -121,7 +135,9 @@ int hfb_hpsi(hfb_ctx* ctx, int prec, const void* psi, int nstates, int npsi0, co
 /* hfb_hpsi with hfb_grstep_weights (job, may be null) in one extra
  * workgroup of the first launch: the gradient step's weights
  * (static.py:152-170, 125-136) are written while the x pass runs, which
- * does not read them; the later passes and hfb_precond_step do. */
+ * does not read them; the later passes and hfb_precond_step do.  job->tail
+ * runs in one extra workgroup of the y-z plane launch (of the first launch
+ * on grids without one); it writes only E, which no pass reads. */
 int hfb_hpsi_ex(hfb_ctx* ctx, int prec, const void* psi, int nstates, int npsi0, const void* upot,
                 const void* bmass, const void* aq, const void* spot, const void* wlspot, const void* v_pair,
                 const double* w, const double* wuv, void* pout, void* pout_mf, void* pout_del, double* partial,

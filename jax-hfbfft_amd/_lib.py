@@ -19,12 +19,19 @@ HFB_F32 = 1
 _lib = None
 
 
+class TailJob(ctypes.Structure):
+    """hfb_tail_job (include/hfb_hip.h)."""
+    _fields_ = [("force", c_void_p), ("sky_part", c_void_p), ("en_part", c_void_p), ("current", c_void_p),
+                ("nstates", c_int), ("wocc", c_void_p), ("wstates", c_void_p), ("sp_kinetic", c_void_p),
+                ("sp_energy", c_void_p), ("sp_orbital", c_void_p), ("sp_spin", c_void_p), ("E", c_void_p)]
+
+
 class WeightsJob(ctypes.Structure):
     """hfb_weights_job (include/hfb_hip.h)."""
     _fields_ = [("nstates", c_int), ("npsi0", c_int), ("tbcs", c_int), ("wocc", c_void_p), ("wstates", c_void_p),
                 ("wguv", c_void_p), ("pairwg", c_void_p), ("v_pair", c_void_p), ("nvox", ctypes.c_longlong),
                 ("vpair_max", c_void_p), ("w", c_void_p), ("wuv", c_void_p), ("pw", c_void_p),
-                ("pshift", c_void_p)]
+                ("pshift", c_void_p), ("tail", ctypes.POINTER(TailJob))]
 
 
 class FinishJob(ctypes.Structure):

@@ -250,9 +250,12 @@ __global__ void __launch_bounds__(256) hpsi_lane_kernel(const cx<T>* __restrict_
                                                         MF<T> mf, const double* __restrict__ w,
                                                         const double* __restrict__ wuv, int npsi0, int nx, int ny,
                                                         int nz, T d, const cx<T>* __restrict__ tw,
-                                                        double* __restrict__ partial, hfb_weights_job job) {
-  if (job.nstates > 0 && blockIdx.y == gridDim.y - 1) {  // the extra row: the step's weights
-    if (blockIdx.x == 0) grstep_weights_body(job);
+                                                        double* __restrict__ partial, WJob job) {
+  // the first row (when there is a job): the step's weights, dispatched
+  // first so that the energy tail before them runs beside the pass
+  const int row0 = job_rows(job);
+  if (row0 && blockIdx.y == 0) {
+    if (blockIdx.x == 0) job_body<4>(job);
     return;
   }
   constexpr int N1 = N / L;
@@ -261,7 +264,7 @@ __global__ void __launch_bounds__(256) hpsi_lane_kernel(const cx<T>* __restrict_
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = lane % L, q = lane / L;
   const int g = (blockIdx.x * 4 + wave) * PW + q;  // pencil
-  const int s = blockIdx.y;
+  const int s = blockIdx.y - row0;
   const int iq = s >= npsi0 ? 1 : 0;
   const int64_t nvox = (int64_t)nx * ny * nz;
   int64_t base, stride;
@@ -523,9 +526,12 @@ __device__ __forceinline__ void load_dfrag(const float* __restrict__ D1, int q, 
 __global__ void __launch_bounds__(256, 3) hpsi_mfma_x_kernel(const cx<float>* __restrict__ psi,
                                                              cx<float>* __restrict__ pmf, MF<float> mf, int npsi0,
                                                              int ny, int nz, float d, const float* __restrict__ D1,
-                                                             hfb_weights_job job) {
-  if (job.nstates > 0 && blockIdx.y == gridDim.y - 1) {  // the extra row: the step's weights
-    if (blockIdx.x == 0) grstep_weights_body(job);
+                                                             WJob job) {
+  // the first row (when there is a job): the step's weights, dispatched
+  // first so that the energy tail before them runs beside the pass
+  const int row0 = job_rows(job);
+  if (row0 && blockIdx.y == 0) {
+    if (blockIdx.x == 0) job_body<4>(job);
     return;
   }
   constexpr int N = 32;
@@ -534,7 +540,7 @@ __global__ void __launch_bounds__(256, 3) hpsi_mfma_x_kernel(const cx<float>* __
   const int nzh = nz >> 4;
   const int tile = blockIdx.x * 4 + wave;  // (y, 16-column z slice)
   const int y = tile / nzh, z = (tile % nzh) * 16 + c;
-  const int s = blockIdx.y;
+  const int s = blockIdx.y - row0;
   const int iq = s >= npsi0 ? 1 : 0;
   const int sx = ny * nz;  // offsets below: unsigned 32-bit from a uniform field base
   const int64_t nvox = (int64_t)N * sx;
@@ -719,14 +725,19 @@ __global__ void __launch_bounds__(N * HP_L) hpsi_plane_kernel(const cx<T>* __res
                                                               MF<T> mf, const double* __restrict__ w,
                                                               const double* __restrict__ wuv, int npsi0, int nx, T d,
                                                               const cx<T>* __restrict__ tw,
-                                                              double* __restrict__ partial) {
+                                                              double* __restrict__ partial, WJob job) {
   constexpr int L = HP_L, N1 = N / L, P = N + 1;  // P: LDS row pitch
+  const int row0 = job_rows(job);
+  if (row0 && blockIdx.y == 0) {  // the first row: the job (the energy tail)
+    if (blockIdx.x == 0) job_body<(N * L + 63) / 64>(job);
+    return;
+  }
   extern __shared__ unsigned char hp_lds[];
   cx<T>* s0 = (cx<T>*)hp_lds;  // [y][z], spin up
   cx<T>* s1 = s0 + N * P;      // spin down
   __shared__ double red[(N * L / 64) * 2];
   const int j = threadIdx.x % L, q = threadIdx.x / L;
-  const int x = blockIdx.x, s = blockIdx.y;
+  const int x = blockIdx.x, s = blockIdx.y - row0;
   const int iq = s >= npsi0 ? 1 : 0;
   const int64_t nvox = (int64_t)nx * N * N;
   const int64_t f0 = (int64_t)(2 * s) * nvox;
@@ -843,13 +854,13 @@ inline int pencils_of_axis(const hfb_ctx* c, int axis) {
 template <typename T, int AXIS>
 int launch_hpsi_axis(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T> mf, const double* w,
                      const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st,
-                     const hfb_weights_job& job = hfb_weights_job{}) {
+                     const WJob& job = WJob{}) {
   constexpr int L = lanes_per_pencil<T>();
   const int ppb = hpsi_pencils_per_block<T>();
   const int np = pencils_of_axis(c, AXIS);
   if (np % ppb != 0) return HFB_E_SIZE;
   const int n = axis_len(c, AXIS);
-  const int rows = ns + (job.nstates > 0 ? 1 : 0);
+  const int rows = ns + job_rows(job);
   HFB_DISPATCH_N(n, {
     hipLaunchKernelGGL((hpsi_lane_kernel<T, N_, AXIS, L>), dim3(np / ppb, rows), dim3(256), 0, st,
                        (const cx<T>*)psi, (cx<T>*)pmf, (cx<T>*)pout, (cx<T>*)pdel, mf, w, wuv, npsi0, c->nx, c->ny,
@@ -866,16 +877,17 @@ inline bool hpsi_plane_path(const hfb_ctx* c, int prec) {
 
 template <typename T>
 int launch_hpsi_plane(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T> mf, const double* w,
-                      const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st) {
+                      const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st,
+                      const WJob& job) {
   HFB_DISPATCH_N(c->ny, {
     constexpr int NP_ = (sizeof(T) == 8 && N_ > 32) ? 32 : N_;  // never taken for FP64 above 32
     if (NP_ != N_) return HFB_E_SIZE;
     const size_t lds = 2 * sizeof(cx<T>) * (size_t)NP_ * (NP_ + 1);
     auto kern = hpsi_plane_kernel<T, NP_>;
     if (lds > 32 * 1024) allow_full_lds((const void*)kern);
-    hipLaunchKernelGGL(kern, dim3(c->nx, ns), dim3(NP_ * HP_L), lds, st, (const cx<T>*)psi, (cx<T>*)pmf,
-                       (cx<T>*)pout, (cx<T>*)pdel, mf, w, wuv, npsi0, c->nx, T(axis_d(c, 1)), twiddles<T>(c, NP_),
-                       partial);
+    hipLaunchKernelGGL(kern, dim3(c->nx, ns + job_rows(job)), dim3(NP_ * HP_L), lds, st, (const cx<T>*)psi,
+                       (cx<T>*)pmf, (cx<T>*)pout, (cx<T>*)pdel, mf, w, wuv, npsi0, c->nx, T(axis_d(c, 1)),
+                       twiddles<T>(c, NP_), partial, job);
   })
   return launch_status();
 }
@@ -889,21 +901,31 @@ inline bool hpsi_mfma_path(const hfb_ctx* c, int prec) {
 #ifndef HFB_HPSI_MFMA_PLANE
 #define HFB_HPSI_MFMA_PLANE 0
 #endif
+#ifndef HFB_TAIL_IN_PLANE
+#define HFB_TAIL_IN_PLANE 1
+#endif
 
 template <typename T>
 int launch_hpsi_plane(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T> mf, const double* w,
-                      const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st);
+                      const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st,
+                      const WJob& job);
 
 int hpsi_mfma(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<float> mf, const double* w,
               const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st,
-              const hfb_weights_job& job) {
-  const int rows = ns + (job.nstates > 0 ? 1 : 0);
+              const WJob& job) {
+  // the weights in the x launch; the energy tail in the plane launch, which
+  // runs longer (the matrix-core plane kernel takes no job)
+  const bool split = !HFB_HPSI_MFMA_PLANE && HFB_TAIL_IN_PLANE;
+  const WJob xj = split ? weights_part(job) : job;
+  const int rows = ns + job_rows(xj);
   hipLaunchKernelGGL(hpsi_mfma_x_kernel, dim3(c->ny * (c->nz / 16) / 4, rows), dim3(256), 0, st,
                      (const cx<float>*)psi, (cx<float>*)pmf, mf, npsi0, c->ny, c->nz, float(c->dx), c->der1f[0],
-                     job);
+                     xj);
   int rc = launch_status();
   if (rc != HFB_OK) return rc;
-  if (!HFB_HPSI_MFMA_PLANE) return launch_hpsi_plane<float>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
+  if (!HFB_HPSI_MFMA_PLANE)
+    return launch_hpsi_plane<float>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st,
+                                    split ? tail_part(job) : WJob{});
   const size_t lds = 6 * sizeof(cx<float>) * 32 * MP;
   allow_full_lds((const void*)hpsi_mfma_plane_kernel);
   hipLaunchKernelGGL(hpsi_mfma_plane_kernel, dim3(c->nx, ns), dim3(256), lds, st, (const cx<float>*)psi,
@@ -914,14 +936,18 @@ int hpsi_mfma(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<float> mf
 
 template <typename T>
 int hpsi_all(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T> mf, const double* w, const double* wuv,
-             void* pout, void* pmf, void* pdel, double* partial, hipStream_t st, const hfb_weights_job& job) {
+             void* pout, void* pmf, void* pdel, double* partial, hipStream_t st, const WJob& job) {
   if constexpr (sizeof(T) == 4) {
     if (hpsi_mfma_path(c, HFB_F32))
       return hpsi_mfma(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st, job);
   }
-  int rc = launch_hpsi_axis<T, 0>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st, job);
-  if (hpsi_plane_path(c, sizeof(T) == 4 ? HFB_F32 : HFB_F64)) {
-    if (rc == HFB_OK) rc = launch_hpsi_plane<T>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
+  // with a plane launch the energy tail runs there, the weights in the x launch
+  const bool plane = hpsi_plane_path(c, sizeof(T) == 4 ? HFB_F32 : HFB_F64);
+  int rc = launch_hpsi_axis<T, 0>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st,
+                                  plane ? weights_part(job) : job);
+  if (plane) {
+    if (rc == HFB_OK)
+      rc = launch_hpsi_plane<T>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st, tail_part(job));
     return rc;
   }
   if (rc == HFB_OK) rc = launch_hpsi_axis<T, 1>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st);
@@ -957,8 +983,8 @@ extern "C" int hfb_hpsi_ex(hfb_ctx* c, int prec, const void* psi, int nstates, i
   if (!c) return HFB_E_NOCTX;
   if (!psi || !pout_mf || !upot || !bmass || !aq || !spot || !wlspot || !v_pair || !w || !wuv || nstates <= 0)
     return HFB_E_ARG;
-  const hfb_weights_job jb = job ? *job : hfb_weights_job{};
-  if (job && !weights_job_ok(jb)) return HFB_E_ARG;
+  WJob jb;
+  if (!wjob_of(c, job, jb)) return HFB_E_ARG;
   hipStream_t st = (hipStream_t)stream;
   if (ntile_out) *ntile_out = hfb_hpsi_ntiles(c, prec);
   if (prec == HFB_F64) {

@@ -5,6 +5,7 @@
 // most 1024) works.
 #pragma once
 #include "hfb_ctx.h"
+#include "hfb_energy.h"
 
 namespace hfb {
 
@@ -18,10 +19,28 @@ inline bool finish_job_ok(const hfb_finish_job& j) {
          j.hsum_ntile >= 0 && j.fsum_ntile >= 0;
 }
 
+// The weights of state s from its occupation inputs and the v_pair maxima
+// per isospin vmax[2].
+__device__ __forceinline__ void weights_state(const hfb_weights_job& job, int s, double wocc, double wstates,
+                                              double wguv, double pairwg, const double* vmax) {
+  const int iq = s >= job.npsi0 ? 1 : 0;
+  if (job.tbcs) {
+    job.w[s] = 1.0;
+    job.wuv[s] = 0.0;
+    job.pw[s] = 1.0;
+    job.pshift[s] = 0.0;  // laplace(..., 1.0, 0.0, ps1, 0.0, e0dmp)
+  } else {
+    job.w[s] = wstates * wocc;
+    job.wuv[s] = wstates * wguv * pairwg;
+    job.pw[s] = fmax(wocc, 0.1);
+    job.pshift[s] = 0.5 * fmax(wguv * pairwg, 0.1) * vmax[iq];
+  }
+}
+
 // Per-state weights of the gradient step (static.py:152-170, 125-136):
 // hpsi weights (w, wuv) and the preconditioner's (max(w,.1), shift).
 __device__ __forceinline__ void grstep_weights_body(const hfb_weights_job& job) {
-  const int ns = job.nstates, npsi0 = job.npsi0, tbcs = job.tbcs;
+  const int ns = job.nstates, tbcs = job.tbcs;
   const double* __restrict__ wocc = job.wocc;
   const double* __restrict__ wstates = job.wstates;
   const double* __restrict__ wguv = job.wguv;
@@ -29,10 +48,6 @@ __device__ __forceinline__ void grstep_weights_body(const hfb_weights_job& job) 
   const double* __restrict__ vpair = job.v_pair;
   const int64_t nvox = job.nvox;
   const double* __restrict__ vpair_max = job.vpair_max;
-  double* __restrict__ w = job.w;
-  double* __restrict__ wuv = job.wuv;
-  double* __restrict__ pw = job.pw;
-  double* __restrict__ pshift = job.pshift;
   __shared__ double vmax_s[2];
   __shared__ double red[16];
   for (int iq = 0; iq < 2; ++iq) {
@@ -73,20 +88,8 @@ __device__ __forceinline__ void grstep_weights_body(const hfb_weights_job& job) 
     }
     __syncthreads();
   }
-  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
-    const int iq = s >= npsi0 ? 1 : 0;
-    if (tbcs) {
-      w[s] = 1.0;
-      wuv[s] = 0.0;
-      pw[s] = 1.0;
-      pshift[s] = 0.0;  // laplace(..., 1.0, 0.0, ps1, 0.0, e0dmp)
-    } else {
-      w[s] = wstates[s] * wocc[s];
-      wuv[s] = wstates[s] * wguv[s] * pairwg[s];
-      pw[s] = fmax(wocc[s], 0.1);
-      pshift[s] = 0.5 * fmax(wguv[s] * pairwg[s], 0.1) * vmax_s[iq];
-    }
-  }
+  for (int s = threadIdx.x; s < ns; s += blockDim.x)
+    weights_state(job, s, wocc[s], wstates[s], wguv[s], pairwg[s], vmax_s);
 }
 
 // Bookkeeping after the step (static.py:173, 211, 247-257).
@@ -155,6 +158,88 @@ __device__ __forceinline__ void grstep_finish_body(const hfb_finish_job& job) {
   if (threadIdx.x == 0) {
     acc[0] += v[0];
     acc[1] += v[1];
+  }
+}
+
+// The jobs an h.psi launch runs in its extra first row of workgroups: the
+// step weights (w.nstates > 0) and/or the energy tail of the previous
+// iteration (tail; hfb_weights_job.tail).  vm_part: the weights take the
+// v_pair maxima from the tail's skyrme partials (when the tail runs in
+// another launch of the same call).
+struct WJob {
+  hfb_weights_job w;
+  TailArgs t;
+  int tail;
+  int vm_part;
+};
+
+// A WJob from the public record (null: no job); false if an argument is
+// missing.
+inline bool wjob_of(const hfb_ctx* c, const hfb_weights_job* job, WJob& out) {
+  out = WJob{};
+  if (!job) return true;
+  out.w = *job;
+  out.w.tail = nullptr;
+  if (!weights_job_ok(out.w)) return false;
+  if (job->tail) {
+    if (!tail_args(c, *job->tail, out.t)) return false;
+    out.tail = 1;
+  }
+  return true;
+}
+
+// Rows of workgroups a launch adds in front of its states for the job.
+__host__ __device__ inline int job_rows(const WJob& j) { return (j.w.nstates > 0 || j.tail) ? 1 : 0; }
+
+// The weights part of a job (no tail) and the tail part (no weights), for
+// calls whose first launch runs the weights and a later one the tail.
+inline WJob weights_part(const WJob& j) {
+  WJob o = j;
+  o.vm_part = j.tail;
+  o.tail = 0;
+  return o;
+}
+inline WJob tail_part(const WJob& j) {
+  WJob o{};
+  o.t = j.t;
+  o.tail = j.tail;
+  return o;
+}
+
+// The state inputs of the weights of state s0 (zeros past the last state).
+struct WIn {
+  double wo, wst, wg, pg;
+};
+__device__ __forceinline__ WIn weights_in(const hfb_weights_job& w, int s) {
+  if (s >= w.nstates) return WIn{0.0, 0.0, 0.0, 0.0};
+  return WIn{w.wocc[s], w.wstates[s], w.wguv[s], w.pairwg[s]};
+}
+
+// The job of one launch, run by one workgroup of at most 64 * MW threads.
+// With the tail or vm_part the weights take the v_pair maxima from the
+// skyrme partials (the values the tail writes to E[HFB_EN_VMAX]) and their
+// inputs are loaded first.
+template <int MW>
+__device__ __forceinline__ void job_body(const WJob& j) {
+  const hfb_weights_job& w = j.w;
+  if (w.nstates > 0 && !j.tail && !j.vm_part) {
+    grstep_weights_body(w);
+    return;
+  }
+  double vm[2];
+  if (w.nstates <= 0) {  // the tail alone
+    energy_tail_body<MW>(j.t, vm);
+    return;
+  }
+  const int s0 = threadIdx.x;
+  WIn in = weights_in(w, s0);
+  if (j.tail)
+    energy_tail_body<MW>(j.t, vm);
+  else
+    vpair_max_of_parts<MW>(j.t, vm);
+  for (int s = s0; s < w.nstates; s += blockDim.x) {
+    if (s != s0) in = weights_in(w, s);
+    weights_state(w, s, in.wo, in.wst, in.wg, in.pg, vm);
   }
 }
 

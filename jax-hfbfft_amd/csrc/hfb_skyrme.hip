@@ -10,6 +10,7 @@
 #include <math.h>
 
 #include "hfb_ctx.h"
+#include "hfb_energy.h"
 
 using namespace hfb;
 
@@ -270,21 +271,6 @@ __global__ void __launch_bounds__(256) skyrme_combine_kernel(SkyArgs a) { skyrme
 // ---------------------------------------------------------------------------
 // Energy integrals (energies.py:90-306): per-block partial sums.
 // ---------------------------------------------------------------------------
-constexpr int NES = 20;
-enum {
-  S_EHF0, S_CRHO0, S_CRHO1, S_EHF3, S_EHF2, S_CDRHO0, S_CDRHO1, S_CTAU0, S_CTAU1, S_EHF1,
-  S_EHFLS, S_CDJ0, S_CDJ1, S_LSODD, S_EHFC, S_ECORC, S_EHFT, S_ECORRP_UNUSED, S_PAD0, S_PAD1
-};
-
-struct EnArgs {
-  hfb_force f;
-  int64_t nvox;
-  const double *rho, *tau, *sdens;
-  const double *divj, *lap, *curlj;
-  const double* wcoul;
-  double* part;  // [block][NES]
-};
-
 // (workgroup vb of vg)
 __device__ __forceinline__ void energy_body(const EnArgs& a, int vb, int vg) {
   const hfb_force& f = a.f;
@@ -346,131 +332,13 @@ __global__ void __launch_bounds__(256) skyrme_energy_kernel(SkyArgs sa, EnArgs e
     energy_body(ea, blockIdx.x - nb, nb);
 }
 
-// The integrals from energy_kernel's block partials (fixed order) into E
-// (energies.py:90-306); one workgroup.
-__device__ void energy_finish_body(const EnArgs& a, int nblk, double wxyz, const double* __restrict__ cur, int nx,
-                                   int ny, int nz, double* __restrict__ E) {
-  __shared__ double red[4 * NES];
-  double s[NES];
-  for (int j = 0; j < NES; ++j) s[j] = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x)
-    for (int j = 0; j < NES; ++j) s[j] += a.part[(int64_t)b * NES + j];
-  block_sum<NES>(s, red);
-  const hfb_force& f = a.f;
-  // energies.py:157-167 index current[:,:,:,0,0] .. current[:,:,:,2,1]: the
-  // (y, z) = (0..2, 0..1) corner of every (iq, component, x) -- kept as written.
-  const int64_t nvox = (int64_t)nx * ny * nz;
-  double cj[2] = {0.0, 0.0};
-  for (int it = threadIdx.x; it < 2 * 3 * nx * 3; it += blockDim.x) {
-    const int y = it % 3;
-    const int x = (it / 3) % nx;
-    const int ic = it / (3 * nx);  // iq*3 + component
-    const double* p = cur + (int64_t)ic * nvox + (int64_t)x * ny * nz + (int64_t)y * nz;
-    const double a0 = p[0], a1 = p[1];
-    cj[0] += (a0 + a1) * (a0 + a1);
-    cj[1] += (-a0 + a1) * (-a0 + a1);
-  }
-  __shared__ double red2[4 * 2];
-  block_sum<2>(cj, red2);
-  if (threadIdx.x != 0) return;
-  const double cj0 = cj[0], cj1 = cj[1];
-  const double ehf0 = wxyz * s[S_EHF0], ehf1 = wxyz * s[S_EHF1], ehf2 = wxyz * s[S_EHF2], ehf3 = wxyz * s[S_EHF3];
-  const double ehflsodd = wxyz * s[S_LSODD];
-  const double ehfls = wxyz * s[S_EHFLS] + ehflsodd;
-  const double ehfc = f.tcoul ? wxyz * s[S_EHFC] : 0.0;
-  const double ecorc = f.tcoul ? wxyz * s[S_ECORC] : 0.0;
-  const double ehft = wxyz * s[S_EHFT];
-  const double ezpe = f.zpe == 1 ? 17.3 / pow(f.mass_number, 0.2) : 0.0;
-  const double epair = E[HFB_EN_EPAIR] + E[HFB_EN_EPAIR + 1];
-  E[HFB_EN_EHFT] = ehft;
-  E[HFB_EN_EHF0] = ehf0;
-  E[HFB_EN_EHF1] = ehf1;
-  E[HFB_EN_EHF2] = ehf2;
-  E[HFB_EN_EHF3] = ehf3;
-  E[HFB_EN_EHFLS] = ehfls;
-  E[HFB_EN_EHFLSODD] = ehflsodd;
-  E[HFB_EN_EHFC] = ehfc;
-  E[HFB_EN_ECORC] = ecorc;
-  E[HFB_EN_E3CORR] = -f.power * ehf3 / 2.0;
-  E[HFB_EN_EZPE] = ezpe;
-  E[HFB_EN_CRHO0] = wxyz * s[S_CRHO0];
-  E[HFB_EN_CRHO1] = wxyz * s[S_CRHO1];
-  E[HFB_EN_CDRHO0] = wxyz * s[S_CDRHO0];
-  E[HFB_EN_CDRHO1] = wxyz * s[S_CDRHO1];
-  E[HFB_EN_CTAU0] = wxyz * s[S_CTAU0];
-  E[HFB_EN_CTAU1] = wxyz * s[S_CTAU1];
-  E[HFB_EN_CDJ0] = wxyz * s[S_CDJ0];
-  E[HFB_EN_CDJ1] = wxyz * s[S_CDJ1];
-  E[HFB_EN_CJ0] = -wxyz * f.Ctau0 * cj0;
-  E[HFB_EN_CJ1] = -wxyz * f.Ctau1 * cj1;
-  E[HFB_EN_EHFINT] = ehft + ehf0 + ehf1 + ehf2 + ehf3 + ehfls + ehfc - epair - ezpe;
-}
-
 __global__ void energy_finish_kernel(EnArgs a, int nblk, double wxyz, const double* __restrict__ cur, int nx,
                                      int ny, int nz, double* __restrict__ E) {
   energy_finish_body(a, nblk, wxyz, cur, nx, ny, nz, E);
 }
 
-// ecorrp from the block partials (meanfield.py:360-363) and the maxima of
-// v_pair per isospin into E[HFB_EN_VMAX..+1].
-__device__ void ecorrp_finish_body(const double* part, int nblk, double wxyz, int ipair, double* E) {
-  __shared__ double red[4];
-  __shared__ double vred[16][2];
-  double s[1] = {0.0};
-  double vm[2] = {-INFINITY, -INFINITY};
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-    s[0] += part[b];
-    vm[0] = fmax(vm[0], part[nblk + b]);
-    vm[1] = fmax(vm[1], part[2 * nblk + b]);
-  }
-  block_sum<1>(s, red);
-  if (threadIdx.x == 0) E[HFB_EN_ECORRP] = ipair == 6 ? -s[0] * wxyz / 2.0 : 0.0;
-  for (int o = 32; o > 0; o >>= 1) {
-    vm[0] = fmax(vm[0], __shfl_xor(vm[0], o, 64));
-    vm[1] = fmax(vm[1], __shfl_xor(vm[1], o, 64));
-  }
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) {
-    vred[wid][0] = vm[0];
-    vred[wid][1] = vm[1];
-  }
-  __syncthreads();
-  if (threadIdx.x < 2) {
-    double m = vred[0][threadIdx.x];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, vred[w][threadIdx.x]);
-    E[HFB_EN_VMAX + threadIdx.x] = m;
-  }
-}
-
 __global__ void ecorrp_finish_kernel(const double* part, int nblk, double wxyz, int ipair, double* E) {
   ecorrp_finish_body(part, nblk, wxyz, ipair, E);
-}
-
-// sum_energy (energies.py:309-371)
-__device__ void sum_energy_body(int ns, const double* wocc, const double* wstates, const double* kin,
-                                const double* spe, const double* orb, const double* spin, double* E) {
-  __shared__ double red[16 * 8];
-  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
-    const double w = wocc[s] * wstates[s];
-    v[0] += w * (kin[s] + spe[s]);
-    v[1] += w * kin[s];
-    for (int c = 0; c < 3; ++c) {
-      v[2 + c] += w * orb[3 * s + c];
-      v[5 + c] += w * spin[3 * s + c];
-    }
-  }
-  block_sum<8>(v, red);
-  if (threadIdx.x == 0) {
-    const double epair = E[HFB_EN_EPAIR] + E[HFB_EN_EPAIR + 1];
-    E[HFB_EN_EHF] = v[0] / 2.0 + E[HFB_EN_E3CORR] + E[HFB_EN_ECORRP] + E[HFB_EN_ECORC] - epair - E[HFB_EN_EZPE];
-    E[HFB_EN_TKE] = v[1];
-    for (int c = 0; c < 3; ++c) {
-      E[HFB_EN_ORB + c] = v[2 + c];
-      E[HFB_EN_SPIN + c] = v[5 + c];
-      E[HFB_EN_JTOT + c] = v[2 + c] + v[5 + c];
-    }
-  }
 }
 
 __global__ void sum_energy_kernel(int ns, const double* wocc, const double* wstates, const double* kin,
@@ -481,15 +349,9 @@ __global__ void sum_energy_kernel(int ns, const double* wocc, const double* wsta
 // The scalar tail of an SCF iteration in one workgroup: ecorrp and the
 // v_pair maxima from the skyrme partials, the energy integrals from the
 // energy partials, then sum_energy on the single-particle properties.
-__global__ void __launch_bounds__(256) energy_tail_kernel(const double* sky_part, int nblk, int ipair, EnArgs a,
-                                                          double wxyz, const double* cur, int nx, int ny, int nz,
-                                                          int ns, const double* wocc, const double* wstates,
-                                                          const double* kin, const double* spe, const double* orb,
-                                                          const double* spin, double* E) {
-  ecorrp_finish_body(sky_part, nblk, wxyz, ipair, E);
-  energy_finish_body(a, nblk, wxyz, cur, nx, ny, nz, E);
-  __syncthreads();
-  sum_energy_body(ns, wocc, wstates, kin, spe, orb, spin, E);
+__global__ void __launch_bounds__(256) energy_tail_kernel(TailArgs t) {
+  double vm[2];
+  energy_tail_body<4>(t, vm);
 }
 
 void push(DerivJobs& J, double* dst, int nterm, const double* s0, int a0, int o0, double c0, const double* s1 = nullptr,
@@ -751,17 +613,11 @@ extern "C" int hfb_energy_tail(hfb_ctx* c, const hfb_force* f, const double* sky
                                const double* sp_kinetic, const double* sp_energy, const double* sp_orbital,
                                const double* sp_spin, double* E, void* stream) {
   if (!c) return HFB_E_NOCTX;
-  if (!f || !sky_part || !en_part || !current || nstates <= 0 || !wocc || !wstates || !sp_kinetic || !sp_energy ||
-      !sp_orbital || !sp_spin || !E)
-    return HFB_E_ARG;
-  EnArgs a{};
-  a.f = *f;
-  a.nvox = (int64_t)c->nx * c->ny * c->nz;
-  a.part = (double*)en_part;
-  const int nb = hfb_skyrme_nblocks(c);
-  hipLaunchKernelGGL(energy_tail_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, sky_part, nb, f->ipair, a,
-                     c->dx * c->dy * c->dz, current, c->nx, c->ny, c->nz, nstates, wocc, wstates, sp_kinetic,
-                     sp_energy, sp_orbital, sp_spin, E);
+  const hfb_tail_job j{f, sky_part, en_part, current, nstates, wocc, wstates, sp_kinetic, sp_energy, sp_orbital,
+                       sp_spin, E};
+  TailArgs t;
+  if (!tail_args(c, j, t)) return HFB_E_ARG;
+  hipLaunchKernelGGL(energy_tail_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, t);
   return launch_status();
 }
 

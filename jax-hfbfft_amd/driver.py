@@ -147,7 +147,7 @@ class HFRun:
         tbcs = True if i <= s.inibcs else self.tbcssav
         tdiag = tbcs or not (i > s.inidiag)
         return (tbcs, tdiag, i > 1 and not tbcs, p.mprint > 0 and i % p.mprint == 0,
-                f.ipair != 0 and i <= 10, self.levels.psi.dtype, s.x0dmp)
+                f.ipair != 0 and i <= 10, self.levels.psi.dtype, s.x0dmp, self.energies.tail is not None)
 
     def iteration(self, i, check=False):
         self.iteration_no = i
@@ -167,6 +167,8 @@ class HFRun:
         self.params.iteration = i
         self.forces.tbcs = sig[0]
         self.static.tdiag = sig[1]
+        # the captured iteration left its energy tail to the next one
+        self.energies.defer_tail(*g.tail)
         from . import kernels
         for graph, label in zip(g.graphs, g.labels):
             if label is not None and kernels.TIMERS is not None:
@@ -198,11 +200,13 @@ class HFRun:
         """Capture the graphs of the given iteration numbers ahead of time
         (capture records kernels without running them, so no state changes)."""
         saved = (self.params.iteration, self.forces.tbcs, self.static.tdiag)
+        tail = (self.energies.tail, self.energies.stale)
         for i in iterations:
             sig = self.signature(i)
             if sig not in self._graphs:
                 self._graphs[sig] = self._capture(i)
         self.params.iteration, self.forces.tbcs, self.static.tdiag = saved
+        self.energies.tail, self.energies.stale = tail
 
     def _capture(self, i):
         from . import kernels
@@ -217,6 +221,7 @@ class HFRun:
             finally:
                 kernels.SEGMENTER = None
                 seg.close()
+        seg.tail = (self.energies._tail_ctx, self.energies.tail)
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
         return seg

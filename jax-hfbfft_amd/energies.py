@@ -25,9 +25,25 @@ class Energies:
         self.efluct2prev = 0.0
         self.ehfprev = 0.0
         self.e_extern = 0.0
+        # energy tail left by the last SCF iteration (kernels.tail_job); the
+        # next iteration's first h psi launch runs it, reads run it here
+        self.tail = None
+        self.stale = False
+        self._tail_ctx = None
+
+    def defer_tail(self, ctx, tail):
+        self._tail_ctx, self.tail, self.stale = ctx, tail, True
+
+    def flush(self):
+        """Run the deferred energy tail, if any (rerunning it is harmless: it
+        rewrites the same slots from the same partials)."""
+        if self.stale:
+            kernels.run_tail(self._tail_ctx, self.tail)
+            self.stale = False
 
     def __getattr__(self, name):
         if name in _SCALARS:
+            self.flush()
             return float(self.vec[EN[_SCALARS[name]]].item())
         raise AttributeError(name)
 
@@ -49,18 +65,22 @@ class Energies:
 
     @property
     def orbital(self):
+        self.flush()
         return self.vec[EN["ORB"]:EN["ORB"] + 3]
 
     @property
     def spin(self):
+        self.flush()
         return self.vec[EN["SPIN"]:EN["SPIN"] + 3]
 
     @property
     def total_angmom(self):
+        self.flush()
         return self.vec[EN["JTOT"]:EN["JTOT"] + 3]
 
     def host(self):
         """All slots as a host dict (one synchronising copy)."""
+        self.flush()
         v = self.vec.cpu().numpy()
         d = {k: float(v[EN[s]]) for k, s in _SCALARS.items()}
         d.update(efluct1=float(v[EN["EFL1"]]), efluct2=float(v[EN["EFL2"]]), ecorrp=float(v[EN["ECORRP"]]),

@@ -173,11 +173,40 @@ def selftest_mfma_f64(a: torch.Tensor, b: torch.Tensor):
 # ---------------------------------------------------------------------------
 # Gradient step (static.py:93-259)
 # ---------------------------------------------------------------------------
-def weights_job(psi_states, npsi0, tbcs, wocc, wstates, wguv, pairwg, v_pair, nvox, out, vpair_max=None):
-    """The arguments of grstep_weights as a record for hpsi(..., weights_job=)."""
-    return _lib.WeightsJob(int(psi_states), int(npsi0), int(bool(tbcs)), ptr(wocc), ptr(wstates), ptr(wguv),
-                           ptr(pairwg), ptr(v_pair), int(nvox), ptr(vpair_max), ptr(out["w"]), ptr(out["wuv"]),
-                           ptr(out["pw"]), ptr(out["pshift"]))
+def weights_job(psi_states, npsi0, tbcs, wocc, wstates, wguv, pairwg, v_pair, nvox, out, vpair_max=None,
+                tail=None):
+    """The arguments of grstep_weights as a record for hpsi(..., weights_job=).
+    ``tail``: a tail_job() the same workgroup runs first (vpair_max may then
+    be the E[VMAX] slots it writes)."""
+    j = _lib.WeightsJob(int(psi_states), int(npsi0), int(bool(tbcs)), ptr(wocc), ptr(wstates), ptr(wguv),
+                        ptr(pairwg), ptr(v_pair), int(nvox), ptr(vpair_max), ptr(out["w"]), ptr(out["wuv"]),
+                        ptr(out["pw"]), ptr(out["pshift"]))
+    if tail is not None:
+        j.tail = ctypes.pointer(tail)
+        j._keep = tail
+    return j
+
+
+def tail_job(ctx, fc, dens, levels, E):
+    """energy_tail's arguments as a record (for weights_job(tail=) and
+    run_tail); keeps the force record alive."""
+    nb = lib().hfb_skyrme_nblocks(ctx.handle)
+    sky = ctx.work("sky_part", (3 * nb,), torch.float64)
+    en = ctx.work("en_part", (nb * 20,), torch.float64)
+    t = _lib.TailJob(ctypes.addressof(fc), ptr(sky), ptr(en), ptr(dens.current), int(levels.nstmax),
+                     ptr(levels.wocc), ptr(levels.wstates), ptr(levels.sp_kinetic), ptr(levels.sp_energy),
+                     ptr(levels.sp_orbital), ptr(levels.sp_spin), ptr(E))
+    t._keep = fc
+    return t
+
+
+def run_tail(ctx, t):
+    """energy_tail from a tail_job() record."""
+    check(lib().hfb_energy_tail(ctx.handle, ctypes.c_void_p(t.force), ctypes.c_void_p(t.sky_part),
+                                ctypes.c_void_p(t.en_part), ctypes.c_void_p(t.current), int(t.nstates),
+                                ctypes.c_void_p(t.wocc), ctypes.c_void_p(t.wstates), ctypes.c_void_p(t.sp_kinetic),
+                                ctypes.c_void_p(t.sp_energy), ctypes.c_void_p(t.sp_orbital),
+                                ctypes.c_void_p(t.sp_spin), ctypes.c_void_p(t.E), stream_handle()), "energy_tail")
 
 
 def grstep_weights(psi_states, npsi0, tbcs, wocc, wstates, wguv, pairwg, v_pair, nvox, out, vpair_max=None):
@@ -456,13 +485,7 @@ def skyrme_energy(ctx, fc, dens, der, wcoul, mf, f32=None):
 def energy_tail(ctx, fc, dens, levels, E):
     """ecorrp, the integrals and sum_energy from the partials that
     skyrme(..., finish=False) and integ_energy(..., finish=False) left."""
-    nb = lib().hfb_skyrme_nblocks(ctx.handle)
-    sky = ctx.work("sky_part", (3 * nb,), torch.float64)
-    en = ctx.work("en_part", (nb * 20,), torch.float64)
-    check(lib().hfb_energy_tail(ctx.handle, ctypes.byref(fc), ptr(sky), ptr(en), ptr(dens.current),
-                                int(levels.nstmax), ptr(levels.wocc), ptr(levels.wstates), ptr(levels.sp_kinetic),
-                                ptr(levels.sp_energy), ptr(levels.sp_orbital), ptr(levels.sp_spin), ptr(E),
-                                stream_handle()), "energy_tail")
+    run_tail(ctx, tail_job(ctx, fc, dens, levels, E))
 
 
 def sum_energy(levels, E):

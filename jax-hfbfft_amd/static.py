@@ -150,11 +150,13 @@ def _ws(ctx, levels):
                 pout=ctx.work(("pout", dt, S), shp, dt))
 
 
-def grstep(forces, grids, levels, meanfield, params, static, defer_finish=False):
+def grstep(forces, grids, levels, meanfield, params, static, defer_finish=False, tail=None):
     """Damped gradient step on all states (static.py:215-259).  With
     ``defer_finish`` the scalar tail (sp_energy, fluctuations, sumflu,
     delesum) is returned as a job for diagstep(..., finish_job=) to run in
-    its first launch: (levels, static, job)."""
+    its first launch: (levels, static, job).  ``tail``: the previous
+    iteration's deferred energy tail (kernels.tail_job), run by the first h psi
+    launch before the step weights that read its v_pair maxima."""
     ctx = grids.ctx
     S = levels.nstmax
     n0 = int(levels.npsi[0])
@@ -162,7 +164,8 @@ def grstep(forces, grids, levels, meanfield, params, static, defer_finish=False)
     # the step's weights are computed by an extra workgroup of the first h psi
     # launch (its x pass does not read them)
     wjob = kernels.weights_job(S, n0, forces.tbcs, levels.wocc, levels.wstates, levels.wguv, levels.pairwg,
-                               meanfield.v_pair, grids.nx * grids.ny * grids.nz, ws, vpair_max=meanfield.vpair_max)
+                               meanfield.v_pair, grids.nx * grids.ny * grids.nz, ws, vpair_max=meanfield.vpair_max,
+                               tail=tail)
     fields = meanfield.fields(levels.psi.dtype)
     # h psi on the old states: ps1 (weighted) and psi_mf -> hampsi (static.py:152-173)
     # the per-state sums stay as tile partials, added up by the step and the
@@ -272,12 +275,15 @@ def prelude(coulomb, densities, energies, forces, grids, levels, meanfield, mome
 
 
 def iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, moment, params, static, pairs,
-            tbcssav, derivs=None, check=True):
-    """One SCF iteration: the loop body of static.py:1755-1882 without output."""
+            tbcssav, derivs=None, check=True, defer_tail=True):
+    """One SCF iteration: the loop body of static.py:1755-1882 without output.
+    With ``defer_tail`` the energy tail (ecorrp, the integrals, sum_energy) is
+    left to the next iteration's first launch, or to the first read of the
+    energies (Energies.flush)."""
     params.iteration = i
     forces.tbcs = True if i <= static.inibcs else tbcssav
     static.tdiag = not (i > static.inidiag)
-    _, _, fin = grstep(forces, grids, levels, meanfield, params, static, defer_finish=True)
+    _, _, fin = grstep(forces, grids, levels, meanfield, params, static, defer_finish=True, tail=energies.tail)
     if forces.tbcs:
         static.tdiag = True
     diagstep(energies, forces, grids, levels, static, static.tdiag, True, finish_job=fin)
@@ -301,7 +307,12 @@ def iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, m
     # integ_energy's integrals come from the same launch as the potentials;
     # its pairing energies go to the energy vector as integ_energy does
     _set_slots(energies.vec, kernels.EN["EPAIR"], pairs.epair)
-    kernels.energy_tail(grids.ctx, forces.to_c(params, levels.mass_number), densities, levels, energies.vec)
+    tail = kernels.tail_job(grids.ctx, forces.to_c(params, levels.mass_number), densities, levels, energies.vec)
+    if defer_tail:
+        energies.defer_tail(grids.ctx, tail)
+    else:
+        kernels.run_tail(grids.ctx, tail)
+        energies.tail, energies.stale = None, False
     converged = False
     if check or static.tvaryx_0:
         e = energies.host()

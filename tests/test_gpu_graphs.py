@@ -55,3 +55,23 @@ def test_graph_replay_pairing(pkg, dev):
         a, b = eager.energies_host(), graphed.energies_host()
         assert a["ehf"] == b["ehf"], (i, a["ehf"], b["ehf"])
     assert torch.equal(eager.levels.wocc, graphed.levels.wocc)
+
+
+@pytest.mark.parametrize("precision,ipair", [("mixed", 0), ("f64", 5)])
+def test_deferred_energy_tail(pkg, dev, precision, ipair):
+    """The energy tail left to the next iteration's first h psi launch (and
+    run again by the read at the end) gives the trajectory of running it at
+    the end of every iteration, bit for bit."""
+    runs = {}
+    for defer in (False, True):
+        r = make(pkg, precision, ipair=ipair)
+        for i in range(1, 9):
+            _, r.derivs = pkg.static.iterate(i, *r.state, r.tbcssav, r.derivs, check=False, defer_tail=defer)
+        assert r.energies.stale == defer
+        runs[defer] = (r.energies_host(), r.levels.psi.clone(), r.levels.sp_energy.clone())
+        assert not r.energies.stale
+    a, b = runs[False], runs[True]
+    for k, v in a[0].items():
+        assert np.array_equal(np.asarray(v), np.asarray(b[0][k])), (k, v, b[0][k])
+    assert torch.equal(a[1], b[1])
+    assert torch.equal(a[2], b[2])
