@@ -323,6 +323,113 @@ __global__ void __launch_bounds__(256) rotate_kernel(RotArgs ra) {
   }
 }
 
+// FP32 rotation on the matrix cores (v_mfma_f32_16x16x4f32) for blocks of
+// at most 24 states.  As a real product, out = C psi with the 2 nst input
+// columns (Re psi_i, Im psi_i) and the out columns (Re, Im of out1_j, then
+// of out2_j) of both matrices stacked: C[(m, j, co)][(i, ci)] = U_m[i][j].x
+// for ci == co, -U_m[i][j].y for (ci, co) = (1, 0), U_m[i][j].y for (0, 1).
+// The workgroup copies U1, U2 (FP32) to LDS; each wavefront keeps C as
+// MT x KS A fragments (rows: out columns, 16 per tile; k-steps: four input
+// columns) and streams tiles of 16 grid points, TPW tiles loaded at a time:
+// B fragment lane l = input column 4 ks + l / 16 at point k0 + l % 16, a
+// 4-byte load straight from psi.  Result lane l, register r = out column
+// 16 mt + 4 (l / 16) + r at point k0 + l % 16, so registers (0, 1) and
+// (2, 3) are the (Re, Im) of one state and store as one 8-byte value, 16
+// consecutive points per state and lane group.  A wavefront reads every
+// input of its points before their first store, so out1 may alias psi.
+typedef float rv4f __attribute__((ext_vector_type(4)));
+constexpr int ROT_MFMA_NMAX = 24;
+#ifndef HFB_ROT_TPW
+#define HFB_ROT_TPW 8  // tiles of 16 points a wavefront loads at once
+#endif
+#ifndef HFB_ROT_BPC
+#define HFB_ROT_BPC 1  // workgroups per CU
+#endif
+
+template <int KS, int MT, int TPW>
+__global__ void __launch_bounds__(256) rotate_mfma_kernel(RotArgs ra) {
+  __shared__ float2 Us[2][ROT_MFMA_NMAX * ROT_MFMA_NMAX];
+  const RotIso& ri = ra.iso[blockIdx.y];
+  const int nst = ri.nst;
+  if (nst <= 0) return;  // uniform over the workgroup
+  const int64_t K = ra.K;
+  const bool two = ri.U2 != nullptr;
+  const int ncol = (two ? 4 : 2) * nst;  // out columns of both matrices
+  for (int e = threadIdx.x; e < nst * nst; e += blockDim.x) {
+    Us[0][e] = float2{float(ri.U1[2 * e]), float(ri.U1[2 * e + 1])};
+    if (two) Us[1][e] = float2{float(ri.U2[2 * e]), float(ri.U2[2 * e + 1])};
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
+  float A[MT][KS];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int co_all = 16 * mt + c16;
+    const bool ok = co_all < ncol;
+    const int m = ok ? co_all / (2 * nst) : 0, j = ok ? (co_all % (2 * nst)) >> 1 : 0, co = co_all & 1;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int i = 2 * ks + (g >> 1), ci = g & 1;
+      float v = 0.0f;
+      if (ok && i < nst) {
+        const float2 u = Us[m][i * nst + j];
+        v = ci == co ? u.x : (ci == 1 ? -u.y : u.y);
+      }
+      A[mt][ks] = v;
+    }
+  }
+  // this lane's out columns: state offset j K into out1 or out2 (or none)
+  int64_t ooff[MT][2];
+  int osel[MT][2];  // 0: out1, 1: out2, -1: none
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int co_all = 16 * mt + 4 * g + 2 * h;
+      osel[mt][h] = co_all < ncol ? co_all / (2 * nst) : -1;
+      ooff[mt][h] = (int64_t)((co_all % (2 * nst)) >> 1) * K;
+    }
+  const float* psi = (const float*)ri.psi;
+  float2* o1 = (float2*)ri.out1;
+  float2* o2 = (float2*)ri.out2;
+  const int64_t ntile = K / 16;
+  const int64_t nwave = (int64_t)gridDim.x * 4;
+  for (int64_t base = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * TPW; base < ntile; base += nwave * TPW) {
+    // the inputs of all TPW tiles are loaded before the first product
+    float B[TPW][KS];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      const int64_t k = (base + t) * 16 + c16;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int i = 2 * ks + (g >> 1);
+        B[t][ks] = (base + t < ntile && i < nst) ? psi[2 * ((int64_t)i * K + k) + (g & 1)] : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      if (base + t >= ntile) break;
+      const int64_t k = (base + t) * 16 + c16;
+      rv4f acc[MT];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) acc[mt] = rv4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+          acc[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mt][ks], B[t][ks], acc[mt], 0, 0, 0);
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          if (osel[mt][h] < 0) continue;
+          float2* o = osel[mt][h] == 0 ? o1 : o2;
+          o[ooff[mt][h] + k] = float2{acc[mt][2 * h], acc[mt][2 * h + 1]};
+        }
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" int hfb_gram_workspace(int nst, long long K) {
@@ -442,6 +549,33 @@ int launch_rotate(int prec, RotArgs ra, int niso, hipStream_t st) {
     nmin = n < nmin ? n : nmin;
   }
   if (nmax == 0) return HFB_OK;
+  if (prec == HFB_F32 && nmax <= ROT_MFMA_NMAX && ra.K % 16 == 0) {
+    // matrix cores: KS k-steps over the 2 nmax input columns, MT tiles over
+    // the (up to) 4 nmax out columns; HFB_ROT_BPC workgroups per CU, each
+    // wavefront loading HFB_ROT_TPW consecutive tiles at a time (measured at
+    // Ca40: 8 tiles, one workgroup per CU 20.3 us; 2 tiles, three 24.3 us)
+    const int ks = (2 * nmax + 3) / 4;
+    bool two = false;
+    for (int iq = 0; iq < niso; ++iq) two = two || (ra.iso[iq].nst > 0 && ra.iso[iq].U2);
+    const int mt = ((two ? 4 : 2) * nmax + 15) / 16;
+    const int64_t ntile = ra.K / 16;
+    constexpr int tpw = HFB_ROT_TPW;
+    int64_t nb = (ntile + 4 * tpw - 1) / (4 * tpw);
+    const int64_t cap = (HFB_ROT_BPC * 256 + niso - 1) / niso;
+    if (nb > cap) nb = cap;
+    const dim3 grid((unsigned)nb, niso);
+#define HFB_ROTM(KSV, MTV)                                                                     \
+  if (ks <= KSV && mt <= MTV) {                                                                \
+    hipLaunchKernelGGL((rotate_mfma_kernel<KSV, MTV, tpw>), grid, dim3(256), 0, st, ra);      \
+    return launch_status();                                                                    \
+  }
+    HFB_ROTM(4, 2)
+    HFB_ROTM(6, 3)
+    HFB_ROTM(8, 4)
+    HFB_ROTM(10, 5)
+    HFB_ROTM(12, 6)
+#undef HFB_ROTM
+  }
   const size_t csz = prec == HFB_F64 ? sizeof(cx<double>) : sizeof(cx<float>);
   // two k per lane while the staged tile fits in LDS, else one
   const int kpl = csz * (size_t)nmax * 128 <= 150 * 1024 ? 2 : 1;
