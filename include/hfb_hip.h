@@ -340,6 +340,15 @@ int hfb_density_derivs(hfb_ctx* ctx, const double* rho, const double* sdens, con
                        const double* sodens, double* divj, double* lap, double* grad, double* curls,
                        double* curlj, double* work, void* stream);
 
+/* hfb_density_derivs and hfb_poisson of rho (coulomb.py:54-75) in one call,
+ * for the SCF loop: on cubic 32^3 and 48^3 grids the sums of the
+ * multi-term derivative fields ride in the Coulomb x-pencil launch (four
+ * launches instead of five); elsewhere the two entry points in turn.  work
+ * as for hfb_density_derivs, pwork as for hfb_poisson. */
+int hfb_density_derivs_coulomb(hfb_ctx* ctx, const double* rho, const double* sdens, const double* current,
+                               const double* sodens, double* divj, double* lap, double* grad, double* curls,
+                               double* curlj, double* work, double e2, double* wcoul, void* pwork, void* stream);
+
 /* divaq = div aq and dbmass = grad bmass (meanfield.py:329-344); work as
  * for hfb_density_derivs. */
 int hfb_meanfield_derivs(hfb_ctx* ctx, const double* aq, const double* bmass, double* divaq, double* dbmass,

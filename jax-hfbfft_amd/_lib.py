@@ -91,6 +91,8 @@ _SIGS = {
                               c_void_p, c_void_p]),
     "hfb_deriv_workspace": (ctypes.c_longlong, [c_void_p]),
     "hfb_density_derivs": (c_int, [c_void_p] + [c_void_p] * 9 + [c_void_p, c_void_p]),
+    "hfb_density_derivs_coulomb": (c_int, [c_void_p] + [c_void_p] * 9 + [c_void_p, c_double, c_void_p, c_void_p,
+                                                                          c_void_p]),
     "hfb_meanfield_derivs": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hfb_skyrme_nblocks": (c_int, [c_void_p]),
     "hfb_skyrme": (c_int, [c_void_p, c_void_p] + [c_void_p] * 11 + [c_void_p] * 6 + [c_void_p] * 6 +

@@ -10,6 +10,7 @@
 #include <math.h>
 
 #include "hfb_ctx.h"
+#include "hfb_poisson.h"
 
 using namespace hfb;
 
@@ -115,107 +116,26 @@ __global__ void __launch_bounds__(128) pois_pass(cx<double>* __restrict__ buf, P
 }
 
 // ----- cubic doubled grid, three launches -------------------------------
-constexpr int PP_L = 8;  // lanes per pencil
-
-// Forward y-z plane of x plane blockIdx.x (< nx): z transforms of the rows
-// y < ny (zero-padded beyond nz), rows y >= ny zero, then y transforms;
-// out[x][ky][kz].
 template <int N>
 __global__ void __launch_bounds__(N * PP_L) pois_plane_fwd(cx<double>* __restrict__ out, const double* __restrict__ rho,
                                                            int nx, int ny, int nz, const cx<double>* __restrict__ tw) {
-  constexpr int L = PP_L, N1 = N / L, P = N + 1;
   extern __shared__ unsigned char pp_lds[];
-  cx<double>* pl = (cx<double>*)pp_lds;  // [y][kz]
-  const int j = threadIdx.x % L, q = threadIdx.x / L;
-  const int x = blockIdx.x;
-  LaneFFT<double, N, L> F;
-  F.init(j, tw);
-  cx<double> v[N1];
-  if (q < ny) {  // z pencil q = y
-#pragma unroll
-    for (int m = 0; m < N1; ++m) {
-      const int z = j + L * m;
-      v[m] = {z < nz ? rho[((int64_t)x * ny + q) * nz + z] : 0.0, 0.0};
-    }
-    F.fwd(v, tw);
-#pragma unroll
-    for (int m = 0; m < N1; ++m) pl[q * P + F.freq(m)] = v[m];
-  } else {
-#pragma unroll
-    for (int m = 0; m < N1; ++m) pl[q * P + j + L * m] = {0.0, 0.0};
-  }
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < N1; ++m) v[m] = pl[(j + L * m) * P + q];  // y pencil q = kz
-  F.fwd(v, tw);
-#pragma unroll
-  for (int m = 0; m < N1; ++m) out[((int64_t)x * N + F.freq(m)) * N + q] = v[m];
+  pois_fwd_body<N>(out, rho, nx, ny, nz, tw, blockIdx.x, (cx<double>*)pp_lds);
 }
 
-// x pencils (ky, kz): forward with x >= nx zero, times scale * qhat,
-// inverse, rows x < nx written back.  PP_PW pencils per 64-lane wavefront.
-constexpr int PP_PW = 64 / PP_L;
-
+// PP_PW pencils per 64-lane wavefront
 template <int N>
 __global__ void __launch_bounds__(64) pois_pencil_mul(cx<double>* __restrict__ buf, const cx<double>* __restrict__ qhat,
                                                       double scale, int nx, const cx<double>* __restrict__ tw) {
-  constexpr int L = PP_L, N1 = N / L;
-  const int j = threadIdx.x % L;
-  const int g = blockIdx.x * PP_PW + threadIdx.x / L;  // pencil = ky * N + kz
-  const int64_t plane = (int64_t)N * N;
-  LaneFFT<double, N, L> F;
-  F.init(j, tw);
-  cx<double> v[N1];
-#pragma unroll
-  for (int m = 0; m < N1; ++m) {
-    const int x = j + L * m;
-    v[m] = x < nx ? buf[x * plane + g] : cx<double>{0.0, 0.0};
-  }
-  F.fwd(v, tw);
-#pragma unroll
-  for (int m = 0; m < N1; ++m) v[m] = cmul(v[m], cscale(qhat[F.freq(m) * plane + g], scale));
-  F.inv(v, tw);
-#pragma unroll
-  for (int m = 0; m < N1; ++m) {
-    const int x = j + L * m;
-    if (x < nx) buf[x * plane + g] = v[m];
-  }
+  pois_mul_body<N>(buf, qhat, scale, nx, tw, blockIdx.x * PP_PW + threadIdx.x / PP_L, threadIdx.x % PP_L);
 }
 
-// Inverse y-z plane of x plane blockIdx.x (< nx): y transforms of every kz,
-// rows y < ny kept in LDS, z transforms of those rows, wcoul[x][y][z] for
-// z < nz (real part, times scale = 1 / N^3).
 template <int N>
 __global__ void __launch_bounds__(N * PP_L) pois_plane_inv(const cx<double>* __restrict__ in, double* __restrict__ wcoul,
                                                            double scale, int ny, int nz,
                                                            const cx<double>* __restrict__ tw) {
-  constexpr int L = PP_L, N1 = N / L, P = N + 1;
   extern __shared__ unsigned char pp_lds[];
-  cx<double>* pl = (cx<double>*)pp_lds;  // [y][kz]
-  const int j = threadIdx.x % L, q = threadIdx.x / L;
-  const int x = blockIdx.x;
-  LaneFFT<double, N, L> F;
-  F.init(j, tw);
-  cx<double> v[N1];
-#pragma unroll
-  for (int m = 0; m < N1; ++m) v[m] = in[((int64_t)x * N + F.freq(m)) * N + q];  // y pencil q = kz
-  F.inv(v, tw);
-#pragma unroll
-  for (int m = 0; m < N1; ++m) {
-    const int y = j + L * m;
-    if (y < ny) pl[y * P + q] = v[m];
-  }
-  __syncthreads();
-  if (q < ny) {  // z pencil q = y
-#pragma unroll
-    for (int m = 0; m < N1; ++m) v[m] = pl[q * P + F.freq(m)];
-    F.inv(v, tw);
-#pragma unroll
-    for (int m = 0; m < N1; ++m) {
-      const int z = j + L * m;
-      if (z < nz) wcoul[((int64_t)x * ny + q) * nz + z] = v[m].x * scale;
-    }
-  }
+  pois_inv_body<N>(in, wcoul, scale, ny, nz, tw, blockIdx.x, (cx<double>*)pp_lds);
 }
 
 int poisson_planes(const hfb_ctx* c, const double* rho_p, double* wcoul, double e2, cx<double>* buf,
@@ -252,9 +172,7 @@ int poisson_planes(const hfb_ctx* c, const double* rho_p, double* wcoul, double 
 #undef HFB_PPL
 }
 
-bool planes_apply(const hfb_ctx* c) {
-  return !c->periodic && c->nx2 == c->ny2 && c->ny2 == c->nz2 && (c->nx2 == 48 || c->nx2 == 64 || c->nx2 == 96);
-}
+bool planes_apply(const hfb_ctx* c) { return pois_planes_apply(c); }
 
 template <int AXIS, int MODE>
 int launch_pass(const hfb_ctx* c, cx<double>* buf, const PoisArgs& a, hipStream_t st) {

@@ -11,6 +11,7 @@
 
 #include "hfb_ctx.h"
 #include "hfb_energy.h"
+#include "hfb_poisson.h"
 
 using namespace hfb;
 
@@ -106,19 +107,20 @@ __device__ __forceinline__ int64_t pencil_off(int ax, int k, int gp) {
 // coef * D src along the term's axis for 64 pencils and writes it to its
 // workspace slot (or to dst for a single-term job); deriv_sum_kernel then
 // adds the terms of each job in term order.
+// Tile tx of term tg by 256 threads (index tid); sS: N * DT_LD doubles of
+// LDS.  An inactive call only takes part in the workgroup barriers.
 template <int N>
-__global__ void __launch_bounds__(256) deriv_tile_kernel(DerivJobs jobs, DerivMats dm, double* __restrict__ work) {
-  __shared__ double sS[N * DT_LD];
-  const int tg = blockIdx.y;
-  const DerivJob& jb = jobs.j[jobs.tjob[tg]];
-  const int term = jobs.tidx[tg];
-  const int slot = jobs.tslot[tg];
+__device__ __forceinline__ void deriv_tile_body(const DerivJobs& jobs, const DerivMats& dm, double* __restrict__ work,
+                                                int tg, int tx, bool active, double* sS, int tid) {
+  const DerivJob& jb = jobs.j[active ? jobs.tjob[tg] : 0];
+  const int term = active ? jobs.tidx[tg] : 0;
+  const int slot = active ? jobs.tslot[tg] : -1;
   const int ax = jb.axis[term];
   const double* __restrict__ src = jb.src[term];
   const double* __restrict__ D = jb.order[term] == 1 ? dm.d1[ax] : dm.d2[ax];
   const double coef = jb.coef[term];
   double* __restrict__ dst = slot < 0 ? jb.dst : work + (int64_t)slot * N * N * N;
-  const int p0 = blockIdx.x * DT_P;
+  const int p0 = tx * DT_P;
   // element (k, p) of the tile; consecutive threads walk the contiguous direction
   auto kp = [&](int e, int& k, int& p) {
     if (ax == 2) {
@@ -129,46 +131,64 @@ __global__ void __launch_bounds__(256) deriv_tile_kernel(DerivJobs jobs, DerivMa
       k = e / DT_P;
     }
   };
-  for (int e = threadIdx.x; e < N * DT_P; e += 256) {
-    int k, p;
-    kp(e, k, p);
-    sS[k * DT_LD + p] = src[pencil_off<N>(ax, k, p0 + p)];
-  }
+  if (active)
+    for (int e = tid; e < N * DT_P; e += 256) {
+      int k, p;
+      kp(e, k, p);
+      sS[k * DT_LD + p] = src[pencil_off<N>(ax, k, p0 + p)];
+    }
   __syncthreads();
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int p = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int p = tid & 63;
   constexpr int R = N / 4;
   double acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  if (active) {
 #pragma unroll
-  for (int l = 0; l < N; ++l) {
-    const double sv = sS[l * DT_LD + p];
+    for (int l = 0; l < N; ++l) {
+      const double sv = sS[l * DT_LD + p];
 #pragma unroll
-    for (int r = 0; r < R; ++r) acc[r] = fma(D[(w + 4 * r) * N + l], sv, acc[r]);
+      for (int r = 0; r < R; ++r) acc[r] = fma(D[(w + 4 * r) * N + l], sv, acc[r]);
+    }
   }
   __syncthreads();
+  if (active) {
 #pragma unroll
-  for (int r = 0; r < R; ++r) sS[(w + 4 * r) * DT_LD + p] = acc[r];
-  __syncthreads();
-  for (int e = threadIdx.x; e < N * DT_P; e += 256) {
-    int k, pp;
-    kp(e, k, pp);
-    dst[pencil_off<N>(ax, k, p0 + pp)] = coef * sS[k * DT_LD + pp];
+    for (int r = 0; r < R; ++r) sS[(w + 4 * r) * DT_LD + p] = acc[r];
   }
+  __syncthreads();
+  if (active)
+    for (int e = tid; e < N * DT_P; e += 256) {
+      int k, pp;
+      kp(e, k, pp);
+      dst[pencil_off<N>(ax, k, p0 + pp)] = coef * sS[k * DT_LD + pp];
+    }
 }
 
-// dst = term 0 + term 1 (+ term 2) for every multi-term job, in that order.
-__global__ void __launch_bounds__(256) deriv_sum_kernel(DerivJobs jobs, const double* __restrict__ work,
-                                                        int64_t nvox) {
-  const DerivJob& jb = jobs.j[blockIdx.y];
+template <int N>
+__global__ void __launch_bounds__(256) deriv_tile_kernel(DerivJobs jobs, DerivMats dm, double* __restrict__ work) {
+  __shared__ double sS[N * DT_LD];
+  deriv_tile_body<N>(jobs, dm, work, blockIdx.y, blockIdx.x, true, sS, threadIdx.x);
+}
+
+// dst = term 0 + term 1 (+ term 2) for every multi-term job, in that order:
+// job jo, workgroup bx of gx (256 threads).
+__device__ __forceinline__ void deriv_sum_body(const DerivJobs& jobs, const double* __restrict__ work, int64_t nvox,
+                                               int bx, int gx, int jo) {
+  const DerivJob& jb = jobs.j[jo];
   if (jb.nterm < 2) return;
-  const double* w0 = work + (int64_t)jobs.slot0[blockIdx.y] * nvox;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nvox; e += (int64_t)gridDim.x * blockDim.x) {
+  const double* w0 = work + (int64_t)jobs.slot0[jo] * nvox;
+  for (int64_t e = bx * (int64_t)256 + threadIdx.x; e < nvox; e += (int64_t)gx * 256) {
     double v = w0[e] + w0[nvox + e];
     if (jb.nterm == 3) v += w0[2 * nvox + e];
     jb.dst[e] = v;
   }
+}
+
+__global__ void __launch_bounds__(256) deriv_sum_kernel(DerivJobs jobs, const double* __restrict__ work,
+                                                        int64_t nvox) {
+  deriv_sum_body(jobs, work, nvox, blockIdx.x, gridDim.x, blockIdx.y);
 }
 
 struct SkyArgs {
@@ -372,22 +392,31 @@ int deriv_slots(const DerivJobs& J) {
   return n;
 }
 
+// The term tables of a job list for deriv_tile_body: returns the number of
+// terms; *nslot receives the workspace slots of the multi-term jobs.
+int prepare_terms(DerivJobs& J, int* nslot) {
+  int nt = 0, slot = 0;
+  for (int j = 0; j < J.njob; ++j) {
+    const bool multi = J.j[j].nterm > 1;
+    J.slot0[j] = multi ? slot : -1;
+    for (int t = 0; t < J.j[j].nterm; ++t) {
+      J.tjob[nt] = j;
+      J.tidx[nt] = t;
+      J.tslot[nt] = multi ? slot++ : -1;
+      ++nt;
+    }
+  }
+  J.nterm_all = nt;
+  *nslot = slot;
+  return nt;
+}
+
 int run_jobs(const hfb_ctx* c, DerivJobs& J, double* work, hipStream_t st) {
   if (c->nx == c->ny && c->ny == c->nz) {
     if (!work && deriv_slots(J) > 0) return HFB_E_ARG;
     DerivMats dm{{c->der1[0], c->der1[1], c->der1[2]}, {c->der2[0], c->der2[1], c->der2[2]}};
-    int nt = 0, slot = 0;
-    for (int j = 0; j < J.njob; ++j) {
-      const bool multi = J.j[j].nterm > 1;
-      J.slot0[j] = multi ? slot : -1;
-      for (int t = 0; t < J.j[j].nterm; ++t) {
-        J.tjob[nt] = j;
-        J.tidx[nt] = t;
-        J.tslot[nt] = multi ? slot++ : -1;
-        ++nt;
-      }
-    }
-    J.nterm_all = nt;
+    int slot = 0;
+    const int nt = prepare_terms(J, &slot);
     const int n = c->nx;
     const int64_t nvox = (int64_t)n * n * n;
     HFB_DISPATCH_N(n, {
@@ -420,13 +449,14 @@ extern "C" long long hfb_deriv_workspace(hfb_ctx* c) {
   return 36LL * c->nx * c->ny * c->nz;  // the 36 terms of the multi-term jobs of hfb_density_derivs
 }
 
-extern "C" int hfb_density_derivs(hfb_ctx* c, const double* rho, const double* sdens, const double* current,
-                                  const double* sodens, double* divj, double* lap, double* grad, double* curls,
-                                  double* curlj, double* work, void* stream) {
-  if (!c) return HFB_E_NOCTX;
-  if (!rho || !sdens || !current || !sodens || !divj || !lap || !grad || !curls || !curlj) return HFB_E_ARG;
+namespace {
+
+// The derived-field jobs of hfb_density_derivs.
+void density_jobs(const hfb_ctx* c, const double* rho, const double* sdens, const double* current,
+                  const double* sodens, double* divj, double* lap, double* grad, double* curls, double* curlj,
+                  DerivJobs& J) {
   const int64_t n = (int64_t)c->nx * c->ny * c->nz;
-  DerivJobs J{};
+  J = DerivJobs{};
   for (int iq = 0; iq < 2; ++iq) {
     const double* r = rho + iq * n;
     const double* so = sodens + iq * 3 * n;
@@ -443,7 +473,110 @@ extern "C" int hfb_density_derivs(hfb_ctx* c, const double* rho, const double* s
     push(J, curlj + (iq * 3 + 1) * n, 2, cu + 0 * n, 2, 1, 1.0, cu + 2 * n, 0, 1, -1.0);
     push(J, curlj + (iq * 3 + 2) * n, 2, cu + 1 * n, 0, 1, 1.0, cu + 0 * n, 1, 1, -1.0);
   }
+}
+
+// Density derivatives and the Coulomb solve in four launches (cubic grid
+// of edge N, doubled grid NP = 2N): the derivative tiles, the forward
+// Poisson planes, then the sums of the multi-term fields with the x-pencil
+// kernel product (workgroups [0, nbp)), then the inverse planes.  (The
+// tiles inside the plane launch ran at one 512-thread workgroup per CU,
+// limited by the plane kernel's registers and LDS: slower than apart.)
+template <int NP>
+__global__ void __launch_bounds__(NP * PP_L) derivs_pois_fwd_kernel(cx<double>* __restrict__ pbuf,
+                                                                     const double* __restrict__ rho_p, int nx, int ny,
+                                                                     int nz, const cx<double>* __restrict__ tw2) {
+  extern __shared__ unsigned char dp_lds[];
+  pois_fwd_body<NP>(pbuf, rho_p, nx, ny, nz, tw2, blockIdx.x, (cx<double>*)dp_lds);
+}
+
+template <int NP>
+__global__ void __launch_bounds__(256) derivs_pois_mul_kernel(DerivJobs jobs, const double* __restrict__ work,
+                                                              int64_t nvox, int gx, int nbp,
+                                                              cx<double>* __restrict__ pbuf,
+                                                              const cx<double>* __restrict__ qhat, double scale,
+                                                              int nx, const cx<double>* __restrict__ tw2) {
+  if ((int)blockIdx.x < nbp) {
+    const int lane = threadIdx.x & 63;
+    const int g = ((int)blockIdx.x * 4 + (threadIdx.x >> 6)) * PP_PW + lane / PP_L;
+    pois_mul_body<NP>(pbuf, qhat, scale, nx, tw2, g, lane % PP_L);
+    return;
+  }
+  const int b = (int)blockIdx.x - nbp;
+  deriv_sum_body(jobs, work, nvox, b % gx, gx, b / gx);
+}
+
+template <int NP>
+__global__ void __launch_bounds__(NP * PP_L) derivs_pois_inv_kernel(const cx<double>* __restrict__ pbuf,
+                                                                     double* __restrict__ wcoul, double scale, int ny,
+                                                                     int nz, const cx<double>* __restrict__ tw2) {
+  extern __shared__ unsigned char dp_lds[];
+  pois_inv_body<NP>(pbuf, wcoul, scale, ny, nz, tw2, blockIdx.x, (cx<double>*)dp_lds);
+}
+
+template <int N, int NP>
+int derivs_coulomb_fused(const hfb_ctx* c, DerivJobs& J, double* work, const double* rho_p, double* wcoul, double e2,
+                         cx<double>* pbuf, hipStream_t st) {
+  const cx<double>* tw2 = twiddles<double>(c, NP);
+  if (!tw2) return HFB_E_SIZE;
+  DerivMats dm{{c->der1[0], c->der1[1], c->der1[2]}, {c->der2[0], c->der2[1], c->der2[2]}};
+  int nslot = 0;
+  const int nt = prepare_terms(J, &nslot);
+  hipLaunchKernelGGL(deriv_tile_kernel<N>, dim3(N * N / DT_P, nt), dim3(256), 0, st, J, dm, work);
+  int rc = launch_status();
+  if (rc != HFB_OK) return rc;
+  const size_t lds = sizeof(cx<double>) * (size_t)NP * (NP + 1);
+  allow_full_lds((const void*)derivs_pois_fwd_kernel<NP>);
+  allow_full_lds((const void*)derivs_pois_inv_kernel<NP>);
+  hipLaunchKernelGGL(derivs_pois_fwd_kernel<NP>, dim3(c->nx), dim3(NP * PP_L), lds, st, pbuf, rho_p, c->nx, c->ny,
+                     c->nz, tw2);
+  rc = launch_status();
+  if (rc != HFB_OK) return rc;
+  const int64_t nvox = (int64_t)N * N * N;
+  int gx = (int)((nvox + 255) / 256);
+  if (gx > 256) gx = 256;
+  const int nbp = NP * NP / (4 * PP_PW);
+  const int nsum = nslot > 0 ? gx * J.njob : 0;
+  hipLaunchKernelGGL(derivs_pois_mul_kernel<NP>, dim3(nbp + nsum), dim3(256), 0, st, J, (const double*)work, nvox, gx,
+                     nbp, pbuf, c->qhat, e2 * c->dx * c->dy * c->dz, c->nx, tw2);
+  rc = launch_status();
+  if (rc != HFB_OK) return rc;
+  hipLaunchKernelGGL(derivs_pois_inv_kernel<NP>, dim3(c->nx), dim3(NP * PP_L), lds, st, (const cx<double>*)pbuf, wcoul,
+                     1.0 / ((double)NP * NP * NP), c->ny, c->nz, tw2);
+  return launch_status();
+}
+
+}  // namespace
+
+extern "C" int hfb_density_derivs(hfb_ctx* c, const double* rho, const double* sdens, const double* current,
+                                  const double* sodens, double* divj, double* lap, double* grad, double* curls,
+                                  double* curlj, double* work, void* stream) {
+  if (!c) return HFB_E_NOCTX;
+  if (!rho || !sdens || !current || !sodens || !divj || !lap || !grad || !curls || !curlj) return HFB_E_ARG;
+  DerivJobs J;
+  density_jobs(c, rho, sdens, current, sodens, divj, lap, grad, curls, curlj, J);
   return run_jobs(c, J, work, (hipStream_t)stream);
+}
+
+extern "C" int hfb_density_derivs_coulomb(hfb_ctx* c, const double* rho, const double* sdens, const double* current,
+                                          const double* sodens, double* divj, double* lap, double* grad,
+                                          double* curls, double* curlj, double* work, double e2, double* wcoul,
+                                          void* pwork, void* stream) {
+  if (!c) return HFB_E_NOCTX;
+  if (!rho || !sdens || !current || !sodens || !divj || !lap || !grad || !curls || !curlj || !work || !wcoul ||
+      !pwork)
+    return HFB_E_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  DerivJobs J;
+  density_jobs(c, rho, sdens, current, sodens, divj, lap, grad, curls, curlj, J);
+  const bool cubic = c->nx == c->ny && c->ny == c->nz;
+  if (cubic && pois_planes_apply(c) && c->nx2 == 2 * c->nx) {
+    const int64_t nvox = (int64_t)c->nx * c->ny * c->nz;
+    if (c->nx == 32) return derivs_coulomb_fused<32, 64>(c, J, work, rho + nvox, wcoul, e2, (cx<double>*)pwork, st);
+    if (c->nx == 48) return derivs_coulomb_fused<48, 96>(c, J, work, rho + nvox, wcoul, e2, (cx<double>*)pwork, st);
+  }
+  const int rc = run_jobs(c, J, work, st);
+  if (rc != HFB_OK) return rc;
+  return hfb_poisson(c, rho, wcoul, e2, pwork, stream);
 }
 
 // divaq = div aq, dbmass = grad bmass (meanfield.py:329-344); computed for

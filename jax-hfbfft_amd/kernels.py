@@ -436,6 +436,17 @@ def density_derivs(ctx, dens, der):
                                    ptr(der["curlj"]), ptr(_deriv_work(ctx)), stream_handle()), "density_derivs")
 
 
+def density_derivs_coulomb(ctx, dens, der, e2, wcoul):
+    """density_derivs and poisson(rho, e2, wcoul) in one call (fewer launches)."""
+    need(dens.rho, torch.float64, (2, ctx.nx, ctx.ny, ctx.nz), "rho")
+    need(wcoul, torch.float64, (ctx.nx, ctx.ny, ctx.nz), "wcoul")
+    pwork = ctx.work("poisson", tuple(ctx.n2), torch.complex128)
+    check(lib().hfb_density_derivs_coulomb(ctx.handle, ptr(dens.rho), ptr(dens.sdens), ptr(dens.current),
+                                           ptr(dens.sodens), ptr(der["divj"]), ptr(der["lap"]), ptr(der["grad"]),
+                                           ptr(der["curls"]), ptr(der["curlj"]), ptr(_deriv_work(ctx)), float(e2),
+                                           ptr(wcoul), ptr(pwork), stream_handle()), "density_derivs_coulomb")
+
+
 def meanfield_derivs(ctx, mf):
     check(lib().hfb_meanfield_derivs(ctx.handle, ptr(mf.aq), ptr(mf.bmass), ptr(mf.divaq_buf), ptr(mf.dbmass_buf),
                                      ptr(_deriv_work(ctx)), stream_handle()), "meanfield_derivs")

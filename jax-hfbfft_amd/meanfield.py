@@ -104,14 +104,22 @@ class DensityDerivs:
         return getattr(self, k)
 
 
-def density_derivs(grids, densities, out=None) -> DensityDerivs:
+def density_derivs(grids, densities, out=None, coulomb=None, e2=None) -> DensityDerivs:
+    """The derived density fields; with ``coulomb`` also its Coulomb
+    potential of these densities into coulomb.wcoul (coulomb.py:54-75), in
+    one call that shares the launches."""
     ctx = grids.ctx
     if out is None:
         dev = densities.rho.device
         s4 = (2, grids.nx, grids.ny, grids.nz)
         s5 = (2, 3, grids.nx, grids.ny, grids.nz)
         out = DensityDerivs(*(torch.empty(s, dtype=torch.float64, device=dev) for s in (s4, s4, s5, s5, s5)))
-    kernels.density_derivs(ctx, densities, out)
+    if coulomb is None:
+        kernels.density_derivs(ctx, densities, out)
+        return out
+    if coulomb.wcoul is None:
+        coulomb.wcoul = torch.empty((grids.nx, grids.ny, grids.nz), dtype=torch.float64, device=densities.rho.device)
+    kernels.density_derivs_coulomb(ctx, densities, out, e2, coulomb.wcoul)
     return out
 
 

@@ -296,12 +296,13 @@ def iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, m
     # (sp_properties, static.py:1831, is finished in the same launch)
     _, spart = add_density_sweep(densities, grids, levels, mix=True, addnew=0.2, moment=moment, want_sp=True,
                                  sp_h2m=forces.h2m)
-    derivs = density_derivs(grids, densities, out=derivs)
+    # the derived fields and the Coulomb potential share their launches
+    derivs = density_derivs(grids, densities, out=derivs, coulomb=coulomb if params.tcoul else None, e2=params.e2)
     # skyrme, sp_properties, integ_energy and sum_energy (static.py:1810-1833)
     # with the scalar finishing steps of the first, third and fourth in one
     # launch at the end
     skyrme(coulomb, densities, forces, grids, meanfield, params, static, derivs=derivs, E=energies.vec,
-           mass_number=levels.mass_number, finish=False, with_energy=True)
+           mass_number=levels.mass_number, finish=False, with_energy=True, wcoul_ready=bool(params.tcoul))
     if static.tsort:
         raise NotImplementedError("sort_states is outside the static HF+BCS path")
     # integ_energy's integrals come from the same launch as the potentials;

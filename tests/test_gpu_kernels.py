@@ -139,6 +139,40 @@ def test_poisson_matches_oracle(pkg, O, dev, n):
     assert relerr(out.cpu().numpy(), ref) < 1e-12
 
 
+@pytest.mark.parametrize("n", [24, 32, 48])
+def test_derivs_coulomb_matches_separate(pkg, O, dev, n):
+    """The combined derivative + Coulomb call (fused launches on 32^3 and
+    48^3) gives the separate calls' fields bit for bit, and the Coulomb
+    potential of the oracle."""
+    from types import SimpleNamespace
+    from jax_hfbfft_amd import _lib
+    d = 0.8 if n == 48 else 1.0
+    g = O.init_grids(n, n, n, d, d, d)
+    coul = O.init_coulomb(g)
+    ctx = _lib.Context(n, n, n, d, d, d)
+    rng = np.random.default_rng(n)
+    X, Y, Z = np.meshgrid(g.x, g.y, g.z, indexing="ij")
+    rho = np.stack([0.08 * np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 8.0),
+                    0.07 * np.exp(-((X - 0.5) ** 2 + Y ** 2 + (Z + 0.3) ** 2) / 9.0)])
+    T = lambda a: torch.tensor(a, dtype=torch.float64, device=dev)
+    dens = SimpleNamespace(rho=T(rho), sdens=T(rng.standard_normal((2, 3, n, n, n))),
+                           current=T(rng.standard_normal((2, 3, n, n, n))),
+                           sodens=T(rng.standard_normal((2, 3, n, n, n))))
+    s4, s5 = (2, n, n, n), (2, 3, n, n, n)
+    mk = lambda: {k: torch.empty(s, dtype=torch.float64, device=dev)
+                  for k, s in (("divj", s4), ("lap", s4), ("grad", s5), ("curls", s5), ("curlj", s5))}
+    a, b = mk(), mk()
+    pkg.kernels.density_derivs(ctx, dens, a)
+    wa = pkg.kernels.poisson(ctx, dens.rho, O.E2)
+    wb = torch.empty((n, n, n), dtype=torch.float64, device=dev)
+    pkg.kernels.density_derivs_coulomb(ctx, dens, b, O.E2, wb)
+    torch.cuda.synchronize()
+    for k in a:
+        assert torch.equal(a[k], b[k]), k
+    assert torch.equal(wa, wb)
+    assert relerr(wb.cpu().numpy(), O.poisson(g, coul, rho)) < 1e-12
+
+
 @pytest.mark.parametrize("nsts", [(8, 8), (20, 20), (21, 7), (36, 30), (40, 0), (64, 50), (126, 82)])
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
 def test_gram2_matches_matmul(pkg, dev, nsts, dtype):
