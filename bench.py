@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=3, help="oracle iterations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch kernels from Python instead of HIP graphs")
+    ap.add_argument("--group", type=int, default=10,
+                    help="timed iterations replayed per HIP graph chain (1: one graph per iteration)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -168,6 +170,7 @@ def main():
             # every iteration replays a HIP graph captured ahead of time
             r.enable_graphs(True)
             r.precapture(range(1, (args.warmup + args.steps if prec == "mixed" else total) + 1))
+            r.precapture_groups(args.warmup + 1, args.steps, args.group)
         runs[prec] = r
     # warmup (untimed) -- iterations 1..W
     for prec, r in runs.items():
@@ -178,7 +181,7 @@ def main():
     for prec, r in runs.items():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        r.run(args.steps, start=args.warmup + 1, check_every=0)
+        r.run(args.steps, start=args.warmup + 1, check_every=0, group=1 if args.eager else args.group)
         torch.cuda.synchronize()
         timing[prec] = time.perf_counter() - t0
 
@@ -217,7 +220,7 @@ def main():
         "config": {"workload": f"{args.workload} {cfg['n']}^3 dx={cfg['d']} {cfg['name']} "
                                f"{cfg['nneut']}N+{cfg['nprot']}Z states, HO start b={cfg['b']} + noise 1e-3",
                    "states": S, "grid": cfg["n"], "refine_iters": args.refine,
-                   "launch": "eager" if args.eager else "hip-graph replay"},
+                   "launch": "eager" if args.eager else f"hip-graph replay, {args.group} iterations per graph"},
         "fp64_iter_s": f64_ips, "speedup_vs_fp64": mixed_ips / f64_ips,
         "eb_err_kev": err_kev, "ehf_mixed": e_mix["ehf"], "ehf_f64": e_64["ehf"],
         "hbm_frac_iter_mixed": iter_bytes(S, nvox, 8, 8) * mixed_ips / HBM_PEAK,

@@ -193,22 +193,23 @@ class HFRun:
         self.graphs_enabled = on
         self.capture_first = capture_first
         if not hasattr(self, "_graphs"):
-            self._graphs, self._seen = {}, set()
+            self._graphs, self._seen, self._groups_n = {}, set(), {}
             self._pool = torch.cuda.graph_pool_handle()
 
     def precapture(self, iterations):
         """Capture the graphs of the given iteration numbers ahead of time
         (capture records kernels without running them, so no state changes)."""
         saved = (self.params.iteration, self.forces.tbcs, self.static.tdiag)
-        tail = (self.energies.tail, self.energies.stale)
+        tail = (self.energies.tail, self.energies.stale, self.energies._tail_ctx)
         for i in iterations:
             sig = self.signature(i)
             if sig not in self._graphs:
                 self._graphs[sig] = self._capture(i)
         self.params.iteration, self.forces.tbcs, self.static.tdiag = saved
-        self.energies.tail, self.energies.stale = tail
+        self.energies.tail, self.energies.stale, self.energies._tail_ctx = tail
 
-    def _capture(self, i):
+    def _capture(self, i, count=1):
+        """Capture iterations i .. i+count-1 into one chain of HIP graphs."""
         from . import kernels
         torch.cuda.synchronize()
         side = torch.cuda.Stream()
@@ -217,7 +218,8 @@ class HFRun:
             seg = _Segmenter(self._pool)
             kernels.SEGMENTER = seg
             try:
-                _, self.derivs = st_mod.iterate(i, *self.state, self.tbcssav, self.derivs, check=False)
+                for j in range(i, i + count):
+                    _, self.derivs = st_mod.iterate(j, *self.state, self.tbcssav, self.derivs, check=False)
             finally:
                 kernels.SEGMENTER = None
                 seg.close()
@@ -226,15 +228,58 @@ class HFRun:
         torch.cuda.synchronize()
         return seg
 
-    def run(self, maxiter, start=1, check_every=10, callback=None):
-        for i in range(start, start + maxiter):
+    def _group_key(self, i, count):
+        return tuple(self.signature(j) for j in range(i, i + count))
+
+    def precapture_groups(self, start, count, group):
+        """Capture the graphs that run(count, start, check_every=0,
+        group=group) replays: runs of `group` consecutive iterations, one
+        graph chain each (fewer graph launches; the same kernels)."""
+        if group <= 1:
+            return
+        saved = (self.params.iteration, self.forces.tbcs, self.static.tdiag)
+        tail = (self.energies.tail, self.energies.stale, self.energies._tail_ctx)
+        if self.energies.tail is None:  # the state every iteration after the first starts in
+            from . import kernels
+            ctx = self.grids.ctx
+            self.energies.defer_tail(ctx, kernels.tail_job(ctx, self.forces.to_c(self.params, self.levels.mass_number),
+                                                           self.densities, self.levels, self.energies.vec))
+        for i in range(max(start, 2), start + count - group + 1, group):
+            key = self._group_key(i, group)
+            if key not in self._groups_n:
+                self._groups_n[key] = self._capture(i, group)
+        self.params.iteration, self.forces.tbcs, self.static.tdiag = saved
+        self.energies.tail, self.energies.stale, self.energies._tail_ctx = tail
+
+    def run(self, maxiter, start=1, check_every=10, callback=None, group=1):
+        """Iterations start .. start+maxiter-1.  With graphs enabled,
+        check_every == 0 and no callback, runs of `group` iterations whose
+        graph chain was captured (precapture_groups) replay as one."""
+        i, end = start, start + maxiter
+        while i < end:
+            if (group > 1 and self.graphs_enabled and check_every == 0 and callback is None and i + group <= end
+                    and self.energies.tail is not None):
+                g = self._groups_n.get(self._group_key(i, group))
+                if g is not None:
+                    last = i + group - 1
+                    sig = self.signature(last)
+                    self.iteration_no = last
+                    self.params.iteration = last
+                    self.forces.tbcs = sig[0]
+                    self.static.tdiag = sig[1]
+                    for graph in g.graphs:
+                        graph.replay()
+                    self.energies.defer_tail(*g.tail)
+                    i += group
+                    continue
             chk = check_every > 0 and (i % check_every == 0)
             conv = self.iteration(i, check=chk)
             if callback is not None:
                 callback(i, self)
             if conv:
                 return i
-        return start + maxiter - 1
+            i += 1
+        return end - 1
 
     def promote(self):
         """Switch a mixed run to complex128 states and FP64 kernels."""
@@ -246,8 +291,9 @@ class HFRun:
         if hasattr(self, "_graphs"):  # captured graphs refer to the old buffers
             # retired graphs stay referenced: dropping every graph of a pool
             # would release the pool that later captures still use
-            self._retired = getattr(self, "_retired", []) + list(self._graphs.values())
-            self._graphs, self._seen = {}, set()
+            self._retired = getattr(self, "_retired", []) + list(self._graphs.values()) + list(
+                self._groups_n.values())
+            self._graphs, self._seen, self._groups_n = {}, set(), {}
 
     def energies_host(self):
         torch.cuda.synchronize()
