@@ -204,7 +204,10 @@ def main():
     mixed_ips = args.steps / timing["mixed"]
     f64_ips = args.steps / timing["f64"]
     ach = hpsi_bytes(S, nvox, 8, 4) / (burst_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": "hpsi (hpsi_mfma_x_kernel x pass on the matrix cores + hpsi_plane_kernel y-z, FP32)", "achieved": ach,
+    g = runs["f64"].grids
+    xpass = ("hpsi_mfma_x_kernel x pass on the matrix cores" if g.nx == g.ny == g.nz == 32
+             else "hpsi_lane_kernel x pass")
+    roof = {"bound": "hbm", "kernel": f"hpsi ({xpass} + hpsi_plane_kernel y-z, FP32)", "achieved": ach,
             "peak": 8000.0,
             "unit": "GB/s", "frac": (ach / 8000.0) if ach else None, "traffic": measured_traffic(args.workload),
             "avg_launch_ms": burst_ms, "launches": 20,
