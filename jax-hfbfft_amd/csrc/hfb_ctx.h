@@ -59,16 +59,19 @@ inline double axis_d(const hfb_ctx* c, int axis) { return axis == 0 ? c->dx : (a
 // last launch status, mapped to the library's codes
 inline int launch_status() { return hip_status(hipGetLastError()); }
 
-// Allow a kernel up to 152 of the 160 KB of LDS as dynamic shared memory; the
-// attribute is set once per kernel (outside any stream capture in practice,
-// since the first call of every kernel is an eager one).
+// Allow a kernel the 160 KB of LDS less its static LDS as dynamic shared
+// memory; the attribute is set once per kernel (outside any stream capture
+// in practice, since the first call of every kernel is an eager one).
 inline void allow_full_lds(const void* kern) {
-  static const void* done[64];
+  static const void* done[128];
   static int ndone = 0;
   for (int i = 0; i < ndone; ++i)
     if (done[i] == kern) return;
-  hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 152 * 1024);  // static LDS fits in the rest
-  if (ndone < 64) done[ndone++] = kern;
+  hipFuncAttributes fa{};
+  size_t stat = 8 * 1024;
+  if (hipFuncGetAttributes(&fa, kern) == hipSuccess) stat = fa.sharedSizeBytes;
+  hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(160 * 1024 - stat));
+  if (ndone < 128) done[ndone++] = kern;
 }
 
 }  // namespace hfb

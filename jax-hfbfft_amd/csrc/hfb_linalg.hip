@@ -203,9 +203,10 @@ __device__ __forceinline__ void mm(cx<double>* __restrict__ C, const cx<double>*
 }
 
 struct JacobiShared {
-  int p[NMAXJ / 2], q[NMAXJ / 2];
-  double c[NMAXJ / 2], s[NMAXJ / 2];
-  cx<double> e[NMAXJ / 2];
+  // rotations of two consecutive rounds (buffer t & 1)
+  int p[2][NMAXJ / 2], q[2][NMAXJ / 2];
+  double c[2][NMAXJ / 2], s[2][NMAXJ / 2];
+  cx<double> e[2][NMAXJ / 2];
   double red[32];
   int perm[NMAXJ];
   double ev[NMAXJ];
@@ -239,7 +240,8 @@ __device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* A2, cx<dou
     }
     block_sum2_all<NT>(off, dia, js.red);
     if (off <= 1e-30 * dia || off == 0.0) break;
-    for (int t = 0; t < m - 1; ++t) {
+    // rotation angles of round t into buffer b (threads < npair)
+    auto rotations = [&](int t, int b) {
       if (tid < npair) {
         // round-robin pairs: index 0 fixed, the others rotate by t
         const int k = tid;
@@ -270,63 +272,70 @@ __device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* A2, cx<dou
         } else {
           q = -1;  // p has no partner this round (odd n)
         }
-        js.p[k] = p;
-        js.q[k] = q;
-        js.c[k] = c;
-        js.s[k] = sn;
-        js.e[k] = ea;
+        js.p[b][k] = p;
+        js.q[b][k] = q;
+        js.c[b][k] = c;
+        js.s[b][k] = sn;
+        js.e[b][k] = ea;
       }
-      __syncthreads();
+    };
+    rotations(0, 0);
+    __syncthreads();
+    for (int t = 0; t < m - 1; ++t) {
+      const int b = t & 1;
+      // every 2x2 block (pair ka, pair kb) of R^H A R into A2
       const int nblk = npair * npair;
-      for (int it = tid; it < nblk + n * npair; it += NT) {
-        if (it < nblk) {  // block (a, b) of R^H A R into A2
-          const int ka = it / npair, kb = it % npair;
-          const int pa = js.p[ka], qa = js.q[ka], pb = js.p[kb], qb = js.q[kb];
-          const double ca = js.c[ka], sa = js.s[ka], cb = js.c[kb], sb = js.s[kb];
-          const cx<double> eca = cscale(js.e[ka], ca), esa = cscale(js.e[ka], sa);
-          const cx<double> ecb = cscale(js.e[kb], cb), esb = cscale(js.e[kb], sb);
-          const cx<double> z = {0.0, 0.0};
-          // rows pa, qa of A, columns pb, qb (absent partners read as zero)
-          const cx<double> x00 = A[pa * n + pb];
-          const cx<double> x01 = qb >= 0 ? A[pa * n + qb] : z;
-          const cx<double> x10 = qa >= 0 ? A[qa * n + pb] : z;
-          const cx<double> x11 = qa >= 0 && qb >= 0 ? A[qa * n + qb] : z;
-          // columns: [xp xq] R_b
-          const cx<double> y00 = csub(cmul(x00, ecb), cscale(x01, sb));
-          const cx<double> y01 = cadd(cmul(x00, esb), cscale(x01, cb));
-          const cx<double> y10 = csub(cmul(x10, ecb), cscale(x11, sb));
-          const cx<double> y11 = cadd(cmul(x10, esb), cscale(x11, cb));
-          // rows: R_a^H
-          cx<double> n00 = csub(cmul(cconj(eca), y00), cscale(y10, sa));
-          cx<double> n01 = csub(cmul(cconj(eca), y01), cscale(y11, sa));
-          cx<double> n10 = cadd(cmul(cconj(esa), y00), cscale(y10, ca));
-          cx<double> n11 = cadd(cmul(cconj(esa), y01), cscale(y11, ca));
-          if (ka == kb && qa >= 0 && sa != 0.0) {  // the rotated pair: diagonal 2x2 block
-            n01 = z;
-            n10 = z;
-            n00.y = 0.0;
-            n11.y = 0.0;
-          }
-          A2[pa * n + pb] = n00;
-          if (qb >= 0) A2[pa * n + qb] = n01;
-          if (qa >= 0) A2[qa * n + pb] = n10;
-          if (qa >= 0 && qb >= 0) A2[qa * n + qb] = n11;
-        } else {  // V <- V R, row r, pair k
-          const int e2 = it - nblk;
-          const int r = e2 / npair, k = e2 % npair;
-          const int p = js.p[k], q = js.q[k];
-          if (q >= 0 && js.s[k] != 0.0) {
-            const cx<double> xp = V[r * n + p], xq = V[r * n + q];
-            const cx<double> ec = cscale(js.e[k], js.c[k]), es = cscale(js.e[k], js.s[k]);
-            V[r * n + p] = csub(cmul(xp, ec), cscale(xq, js.s[k]));
-            V[r * n + q] = cadd(cmul(xp, es), cscale(xq, js.c[k]));
-          }
+      for (int it = tid; it < nblk; it += NT) {
+        const int ka = it / npair, kb = it % npair;
+        const int pa = js.p[b][ka], qa = js.q[b][ka], pb = js.p[b][kb], qb = js.q[b][kb];
+        const double ca = js.c[b][ka], sa = js.s[b][ka], cb = js.c[b][kb], sb = js.s[b][kb];
+        const cx<double> eca = cscale(js.e[b][ka], ca), esa = cscale(js.e[b][ka], sa);
+        const cx<double> ecb = cscale(js.e[b][kb], cb), esb = cscale(js.e[b][kb], sb);
+        const cx<double> z = {0.0, 0.0};
+        // rows pa, qa of A, columns pb, qb (absent partners read as zero)
+        const cx<double> x00 = A[pa * n + pb];
+        const cx<double> x01 = qb >= 0 ? A[pa * n + qb] : z;
+        const cx<double> x10 = qa >= 0 ? A[qa * n + pb] : z;
+        const cx<double> x11 = qa >= 0 && qb >= 0 ? A[qa * n + qb] : z;
+        // columns: [xp xq] R_b
+        const cx<double> y00 = csub(cmul(x00, ecb), cscale(x01, sb));
+        const cx<double> y01 = cadd(cmul(x00, esb), cscale(x01, cb));
+        const cx<double> y10 = csub(cmul(x10, ecb), cscale(x11, sb));
+        const cx<double> y11 = cadd(cmul(x10, esb), cscale(x11, cb));
+        // rows: R_a^H
+        cx<double> n00 = csub(cmul(cconj(eca), y00), cscale(y10, sa));
+        cx<double> n01 = csub(cmul(cconj(eca), y01), cscale(y11, sa));
+        cx<double> n10 = cadd(cmul(cconj(esa), y00), cscale(y10, ca));
+        cx<double> n11 = cadd(cmul(cconj(esa), y01), cscale(y11, ca));
+        if (ka == kb && qa >= 0 && sa != 0.0) {  // the rotated pair: diagonal 2x2 block
+          n01 = z;
+          n10 = z;
+          n00.y = 0.0;
+          n11.y = 0.0;
         }
+        A2[pa * n + pb] = n00;
+        if (qb >= 0) A2[pa * n + qb] = n01;
+        if (qa >= 0) A2[qa * n + pb] = n10;
+        if (qa >= 0 && qb >= 0) A2[qa * n + qb] = n11;
       }
       __syncthreads();
       cx<double>* sw = A;
       A = A2;
       A2 = sw;
+      // the next round's angles (first wavefront) beside V <- V R of this
+      // round (the other wavefronts): different buffers, one barrier
+      if (t + 1 < m - 1) rotations(t + 1, b ^ 1);
+      for (int it = tid - 64; it >= 0 && it < n * npair; it += NT - 64) {
+        const int r = it / npair, k = it % npair;
+        const int p = js.p[b][k], q = js.q[b][k];
+        if (q >= 0 && js.s[b][k] != 0.0) {
+          const cx<double> xp = V[r * n + p], xq = V[r * n + q];
+          const cx<double> ec = cscale(js.e[b][k], js.c[b][k]), es = cscale(js.e[b][k], js.s[b][k]);
+          V[r * n + p] = csub(cmul(xp, ec), cscale(xq, js.s[b][k]));
+          V[r * n + q] = cadd(cmul(xp, es), cscale(xq, js.c[b][k]));
+        }
+      }
+      __syncthreads();
     }
   }
   for (int i = tid; i < n; i += NT) js.ev[i] = A[i * n + i].x;
