@@ -341,16 +341,19 @@ __device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* A2, cx<dou
   for (int i = tid; i < n; i += NT) js.ev[i] = A[i * n + i].x;
   __syncthreads();
   if (sort) {
-    if (tid == 0) {
-      for (int i = 0; i < n; ++i) js.perm[i] = i;
-      for (int i = 0; i < n; ++i) {  // stable selection sort
-        int b = i;
-        for (int j = i + 1; j < n; ++j)
-          if (js.ev[js.perm[j]] < js.ev[js.perm[b]]) b = j;
-        const int tmp = js.perm[b];
-        for (int j = b; j > i; --j) js.perm[j] = js.perm[j - 1];
-        js.perm[i] = tmp;
+    // stable ascending order by rank: index i goes to the number of
+    // eigenvalues below it, ties kept in index order (the identity first, so
+    // that every entry stays an index below n whatever the values)
+    for (int i = tid; i < n; i += NT) js.perm[i] = i;
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) {
+      const double e = js.ev[i];
+      int r = 0;
+      for (int j = 0; j < n; ++j) {
+        const double f = js.ev[j];
+        r += (f < e || (f == e && j < i)) ? 1 : 0;
       }
+      js.perm[r] = i;
     }
     __syncthreads();
     tile_for<NP, NT>(n, [&](int i, int j) { T[i * n + j] = V[i * n + js.perm[j]]; });
