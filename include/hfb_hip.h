@@ -250,11 +250,12 @@ int hfb_gram2_ex(int prec, const void* const* A, const void* const* B, int nm, c
  * lambda_lin; static.py:484-577 construct_hfb_matrices).  mats[iq*4 + m]:
  * m=0 S, 1 H_old, 2 lambda_lin (or null), 3 G_old (or null), each
  * nst_iq x nst_iq complex128 from hfb_gram2.  uo[iq*2] receives the
- * unitary U (real and imaginary planes for hfb_rotate), uo[iq*2+1] the
- * matrix lambda+ that maps the rotated states to lagrange.  Writes sp_norm,
+ * unitary U (nst x nst complex128, row-major: psi_new = psi_old U, the
+ * input of hfb_rotate), uo[iq*2+1] M2 = U lambda+, which maps the states
+ * before the rotation to lagrange.  Writes sp_norm,
  * sp_energy, deltaf (ipair), hmatrix, gapmatrix, symcond, lambda_save
  * (2, nmax, nmax) complex128 and efluct1q/2q (2), efluct1/2 (1).
- * nst0, nst1 <= 128 (HFB_E_SIZE above).  work: null when both blocks
+ * nst0, nst1 <= 256 (HFB_E_SIZE above).  work: null when both blocks
  * have at most 36 states (the work matrices then live in LDS), else
  * 10 * nmax^2 complex doubles.  stats (optional, 4
  * doubles): Jacobi sweeps of the lambda_lin eigensolve per isospin, then
@@ -269,6 +270,30 @@ int hfb_diag_small(hfb_ctx* c, int nst0, int nst1, int nmax, const double* const
                    double* gapmatrix, double* symcond, double* lambda_save, double* efluct1q, double* efluct2q,
                    double* efluct1, double* efluct2, double* const* uo, double* work, double* stats,
                    void* stream);
+
+/* loewdin_orthonormalize (static.py:463-482) of one block from its overlap
+ * S = wxyz psi_2d^H psi_2d (nst x nst complex128, e.g. from hfb_gram): U
+ * (nst x nst complex128, row-major, psi_new = psi_2d U) = S^{-1/2} and
+ * sp_norm = Re diag S.  Same solver as hfb_diag_small (binomial series or
+ * Newton-Schulz near the identity, else the eigen-decomposition with the
+ * reference's eigenvalue clamp).  nst <= 256; work: null up to 36 states,
+ * else 5 nst^2 complex doubles. */
+int hfb_loewdin(int nst, const double* S, double* U, double* sp_norm, double* work, void* stream);
+
+/* construct_hfb_matrices (static.py:484-577) of isospin iq (states start ..
+ * start+nst-1, already rotated by U) with the caller's unitary U and
+ * H_old = wxyz psi_old^H hmfpsi, G_old = wxyz psi_old^H delpsi (ipair;
+ * may be null otherwise), all nst x nst complex128: writes rows iq of
+ * hmatrix, gapmatrix (diagonal only when tbcs), symcond, lambda_save
+ * (2, nmax, nmax), sp_energy and deltaf (ipair) of those states,
+ * efluct1q/2q[iq], efluct1 = max and efluct2 = mean over both isospins'
+ * efluct1q/2q, and lam = lambda+ (nst x nst) so that lagrange = psi lam
+ * (e.g. hfb_rotate).  nst <= 256; work as for hfb_loewdin. */
+int hfb_hfb_matrices(hfb_ctx* c, int iq, int start, int nst, int nmax, const double* U, const double* Hold,
+                     const double* Gold, int ipair, int tbcs, const double* wocc, const double* wstates,
+                     const double* wguv, const double* pairwg, double* sp_energy, double* deltaf, double* hmatrix,
+                     double* gapmatrix, double* symcond, double* lambda_save, double* efluct1q, double* efluct2q,
+                     double* efluct1, double* efluct2, double* lam, double* work, void* stream);
 
 /* out1 = psi_2d U1 and out2 = psi_2d U2 over the nst states of one block
  * (static.py:316 psi_2d @ unitary; static.py:557 lagrange = psi_new @
@@ -422,8 +447,9 @@ int hfb_sum_energy(int nstates, const double* wocc, const double* wstates, const
  * pairs.py:47-61; here a bisection to the same 1e-14 tolerance), v2, uv,
  * wocc, soft cutoffs, and eferm, epair, avdelt, avdeltv2, avg (2) into
  * their arrays and epair into E[HFB_EN_EPAIR..+1] (pairs.py:66-131).
- * status[iq] != 0 flags a bracket without a sign change (the reference's
- * brentq ValueError). */
+ * status[0] is set to 1 when the bracket of either isospin has no sign
+ * change (the reference's brentq ValueError); it is never cleared here.
+ * Blocks of any size (levels staged in LDS up to 512 states). */
 int hfb_bcs(int nst0, int nst1, double nneut, double nprot, const double* pair_cutoff,
             const double* state_cutoff, double softcut, double const_gap, const double* sp_energy,
             double* deltaf, double* wocc, double* wguv, double* pairwg, double* wstates, double* eferm,

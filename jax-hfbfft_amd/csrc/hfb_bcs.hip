@@ -16,6 +16,8 @@ using namespace hfb;
 
 namespace {
 
+constexpr int BCS_LDS = 512;  // largest block whose levels are staged in LDS
+
 struct BcsIso {
   int start, nst;
   double target;      // particle number N or Z
@@ -58,14 +60,22 @@ __global__ void __launch_bounds__(256) bcs_kernel(BcsArgs a) {
   const BcsIso b = a.iso[iq];
   const int n = b.nst;
   const int tid = threadIdx.x;
-  __shared__ double e[64], dl[64], ws[64];
+  // the levels of a block up to BCS_LDS states are staged in LDS, larger
+  // blocks are read from global memory (L2) by every trial evaluation
+  __shared__ double e_l[BCS_LDS], dl_l[BCS_LDS], ws_l[BCS_LDS];
   __shared__ double fval[256];
   __shared__ double lo, hi, flo;
   __shared__ int bad;
-  for (int k = tid; k < n; k += blockDim.x) {
-    e[k] = a.sp_energy[b.start + k];
-    dl[k] = a.deltaf[b.start + k];
-    ws[k] = a.wstates[b.start + k];
+  const bool staged = n <= BCS_LDS;
+  const double* e = staged ? e_l : a.sp_energy + b.start;
+  const double* dl = staged ? dl_l : a.deltaf + b.start;
+  const double* ws = staged ? ws_l : a.wstates + b.start;
+  if (staged) {
+    for (int k = tid; k < n; k += blockDim.x) {
+      e_l[k] = a.sp_energy[b.start + k];
+      dl_l[k] = a.deltaf[b.start + k];
+      ws_l[k] = a.wstates[b.start + k];
+    }
   }
   if (tid == 0) {
     lo = -100.0;
@@ -114,7 +124,7 @@ __global__ void __launch_bounds__(256) bcs_kernel(BcsArgs a) {
   __shared__ double ef;
   if (tid == 0) {
     ef = 0.5 * (lo + hi);
-    if (bad) a.status[0] = 1;
+    if (bad) a.status[0] = 1;  // either isospin
   }
   __syncthreads();
   const double lam = ef;
@@ -168,7 +178,7 @@ extern "C" int hfb_bcs(int nst0, int nst1, double nneut, double nprot, const dou
                        double* deltaf, double* wocc, double* wguv, double* pairwg, double* wstates, double* eferm,
                        double* epair, double* avdelt, double* avdeltv2, double* avg, double* E, int* status,
                        void* stream) {
-  if (nst0 < 0 || nst1 < 0 || nst0 > 64 || nst1 > 64) return HFB_E_SIZE;
+  if (nst0 < 0 || nst1 < 0) return HFB_E_SIZE;
   if (!pair_cutoff || !state_cutoff || !sp_energy || !deltaf || !wocc || !wguv || !pairwg || !wstates || !eferm ||
       !epair || !avdelt || !avdeltv2 || !avg || !status)
     return HFB_E_ARG;

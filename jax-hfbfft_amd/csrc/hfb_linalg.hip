@@ -17,7 +17,7 @@ using namespace hfb;
 
 namespace {
 
-constexpr int NMAXJ = 128;
+constexpr int NMAXJ = 256;
 
 struct DiagIso {
   int nst, start;
@@ -51,7 +51,10 @@ struct DiagArgs {
   unsigned int* sync;  // arrival counter of the two isospin blocks
   double* stats;  // optional: [Jacobi sweeps (2), Newton-Schulz iterations (2, -1 = fell back)]
   int tdiag, ipair, tbcs, construct;
+  int mode;  // DIAG_FULL, DIAG_LOEWDIN (S -> U only) or DIAG_GIVEN_U (construct from U in d.U)
 };
+
+enum { DIAG_FULL = 0, DIAG_LOEWDIN = 1, DIAG_GIVEN_U = 2 };
 
 
 // Work split of an n x n matrix (n <= NP) over the NT threads: thread t
@@ -369,7 +372,7 @@ __device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* A2, cx<dou
 }
 
 constexpr int NDIAG_LDS = 36;   // blocks up to this size keep the five work matrices in LDS
-constexpr int NDIAG_MAX = 128;  // largest block hfb_diag_small accepts
+constexpr int NDIAG_MAX = 256;  // largest block hfb_diag_small accepts
 
 // Inverse square root of the Hermitian positive matrix in Y (scaled copy of
 // S) by the coupled Newton-Schulz iteration Y <- Y T, Z <- T Z,
@@ -444,100 +447,107 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
   cx<double>* const B3 = B2 + n * n;
   cx<double>* const B4 = B3 + n * n;
 
-  // Loewdin S^{-1/2} (static.py:463-482)
-  for (int i = tid; i < n; i += NT) a.sp_norm[d.start + i] = d.S[i * n + i].x;
-  load_lower_herm<NP, NT>(B0, d.S, n);
-  cx<double>*Y = B0, *Z = B1, *Y2 = B3, *Z2 = B4;
-  int ns_iters = 0;
-  double cs = 1.0;
-  // Near convergence the overlap is S = I + E with a small E: the binomial
-  // series (I + E)^{-1/2} = sum_k c_k E^k, c_k = (-1)^k C(2k,k)/4^k, is then
-  // exact to round-off in fewer products than the Newton-Schulz iteration
-  // (three per step): to degree 4 for |E|_F < 1e-3 (first omitted term below
-  // 0.25e-15) in 3 products, to degree 7 for |E|_F < 1e-2 (below 0.2e-16) in
-  // 6.  Horner: T = c_{d-1} I + c_d E, then T <- c_k I + E T.  The series
-  // works on S itself; Newton-Schulz on S scaled by a Gershgorin bound.
-  bool ok;
-  {
-    double fr = 0.0, dummy = 0.0;
-    HFB_TILE_LOOP(NP, NT, n, i, j) {
-      const cx<double> v = B0[i * n + j];
-      const double e = i == j ? v.x - 1.0 : v.x;
-      fr += e * e + v.y * v.y;
-    }
-    block_sum2_all<NT>(fr, dummy, red);
-    const int deg = fr < 1e-6 ? 4 : (fr < 1e-4 ? 7 : 0);
-    if (deg > 0) {
-      constexpr double ck[8] = {1.0, -0.5, 0.375, -0.3125, 0.2734375, -0.24609375, 0.2255859375, -0.20947265625};
-      // E in B0, Horner terms in B2 (products in B3), result Z in B1
-      tile_for<NP, NT>(n, [&](int i, int j) {
-        if (i == j) B0[i * n + j].x -= 1.0;
-        const cx<double> e = B0[i * n + j];
-        B2[i * n + j] = cx<double>{(i == j ? ck[deg - 1] : 0.0) + ck[deg] * e.x, ck[deg] * e.y};
-      });
-      __syncthreads();
-      for (int k = deg - 2; k >= 0; --k) {
-        mm<NP, NT, false>(B3, B0, B2, n);
-        cx<double>* dst = k == 0 ? B1 : B2;
-        tile_for<NP, NT>(n, [&](int i, int j) {
-          const cx<double> v = B3[i * n + j];
-          dst[i * n + j] = cx<double>{v.x + (i == j ? ck[k] : 0.0), v.y};
-        });
-        __syncthreads();
-      }
-      Z = B1;
-      ok = true;
-    } else {
-      double rowsum = 0.0;  // Gershgorin bound on the largest eigenvalue
-      if (tid < n)
-        for (int j = 0; j < n; ++j) rowsum += sqrt(cabs2(B0[tid * n + j]));
-      cs = block_max_all<NT>(rowsum, red);
-      if (!(cs > 0.0)) cs = 1.0;
-      tile_for<NP, NT>(n, [&](int i, int j) { B0[i * n + j] = cscale(B0[i * n + j], 1.0 / cs); });
-      __syncthreads();
-      ok = newton_schulz<NP, NT>(Y, Z, B2, Y2, Z2, n, red, ns_iters);
-    }
-  }
   cx<double>* Urho = B4;
-  if (ok) {
-    const double f = 1.0 / sqrt(cs);
-    // Z may live in B4: the scaling is elementwise, so in place is safe
-    tile_for<NP, NT>(n, [&](int i, int j) { Urho[i * n + j] = cscale(Z[i * n + j], f); });
+  cx<double>* U = Urho;
+  if (a.mode == DIAG_GIVEN_U) {
+    // construct_hfb_matrices with the caller's unitary (static.py:484-577)
+    tile_for<NP, NT>(n, [&](int i, int j) { Urho[i * n + j] = d.U[i * n + j]; });
     __syncthreads();
   } else {
+    // Loewdin S^{-1/2} (static.py:463-482)
+    for (int i = tid; i < n; i += NT) a.sp_norm[d.start + i] = d.S[i * n + i].x;
     load_lower_herm<NP, NT>(B0, d.S, n);
-    jacobi_eigh<NP, NT>(B0, B3, B1, n, js, false, B2);
-    __shared__ double fac[NMAXJ];
-    if (tid == 0) {
-      double wmax = js.ev[0];
-      for (int i = 1; i < n; ++i) wmax = fmax(wmax, js.ev[i]);
-      const double thr = fmax(1e-12, 2.220446049250313e-16 * wmax * n);
-      for (int i = 0; i < n; ++i) fac[i] = 1.0 / sqrt(fmax(js.ev[i], thr));
+    cx<double>*Y = B0, *Z = B1, *Y2 = B3, *Z2 = B4;
+    int ns_iters = 0;
+    double cs = 1.0;
+    // Near convergence the overlap is S = I + E with a small E: the binomial
+    // series (I + E)^{-1/2} = sum_k c_k E^k, c_k = (-1)^k C(2k,k)/4^k, is then
+    // exact to round-off in fewer products than the Newton-Schulz iteration
+    // (three per step): to degree 4 for |E|_F < 1e-3 (first omitted term below
+    // 0.25e-15) in 3 products, to degree 7 for |E|_F < 1e-2 (below 0.2e-16) in
+    // 6.  Horner: T = c_{d-1} I + c_d E, then T <- c_k I + E T.  The series
+    // works on S itself; Newton-Schulz on S scaled by a Gershgorin bound.
+    bool ok;
+    {
+      double fr = 0.0, dummy = 0.0;
+      HFB_TILE_LOOP(NP, NT, n, i, j) {
+        const cx<double> v = B0[i * n + j];
+        const double e = i == j ? v.x - 1.0 : v.x;
+        fr += e * e + v.y * v.y;
+      }
+      block_sum2_all<NT>(fr, dummy, red);
+      const int deg = fr < 1e-6 ? 4 : (fr < 1e-4 ? 7 : 0);
+      if (deg > 0) {
+        constexpr double ck[8] = {1.0, -0.5, 0.375, -0.3125, 0.2734375, -0.24609375, 0.2255859375, -0.20947265625};
+        // E in B0, Horner terms in B2 (products in B3), result Z in B1
+        tile_for<NP, NT>(n, [&](int i, int j) {
+          if (i == j) B0[i * n + j].x -= 1.0;
+          const cx<double> e = B0[i * n + j];
+          B2[i * n + j] = cx<double>{(i == j ? ck[deg - 1] : 0.0) + ck[deg] * e.x, ck[deg] * e.y};
+        });
+        __syncthreads();
+        for (int k = deg - 2; k >= 0; --k) {
+          mm<NP, NT, false>(B3, B0, B2, n);
+          cx<double>* dst = k == 0 ? B1 : B2;
+          tile_for<NP, NT>(n, [&](int i, int j) {
+            const cx<double> v = B3[i * n + j];
+            dst[i * n + j] = cx<double>{v.x + (i == j ? ck[k] : 0.0), v.y};
+          });
+          __syncthreads();
+        }
+        Z = B1;
+        ok = true;
+      } else {
+        double rowsum = 0.0;  // Gershgorin bound on the largest eigenvalue
+        if (tid < n)
+          for (int j = 0; j < n; ++j) rowsum += sqrt(cabs2(B0[tid * n + j]));
+        cs = block_max_all<NT>(rowsum, red);
+        if (!(cs > 0.0)) cs = 1.0;
+        tile_for<NP, NT>(n, [&](int i, int j) { B0[i * n + j] = cscale(B0[i * n + j], 1.0 / cs); });
+        __syncthreads();
+        ok = newton_schulz<NP, NT>(Y, Z, B2, Y2, Z2, n, red, ns_iters);
+      }
     }
-    __syncthreads();
-    const cx<double>* V = B1;
-    tile_for<NP, NT>(n, [&](int i, int j) {  // U = V diag(fac) V^H
-      cx<double> acc = {0.0, 0.0};
-      for (int k = 0; k < n; ++k) acc = cadd(acc, cscale(cmul(V[i * n + k], cconj(V[j * n + k])), fac[k]));
-      Urho[i * n + j] = acc;
-    });
-    __syncthreads();
-  }
+    if (ok) {
+      const double f = 1.0 / sqrt(cs);
+      // Z may live in B4: the scaling is elementwise, so in place is safe
+      tile_for<NP, NT>(n, [&](int i, int j) { Urho[i * n + j] = cscale(Z[i * n + j], f); });
+      __syncthreads();
+    } else {
+      load_lower_herm<NP, NT>(B0, d.S, n);
+      jacobi_eigh<NP, NT>(B0, B3, B1, n, js, false, B2);
+      __shared__ double fac[NMAXJ];
+      if (tid == 0) {
+        double wmax = js.ev[0];
+        for (int i = 1; i < n; ++i) wmax = fmax(wmax, js.ev[i]);
+        const double thr = fmax(1e-12, 2.220446049250313e-16 * wmax * n);
+        for (int i = 0; i < n; ++i) fac[i] = 1.0 / sqrt(fmax(js.ev[i], thr));
+      }
+      __syncthreads();
+      const cx<double>* V = B1;
+      tile_for<NP, NT>(n, [&](int i, int j) {  // U = V diag(fac) V^H
+        cx<double> acc = {0.0, 0.0};
+        for (int k = 0; k < n; ++k) acc = cadd(acc, cscale(cmul(V[i * n + k], cconj(V[j * n + k])), fac[k]));
+        Urho[i * n + j] = acc;
+      });
+      __syncthreads();
+    }
 
-  // subspace diagonalisation (static.py:300-303): U <- U_rho U_lam
-  if (a.stats && tid == 0) {
-    a.stats[2 + iq] = ok ? ns_iters : -1;
-    a.stats[iq] = 0;
-  }
-  cx<double>* U = Urho;
-  if (a.tdiag && d.Lam) {
-    load_lower_herm<NP, NT>(B0, d.Lam, n);
-    const int sw = jacobi_eigh<NP, NT>(B0, B3, B1, n, js, true, B2);
-    if (a.stats && tid == 0) a.stats[iq] = sw;
-    mm<NP, NT, false>(B3, Urho, B1, n);
-    U = B3;
-  }
-  tile_for<NP, NT>(n, [&](int i, int j) { d.U[i * n + j] = U[i * n + j]; });
+    // subspace diagonalisation (static.py:300-303): U <- U_rho U_lam
+    if (a.stats && tid == 0) {
+      a.stats[2 + iq] = ok ? ns_iters : -1;
+      a.stats[iq] = 0;
+    }
+    if (a.tdiag && d.Lam) {
+      load_lower_herm<NP, NT>(B0, d.Lam, n);
+      const int sw = jacobi_eigh<NP, NT>(B0, B3, B1, n, js, true, B2);
+      if (a.stats && tid == 0) a.stats[iq] = sw;
+      mm<NP, NT, false>(B3, Urho, B1, n);
+      U = B3;
+    }
+    tile_for<NP, NT>(n, [&](int i, int j) { d.U[i * n + j] = U[i * n + j]; });
+    if (a.mode == DIAG_LOEWDIN) return;
+  }  // Loewdin and the optional diagonalisation
 
   if (a.construct) {  // construct_hfb_matrices (static.py:484-577)
     cx<double>* H = B2;
@@ -605,13 +615,43 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
     tile_for<NP, NT>(n, [&](int i, int j) { B1[i * n + j] = d.Ls[i * nm + j]; });
     __syncthreads();
   }
-  // M2 = U lambda+ (static.py:557-570)
-  mm<NP, NT, false>(B0, U, B1, n);
-  tile_for<NP, NT>(n, [&](int i, int j) { d.M2[i * n + j] = B0[i * n + j]; });
+  // M2 = U lambda+ (static.py:557-570); with the caller's unitary the
+  // states are already rotated and M2 = lambda+
+  if (a.mode == DIAG_GIVEN_U) {
+    tile_for<NP, NT>(n, [&](int i, int j) { d.M2[i * n + j] = B1[i * n + j]; });
+  } else {
+    mm<NP, NT, false>(B0, U, B1, n);
+    tile_for<NP, NT>(n, [&](int i, int j) { d.M2[i * n + j] = B0[i * n + j]; });
+  }
   if (a.construct) diag_finish_arrive(a);
 }
 
 }  // namespace
+
+// One workgroup per isospin; the work matrices of the larger block decide
+// the instance (LDS up to NDIAG_LDS states, global memory above).
+static int launch_diag(const DiagArgs& a, int nbig, hipStream_t st) {
+#define HFB_DIAG(NPV, NTV, LDSV)                                                            \
+  {                                                                                         \
+    if ((LDSV) > 32 * 1024) allow_full_lds((const void*)diag_kernel<NPV, NTV>);             \
+    hipLaunchKernelGGL((diag_kernel<NPV, NTV>), dim3(2), dim3(NTV), (LDSV), st, a);         \
+  }
+  if (nbig <= NDIAG_LDS) {
+    const size_t lds = sizeof(cx<double>) * 5 * (size_t)nbig * nbig;
+    if (nbig <= 16)
+      HFB_DIAG(16, 256, lds)
+    else if (nbig <= 32)
+      HFB_DIAG(32, 512, lds)
+    else
+      HFB_DIAG(64, 512, lds)
+  } else if (nbig <= 128) {
+    HFB_DIAG(128, 1024, 0)
+  } else {
+    HFB_DIAG(256, 1024, 0)
+  }
+#undef HFB_DIAG
+  return launch_status();
+}
 
 // mats[iq*4 + m]: m=0 S, 1 H_old, 2 lambda_lin (or null), 3 G_old (or null),
 // each nst_iq x nst_iq complex; uo[iq*2]: U, uo[iq*2+1]: M2 outputs.
@@ -668,24 +708,74 @@ extern "C" int hfb_diag_small(hfb_ctx* c, int nst0, int nst1, int nmax, const do
   a.tbcs = tbcs;
   a.construct = construct;
   a.stats = stats;
-  const int nbig = nst0 > nst1 ? nst0 : nst1;
-  hipStream_t st = (hipStream_t)stream;
-#define HFB_DIAG(NPV, NTV, LDSV)                                                            \
-  {                                                                                         \
-    if ((LDSV) > 32 * 1024) allow_full_lds((const void*)diag_kernel<NPV, NTV>);             \
-    hipLaunchKernelGGL((diag_kernel<NPV, NTV>), dim3(2), dim3(NTV), (LDSV), st, a);         \
-  }
-  if (nbig <= NDIAG_LDS) {
-    const size_t lds = sizeof(cx<double>) * 5 * (size_t)nbig * nbig;
-    if (nbig <= 16)
-      HFB_DIAG(16, 256, lds)
-    else if (nbig <= 32)
-      HFB_DIAG(32, 512, lds)
-    else
-      HFB_DIAG(64, 512, lds)
-  } else {
-    HFB_DIAG(128, 1024, 0)
-  }
-#undef HFB_DIAG
-  return launch_status();
+  return launch_diag(a, nst0 > nst1 ? nst0 : nst1, (hipStream_t)stream);
+}
+
+// Loewdin S^{-1/2} of one block (static.py:463-482): U (nst x nst complex)
+// and sp_norm = Re diag S; work: null up to NDIAG_LDS states, else 5 nst^2
+// complex doubles.
+extern "C" int hfb_loewdin(int nst, const double* S, double* U, double* sp_norm, double* work, void* stream) {
+  if (!S || !U || !sp_norm) return HFB_E_ARG;
+  if (nst <= 0 || nst > NDIAG_MAX) return HFB_E_SIZE;
+  if (nst > NDIAG_LDS && !work) return HFB_E_ARG;
+  DiagArgs a{};
+  a.nmax = nst;
+  DiagIso& d = a.iso[0];
+  d.nst = nst;
+  d.S = (const cx<double>*)S;
+  d.U = (cx<double>*)U;
+  d.W = (cx<double>*)work;
+  a.sp_norm = sp_norm;
+  a.mode = DIAG_LOEWDIN;
+  return launch_diag(a, nst, (hipStream_t)stream);
+}
+
+// construct_hfb_matrices (static.py:484-577) of isospin iq with the
+// caller's unitary U (states start .. start+nst-1 already rotated):
+// H = U^H H_old U, G likewise (ipair), hmatrix/gapmatrix/symcond/lambda_save
+// rows of iq, sp_energy, deltaf, efluct1q/2q[iq], efluct1/2 over both
+// isospins, and lam = lambda+ (nst x nst) with lagrange = psi lam.
+extern "C" int hfb_hfb_matrices(hfb_ctx* c, int iq, int start, int nst, int nmax, const double* U, const double* Hold,
+                                const double* Gold, int ipair, int tbcs, const double* wocc, const double* wstates,
+                                const double* wguv, const double* pairwg, double* sp_energy, double* deltaf,
+                                double* hmatrix, double* gapmatrix, double* symcond, double* lambda_save,
+                                double* efluct1q, double* efluct2q, double* efluct1, double* efluct2, double* lam,
+                                double* work, void* stream) {
+  if (!c) return HFB_E_NOCTX;
+  if (!U || !Hold || (ipair && !Gold) || !wocc || !wstates || !wguv || !pairwg || !sp_energy || !deltaf ||
+      !hmatrix || !gapmatrix || !symcond || !lambda_save || !efluct1q || !efluct2q || !efluct1 || !efluct2 || !lam)
+    return HFB_E_ARG;
+  if (iq < 0 || iq > 1 || start < 0 || nst <= 0 || nst > nmax || nst > NDIAG_MAX) return HFB_E_SIZE;
+  if (nst > NDIAG_LDS && !work) return HFB_E_ARG;
+  DiagArgs a{};
+  a.nmax = nmax;
+  DiagIso& d = a.iso[iq];
+  d.nst = nst;
+  d.start = start;
+  d.U = (cx<double>*)U;
+  d.Hold = (const cx<double>*)Hold;
+  d.Gold = ipair ? (const cx<double>*)Gold : nullptr;
+  d.M2 = (cx<double>*)lam;
+  d.W = (cx<double>*)work;
+  const int64_t nm2 = (int64_t)nmax * nmax;
+  d.Hm = (cx<double>*)hmatrix + iq * nm2;
+  d.Gm = (cx<double>*)gapmatrix + iq * nm2;
+  d.Sym = (cx<double>*)symcond + iq * nm2;
+  d.Ls = (cx<double>*)lambda_save + iq * nm2;
+  a.wocc = wocc;
+  a.wstates = wstates;
+  a.wguv = wguv;
+  a.pairwg = pairwg;
+  a.sp_energy = sp_energy;
+  a.deltaf = deltaf;
+  a.efl1q = efluct1q;
+  a.efl2q = efluct2q;
+  a.efl1 = efluct1;
+  a.efl2 = efluct2;
+  a.sync = c->sync + HFB_SYNC_DIAG;
+  a.ipair = ipair;
+  a.tbcs = tbcs;
+  a.construct = 1;
+  a.mode = DIAG_GIVEN_U;
+  return launch_diag(a, nst, (hipStream_t)stream);
 }

@@ -151,7 +151,9 @@ class HFRun:
 
     def iteration(self, i, check=False):
         self.iteration_no = i
-        if not self.graphs_enabled:
+        # the adaptive step size (tvaryx_0) reads the energies on the host
+        # after every iteration and changes x0dmp: such iterations run eagerly
+        if not self.graphs_enabled or self.static.tvaryx_0:
             conv, self.derivs = st_mod.iterate(i, *self.state, self.tbcssav, self.derivs, check=check)
             return conv
         sig = self.signature(i)
@@ -189,7 +191,9 @@ class HFRun:
         default a signature is captured on its second occurrence (the first
         runs eagerly and allocates its workspaces); with capture_first the
         first occurrence is captured directly (new workspaces then come from
-        the graph pool)."""
+        the graph pool).  With the adaptive step size (static.tvaryx_0) every
+        iteration runs eagerly: it reads the energies on the host and
+        changes x0dmp after each one (static.py:1870-1882)."""
         self.graphs_enabled = on
         self.capture_first = capture_first
         if not hasattr(self, "_graphs"):
@@ -199,6 +203,8 @@ class HFRun:
     def precapture(self, iterations):
         """Capture the graphs of the given iteration numbers ahead of time
         (capture records kernels without running them, so no state changes)."""
+        if self.static.tvaryx_0:
+            return
         saved = (self.params.iteration, self.forces.tbcs, self.static.tdiag)
         tail = (self.energies.tail, self.energies.stale, self.energies._tail_ctx)
         for i in iterations:
@@ -235,7 +241,7 @@ class HFRun:
         """Capture the graphs that run(count, start, check_every=0,
         group=group) replays: runs of `group` consecutive iterations, one
         graph chain each (fewer graph launches; the same kernels)."""
-        if group <= 1:
+        if group <= 1 or self.static.tvaryx_0:
             return
         saved = (self.params.iteration, self.forces.tbcs, self.static.tdiag)
         tail = (self.energies.tail, self.energies.stale, self.energies._tail_ctx)

@@ -285,7 +285,10 @@ def gram(ctx, A, Bs, scale, out):
         need(b, A.dtype, A.shape, "B")
     need(out, torch.complex128, (nm, nst, nst), "gram_out")
     nwork = lib().hfb_gram_workspace(nst, K)
-    part = ctx.work(("gram_part", nst), (nwork,), torch.float64)
+    if ctx is not None:
+        part = ctx.work(("gram_part", nst), (nwork,), torch.float64)
+    else:
+        part = torch.empty(nwork, dtype=torch.float64, device=A.device)
     barr = (c_void_p * nm)(*[b.data_ptr() for b in Bs])
     check(lib().hfb_gram(prec, ptr(A), barr, nm, nst, K, float(scale), ptr(part), ptr(out), stream_handle()), "gram")
     return out
@@ -323,6 +326,47 @@ def gram2(ctx, blocks, scale, outs, finish_job=None):
     check(lib().hfb_gram2_ex(prec, As, Bs, nm, ns, K, float(scale), parts, op,
                              ctypes.byref(finish_job) if finish_job is not None else None, stream_handle()), "gram2")
     return outs
+
+
+def _diag_work(nst, device):
+    """Work matrices of the subspace kernel for one block above 36 states."""
+    return torch.empty(5 * nst * nst, dtype=torch.complex128, device=device) if nst > 36 else None
+
+
+def loewdin(S, U=None, sp_norm=None):
+    """U = S^{-1/2} and sp_norm = Re diag S of one block's overlap S
+    (loewdin_orthonormalize, static.py:463-482)."""
+    nst = S.shape[0]
+    need(S, torch.complex128, (nst, nst), "rhomatr")
+    dev = S.device
+    if U is None:
+        U = torch.empty((nst, nst), dtype=torch.complex128, device=dev)
+    if sp_norm is None:
+        sp_norm = torch.empty(nst, dtype=torch.float64, device=dev)
+    check(lib().hfb_loewdin(nst, ptr(S), ptr(U), ptr(sp_norm), ptr(_diag_work(nst, dev)), stream_handle()),
+          "loewdin")
+    return U, sp_norm
+
+
+def hfb_matrices(ctx, iq, start, nst, U, Hold, Gold, ipair, tbcs, levels, static, E, lam):
+    """construct_hfb_matrices (static.py:484-577) with the caller's unitary;
+    lam receives lambda+ (lagrange = psi_new lam)."""
+    nmax = static.hmatrix.shape[-1]
+    for t, nm in ((U, "unitary"), (Hold, "h_matrix_old"), (lam, "lambda")):
+        need(t, torch.complex128, (nst, nst), nm)
+    if ipair:
+        need(Gold, torch.complex128, (nst, nst), "gap_matrix_old")
+    Ep = E.data_ptr()
+    sz = 8
+    check(lib().hfb_hfb_matrices(ctx.handle, int(iq), int(start), int(nst), int(nmax), ptr(U), ptr(Hold),
+                                 ptr(Gold) if ipair else None, int(bool(ipair)), int(bool(tbcs)), ptr(levels.wocc),
+                                 ptr(levels.wstates), ptr(levels.wguv), ptr(levels.pairwg), ptr(levels.sp_energy),
+                                 ptr(levels.deltaf), ptr(static.hmatrix), ptr(static.gapmatrix), ptr(static.symcond),
+                                 ptr(static.lambda_save), c_void_p(Ep + sz * EN["EFL1Q"]),
+                                 c_void_p(Ep + sz * EN["EFL2Q"]), c_void_p(Ep + sz * EN["EFL1"]),
+                                 c_void_p(Ep + sz * EN["EFL2"]), ptr(lam), ptr(_diag_work(nst, U.device)),
+                                 stream_handle()), "hfb_matrices")
+    return lam
 
 
 def rotate(A, U1, U2, out1, out2):

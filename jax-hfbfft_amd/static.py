@@ -200,9 +200,142 @@ def grstep(forces, grids, levels, meanfield, params, static, defer_finish=False,
     return levels, static
 
 
+def _one_state_weights(forces, grids, levels, meanfield, s, iq):
+    """hpsi and preconditioner weights of state s (isospin iq) as 1-element
+    device vectors: the HFB weights, or (1, 0) in the BCS branch
+    (static.py:152-170, 128-137)."""
+    f64 = torch.float64
+    dev = levels.psi.device
+    out = {k: torch.empty(1, dtype=f64, device=dev) for k in ("w", "wuv", "pw", "pshift")}
+    kernels.grstep_weights(1, 1 if iq == 0 else 0, forces.tbcs, levels.wocc[s:s + 1], levels.wstates[s:s + 1],
+                           levels.wguv[s:s + 1], levels.pairwg[s:s + 1], meanfield.v_pair,
+                           grids.nx * grids.ny * grids.nz, out)
+    return out
+
+
+def _one_state_step(forces, grids, levels, params, static, s, psi1, ps1, lag1, hsum, wts):
+    """psi1 <- psi1 - x0dmp * laplace(residual) in place for one state, and
+    its fluctuation norms (static.py:93-146)."""
+    want_fl = params.mprint > 0 and params.iteration % params.mprint == 0
+    fl = torch.empty((1, 2), dtype=torch.float64, device=psi1.device) if want_fl else None
+    use_lag = (params.iteration > 1) and (not forces.tbcs)
+    kernels.precond_step(grids.ctx, psi1, ps1, lag1, hsum, grids.wxyz, use_lag, wts["pw"], wts["pshift"],
+                         static.e0dmp, forces.h2ma, static.x0dmp, fluct=fl)
+    if want_fl:
+        return fl[0, 0].sqrt(), fl[0, 1].sqrt()
+    return levels.sp_efluct1[s].clone(), levels.sp_efluct2[s].clone()
+
+
+def _one(t, shape):
+    return t.reshape((1,) + tuple(shape[-4:])).contiguous()
+
+
+def e0dmp_gt_zero(args):
+    """Damped, preconditioned step of one state (static.py:93-146):
+    args = (nst, iq, psin, ps1, lagrange, forces, grids, levels, meanfield,
+    params, static) with nst the state index, psin/ps1/lagrange that state's
+    (2, nx, ny, nz) spinors; returns (psin_new, sp_efluct1, sp_efluct2)."""
+    nst, iq, psin, ps1, lagrange, forces, grids, levels, meanfield, params, static = args
+    s, iq = int(nst), int(iq)
+    psi1 = psin.reshape((1,) + tuple(psin.shape[-4:])).clone()
+    p1 = _one(ps1, psin.shape)
+    l1 = _one(lagrange, psin.shape)
+    # h_exp = Re<psi|ps1>: the unscaled overlap enters the step kernel's sums
+    g = torch.empty((1, 1, 1), dtype=torch.complex128, device=psi1.device)
+    kernels.gram(grids.ctx, psi1, [p1], 1.0, g)
+    hsum = torch.zeros((1, 2), dtype=torch.float64, device=psi1.device)
+    hsum[0, 1] = g.real.reshape(())
+    wts = _one_state_weights(forces, grids, levels, meanfield, s, iq)
+    efl1, efl2 = _one_state_step(forces, grids, levels, params, static, s, psi1, p1, l1, hsum, wts)
+    return psi1[0], efl1, efl2
+
+
+def grstep_helper(nst, iq, spe_mf, psin, lagrange, forces, grids, levels, meanfield, params, static):
+    """Gradient step of one state (static.py:148-213): h psi with the weights
+    of the BCS or HFB branch, the damped step (e0dmp_gt_zero, or the simple
+    step when e0dmp <= 0), h psi of the new state.  Returns (psin, psi_mf,
+    hmfpsi, delpsi, denerg, spe_mf_new, sp_efluct1, sp_efluct2)."""
+    ctx = grids.ctx
+    s, iq = int(nst), int(iq)
+    psi1 = psin.reshape((1,) + tuple(psin.shape[-4:])).clone()
+    l1 = _one(lagrange, psin.shape)
+    n0 = 1 if iq == 0 else 0
+    wts = _one_state_weights(forces, grids, levels, meanfield, s, iq)
+    fields = meanfield.fields(psi1.dtype)
+    ps1, psi_mf, _, hsum = kernels.hpsi(ctx, psi1, n0, *fields, wts["w"], wts["wuv"], want_del=False)
+    hsum = hsum.clone()
+    spe_new = grids.wxyz * hsum[0, 0]
+    spe_old = torch.as_tensor(spe_mf, dtype=torch.float64, device=psi1.device)
+    if static.e0dmp > 0.0:
+        efl1, efl2 = _one_state_step(forces, grids, levels, params, static, s, psi1, ps1, l1, hsum, wts)
+    else:  # static.py:181-185
+        fac = (1.0 + static.x0dmp * (spe_new - spe_old)).to(psi1.real.dtype)
+        psi1.mul_(fac).sub_(static.x0dmp * ps1)
+        efl1, efl2 = levels.sp_efluct1[s].clone(), levels.sp_efluct2[s].clone()
+    _, hmf, dl, _ = kernels.hpsi(ctx, psi1, n0, *fields, wts["w"], wts["wuv"], want_pout=False, want_del=True,
+                                 want_partial=False)
+    denerg = (spe_old - spe_new) / spe_new.abs()
+    return psi1[0], psi_mf[0], hmf[0], dl[0], denerg, spe_new, efl1, efl2
+
+
 # ---------------------------------------------------------------------------
 # Orthonormalisation / subspace (static.py:261-577)
 # ---------------------------------------------------------------------------
+def _state_major(psi_2d, nst):
+    """The (nst, K) state-major block behind a reference-style psi_2d (K, nst)
+    (static.py:277-284), or a state block (nst, 2, nx, ny, nz) as it is.
+    A transposed view of a state block is used without a copy."""
+    if psi_2d.dim() == 2 and psi_2d.shape[1] == nst:
+        a = psi_2d.transpose(0, 1)
+        return a if a.is_contiguous() else a.contiguous()
+    if psi_2d.shape[0] != nst:
+        raise ValueError(f"psi_2d of shape {tuple(psi_2d.shape)} does not hold {nst} states")
+    a = psi_2d.reshape(nst, -1)
+    return a if a.is_contiguous() else a.contiguous()
+
+
+def loewdin_orthonormalize(psi_2d, wxyz, nst):
+    """S = wxyz psi_2d^H psi_2d, U = S^{-1/2} with the eigenvalue clamp, and
+    sp_norm = Re diag S (static.py:463-482).  psi_2d: (2N, nst) as in the
+    reference (e.g. block.reshape(nst, -1).T) or a state block; U is
+    (nst, nst) complex128 with psi_new = psi_2d U."""
+    nst = int(nst)
+    A = _state_major(psi_2d, nst)
+    S = torch.empty((1, nst, nst), dtype=torch.complex128, device=A.device)
+    kernels.gram(None, A, [A], float(wxyz), S)
+    return kernels.loewdin(S[0])
+
+
+def construct_hfb_matrices(unitary, psi_2d, energies, forces, grids, levels, static, iq, start, end, nst):
+    """H = U^H (wxyz psi_2d^H hmfpsi) U, the gap matrix likewise, sp_energy,
+    deltaf, lambda and its Hermitian/anti-Hermitian parts, the fluctuation
+    measures and lagrange = psi_new lambda+ for isospin iq, states
+    start:end (static.py:484-577).  psi_2d holds the states before the
+    rotation; levels.psi[start:end] the rotated ones."""
+    nst = int(nst)
+    start, end = int(start), int(end)
+    if end - start != nst:
+        raise ValueError("end - start must equal nst")
+    ctx = grids.ctx
+    A = _state_major(psi_2d, nst)
+    ipair = forces.ipair != 0
+    Bs = [levels.hmfpsi[start:end].reshape(nst, -1)]
+    if ipair:
+        Bs.append(levels.delpsi[start:end].reshape(nst, -1))
+    for b in Bs:
+        if b.dtype != A.dtype:
+            raise ValueError("psi_2d and the level arrays need the same dtype")
+    M = torch.empty((len(Bs), nst, nst), dtype=torch.complex128, device=A.device)
+    kernels.gram(ctx, A, Bs, grids.wxyz, M)
+    U = unitary.to(torch.complex128).contiguous()
+    lam = torch.empty((nst, nst), dtype=torch.complex128, device=A.device)
+    kernels.hfb_matrices(ctx, int(iq), start, nst, U, M[0], M[1] if ipair else None, ipair, forces.tbcs, levels,
+                         static, energies.vec, lam)
+    kernels.rotate(levels.psi[start:end], lam, None, levels.lagrange[start:end], None)
+    return energies, levels, static
+
+
+
 def diagstep(energies, forces, grids, levels, static, diagonalize=False, construct=True, npsi_neutron=None,
              npsi_proton=None, finish_job=None):
     """Loewdin orthonormalisation, optional subspace diagonalisation and the

@@ -860,6 +860,48 @@ def grstep(forces, grids, levels, meanfield, params, static):
     return levels, static
 
 
+def e0dmp_gt_zero(args):
+    """Preconditioned damped step of one state (static.py:93-146)."""
+    nst, iq, psin, ps1, lagrange, forces, grids, levels, meanfield, params, static = args
+    h_exp = np.real(overlap(psin, ps1, grids.wxyz))
+    if (params.iteration > 1) and (not forces.tbcs):
+        ps1o = ps1 - lagrange
+    else:
+        ps1o = ps1 - h_exp * psin
+    if params.mprint > 0 and params.iteration % params.mprint == 0:
+        efl1 = np.sqrt(rpsnorm(ps1o, grids.wxyz))
+        efl2 = np.sqrt(rpsnorm(lagrange - h_exp * psin, grids.wxyz))
+    else:
+        efl1, efl2 = levels.sp_efluct1[nst], levels.sp_efluct2[nst]
+    if forces.tbcs:
+        ps2 = laplace(grids, forces.h2ma, 1.0, 0.0, ps1o, 0.0, static.e0dmp)
+    else:
+        ps2 = laplace(grids, forces.h2ma, levels.wocc[nst], levels.wguv[nst] * levels.pairwg[nst], ps1o,
+                      np.max(meanfield.v_pair[iq]), static.e0dmp)
+    return psin - static.x0dmp * ps2, efl1, efl2
+
+
+def grstep_helper(nst, iq, spe_mf, psin, lagrange, forces, grids, levels, meanfield, params, static):
+    """Gradient step of one state (static.py:148-213); returns (psin, psi_mf,
+    hmfpsi, delpsi, denerg, spe_mf_new, sp_efluct1, sp_efluct2)."""
+    if forces.tbcs:
+        w, wuv = 1.0, 0.0
+    else:
+        w = levels.wstates[nst] * levels.wocc[nst]
+        wuv = levels.wstates[nst] * levels.wguv[nst] * levels.pairwg[nst]
+    ps1, psi_mf, _ = hpsi01(grids, meanfield, iq, w, wuv, psin)
+    spe_new = np.real(overlap(psin, psi_mf, grids.wxyz))
+    if static.e0dmp > 0.0:
+        pnew, efl1, efl2 = e0dmp_gt_zero((nst, iq, psin, ps1, lagrange, forces, grids, levels, meanfield, params,
+                                          static))
+    else:
+        pnew = (1.0 + static.x0dmp * (spe_new - spe_mf)) * psin - static.x0dmp * ps1
+        efl1, efl2 = levels.sp_efluct1[nst], levels.sp_efluct2[nst]
+    _, hmf, dl = hpsi01(grids, meanfield, iq, w, wuv, pnew)
+    denerg = (spe_mf - spe_new) / np.abs(spe_new)
+    return pnew, psi_mf, hmf, dl, denerg, spe_new, efl1, efl2
+
+
 # ---------------------------------------------------------------------------
 # Orthonormalisation / subspace                             static.py:261-577
 # ---------------------------------------------------------------------------

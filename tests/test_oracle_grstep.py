@@ -251,3 +251,28 @@ def test_energy_decreases_late_phase(O):
     hist = O.statichf(st, maxiter=24, callback=cb)
     assert all(b <= a + 1e-6 for a, b in zip(hist[3:], hist[4:])), hist
     assert e1[-1] < 0.5 * e1[3], e1
+
+
+@pytest.mark.parametrize("case", ["late", "hfb", "bcs", "simple"])
+def test_grstep_helper_equals_batched_grstep(O, case):
+    """The per-state restatement (grstep_helper / e0dmp_gt_zero,
+    static.py:93-213) gives the batched grstep's state for every state."""
+    G, F, lv, st, mf, basis = _setup(O, seed=7)
+    if case != "late":
+        _set_weights(O, lv, _rng(70))
+    if case == "simple":
+        st.e0dmp, st.x0dmp = 0.0, 0.01
+    F.tbcs = case == "bcs"
+    P = O.Params(iteration=6, mprint=3)
+    import copy
+    lv0 = copy.deepcopy(lv)
+    per = [O.grstep_helper(s, int(lv0.isospin[s]), lv0.sp_energy[s], lv0.psi[s], lv0.lagrange[s], F, G, lv0, mf, P,
+                           st) for s in range(lv0.nstmax)]
+    O.grstep(F, G, lv, mf, P, st)
+    for s, (pn, pmf, hmf, dl, den, spe, e1, e2) in enumerate(per):
+        assert np.max(np.abs(pn - lv.psi[s])) < 1e-12
+        assert np.max(np.abs(pmf - lv.hampsi[s])) < 1e-10
+        assert np.max(np.abs(hmf - lv.hmfpsi[s])) < 1e-10
+        assert np.max(np.abs(dl - lv.delpsi[s])) < 1e-12
+        assert abs(spe - lv.sp_energy[s]) < 1e-11
+        assert abs(e1 - lv.sp_efluct1[s]) < 1e-11 and abs(e2 - lv.sp_efluct2[s]) < 1e-11
