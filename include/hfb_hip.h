@@ -170,6 +170,21 @@ int hfb_selftest_mfma_f32(const float* a, const float* b, float* d, void* stream
  * rho[1] is read.  work: (2nx)(2ny)(2nz) complex doubles. */
 int hfb_poisson(hfb_ctx* ctx, const double* rho, double* wcoul, double e2, void* work, void* stream);
 
+/* Largest doubled Coulomb grid (points) of hfb_poisson_noalloc: 96^3, i.e.
+ * physical grids up to 48^3 (14.2 MB of device memory per GPU). */
+#define HFB_POISSON_SCRATCH_MAX (96LL * 96 * 96)
+
+/* hfb_poisson without a work argument and without any allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The doubled-grid work array is a static device array
+ * of HFB_POISSON_SCRATCH_MAX complex doubles on each GPU, held until the
+ * process ends; a context whose doubled grid is larger gets HFB_E_SIZE.
+ * Every call on one GPU shares that array: calls that overlap in time on
+ * one GPU (other streams, other threads, other contexts) race and must be
+ * ordered by the caller.  Nothing in it needs to start at zero. */
+int hfb_poisson_noalloc(hfb_ctx* ctx, const double* rho, double* wcoul, double e2, void* stream);
+
 /* ---------------------------------------------------------------------
  * Gradient step (static.py:93-259: grstep, grstep_helper, e0dmp_gt_zero;
  * levels.py:397-437: laplace)

@@ -77,6 +77,7 @@ bool supported_n(int n) { return n == 24 || n == 32 || n == 48; }
 }  // namespace
 
 int hfb_coulomb_setup(hfb_ctx* c);  // hfb_poisson.hip
+int hfb_poisson_scratch_setup(int device);  // hfb_poisson.hip
 
 extern "C" int hfb_ctx_create(hfb_ctx** out, int device, int nx, int ny, int nz, double dx, double dy,
                               double dz, int periodic) {
@@ -118,6 +119,7 @@ extern "C" int hfb_ctx_create(hfb_ctx** out, int device, int nx, int ny, int nz,
       rc = HFB_E_HIP;
   }
   if (rc == HFB_OK) rc = hfb_coulomb_setup(c);
+  if (rc == HFB_OK) rc = hfb_poisson_scratch_setup(device);
   if (rc == HFB_OK) {
     const size_t bytes = sizeof(unsigned int) * HFB_SYNC_COUNT;
     if (hipMalloc(&c->sync, bytes) != hipSuccess || hipMemset(c->sync, 0, bytes) != hipSuccess ||

@@ -9,6 +9,8 @@
 // kept), y-z planes again (inverse along y then z, corner extracted).
 #include <math.h>
 
+#include <mutex>
+
 #include "hfb_ctx.h"
 #include "hfb_poisson.h"
 
@@ -247,4 +249,46 @@ extern "C" int hfb_poisson(hfb_ctx* c, const double* rho, double* wcoul, double 
   a.scale = 1.0 / ((double)c->nx2 * c->ny2 * c->nz2);
   if (rc == HFB_OK) rc = launch_pass<2, PM_EXTRACT>(c, buf, a, st);
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// No-allocation variant: the doubled-grid work array lives in a file-scope
+// device array (one copy per GPU, held until the process ends) instead of a
+// caller workspace.  In host code the array's name is not a device address:
+// its address on each device comes from hipGetSymbolAddress, taken once per
+// device when a context is created there (hfb_ctx_create), and only that
+// address is ever passed to kernels.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[2 * HFB_POISSON_SCRATCH_MAX];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Called by hfb_ctx_create with `device` current.
+int hfb_poisson_scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return HFB_E_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return HFB_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return HFB_E_HIP;
+  g_scratch_ptr[device] = p;
+  return HFB_OK;
+}
+
+extern "C" int hfb_poisson_noalloc(hfb_ctx* c, const double* rho, double* wcoul, double e2, void* stream) {
+  if (!c) return HFB_E_NOCTX;
+  if (!rho || !wcoul) return HFB_E_ARG;
+  if ((int64_t)c->nx2 * c->ny2 * c->nz2 > HFB_POISSON_SCRATCH_MAX) return HFB_E_SIZE;
+  void* p = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (c->dev >= 0 && c->dev < kMaxDevices) p = g_scratch_ptr[c->dev];
+  }
+  if (!p) return HFB_E_NOCTX;
+  // every element of the work array that a pass reads is written earlier in
+  // the same call, so nothing is cleared here
+  return hfb_poisson(c, rho, wcoul, e2, p, stream);
 }

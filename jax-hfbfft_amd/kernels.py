@@ -153,6 +153,15 @@ def poisson(ctx, rho: torch.Tensor, e2: float, wcoul: torch.Tensor | None = None
     return wcoul
 
 
+def poisson_noalloc(ctx, rho: torch.Tensor, e2: float, wcoul: torch.Tensor):
+    """poisson with the library's static device scratch (no allocation in the
+    library; wcoul is the caller's)."""
+    need(rho, torch.float64, (2, ctx.nx, ctx.ny, ctx.nz), "rho")
+    need(wcoul, torch.float64, (ctx.nx, ctx.ny, ctx.nz), "wcoul")
+    check(lib().hfb_poisson_noalloc(ctx.handle, ptr(rho), ptr(wcoul), float(e2), stream_handle()), "poisson_noalloc")
+    return wcoul
+
+
 def selftest_mfma_f32(a: torch.Tensor, b: torch.Tensor):
     """Raw (64 lanes, 4 registers) result of one 16x16x4 FP32 matrix-core product."""
     need(a, torch.float32, (16, 4), "a")
