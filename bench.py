@@ -7,11 +7,16 @@ reference's statichf: gradient step, orthonormalisation/diagonalisation,
 densities with mixing, potentials incl. Coulomb, single-particle properties,
 energies).  Two runs from the same initial states: mixed precision (FP32
 transforms and h psi, FP64 subspace algebra, potentials and energies) and
-all-FP64.  After warmup+steps iterations the mixed run is promoted to FP64
-and both runs do `--refine` more FP64 iterations; the binding energy E_B =
-ehf is then compared.
+all-FP64; `value` is the mixed run's iterations/s over the timed steps.
 
-Prints one JSON line.  `value` is the mixed run's iterations/s.
+Accuracy: both runs continue to the iteration where the FP64 run has
+converged (ACCURACY_ITERS, or --accuracy-iters), the mixed run in mixed
+precision; E_B = ehf, the rms radii, Q20 and (with pairing) the mean gaps
+and Fermi energies are compared there ("unrefined"), then the mixed run is
+promoted to FP64 and both runs do `--refine` more FP64 iterations before the
+reported comparison.
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -33,9 +38,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--refine", type=int, default=10)
+    ap.add_argument("--refine", type=int, default=50)
     ap.add_argument("--workload", default="Ca40")
-    ap.add_argument("--cpu-iters", type=int, default=3, help="oracle iterations timed for the CPU baseline")
+    ap.add_argument("--cpu-iters", type=int, default=10, help="oracle iterations timed for the CPU baseline")
+    ap.add_argument("--accuracy-iters", type=int, default=None,
+                    help="iterations of both runs before the observables are compared (default per workload: "
+                         "where the FP64 run has converged)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch kernels from Python instead of HIP graphs")
     ap.add_argument("--group", type=int, default=10,
@@ -58,11 +66,35 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_mib": mib, "launches": 5}))
 
 
-def iter_bytes(S, nvox, c, r, P=11):
+# iterations after which the FP64 run of each workload has converged on the
+# MI355X (|delta ehf| below 1e-8 MeV per iteration, largest single-particle
+# residual below 1e-5 MeV); Cr48's BCS iteration drifts slowly instead and is
+# compared at the same iteration count
+ACCURACY_ITERS = {"O16": 800, "Ca40": 1000, "Cr48": 1500, "Pb208": 1500}
+
+
+def iter_bytes(S, nvox, c, r, tdiag=False, ipair=False):
     """Algorithmic HBM bytes of one iteration: P spinor-array passes plus
-    density, potential and Coulomb traffic (B_iter = P S 2N c + F)."""
+    density, potential and Coulomb traffic (B_iter = P S 2N c + F), with
+    P = 11, +1 for the hampsi read of the subspace diagonalisation and +2 for
+    the delpsi write and read with pairing."""
+    P = 11 + (1 if tdiag else 0) + (2 if ipair else 0)
     F = (2 * 12 + 2 * 16 + 2 * 12) * nvox * r + 2 * 2 * 8 * nvox * 16
     return P * S * 2 * nvox * c + F
+
+
+def window_bytes(run, args, c):
+    """Algorithmic bytes of the timed iterations warmup+1 .. warmup+steps
+    (subspace diagonalisation while i <= inidiag or BCS is on)."""
+    st, f = run.static, run.forces
+    S = run.levels.nstmax
+    nvox = run.grids.nx * run.grids.ny * run.grids.nz
+    tot = 0
+    for i in range(args.warmup + 1, args.warmup + args.steps + 1):
+        tbcs = True if i <= st.inibcs else run.tbcssav
+        tdiag = tbcs or i <= st.inidiag
+        tot += iter_bytes(S, nvox, c, 8, tdiag=tdiag, ipair=f.ipair != 0)
+    return tot
 
 
 def hpsi_bytes(S, nvox, c, r):
@@ -122,6 +154,44 @@ def hpsi_burst(pkg, run, reps=20):
     return a.elapsed_time(b) / reps
 
 
+def snapshot(run):
+    """E_B and the observables of a run (output.py:177-293, pairs.py:125-127)."""
+    e = run.energies_host()
+    ob = run.observables()
+    out = dict(ehf=e["ehf"], ehfint=e["ehfint"], rms=[float(x) for x in ob["rms"]], rmstot=float(ob["rmstot"]),
+               q20=[float(x) for x in ob["q20"]], q20tot=float(ob["q20tot"]), beta=float(ob["beta"]),
+               sp_efluct1_max=float(run.levels.sp_efluct1.max()))
+    if run.forces.ipair != 0:
+        out["avdelt"] = [float(x) for x in run.pairs.avdelt.cpu()]
+        out["eferm"] = [float(x) for x in run.pairs.eferm.cpu()]
+    return out
+
+
+def compare(unref, ref, acc_it, refine, pairing):
+    """Errors of the mixed run against the FP64 run after acc_it iterations
+    (mixed precision throughout: `unrefined`) and after `refine` more FP64
+    iterations of both (the reported figures)."""
+    def errs(m, f):
+        d = {"eb_err_kev": abs(m["ehf"] - f["ehf"]) * 1e3,
+             "rms_err_fm": max(abs(a - b) for a, b in zip(m["rms"] + [m["rmstot"]], f["rms"] + [f["rmstot"]])),
+             "q20_err_fm2": abs(m["q20tot"] - f["q20tot"])}
+        if pairing:
+            d["gap_err_mev"] = max(abs(a - b) for a, b in zip(m["avdelt"], f["avdelt"]))
+            d["eferm_err_mev"] = max(abs(a - b) for a, b in zip(m["eferm"], f["eferm"]))
+        return d
+    out = errs(ref["mixed"], ref["f64"])
+    out["unrefined"] = errs(unref["mixed"], unref["f64"])
+    out["iterations"] = acc_it
+    out["refine_iters"] = refine
+    f = ref["f64"]
+    out["f64"] = {k: f[k] for k in f}
+    # convergence of the FP64 run: energy change per iteration over the
+    # refinement window and the largest single-particle residual norm
+    out["f64_dehf_per_iter"] = abs(f["ehf"] - unref["f64"]["ehf"]) / max(refine, 1)
+    out["f64_converged"] = bool(out["f64_dehf_per_iter"] < 1e-8 and f["sp_efluct1_max"] < 1e-4)
+    return out
+
+
 def cpu_baseline(cfg, iters):
     sys.path.insert(0, str(ROOT))
     from oracle import hfb_oracle as O
@@ -162,14 +232,13 @@ def main():
     dev = "cuda:0"
 
     runs = {}
-    total = args.warmup + args.steps + args.refine
     for prec in ("mixed", "f64"):
         r = drv.HFRun.from_config(precision=prec, device=dev, **cfg)
         r.prelude()
         if not args.eager:
             # every iteration replays a HIP graph captured ahead of time
             r.enable_graphs(True)
-            r.precapture(range(1, (args.warmup + args.steps if prec == "mixed" else total) + 1))
+            r.precapture(range(1, args.warmup + args.steps + 1))
             r.precapture_groups(args.warmup + 1, args.steps, args.group)
         runs[prec] = r
     # warmup (untimed) -- iterations 1..W
@@ -187,16 +256,27 @@ def main():
 
     burst_ms = hpsi_burst(pkg, runs["mixed"])
 
-    # FP64 refinement of the mixed run; the FP64 run continues in step
-    start = args.warmup + args.steps + 1
+    # accuracy: both runs continue (the mixed run in mixed precision) to the
+    # iteration where the FP64 run has converged; then the mixed run is
+    # promoted to FP64 for `refine` more iterations and the FP64 run continues
+    # in step
+    done = args.warmup + args.steps
+    acc_it = max(done, args.accuracy_iters if args.accuracy_iters is not None
+                 else ACCURACY_ITERS.get(args.workload, 1000))
+    for prec, r in runs.items():
+        if acc_it > done:
+            r.run(acc_it - done, start=done + 1, check_every=0)
+    unref = {prec: snapshot(r) for prec, r in runs.items()}
+    start = acc_it + 1
     runs["mixed"].promote()
     if not args.eager:
         runs["mixed"].precapture(range(start, start + args.refine))
     for prec, r in runs.items():
         r.run(args.refine, start=start, check_every=0)
-    e_mix = runs["mixed"].energies_host()
-    e_64 = runs["f64"].energies_host()
-    err_kev = abs(e_mix["ehf"] - e_64["ehf"]) * 1e3
+    ref = {prec: snapshot(r) for prec, r in runs.items()}
+    accuracy = compare(unref, ref, acc_it, args.refine, bool(cfg.get("ipair")))
+    e_mix, e_64 = ref["mixed"], ref["f64"]
+    err_kev = accuracy["eb_err_kev"]
 
     lv = runs["f64"].levels
     S = lv.nstmax
@@ -225,9 +305,10 @@ def main():
                    "states": S, "grid": cfg["n"], "refine_iters": args.refine,
                    "launch": "eager" if args.eager else f"hip-graph replay, {args.group} iterations per graph"},
         "fp64_iter_s": f64_ips, "speedup_vs_fp64": mixed_ips / f64_ips,
-        "eb_err_kev": err_kev, "ehf_mixed": e_mix["ehf"], "ehf_f64": e_64["ehf"],
-        "hbm_frac_iter_mixed": iter_bytes(S, nvox, 8, 8) * mixed_ips / HBM_PEAK,
-        "hbm_frac_iter_f64": iter_bytes(S, nvox, 16, 8) * f64_ips / HBM_PEAK,
+        "eb_err_kev": err_kev, "eb_err_kev_unrefined": accuracy["unrefined"]["eb_err_kev"],
+        "ehf_mixed": e_mix["ehf"], "ehf_f64": e_64["ehf"], "accuracy": accuracy,
+        "hbm_frac_iter_mixed": window_bytes(runs["f64"], args, 8) * mixed_ips / args.steps / HBM_PEAK,
+        "hbm_frac_iter_f64": window_bytes(runs["f64"], args, 16) * f64_ips / args.steps / HBM_PEAK,
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
