@@ -206,7 +206,17 @@ __device__ __forceinline__ void mm(cx<double>* __restrict__ C, const cx<double>*
 }
 
 struct JacobiShared {
-  // rotations of two consecutive rounds (buffer t & 1)
+  // rotations of the current round
+  int p[NMAXJ / 2], q[NMAXJ / 2];
+  double c[NMAXJ / 2], s[NMAXJ / 2];
+  cx<double> e[NMAXJ / 2];
+  double red[32];
+  int perm[NMAXJ];
+  double ev[NMAXJ];
+};
+
+// Rotations of two consecutive rounds (buffer t & 1) for jacobi_eigh.
+struct JacobiShared2 {
   int p[2][NMAXJ / 2], q[2][NMAXJ / 2];
   double c[2][NMAXJ / 2], s[2][NMAXJ / 2];
   cx<double> e[2][NMAXJ / 2];
@@ -215,7 +225,8 @@ struct JacobiShared {
   double ev[NMAXJ];
 };
 
-// Eigen-decomposition of the Hermitian A (destroyed); V gets the
+// Eigen-decomposition of the Hermitian A (n x n, dense; destroyed) for
+// blocks whose work matrices live in LDS; V gets the
 // eigenvectors as columns, js.ev the eigenvalues; sorted ascending when
 // `sort` (T is scratch for the column permutation).  Parallel cyclic Jacobi:
 // in each round the n/2 disjoint pairs (p_k, q_k) of a round-robin ordering
@@ -225,7 +236,7 @@ struct JacobiShared {
 // second buffer A2 and V <- V R in place; the buffers then swap.  For odd n
 // one index per round has no partner and is carried unrotated.
 template <int NP, int NT>
-__device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* A2, cx<double>* V, int n, JacobiShared& js,
+__device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* A2, cx<double>* V, int n, JacobiShared2& js,
                                            bool sort, cx<double>* T) {
   const int tid = threadIdx.x;
   tile_for<NP, NT>(n, [&](int i, int j) { V[i * n + j] = cx<double>{i == j ? 1.0 : 0.0, 0.0}; });
@@ -371,6 +382,205 @@ __device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* A2, cx<dou
   return sweep;
 }
 
+// Packed lower triangle of a Hermitian n x n matrix: entry (i, j), i >= j,
+// at i (i + 1) / 2 + j; an upper entry is the conjugate of its mirror.
+__device__ __forceinline__ int ploff(int i) { return (i * (i + 1)) >> 1; }
+__device__ __forceinline__ cx<double> pld(const cx<double>* A, int i, int j) {
+  return i >= j ? A[ploff(i) + j] : cconj(A[ploff(j) + i]);
+}
+__device__ __forceinline__ void pst(cx<double>* A, int i, int j, cx<double> v) {
+  if (i >= j)
+    A[ploff(i) + j] = v;
+  else
+    A[ploff(j) + i] = cconj(v);
+}
+// Row of linear index e of the packed triangle: the largest i with
+// ploff(i) <= e (a floating-point estimate, then corrected).
+__device__ __forceinline__ int prow(int e) {
+  int i = (int)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
+  if (ploff(i + 1) <= e) ++i;
+  if (ploff(i) > e) --i;
+  return i;
+}
+
+// A (packed, n (n + 1) / 2) <- the Hermitian matrix of M's lower triangle
+// (M dense, leading dimension n), diagonal made real.
+template <int NT>
+__device__ void load_lower_packed(cx<double>* A, const cx<double>* M, int n) {
+  const int tot = ploff(n);
+  for (int e = threadIdx.x; e < tot; e += NT) {
+    const int i = prow(e), j = e - ploff(i);
+    const cx<double> v = M[i * n + j];
+    A[e] = i == j ? cx<double>{v.x, 0.0} : v;
+  }
+  __syncthreads();
+}
+
+// Eigen-decomposition of the Hermitian matrix in A (packed lower triangle,
+// destroyed): Vout (n x n, leading dimension n) gets the eigenvectors as
+// columns, js.ev the eigenvalues, in ascending order when `sort`.  Vt (n x n)
+// is scratch holding V transposed (row c = eigenvector c) while the
+// rotations are applied, so that the update of a column pair is a
+// contiguous row pair.  Parallel cyclic Jacobi: in each round the n/2
+// disjoint pairs (p_k, q_k) of a round-robin ordering are rotated together,
+// A <- R^H A R with R block-diagonal over the pairs.  A round computes the
+// rotations (one thread per pair), then every 2x2 block (pair a, pair b),
+// a >= b, of R_a^H A[a,b] R_b in place (the blocks of one round are
+// disjoint, and the packed triangle holds each entry once) beside
+// V <- V R.  For odd n one index per round has no partner and is carried
+// unrotated.
+template <int NT>
+__device__ __forceinline__ int jacobi_packed(cx<double>* __restrict__ A, cx<double>* __restrict__ Vt,
+                                             cx<double>* __restrict__ Vout, int n, JacobiShared& js, bool sort) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < n * n; e += NT) {
+    const int r = e / n, c = e - r * n;
+    Vt[e] = cx<double>{r == c ? 1.0 : 0.0, 0.0};
+  }
+  const int m = n + (n & 1);
+  const int npair = m / 2;
+  const int nblk = ploff(npair);
+  const int nv = n * npair;
+  __syncthreads();
+  int sweep = 0;
+  for (; sweep < 40; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    for (int e = tid; e < ploff(n); e += NT) {
+      const int i = prow(e), j = e - ploff(i);
+      const double v = cabs2(A[e]);
+      if (i == j)
+        dia += v;
+      else
+        off += 2.0 * v;
+    }
+    block_sum2_all<NT>(off, dia, js.red);
+    if (off <= 1e-30 * dia || off == 0.0) break;
+    for (int t = 0; t < m - 1; ++t) {
+      // rotation angles of round t (threads < npair)
+      if (tid < npair) {
+        // round-robin pairs: index 0 fixed, the others rotate by t
+        const int k = tid;
+        const int m1 = m - 1;
+        int xa = k - 1 + t;  // < 2 (m - 1)
+        xa = xa >= m1 ? xa - m1 : xa;
+        int xb = m1 - k - 1 + t;
+        xb = xb >= m1 ? xb - m1 : xb;
+        const int a0 = (k == 0) ? 0 : xa + 1;
+        const int b0 = xb + 1;
+        int p = a0 < b0 ? a0 : b0, q = a0 < b0 ? b0 : a0;
+        double c = 1.0, sn = 0.0;
+        cx<double> ea = {1.0, 0.0};
+        if (q < n) {
+          const double app = A[ploff(p) + p].x, aqq = A[ploff(q) + q].x;
+          const cx<double> apq = cconj(A[ploff(q) + p]);
+          const double r2 = cabs2(apq), den = fabs(app) + fabs(aqq);
+          if (r2 > 0.0 && r2 > 1e-36 * den * den) {
+            const double ri = rsq_full(r2);  // 1 / |a_pq|
+            ea = {apq.x * ri, apq.y * ri};
+            const double th = 0.5 * (aqq - app) * ri;
+            const double h = 1.0 + th * th;
+            const double rt = h * rsq_full(h);  // sqrt(1 + th^2)
+            const double tt = (th >= 0.0 ? 1.0 : -1.0) * rcp_full(fabs(th) + rt);
+            c = rsq_full(1.0 + tt * tt);
+            sn = tt * c;
+          }
+        } else {
+          q = -1;  // p has no partner this round (odd n)
+        }
+        js.p[k] = p;
+        js.q[k] = q;
+        js.c[k] = c;
+        js.s[k] = sn;
+        js.e[k] = ea;
+      }
+      __syncthreads();
+      // the blocks (ka, kb), ka >= kb, of R^H A R, in place
+      for (int it = tid; it < nblk; it += NT) {
+        const int ka = prow(it), kb = it - ploff(ka);
+        const int pa = js.p[ka], qa = js.q[ka], pb = js.p[kb], qb = js.q[kb];
+        const double ca = js.c[ka], sa = js.s[ka], cb = js.c[kb], sb = js.s[kb];
+        const cx<double> eca = cscale(js.e[ka], ca), esa = cscale(js.e[ka], sa);
+        const cx<double> ecb = cscale(js.e[kb], cb), esb = cscale(js.e[kb], sb);
+        const cx<double> z = {0.0, 0.0};
+        // rows pa, qa of A, columns pb, qb (absent partners read as zero)
+        const cx<double> x00 = pld(A, pa, pb);
+        const cx<double> x01 = qb >= 0 ? pld(A, pa, qb) : z;
+        const cx<double> x10 = qa >= 0 ? pld(A, qa, pb) : z;
+        const cx<double> x11 = qa >= 0 && qb >= 0 ? pld(A, qa, qb) : z;
+        // columns: [xp xq] R_b
+        const cx<double> y00 = csub(cmul(x00, ecb), cscale(x01, sb));
+        const cx<double> y01 = cadd(cmul(x00, esb), cscale(x01, cb));
+        const cx<double> y10 = csub(cmul(x10, ecb), cscale(x11, sb));
+        const cx<double> y11 = cadd(cmul(x10, esb), cscale(x11, cb));
+        // rows: R_a^H
+        cx<double> n00 = csub(cmul(cconj(eca), y00), cscale(y10, sa));
+        cx<double> n01 = csub(cmul(cconj(eca), y01), cscale(y11, sa));
+        cx<double> n10 = cadd(cmul(cconj(esa), y00), cscale(y10, ca));
+        cx<double> n11 = cadd(cmul(cconj(esa), y01), cscale(y11, ca));
+        if (ka == kb) {
+          // diagonal block: Hermitian, the rotated pair's coupling removed
+          if (qa >= 0 && sa != 0.0) n10 = z;
+          n00.y = 0.0;
+          n11.y = 0.0;
+          A[ploff(pa) + pa] = n00;
+          if (qa >= 0) {
+            A[ploff(qa) + pa] = n10;
+            A[ploff(qa) + qa] = n11;
+          }
+        } else {
+          pst(A, pa, pb, n00);
+          if (qb >= 0) pst(A, pa, qb, n01);
+          if (qa >= 0) pst(A, qa, pb, n10);
+          if (qa >= 0 && qb >= 0) pst(A, qa, qb, n11);
+        }
+      }
+      // V <- V R: rows p, q of Vt
+      for (int it = tid; it < nv; it += NT) {
+        const int k = it / n, r = it - k * n;
+        const int p = js.p[k], q = js.q[k];
+        const double sk = js.s[k];
+        if (q >= 0 && sk != 0.0) {
+          const double ck = js.c[k];
+          const cx<double> ek = js.e[k];
+          const cx<double> xp = Vt[p * n + r], xq = Vt[q * n + r];
+          Vt[p * n + r] = csub(cmul(xp, cscale(ek, ck)), cscale(xq, sk));
+          Vt[q * n + r] = cadd(cmul(xp, cscale(ek, sk)), cscale(xq, ck));
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < n; i += NT) js.ev[i] = A[ploff(i) + i].x;
+  for (int i = tid; i < n; i += NT) js.perm[i] = i;
+  __syncthreads();
+  if (sort) {
+    // stable ascending order by rank: index i goes to the number of
+    // eigenvalues below it, ties kept in index order (every entry stays an
+    // index below n whatever the values)
+    for (int i = tid; i < n; i += NT) {
+      const double e = js.ev[i];
+      int r = 0;
+      for (int j = 0; j < n; ++j) {
+        const double f = js.ev[j];
+        r += (f < e || (f == e && j < i)) ? 1 : 0;
+      }
+      js.perm[r] = i;
+    }
+    __syncthreads();
+  }
+  // Vout[i][j] = component i of eigenvector perm[j]
+  for (int e = tid; e < n * n; e += NT) {
+    const int i = e / n, j = e - i * n;
+    Vout[e] = Vt[js.perm[j] * n + i];
+  }
+  double evs = 0.0;
+  if (sort && tid < n) evs = js.ev[js.perm[tid]];
+  __syncthreads();
+  if (sort && tid < n) js.ev[tid] = evs;
+  __syncthreads();
+  return sweep;
+}
+
 constexpr int NDIAG_LDS = 36;   // blocks up to this size keep the five work matrices in LDS
 constexpr int NDIAG_MAX = 256;  // largest block hfb_diag_small accepts
 
@@ -420,10 +630,22 @@ __device__ __forceinline__ void diag_finish_arrive(const DiagArgs& a) {
   }
 }
 
-template <int NP, int NT>
+// Storage of the work matrices B0..B4 and of the Jacobi eigensolver's
+// packed matrix Ap and transposed eigenvectors Vt:
+//   WM_LDS:  all in LDS (blocks up to NDIAG_LDS states)
+//   WM_GL1:  B global, Ap and Vt in LDS (up to 64 states)
+//   WM_GL2:  B global, Ap in LDS, Vt global (up to 128 states)
+//   WM_GL3:  all global
+enum { WM_LDS = 0, WM_GL1 = 1, WM_GL2 = 2, WM_GL3 = 3 };
+
+template <int NP, int NT, int WM>
 __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
   extern __shared__ double lds_raw[];
-  __shared__ JacobiShared js;
+  // one LDS area for the eigensolver's rotation tables: dense-matrix rounds
+  // (work matrices in LDS) or packed-matrix rounds (the larger blocks)
+  __shared__ JacobiShared2 js2;
+  JacobiShared& js = reinterpret_cast<JacobiShared&>(js2);
+  double* const jev = WM == WM_LDS ? js2.ev : js.ev;
   __shared__ double red[32];
   const int iq = blockIdx.x;
   const DiagIso d = a.iso[iq];
@@ -438,7 +660,7 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
   // resident) for large ones; each instance sees one kind of pointer only,
   // so the compiler uses LDS or global instructions, never flat ones
   cx<double>* B0;
-  if constexpr (NP <= 64)
+  if constexpr (WM == WM_LDS)
     B0 = (cx<double>*)lds_raw;
   else
     B0 = d.W;
@@ -446,6 +668,18 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
   cx<double>* const B2 = B1 + n * n;
   cx<double>* const B3 = B2 + n * n;
   cx<double>* const B4 = B3 + n * n;
+  cx<double>* Ap;
+  cx<double>* Vt;
+  if constexpr (WM == WM_LDS || WM == WM_GL3) {
+    Ap = B0;
+    Vt = B3;
+  } else if constexpr (WM == WM_GL1) {
+    Ap = (cx<double>*)lds_raw;
+    Vt = Ap + ploff(n);
+  } else {
+    Ap = (cx<double>*)lds_raw;
+    Vt = B3;
+  }
 
   cx<double>* Urho = B4;
   cx<double>* U = Urho;
@@ -514,14 +748,19 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
       tile_for<NP, NT>(n, [&](int i, int j) { Urho[i * n + j] = cscale(Z[i * n + j], f); });
       __syncthreads();
     } else {
-      load_lower_herm<NP, NT>(B0, d.S, n);
-      jacobi_eigh<NP, NT>(B0, B3, B1, n, js, false, B2);
+      if constexpr (WM == WM_LDS) {
+        load_lower_herm<NP, NT>(B0, d.S, n);
+        jacobi_eigh<NP, NT>(B0, B3, B1, n, js2, false, B2);
+      } else {
+        load_lower_packed<NT>(Ap, d.S, n);
+        jacobi_packed<NT>(Ap, Vt, B1, n, js, false);
+      }
       __shared__ double fac[NMAXJ];
       if (tid == 0) {
-        double wmax = js.ev[0];
-        for (int i = 1; i < n; ++i) wmax = fmax(wmax, js.ev[i]);
+        double wmax = jev[0];
+        for (int i = 1; i < n; ++i) wmax = fmax(wmax, jev[i]);
         const double thr = fmax(1e-12, 2.220446049250313e-16 * wmax * n);
-        for (int i = 0; i < n; ++i) fac[i] = 1.0 / sqrt(fmax(js.ev[i], thr));
+        for (int i = 0; i < n; ++i) fac[i] = 1.0 / sqrt(fmax(jev[i], thr));
       }
       __syncthreads();
       const cx<double>* V = B1;
@@ -539,8 +778,14 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
       a.stats[iq] = 0;
     }
     if (a.tdiag && d.Lam) {
-      load_lower_herm<NP, NT>(B0, d.Lam, n);
-      const int sw = jacobi_eigh<NP, NT>(B0, B3, B1, n, js, true, B2);
+      int sw;
+      if constexpr (WM == WM_LDS) {
+        load_lower_herm<NP, NT>(B0, d.Lam, n);
+        sw = jacobi_eigh<NP, NT>(B0, B3, B1, n, js2, true, B2);
+      } else {
+        load_lower_packed<NT>(Ap, d.Lam, n);
+        sw = jacobi_packed<NT>(Ap, Vt, B1, n, js, true);
+      }
       if (a.stats && tid == 0) a.stats[iq] = sw;
       mm<NP, NT, false>(B3, Urho, B1, n);
       U = B3;
@@ -631,23 +876,27 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
 // One workgroup per isospin; the work matrices of the larger block decide
 // the instance (LDS up to NDIAG_LDS states, global memory above).
 static int launch_diag(const DiagArgs& a, int nbig, hipStream_t st) {
-#define HFB_DIAG(NPV, NTV, LDSV)                                                            \
+#define HFB_DIAG(NPV, NTV, WMV, LDSV)                                                        \
   {                                                                                         \
-    if ((LDSV) > 32 * 1024) allow_full_lds((const void*)diag_kernel<NPV, NTV>);             \
-    hipLaunchKernelGGL((diag_kernel<NPV, NTV>), dim3(2), dim3(NTV), (LDSV), st, a);         \
+    if ((LDSV) > 32 * 1024) allow_full_lds((const void*)diag_kernel<NPV, NTV, WMV>);        \
+    hipLaunchKernelGGL((diag_kernel<NPV, NTV, WMV>), dim3(2), dim3(NTV), (LDSV), st, a);    \
   }
+  const size_t cz = sizeof(cx<double>);
+  const size_t tri = (size_t)nbig * (nbig + 1) / 2;
   if (nbig <= NDIAG_LDS) {
-    const size_t lds = sizeof(cx<double>) * 5 * (size_t)nbig * nbig;
+    const size_t lds = cz * 5 * (size_t)nbig * nbig;
     if (nbig <= 16)
-      HFB_DIAG(16, 256, lds)
+      HFB_DIAG(16, 256, WM_LDS, lds)
     else if (nbig <= 32)
-      HFB_DIAG(32, 512, lds)
+      HFB_DIAG(32, 512, WM_LDS, lds)
     else
-      HFB_DIAG(64, 512, lds)
+      HFB_DIAG(64, 512, WM_LDS, lds)
+  } else if (nbig <= 64) {
+    HFB_DIAG(64, 512, WM_GL1, cz * (tri + (size_t)nbig * nbig))
   } else if (nbig <= 128) {
-    HFB_DIAG(128, 1024, 0)
+    HFB_DIAG(128, 1024, WM_GL2, cz * tri)
   } else {
-    HFB_DIAG(256, 1024, 0)
+    HFB_DIAG(256, 1024, WM_GL3, 0)
   }
 #undef HFB_DIAG
   return launch_status();

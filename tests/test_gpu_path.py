@@ -62,7 +62,8 @@ _FAULTED = pytest.mark.skip(reason="this case faulted the device with a since-re
 
 
 @pytest.mark.parametrize("npsi", [[12, 10], [21, 15], [36, 30], pytest.param([40, 33], marks=_FAULTED),
-                                  pytest.param([64, 50], marks=_FAULTED), [90, 60], [91, 61], [126, 82]])
+                                  pytest.param([64, 50], marks=_FAULTED), [56, 47], [90, 60], [91, 61], [100, 70],
+                                  [126, 82]])
 def test_diagstep_matches_oracle(pkg, O, dev, npsi):
     """Loewdin + subspace diagonalisation + HFB matrices on the same states;
     block sizes of O16-BCS and Cr48 (work matrices in LDS) and of the
