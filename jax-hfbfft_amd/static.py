@@ -388,9 +388,12 @@ class LoopState:
 
 
 def prelude(coulomb, densities, energies, forces, grids, levels, meanfield, moment, params, static, pairs):
-    """Steps before the loop (static.py:1700-1744)."""
-    params.iteration = 0
-    diagstep(energies, forces, grids, levels, static, False, True)
+    """Steps before the loop (static.py:1700-1744).  With params.trestart
+    (a run continued from load_checkpoint) the iteration number is kept and
+    the first orthonormalisation is skipped (static.py:1700-1706)."""
+    if not params.trestart:
+        params.iteration = 0
+        diagstep(energies, forces, grids, levels, static, False, True)
     add_density(densities, grids, levels)
     derivs = density_derivs(grids, densities)
     skyrme(coulomb, densities, forces, grids, meanfield, params, static, derivs=derivs, E=energies.vec,
@@ -436,8 +439,9 @@ def iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, m
     # launch at the end
     skyrme(coulomb, densities, forces, grids, meanfield, params, static, derivs=derivs, E=energies.vec,
            mass_number=levels.mass_number, finish=False, with_energy=True, wcoul_ready=bool(params.tcoul))
-    if static.tsort:
-        raise NotImplementedError("sort_states is outside the static HF+BCS path")
+    if static.tsort:  # static.py:1829-1831
+        for iq in range(2):
+            sort_states(forces, grids, levels, iq)
     # integ_energy's integrals come from the same launch as the potentials;
     # its pairing energies go to the energy vector as integ_energy does
     _set_slots(energies.vec, kernels.EN["EPAIR"], pairs.epair)
@@ -467,11 +471,21 @@ def statichf(coulomb, densities, energies, forces, grids, levels, meanfield, mom
     """Prelude + SCF loop (static.py:1394-1601).  ``output_writer`` is not
     required (the reference needs one, Q1); ``callback(i, state)`` runs after
     every iteration."""
+    firstiter = params.iteration + 1 if params.trestart else 1
     derivs = prelude(coulomb, densities, energies, forces, grids, levels, meanfield, moment, params, static, pairs)
     tbcssav = forces.tbcs
-    for i in range(1, static.maxiter + 1):
+    if output_writer is not None:
+        from .output import complete_sinfo_with_fortran_output
+        complete_sinfo_with_fortran_output(coulomb, densities, energies, forces, grids, levels, meanfield, moment,
+                                           params, static, pairs, output_writer)
+    for i in range(firstiter, static.maxiter + 1):
         conv, derivs = iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, moment, params,
                                static, pairs, tbcssav, derivs)
+        if output_writer is not None and i % output_interval == 0:
+            complete_sinfo_with_fortran_output(coulomb, densities, energies, forces, grids, levels, meanfield,
+                                               moment, params, static, pairs, output_writer)
+        if params.mrest > 0 and params.wffile not in ("", "none") and i % params.mrest == 0:
+            save_checkpoint(params.wffile, levels, params, static, pairs)
         if callback is not None:
             callback(i, levels, energies)
         if conv:
@@ -483,10 +497,11 @@ def statichf_with_benchmark(coulomb, densities, energies, forces, grids, levels,
                             pairs, output_writer=None, output_interval=10):
     """statichf with per-iteration wall times (static.py:1666-1927); returns
     the reference's tuple plus the list of iteration durations."""
+    firstiter = params.iteration + 1 if params.trestart else 1
     derivs = prelude(coulomb, densities, energies, forces, grids, levels, meanfield, moment, params, static, pairs)
     tbcssav = forces.tbcs
     times = []
-    for i in range(1, static.maxiter + 1):
+    for i in range(firstiter, static.maxiter + 1):
         t0 = time.perf_counter()
         conv, derivs = iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, moment, params,
                                static, pairs, tbcssav, derivs)
@@ -495,3 +510,124 @@ def statichf_with_benchmark(coulomb, densities, energies, forces, grids, levels,
         if conv:
             break
     return (coulomb, densities, energies, forces, grids, levels, meanfield, moment, params, static), times
+
+
+# ---------------------------------------------------------------------------
+# State ordering, plane-wave start, checkpoints
+# (static.py:807-918, 1338-1392, 1700-1701)
+# ---------------------------------------------------------------------------
+SORTED_FIELDS = ("psi", "lagrange", "sp_energy", "deltaf", "sp_norm", "sp_efluct1", "sp_efluct2", "wocc", "wguv",
+                 "wstates", "pairwg")
+# per-state results of sp_properties that this build computes before the
+# sort (in the density sweep); moving them along is the same as computing
+# them after it, as the reference does
+SP_FIELDS = ("sp_kinetic", "sp_orbital", "sp_spin", "sp_parity")
+
+
+def sort_states(forces, grids, levels, isospin):
+    """Order the states of one isospin block by sp_energy, stable, moving
+    psi, lagrange and the per-state arrays along (static.py:1338-1392).  The
+    block is the isospin's npsi states (the reference slices by nneut/nprot,
+    the same block only when npsi = (N, Z)).  Arrays are permuted in place on
+    the device."""
+    n0 = int(levels.npsi[0])
+    a, b = (0, n0) if isospin == 0 else (n0, n0 + int(levels.npsi[1]))
+    if b - a <= 1:
+        return levels
+    perm = torch.argsort(levels.sp_energy[a:b], stable=True) + a
+    for name in SORTED_FIELDS + SP_FIELDS:
+        t = getattr(levels, name)
+        t[a:b].copy_(t.index_select(0, perm))
+    return levels
+
+
+def planewaves_host(grids, npsi, nneut, nprot, nucleon_mass, hbc=197.328910, nmax=7):
+    """Plane-wave states (static.py:807-918) as a host array: per isospin
+    block the wave numbers k_a in -nmax//2 .. nmax//2 ordered (stable) by
+    hbc^2/2m sum_a ((2 pi k_a + bang_a)/(n_a d_a))^2, each first with spin up,
+    then spin down, normalised."""
+    r = range(-nmax // 2, nmax // 2 + 1)
+    ks = [(kx, ky, kz) for kx in r for ky in r for kz in r]
+    bang = (grids.bangx, grids.bangy, grids.bangz)
+    ns = (grids.nx, grids.ny, grids.nz)
+    ds = (grids.dx, grids.dy, grids.dz)
+    en = np.array([hbc ** 2 / (2.0 * nucleon_mass) * sum(((2.0 * math.pi * k[c] + bang[c]) / (ns[c] * ds[c])) ** 2
+                                                         for c in range(3)) for k in ks])
+    order = [ks[i] for i in np.argsort(en, kind="stable")]
+    I = np.arange(grids.nx)[:, None, None]
+    J = np.arange(grids.ny)[None, :, None]
+    K = np.arange(grids.nz)[None, None, :]
+    S = int(npsi[0] + npsi[1])
+    psi = np.zeros((S, 2, grids.nx, grids.ny, grids.nz), dtype=np.complex128)
+    start = 0
+    for iq in range(2):
+        for m in range(int(npsi[iq])):
+            k = order[m // 2]
+            ph = (I * ((2.0 * math.pi * k[0] + bang[0]) / grids.nx) + J * ((2.0 * math.pi * k[1] + bang[1]) / grids.ny)
+                  + K * ((2.0 * math.pi * k[2] + bang[2]) / grids.nz))
+            psi[start + m, m % 2] = np.cos(ph) + 1j * np.sin(ph)
+            psi[start + m] /= math.sqrt(np.sum(np.abs(psi[start + m]) ** 2) * grids.wxyz)
+        start += int(npsi[iq])
+    return psi
+
+
+def planewaves(grids, levels, params, static, forces):
+    """Plane-wave initial states and occupations on the device
+    (static.py:807-918; the reference reads an undefined global for the
+    nucleon mass, here it is forces.nucleon_mass)."""
+    psi = planewaves_host(grids, levels.npsi, levels.nneut, levels.nprot, forces.nucleon_mass, params.hbc)
+    levels.psi.copy_(torch.from_numpy(psi).to(levels.psi.dtype))
+    levels.wocc.zero_()
+    levels.wocc[:levels.nneut] = 1.0
+    n0 = int(levels.npsi[0])
+    levels.wocc[n0:n0 + levels.nprot] = 1.0
+    levels.wguv.zero_()
+    return levels
+
+
+CHECKPOINT_ARRAYS = ("psi", "lagrange", "hampsi", "hmfpsi", "delpsi", "sp_energy", "sp_norm", "sp_efluct1",
+                     "sp_efluct2", "sp_kinetic", "sp_orbital", "sp_spin", "sp_parity", "wocc", "wguv", "pairwg",
+                     "wstates", "deltaf")
+CHECKPOINT_STATIC = ("hmatrix", "gapmatrix", "symcond", "lambda_save", "acc")
+CHECKPOINT_PAIRS = ("eferm", "epair", "avdelt", "avdeltv2", "avg")
+
+
+def save_checkpoint(path, levels, params, static, pairs):
+    """Write the wave functions and the solver state a restart needs (the
+    reference has the restart flag params.trestart and the file name
+    params.wffile, static.py:1700-1701, but no writer): one .npz with the
+    level arrays, the subspace matrices, the pairing results, the iteration
+    number and x0dmp."""
+    out = {f"levels.{k}": getattr(levels, k).detach().cpu().numpy() for k in CHECKPOINT_ARRAYS}
+    out.update({f"static.{k}": getattr(static, k).detach().cpu().numpy() for k in CHECKPOINT_STATIC})
+    out.update({f"pairs.{k}": getattr(pairs, k).detach().cpu().numpy() for k in CHECKPOINT_PAIRS})
+    out["npsi"] = np.asarray(levels.npsi, dtype=np.int64)
+    out["iteration"] = np.int64(params.iteration)
+    out["x0dmp"] = np.float64(static.x0dmp)
+    out["pairs_firstcall"] = np.int64(bool(pairs.firstcall))
+    tmp = f"{path}.tmp.npz"
+    np.savez(tmp, **out)
+    import os
+    os.replace(tmp, path)
+
+
+def load_checkpoint(path, levels, params, static, pairs):
+    """Read a save_checkpoint file into the existing device arrays (shapes and
+    precision of this run; complex128 data is narrowed for a mixed run) and
+    set params.trestart and params.iteration for the restart."""
+    with np.load(path, allow_pickle=False) as d:
+        if tuple(d["npsi"]) != tuple(int(v) for v in levels.npsi):
+            raise ValueError(f"checkpoint has npsi {tuple(d['npsi'])}, this run {tuple(levels.npsi)}")
+        for pre, obj, names in (("levels", levels, CHECKPOINT_ARRAYS), ("static", static, CHECKPOINT_STATIC),
+                                ("pairs", pairs, CHECKPOINT_PAIRS)):
+            for k in names:
+                t = getattr(obj, k)
+                a = d[f"{pre}.{k}"]
+                if tuple(a.shape) != tuple(t.shape):
+                    raise ValueError(f"checkpoint array {pre}.{k} has shape {a.shape}, expected {tuple(t.shape)}")
+                t.copy_(torch.from_numpy(a).to(t.dtype))
+        params.iteration = int(d["iteration"])
+        static.x0dmp = float(d["x0dmp"])
+        pairs.firstcall = bool(int(d["pairs_firstcall"]))
+    params.trestart = True
+    return levels, params, static, pairs

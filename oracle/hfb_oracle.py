@@ -1219,6 +1219,112 @@ def observables(densities, grids, nneut, nprot):
                 beta=np.sqrt(beta20 ** 2 + 2.0 * beta22 ** 2))
 
 
+def complete_observables(densities, levels, grids, current=None):
+    """Every observable of output.py:177-354 as a dict: N, c.m. (per isospin
+    and total), isovector dipole, rms radii, second moments, quadrupole
+    moments (per isospin, total, isovector q20T1), beta20/22, beta, gamma,
+    <L>, <S>, <J> summed over states with wocc > 1e-10, and the momenta from
+    the current when it is given."""
+    ob = observables(densities, grids, levels.nneut, levels.nprot)
+    rho = densities.rho
+    w = grids.wxyz
+    X = grids.x[:, None, None]
+    Y = grids.y[None, :, None]
+    Z = grids.z[None, None, :]
+    pnr, cm = ob["pnr"], ob["cm"]
+    pnrtot = pnr[0] + pnr[1]
+    cmtot = (pnr[0] * cm[:, 0] + pnr[1] * cm[:, 1]) / pnrtot
+    x2m = np.zeros((3, 2))
+    for iq in range(2):
+        for i, c in enumerate((X, Y, Z)):
+            x2m[i, iq] = np.sum((c - cm[i, iq]) ** 2 * rho[iq]) * w / pnr[iq]
+    x2mtot = (pnr[0] * x2m[:, 0] + pnr[1] * x2m[:, 1]) / pnrtot
+    q20T1 = -ob["q20"][0] / levels.nneut + ob["q20"][1] / levels.nprot
+    b20, b22 = ob["beta20tot"], ob["beta22tot"]
+    gamma = abs(math.atan2(math.sqrt(2.0) * b22, b20)) * 180.0 / math.pi
+    if gamma > 120.0:
+        gamma -= 120.0
+    if 60.0 < gamma <= 120.0:
+        gamma = 120.0 - gamma
+    dip = np.array([np.sum(c * (rho[0] / pnr[0] - rho[1] / pnr[1])) * w for c in (X, Y, Z)])
+    sel = levels.wocc > 1e-10
+    wt = (levels.wocc * levels.wstates)[sel]
+    orb = (wt[:, None] * levels.sp_orbital[sel]).sum(axis=0)
+    spn = (wt[:, None] * levels.sp_spin[sel]).sum(axis=0)
+    pcm = np.zeros((3, 2))
+    if current is not None:
+        for iq in range(2):
+            for i in range(3):
+                pcm[i, iq] = np.sum(current[iq, i]) * w
+    out = dict(ob)
+    out.update(pnrtot=pnrtot, cmtot=cmtot, x2m=x2m, x2mtot=x2mtot, q20T1=q20T1, gamma=gamma,
+               dipole_isovector=dip, orbital_momentum=orb, spin_momentum=spn, total_momentum=orb + spn, pcm=pcm)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# State ordering and plane-wave start              static.py:807-918, 1338-1392
+# ---------------------------------------------------------------------------
+SORTED_FIELDS = ("psi", "lagrange", "sp_energy", "deltaf", "sp_norm", "sp_efluct1", "sp_efluct2", "wocc", "wguv",
+                 "wstates", "pairwg")
+
+
+def sort_states(forces, grids, levels, isospin):
+    """Order the states of one isospin block by sp_energy (stable), with the
+    eleven per-state arrays the reference moves along (static.py:1338-1392).
+    The block is the isospin's npsi states (the reference slices by
+    nneut/nprot, which is the same block only when npsi = (N, Z))."""
+    sl = levels.block(isospin)
+    perm = sl.start + np.argsort(levels.sp_energy[sl], kind="stable")
+    for name in SORTED_FIELDS:
+        a = getattr(levels, name)
+        a[sl] = a[perm]
+    return levels
+
+
+def planewave_order(grids, nucleon_mass, nmax=7):
+    """Wave-number triples (kx, ky, kz), each in -nmax//2 .. nmax//2 with kx
+    outermost, ordered by the kinetic energy hbc^2/2m sum_a ((2 pi k_a +
+    bang_a) / (n_a d_a))^2, stable (static.py:807-845)."""
+    r = range(-nmax // 2, nmax // 2 + 1)
+    ks = [(kx, ky, kz) for kx in r for ky in r for kz in r]
+    bang = (getattr(grids, "bangx", 0.0), getattr(grids, "bangy", 0.0), getattr(grids, "bangz", 0.0))
+    ns = (grids.nx, grids.ny, grids.nz)
+    ds = (grids.dx, grids.dy, grids.dz)
+    en = np.array([HBC ** 2 / (2.0 * nucleon_mass) * sum(((2.0 * math.pi * k[a] + bang[a]) / (ns[a] * ds[a])) ** 2
+                                                         for a in range(3)) for k in ks])
+    return [ks[i] for i in np.argsort(en, kind="stable")]
+
+
+def planewaves(grids, levels, forces):
+    """Plane-wave initial states (static.py:807-918): per isospin block, the
+    wave numbers in the order of planewave_order, each first with spin up
+    then spin down, normalised; wocc = 1 for the first N (Z) states of each
+    block, wguv = 0."""
+    order = planewave_order(grids, forces.nucleon_mass)
+    I = np.arange(grids.nx)[:, None, None]
+    J = np.arange(grids.ny)[None, :, None]
+    K = np.arange(grids.nz)[None, None, :]
+    bang = (getattr(grids, "bangx", 0.0), getattr(grids, "bangy", 0.0), getattr(grids, "bangz", 0.0))
+    psi = np.zeros_like(levels.psi)
+    for iq in range(2):
+        sl = levels.block(iq)
+        for m in range(sl.stop - sl.start):
+            k = order[m // 2]
+            ph = (I * ((2.0 * math.pi * k[0] + bang[0]) / grids.nx) + J * ((2.0 * math.pi * k[1] + bang[1]) / grids.ny)
+                  + K * ((2.0 * math.pi * k[2] + bang[2]) / grids.nz))
+            s = sl.start + m
+            psi[s, m % 2] = np.cos(ph) + 1j * np.sin(ph)
+            psi[s] /= math.sqrt(np.sum(np.abs(psi[s]) ** 2) * grids.wxyz)
+    levels.psi = psi
+    levels.wocc = np.zeros(levels.nstmax)
+    levels.wocc[:levels.nneut] = 1.0
+    n0 = int(levels.npsi[0])
+    levels.wocc[n0:n0 + levels.nprot] = 1.0
+    levels.wguv = np.zeros(levels.nstmax)
+    return levels
+
+
 # ---------------------------------------------------------------------------
 # Initial states                                             static.py:580-738
 # ---------------------------------------------------------------------------
@@ -1325,10 +1431,13 @@ def setup(name="SLy4", n=24, d=1.0, nneut=8, nprot=8, npsi=None, b=1.8, beta2=0.
                    Params(mprint=mprint), Energies(), Pairs())
 
 
-def prelude(st: HFState):
-    """Steps before the loop (static.py:1700-1740)."""
-    st.params.iteration = 0
-    diagstep(st.energies, st.forces, st.grids, st.levels, st.static, False, True)
+def prelude(st: HFState, restart=False):
+    """Steps before the loop (static.py:1700-1740).  restart: the reference's
+    params.trestart branch (static.py:1700-1706), which keeps the iteration
+    number and skips the first orthonormalisation."""
+    if not restart:
+        st.params.iteration = 0
+        diagstep(st.energies, st.forces, st.grids, st.levels, st.static, False, True)
     add_density(st.densities, st.grids, st.levels)
     skyrme(st.coulomb, st.densities, st.forces, st.grids, st.meanfield)
     grstep(st.forces, st.grids, st.levels, st.meanfield, st.params, st.static)
@@ -1362,6 +1471,9 @@ def iteration(st: HFState, i: int):
     d.tau = 0.2 * d.tau + 0.8 * old_tau
     d.chi = 0.2 * d.chi + 0.8 * old_chi
     skyrme(st.coulomb, d, f, st.grids, st.meanfield)
+    if s.tsort:  # static.py:1829-1831
+        for iq in range(2):
+            sort_states(f, st.grids, st.levels, iq)
     sp_properties(f, st.grids, st.levels)
     integ_energy(st.coulomb, d, st.energies, f, st.grids, st.levels, st.pairs)
     sum_energy(st.energies, st.levels, st.meanfield, st.pairs)
