@@ -310,6 +310,37 @@ int hfb_hfb_matrices(hfb_ctx* c, int iq, int start, int nst, int nmax, const dou
                      double* gapmatrix, double* symcond, double* lambda_save, double* efluct1q, double* efluct2q,
                      double* efluct1, double* efluct2, double* lam, double* work, void* stream);
 
+/* hfb_natural (static.py:920-1070): gradient sub-iterations of the
+ * canonical basis of isospin iq (states start .. start+nst-1; the
+ * reference slices by nneut/nprot) in configuration space.  phase 0 runs
+ * the iternat sub-iterations: cm <- QR(cm - dampstep * symcond) (Householder
+ * with LAPACK's conventions, the Q of jnp.linalg.qr), htran = cm^H h cm,
+ * gaptran likewise, sp_energy and deltaf of the block, the one-isospin
+ * pairing update between sub-iterations (static.py:1170-1336; its brentq
+ * call always raises, so the Fermi energy is the midpoint of levels N-1 and
+ * N), symcond <- w hcm - wuv gapcm - cm lambda.  The caller then runs the
+ * full pair (hfb_bcs) of the last sub-iteration when ipair, and phase 1
+ * writes lambda_save, hmatrix, gapmatrix of iq, cm_out = cm and m2_out =
+ * cm htran (nst x nst complex128) for psi <- psi cm and lagrange <- psi cm
+ * htran (hfb_rotate).  hmatrix..lambda_save: (2, nmax, nmax) complex128;
+ * eferm..avg: (2) doubles.  work: 10 nst^2 + nst complex doubles, kept
+ * between the phases.  nst <= 256. */
+int hfb_natural(int phase, int iq, int start, int nst, int nmax, int iternat, int ipair, double delstepbas,
+                double e0bas, double npart, double pair_cut, double state_cut, double softcut, double* hmatrix,
+                double* gapmatrix, double* symcond, double* lambda_save, double* sp_energy, double* deltaf,
+                double* wocc, double* wguv, double* pairwg, double* wstates, double* eferm, double* epair,
+                double* avdelt, double* avdeltv2, double* avg, double* work, double* cm_out, double* m2_out,
+                void* stream);
+
+/* lastdiag (static.py:1072-1168) of isospin iq over states start ..
+ * start+nst-1: eigenvalues of the Hermitian part of (h - eferm) xi + Delta
+ * eta in the canonical basis (xi_ij = -sqrt(v_i v_j) + sqrt((1-v_i)(1-v_j)),
+ * eta_ij = sqrt(v_i(1-v_j)) + sqrt(v_j(1-v_i)), v = wocc) in ascending
+ * order into qp_energy, the norms of their lower components into qp_norm
+ * (nst doubles each).  work: nst (nst + 1) / 2 + 2 nst^2 complex doubles. */
+int hfb_lastdiag(int iq, int start, int nst, int nmax, double eferm, const double* hmatrix, const double* gapmatrix,
+                 const double* wocc, double* qp_energy, double* qp_norm, double* work, void* stream);
+
 /* out1 = psi_2d U1 and out2 = psi_2d U2 over the nst states of one block
  * (static.py:316 psi_2d @ unitary; static.py:557 lagrange = psi_new @
  * lambda+), psi (nst, K), U1/U2 (nst, nst) complex128.  out1 may alias psi

@@ -84,6 +84,8 @@ _SIGS = {
     "hfb_loewdin": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hfb_hfb_matrices": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int]
                          + [c_void_p] * 15 + [c_void_p]),
+    "hfb_natural": (c_int, [c_int] * 7 + [c_double] * 6 + [c_void_p] * 18 + [c_void_p]),
+    "hfb_lastdiag": (c_int, [c_int, c_int, c_int, c_int, c_double] + [c_void_p] * 6 + [c_void_p]),
     "hfb_density_sizes": (c_int, [c_void_p, c_int, POINTER(ctypes.c_longlong), POINTER(ctypes.c_longlong)]),
     "hfb_density": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),

@@ -871,6 +871,301 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
   if (a.construct) diag_finish_arrive(a);
 }
 
+// ---------------------------------------------------------------------------
+// Configuration-space sub-iterations of the canonical basis (static.py:920-
+// 1070, hfb_natural) and the quasi-particle diagonalisation (static.py:1072-
+// 1168, lastdiag), one workgroup, matrices in global memory (L2 resident).
+// ---------------------------------------------------------------------------
+constexpr int NAT_NT = 256;
+
+struct NatArgs {
+  int iq, start, n, nmax, iternat, ipair, phase, e0bas_zero;
+  double delstepbas, e0bas, npart, pair_cut, state_cut, softcut;
+  cx<double>* hm;  // static arrays, rows of iq, leading dimension nmax
+  cx<double>* gm;
+  cx<double>* sym;
+  cx<double>* ls;
+  double *sp_energy, *deltaf, *wocc, *wguv, *pairwg, *wstates;
+  double *eferm, *epair, *avdelt, *avdeltv2, *avg;  // (2)
+  cx<double>* W;       // 10 n^2 + n complex: cm h g ht gt hcm gcm lam tmp q, tau
+  cx<double>* cm_out;  // phase 1: cm
+  cx<double>* m2_out;  // phase 1: cm htran
+};
+
+__device__ __forceinline__ cx<double> cdiv(cx<double> a, cx<double> b) {
+  const double d = b.x * b.x + b.y * b.y;
+  return {(a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d};
+}
+
+// Q of the QR factorisation of A (n x n, overwritten by the reflectors):
+// Householder reflections with LAPACK's zgeqr2/zlarfg conventions (real
+// R_kk = beta = -sign(Re alpha) |column|, tau = (beta - alpha)/beta,
+// v_k = 1, H_k^H applied from the left) and Q = H_0 H_1 ... H_{n-1}
+// accumulated backwards as zung2r does, i.e. the Q of numpy.linalg.qr and
+// jnp.linalg.qr.
+template <int NT>
+__device__ void householder_q(cx<double>* A, cx<double>* Q, cx<double>* tau, int n, double* red) {
+  const int tid = threadIdx.x;
+  for (int k = 0; k < n; ++k) {
+    double xn2 = 0.0, dummy = 0.0;
+    for (int i = k + 1 + tid; i < n; i += NT) xn2 += cabs2(A[i * n + k]);
+    block_sum2_all<NT>(xn2, dummy, red);
+    const cx<double> alpha = A[k * n + k];
+    cx<double> tk = {0.0, 0.0};
+    if (!(xn2 == 0.0 && alpha.y == 0.0)) {
+      const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+      const double beta = alpha.x >= 0.0 ? -nrm : nrm;
+      tk = {(beta - alpha.x) / beta, -alpha.y / beta};
+      const cx<double> sc = cdiv(cx<double>{1.0, 0.0}, cx<double>{alpha.x - beta, alpha.y});
+      for (int i = k + 1 + tid; i < n; i += NT) A[i * n + k] = cmul(A[i * n + k], sc);
+      __syncthreads();
+      if (tid == 0) A[k * n + k] = {beta, 0.0};
+    }
+    if (tid == 0) tau[k] = tk;
+    __syncthreads();
+    // columns j > k: A <- (I - conj(tau) v v^H) A
+    const cx<double> ct = cconj(tk);
+    for (int j = k + 1 + tid; j < n; j += NT) {
+      cx<double> w = A[k * n + j];
+      for (int i = k + 1; i < n; ++i) w = cadd(w, cmul(cconj(A[i * n + k]), A[i * n + j]));
+      const cx<double> f = cmul(ct, w);
+      A[k * n + j] = csub(A[k * n + j], f);
+      for (int i = k + 1; i < n; ++i) A[i * n + j] = csub(A[i * n + j], cmul(A[i * n + k], f));
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < n * n; e += NT) Q[e] = cx<double>{(e / n) == (e % n) ? 1.0 : 0.0, 0.0};
+  __syncthreads();
+  for (int k = n - 1; k >= 0; --k) {
+    const cx<double> tk = tau[k];
+    for (int j = k + tid; j < n; j += NT) {
+      cx<double> w = Q[k * n + j];
+      for (int i = k + 1; i < n; ++i) w = cadd(w, cmul(cconj(A[i * n + k]), Q[i * n + j]));
+      const cx<double> f = cmul(tk, w);
+      Q[k * n + j] = csub(Q[k * n + j], f);
+      for (int i = k + 1; i < n; ++i) Q[i * n + j] = csub(Q[i * n + j], cmul(A[i * n + k], f));
+    }
+    __syncthreads();
+  }
+}
+
+// L (n x n) <- Hermitian part of w_j H_ij - wuv_j G_ij (static.py:1036-1053)
+template <int NT>
+__device__ void nat_lambda(const NatArgs& a, cx<double>* L, const cx<double>* H, const cx<double>* G) {
+  const int n = a.n;
+  for (int e = threadIdx.x; e < n * n; e += NT) {
+    const int i = e / n, j = e - i * n;
+    const int si = a.start + i, sj = a.start + j;
+    const double wj = a.wocc[sj] * a.wstates[sj], uj = a.wguv[sj] * a.pairwg[sj] * a.wstates[sj];
+    const double wi = a.wocc[si] * a.wstates[si], ui = a.wguv[si] * a.pairwg[si] * a.wstates[si];
+    const cx<double> lij = csub(cscale(H[i * n + j], wj), cscale(G[i * n + j], uj));
+    const cx<double> lji = csub(cscale(H[j * n + i], wi), cscale(G[j * n + i], ui));
+    L[e] = cscale(cadd(lij, cconj(lji)), 0.5);
+  }
+  __syncthreads();
+}
+
+// BCS of the block inside the sub-iterations (static.py:1170-1336): the
+// reference's brentq call (rtol=1e-16, below SciPy's minimum) always
+// raises, so the Fermi energy is its fallback, the midpoint of block levels
+// N-1 and N (or their mean when the block has no such pair).
+template <int NT>
+__device__ void nat_pair(const NatArgs& a, double* red) {
+  const int n = a.n, tid = threadIdx.x;
+  const double* e = a.sp_energy + a.start;
+  const int nf = (int)llround(a.npart) - 1;
+  double ef;
+  if (nf >= 0 && nf < n - 1) {
+    ef = 0.5 * (e[nf] + e[nf + 1]);
+  } else {
+    double sm = 0.0, dummy = 0.0;
+    for (int i = tid; i < n; i += NT) sm += e[i];
+    block_sum2_all<NT>(sm, dummy, red);
+    ef = sm / n;
+  }
+  double v[4] = {0.0, 0.0, 0.0, 0.0};  // sumuv, sumduv, sumv2, sumdv2
+  for (int i = tid; i < n; i += NT) {
+    const int s = a.start + i;
+    const double dl = a.deltaf[s], pw = a.pairwg[s], ws = a.wstates[s];
+    const double edif = e[i] - ef, dsq = dl * pw;
+    const double eq = fmax(sqrt(edif * edif + dsq * dsq), 1e-20);
+    const double v2 = fmin(fmax(0.5 * (1.0 - edif / eq), 1e-6), 1.0 - 1e-6);
+    const double uv = sqrt(fmax(v2 * (1.0 - v2), 1e-6));
+    double wsn = ws;
+    if (a.pair_cut > 0.0)
+      a.pairwg[s] = pw / (1.0 + exp((e[i] - (ef + a.pair_cut)) / (a.softcut * a.pair_cut)));
+    if (a.state_cut > 0.0) {
+      wsn = ws / (1.0 + exp((e[i] - (ef + a.state_cut)) / (a.softcut * a.state_cut)));
+      a.wstates[s] = wsn;
+    }
+    a.wocc[s] = v2;
+    a.wguv[s] = uv;
+    const double vol = 0.5 * uv * wsn;
+    v[0] += vol;
+    v[1] += vol * dl;
+    v[2] += v2 * wsn;
+    v[3] += dl * v2 * wsn;
+  }
+  block_sum2_all<NT>(v[0], v[1], red);
+  block_sum2_all<NT>(v[2], v[3], red);
+  if (tid == 0) {
+    const int q = a.iq;
+    const double suv = fmax(v[0], 1e-20);
+    a.eferm[q] = ef;
+    a.avdelt[q] = v[1] / suv;
+    a.avdeltv2[q] = v[3] / v[2];
+    a.epair[q] = v[1];
+    a.avg[q] = v[1] / (suv * suv);
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(NAT_NT) natural_kernel(NatArgs a) {
+  constexpr int NT = NAT_NT;
+  __shared__ double red[32];
+  __shared__ double damp[NMAXJ];
+  const int n = a.n, nm = a.nmax, tid = threadIdx.x;
+  const int n2 = n * n;
+  cx<double>* const CM = a.W;
+  cx<double>* const H = CM + n2;
+  cx<double>* const G = H + n2;
+  cx<double>* const HT = G + n2;
+  cx<double>* const GT = HT + n2;
+  cx<double>* const HCM = GT + n2;
+  cx<double>* const GCM = HCM + n2;
+  cx<double>* const LAM = GCM + n2;
+  cx<double>* const TMP = LAM + n2;
+  cx<double>* const QQ = TMP + n2;
+  cx<double>* const TAU = QQ + n2;
+  if (a.phase == 1) {
+    // after the full pair of the last sub-iteration: lambda_save, hmatrix,
+    // gapmatrix, cm and cm htran for the rotation of psi and lagrange
+    nat_lambda<NT>(a, LAM, HT, GT);
+    for (int e = tid; e < n2; e += NT) {
+      const int i = e / n, j = e - i * n;
+      a.ls[i * nm + j] = LAM[e];
+      a.hm[i * nm + j] = HT[e];
+      a.gm[i * nm + j] = GT[e];
+      a.cm_out[e] = CM[e];
+    }
+    mm<NMAXJ, NT, false>(TMP, CM, HT, n);
+    for (int e = tid; e < n2; e += NT) a.m2_out[e] = TMP[e];
+    return;
+  }
+  for (int e = tid; e < n2; e += NT) {
+    const int i = e / n, j = e - i * n;
+    CM[e] = cx<double>{i == j ? 1.0 : 0.0, 0.0};
+    H[e] = a.hm[i * nm + j];
+    G[e] = a.gm[i * nm + j];
+  }
+  // damping per state (static.py:935-944)
+  {
+    const double* e = a.sp_energy + a.start;
+    double mx = -1e300, mn = 1e300;
+    for (int i = tid; i < n; i += NT) {
+      mx = fmax(mx, e[i]);
+      mn = fmin(mn, e[i]);
+    }
+    mx = block_max_all<NT>(mx, red);
+    mn = -block_max_all<NT>(-mn, red);
+    for (int i = tid; i < n; i += NT)
+      damp[i] = a.e0bas_zero ? a.delstepbas / (mx - mn) : a.delstepbas / (e[i] - e[0] + a.e0bas);
+  }
+  __syncthreads();
+  for (int it = 0; it < a.iternat; ++it) {
+    const bool last = it == a.iternat - 1;
+    for (int e = tid; e < n2; e += NT) {
+      const int j = e / n, k = e - j * n;
+      TMP[e] = csub(CM[e], cscale(a.sym[j * nm + k], damp[j]));
+    }
+    __syncthreads();
+    householder_q<NT>(TMP, CM, TAU, n, red);
+    mm<NMAXJ, NT, false>(HCM, H, CM, n);
+    mm<NMAXJ, NT, false>(GCM, G, CM, n);
+    mm<NMAXJ, NT, true>(HT, CM, HCM, n);
+    mm<NMAXJ, NT, true>(GT, CM, GCM, n);
+    if (a.ipair || last) {
+      for (int i = tid; i < n; i += NT) {
+        a.sp_energy[a.start + i] = HT[i * n + i].x;
+        if (a.ipair) a.deltaf[a.start + i] = GT[i * n + i].x * a.pairwg[a.start + i];
+      }
+      __syncthreads();
+      if (a.ipair && !last) nat_pair<NT>(a, red);
+    }
+    if (!last) {
+      nat_lambda<NT>(a, LAM, HT, GT);
+      mm<NMAXJ, NT, false>(TMP, CM, LAM, n);
+      for (int e = tid; e < n2; e += NT) {
+        const int i = e / n, j = e - i * n;
+        const int sj = a.start + j;
+        const double wj = a.wocc[sj] * a.wstates[sj], uj = a.wguv[sj] * a.pairwg[sj] * a.wstates[sj];
+        a.sym[i * nm + j] = csub(csub(cscale(HCM[e], wj), cscale(GCM[e], uj)), TMP[e]);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+struct LastArgs {
+  int iq, start, n;
+  double eferm;
+  const cx<double>* hm;  // rows of iq, leading dimension nmax
+  const cx<double>* gm;
+  int nmax;
+  const double* wocc;
+  double* qp_energy;
+  double* qp_norm;
+  cx<double>* W;  // packed n(n+1)/2 + 2 n^2
+};
+
+__global__ void __launch_bounds__(NAT_NT) lastdiag_kernel(LastArgs a) {
+  constexpr int NT = NAT_NT;
+  __shared__ JacobiShared js;
+  const int n = a.n, nm = a.nmax, tid = threadIdx.x;
+  cx<double>* const Ap = a.W;
+  cx<double>* const Vt = Ap + ploff(n);
+  cx<double>* const V = Vt + n * n;
+  // (h - eferm) xi + Delta eta, its Hermitian part, lower triangle packed
+  for (int e = tid; e < ploff(n); e += NT) {
+    const int i = prow(e), j = e - ploff(i);
+    const double vi = a.wocc[a.start + i], vj = a.wocc[a.start + j];
+    const double xi = -sqrt(vi * vj) + sqrt((1.0 - vi) * (1.0 - vj));
+    const double eta = sqrt(vi * (1.0 - vj)) + sqrt(vj * (1.0 - vi));
+    cx<double> hij = a.hm[i * nm + j], hji = a.hm[j * nm + i];
+    if (i == j) {
+      hij.x -= a.eferm;
+      hji.x -= a.eferm;
+    }
+    const cx<double> m1 = cadd(cscale(hij, xi), cscale(a.gm[i * nm + j], eta));
+    const cx<double> m2 = cadd(cscale(hji, xi), cscale(a.gm[j * nm + i], eta));
+    cx<double> v = cscale(cadd(m1, cconj(m2)), 0.5);
+    if (i == j) v.y = 0.0;
+    Ap[e] = v;
+  }
+  __syncthreads();
+  jacobi_packed<NT>(Ap, Vt, V, n, js, true);
+  // lower components: row i of the coefficient matrix has one entry, at the
+  // time-reversed partner (global index parity, static.py:1120-1131)
+  for (int j = tid; j < n; j += NT) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const int gi = a.start + i;
+      int pj = -1;
+      double c = 0.0;
+      if (gi % 2 == 0 && i >= 1) {
+        pj = i - 1;
+        c = -sqrt(a.wocc[gi]);
+      } else if (gi % 2 == 1 && i + 1 < n) {
+        pj = i + 1;
+        c = sqrt(a.wocc[gi]);
+      }
+      if (pj >= 0) s += c * c * cabs2(V[pj * n + j]);
+    }
+    a.qp_energy[j] = js.ev[j];
+    a.qp_norm[j] = sqrt(s);
+  }
+}
+
 }  // namespace
 
 // One workgroup per isospin; the work matrices of the larger block decide
@@ -1027,4 +1322,87 @@ extern "C" int hfb_hfb_matrices(hfb_ctx* c, int iq, int start, int nst, int nmax
   a.construct = 1;
   a.mode = DIAG_GIVEN_U;
   return launch_diag(a, nst, (hipStream_t)stream);
+}
+
+// Sub-iterations of hfb_natural (static.py:920-1070) for isospin iq, states
+// start .. start+nst-1.  phase 0: the iternat sub-iterations up to the
+// full pair of the last one (sp_energy, deltaf, wocc, ..., symcond and the
+// pairing results of iq updated on the way); the caller then runs the full
+// pair (hfb_bcs) when ipair; phase 1: lambda_save, hmatrix, gapmatrix of iq,
+// cm_out = cm and m2_out = cm htran (psi <- psi cm, lagrange <- psi cm htran
+// by hfb_rotate).  work: 10 nst^2 + nst complex doubles, kept between the
+// phases.
+extern "C" int hfb_natural(int phase, int iq, int start, int nst, int nmax, int iternat, int ipair, double delstepbas,
+                           double e0bas, double npart, double pair_cut, double state_cut, double softcut,
+                           double* hmatrix, double* gapmatrix, double* symcond, double* lambda_save,
+                           double* sp_energy, double* deltaf, double* wocc, double* wguv, double* pairwg,
+                           double* wstates, double* eferm, double* epair, double* avdelt, double* avdeltv2,
+                           double* avg, double* work, double* cm_out, double* m2_out, void* stream) {
+  if (phase != 0 && phase != 1) return HFB_E_ARG;
+  if (iq < 0 || iq > 1 || start < 0 || nst <= 0 || nst > nmax || nst > NDIAG_MAX || iternat <= 0) return HFB_E_SIZE;
+  if (!hmatrix || !gapmatrix || !symcond || !lambda_save || !sp_energy || !deltaf || !wocc || !wguv || !pairwg ||
+      !wstates || !eferm || !epair || !avdelt || !avdeltv2 || !avg || !work)
+    return HFB_E_ARG;
+  if (phase == 1 && (!cm_out || !m2_out)) return HFB_E_ARG;
+  NatArgs a{};
+  a.iq = iq;
+  a.start = start;
+  a.n = nst;
+  a.nmax = nmax;
+  a.iternat = iternat;
+  a.ipair = ipair;
+  a.phase = phase;
+  a.e0bas_zero = e0bas == 0.0;
+  a.delstepbas = delstepbas;
+  a.e0bas = e0bas;
+  a.npart = npart;
+  a.pair_cut = pair_cut;
+  a.state_cut = state_cut;
+  a.softcut = softcut;
+  const int64_t nm2 = (int64_t)nmax * nmax;
+  a.hm = (cx<double>*)hmatrix + iq * nm2;
+  a.gm = (cx<double>*)gapmatrix + iq * nm2;
+  a.sym = (cx<double>*)symcond + iq * nm2;
+  a.ls = (cx<double>*)lambda_save + iq * nm2;
+  a.sp_energy = sp_energy;
+  a.deltaf = deltaf;
+  a.wocc = wocc;
+  a.wguv = wguv;
+  a.pairwg = pairwg;
+  a.wstates = wstates;
+  a.eferm = eferm;
+  a.epair = epair;
+  a.avdelt = avdelt;
+  a.avdeltv2 = avdeltv2;
+  a.avg = avg;
+  a.W = (cx<double>*)work;
+  a.cm_out = (cx<double>*)cm_out;
+  a.m2_out = (cx<double>*)m2_out;
+  hipLaunchKernelGGL(natural_kernel, dim3(1), dim3(NAT_NT), 0, (hipStream_t)stream, a);
+  return launch_status();
+}
+
+// lastdiag (static.py:1072-1168) of isospin iq over states start ..
+// start+nst-1: quasi-particle energies (ascending) and the norms of their
+// lower components.  work: nst (nst + 1) / 2 + 2 nst^2 complex doubles.
+extern "C" int hfb_lastdiag(int iq, int start, int nst, int nmax, double eferm, const double* hmatrix,
+                            const double* gapmatrix, const double* wocc, double* qp_energy, double* qp_norm,
+                            double* work, void* stream) {
+  if (iq < 0 || iq > 1 || start < 0 || nst <= 0 || nst > nmax || nst > NDIAG_MAX) return HFB_E_SIZE;
+  if (!hmatrix || !gapmatrix || !wocc || !qp_energy || !qp_norm || !work) return HFB_E_ARG;
+  LastArgs a{};
+  a.iq = iq;
+  a.start = start;
+  a.n = nst;
+  a.nmax = nmax;
+  a.eferm = eferm;
+  const int64_t nm2 = (int64_t)nmax * nmax;
+  a.hm = (const cx<double>*)hmatrix + iq * nm2;
+  a.gm = (const cx<double>*)gapmatrix + iq * nm2;
+  a.wocc = wocc;
+  a.qp_energy = qp_energy;
+  a.qp_norm = qp_norm;
+  a.W = (cx<double>*)work;
+  hipLaunchKernelGGL(lastdiag_kernel, dim3(1), dim3(NAT_NT), 0, (hipStream_t)stream, a);
+  return launch_status();
 }

@@ -378,6 +378,32 @@ def hfb_matrices(ctx, iq, start, nst, U, Hold, Gold, ipair, tbcs, levels, static
     return lam
 
 
+def natural(phase, iq, start, nst, iternat, ipair, delstepbas, e0bas, npart, pair_cut, state_cut, softcut, levels,
+            static, pairs, work, cm_out=None, m2_out=None):
+    """One phase of hfb_natural's configuration-space sub-iterations."""
+    nmax = static.hmatrix.shape[-1]
+    check(lib().hfb_natural(int(phase), int(iq), int(start), int(nst), int(nmax), int(iternat), int(bool(ipair)),
+                            float(delstepbas), float(e0bas), float(npart), float(pair_cut), float(state_cut),
+                            float(softcut), ptr(static.hmatrix), ptr(static.gapmatrix), ptr(static.symcond),
+                            ptr(static.lambda_save), ptr(levels.sp_energy), ptr(levels.deltaf), ptr(levels.wocc),
+                            ptr(levels.wguv), ptr(levels.pairwg), ptr(levels.wstates), ptr(pairs.eferm),
+                            ptr(pairs.epair), ptr(pairs.avdelt), ptr(pairs.avdeltv2), ptr(pairs.avg), ptr(work),
+                            ptr(cm_out), ptr(m2_out), stream_handle()), "natural")
+
+
+def lastdiag(iq, start, nst, eferm, static, wocc):
+    """Quasi-particle energies and lower-component norms of one isospin."""
+    nmax = static.hmatrix.shape[-1]
+    dev = wocc.device
+    work = torch.empty(nst * (nst + 1) // 2 + 2 * nst * nst, dtype=torch.complex128, device=dev)
+    qe = torch.empty(nst, dtype=torch.float64, device=dev)
+    qn = torch.empty(nst, dtype=torch.float64, device=dev)
+    check(lib().hfb_lastdiag(int(iq), int(start), int(nst), int(nmax), float(eferm), ptr(static.hmatrix),
+                             ptr(static.gapmatrix), ptr(wocc), ptr(qe), ptr(qn), ptr(work), stream_handle()),
+          "lastdiag")
+    return qe, qn
+
+
 def rotate(A, U1, U2, out1, out2):
     """out1 = A U1, out2 = A U2 over states (psi_2d @ unitary, static.py:316, 557)."""
     prec = _lib.prec_of(A.dtype)

@@ -52,6 +52,7 @@ class Static:
     symcond: torch.Tensor = None
     lambda_save: torch.Tensor = None
     acc: torch.Tensor = None  # device (sumflu, delesum)
+    qp: list = None  # [(quasi-particle energies, lower norms)] per isospin after the loop (lastdiag)
 
     @property
     def sumflu(self):
@@ -426,7 +427,9 @@ def iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, m
     if forces.ipair != 0:
         pair(levels, meanfield, forces, params, grids, pairs, E=energies.vec, check_status=False)
     if forces.ipair != 0 and static.iternat > 0 and i > static.iternat_start and not forces.tbcs:
-        raise NotImplementedError("hfb_natural sub-iterations are outside the static HF+BCS path")
+        # configuration-space sub-iterations (static.py:1795-1797)
+        for iq in range(2):
+            hfb_natural(forces, grids, levels, meanfield, params, static, iq, static.iternat, pairs=pairs)
     # density with the 0.2/0.8 mixing of rho, tau, chi and the per-state
     # properties from the same transforms (static.py:1800-1833)
     # (sp_properties, static.py:1831, is finished in the same launch)
@@ -490,6 +493,8 @@ def statichf(coulomb, densities, energies, forces, grids, levels, meanfield, mom
             callback(i, levels, energies)
         if conv:
             break
+    # final quasi-particle diagonalisation (static.py:1912-1925)
+    static.qp = [lastdiag(forces, grids, levels, params, static, pairs, iq) for iq in range(2)]
     return coulomb, densities, energies, forces, grids, levels, meanfield, moment, params, static
 
 
@@ -509,6 +514,7 @@ def statichf_with_benchmark(coulomb, densities, energies, forces, grids, levels,
         times.append(time.perf_counter() - t0)
         if conv:
             break
+    static.qp = [lastdiag(forces, grids, levels, params, static, pairs, iq) for iq in range(2)]
     return (coulomb, densities, energies, forces, grids, levels, meanfield, moment, params, static), times
 
 
@@ -631,3 +637,56 @@ def load_checkpoint(path, levels, params, static, pairs):
         pairs.firstcall = bool(int(d["pairs_firstcall"]))
     params.trestart = True
     return levels, params, static, pairs
+
+
+# ---------------------------------------------------------------------------
+# HFB in the canonical basis (static.py:920-1168)
+# ---------------------------------------------------------------------------
+def _block(levels, isospin):
+    n0 = int(levels.npsi[0])
+    return (0, n0) if isospin == 0 else (n0, n0 + int(levels.npsi[1]))
+
+
+def hfb_natural(forces, grids, levels, meanfield, params, static, isospin, iternat, pairs=None):
+    """Sub-iterations of the canonical basis of one isospin in configuration
+    space (static.py:920-1070) over its npsi states: damped steps along
+    symcond with QR re-orthonormalisation, transformed h and gap matrices,
+    s.p. energies and gaps, the one-isospin pairing update between
+    sub-iterations and the full pair at the last one, then psi <- psi cm and
+    lagrange <- psi_new htran (as the reference does), hmatrix, gapmatrix,
+    lambda_save.  The reference's version slices by nneut/nprot and calls the
+    pairing with no pairing data; here ``pairs`` is that data (required with
+    pairing)."""
+    a, b = _block(levels, isospin)
+    nst = b - a
+    if nst <= 0:
+        return levels, static
+    ipair = forces.ipair != 0
+    if pairs is None:
+        if ipair:
+            raise ValueError("hfb_natural with pairing needs the pairing data (pairs=)")
+        from .pairs import Pairs
+        pairs = Pairs()
+    ctx = grids.ctx
+    c128 = torch.complex128
+    work = ctx.work(("natural", nst), (10 * nst * nst + nst,), c128)
+    npart = levels.nneut if isospin == 0 else levels.nprot
+    args = (isospin, a, nst, int(iternat), ipair, static.delstepbas, static.e0bas, npart,
+            forces.pair_cutoff[isospin], forces.state_cutoff[isospin], forces.softcut_range, levels, static, pairs,
+            work)
+    kernels.natural(0, *args)
+    if ipair:
+        pair(levels, meanfield, forces, params, grids, pairs, check_status=False)
+    cm = ctx.work(("natural_cm", nst), (nst, nst), c128)
+    m2 = ctx.work(("natural_m2", nst), (nst, nst), c128)
+    kernels.natural(1, *args, cm_out=cm, m2_out=m2)
+    kernels.rotate(levels.psi[a:b], cm, m2, levels.psi[a:b], levels.lagrange[a:b])
+    return levels, static
+
+
+def lastdiag(forces, grids, levels, params, static, pairs, isospin):
+    """Quasi-particle energies (ascending) and the norms of their lower
+    components for one isospin (static.py:1072-1168) over its npsi states;
+    device tensors (nst,), (nst,)."""
+    a, b = _block(levels, isospin)
+    return kernels.lastdiag(isospin, a, b - a, float(pairs.eferm[isospin]), static, levels.wocc)

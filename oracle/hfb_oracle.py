@@ -716,6 +716,8 @@ class Static:
     maxiter: int = 200
     iternat: int = 100
     iternat_start: int = 40
+    delstepbas: float = 2.0
+    e0bas: float = 10.0
     iteranneal: int = 0
     pairenhance: float = 0.0
     inibcs: int = 30
@@ -1062,6 +1064,149 @@ def pair(levels, meanfield, forces, params, grids, pairs):
         if forces.state_cutoff[iq] > 0.0:
             levels.wstates = wstates_full
     return levels, pairs
+
+
+# ---------------------------------------------------------------------------
+# HFB sub-iterations in configuration space                 static.py:920-1336
+# ---------------------------------------------------------------------------
+def lambda_matrix(hmat, gapmat, levels, sl):
+    """Hermitian part of w_j H_ij - wuv_j G_ij (static.py:1036-1053)."""
+    w = levels.wocc[sl] * levels.wstates[sl]
+    wuv = levels.wguv[sl] * levels.pairwg[sl] * levels.wstates[sl]
+    lt = w[None, :] * hmat - wuv[None, :] * gapmat
+    return 0.5 * (lt + np.conj(lt.T))
+
+
+def pair_subiter(levels, forces, pairs, iq):
+    """BCS of one isospin inside the sub-iterations (static.py:1170-1336).
+    The reference calls brentq with rtol=1e-16, below SciPy's minimum of
+    4 eps, so brentq raises and the reference always takes its fallback:
+    the midpoint of levels N-1 and N of the block (in block order)."""
+    sl = levels.block(iq)
+    pn = float(levels.nneut if iq == 0 else levels.nprot)
+    e = levels.sp_energy[sl].copy()
+    dl = levels.deltaf[sl].copy()
+    ws = levels.wstates[sl].copy()
+    pw = levels.pairwg[sl].copy()
+    nf = int(round(pn)) - 1
+    ef0 = 0.5 * (e[nf] + e[nf + 1]) if 0 <= nf < len(e) - 1 else float(np.mean(e))
+
+    def f(x):
+        edif = e - x
+        eq = np.maximum(np.sqrt(edif ** 2 + dl ** 2), 1e-20)
+        return float(np.sum(np.clip(0.5 * (1.0 - edif / eq), 1e-10, 1.0 - 1e-10) * ws)) - pn
+    try:
+        ef = brentq(f, a=-100.0, b=100.0, xtol=1e-14, rtol=1e-16)
+    except ValueError:
+        ef = float(ef0)
+    edif = e - ef
+    dsq = dl * pw
+    eq = np.maximum(np.sqrt(edif ** 2 + dsq ** 2), 1e-20)
+    v2 = np.clip(0.5 * (1.0 - edif / eq), 1e-6, 1.0 - 1e-6)
+    wguv = np.sqrt(np.maximum(v2 * (1.0 - v2), 1e-6))
+    wsn, pwn = ws, pw
+    if forces.pair_cutoff[iq] > 0.0:
+        pwn = pw * soft_cutoff(e, ef + forces.pair_cutoff[iq], forces.softcut_range * forces.pair_cutoff[iq])
+    if forces.state_cutoff[iq] > 0.0:
+        wsn = ws * soft_cutoff(e, ef + forces.state_cutoff[iq], forces.softcut_range * forces.state_cutoff[iq])
+    vol = 0.5 * wguv * wsn
+    sumuv = max(float(np.sum(vol)), 1e-20)
+    sumduv = float(np.sum(vol * dl))
+    sumv2 = float(np.sum(v2 * wsn))
+    sumdv2 = float(np.sum(dl * v2 * wsn))
+    pairs.eferm[iq] = ef
+    pairs.avdelt[iq] = sumduv / sumuv
+    pairs.avdeltv2[iq] = sumdv2 / sumv2
+    pairs.epair[iq] = sumduv
+    pairs.avg[iq] = sumduv / (sumuv * sumuv)
+    levels.wocc[sl] = v2
+    levels.wguv[sl] = wguv
+    if forces.pair_cutoff[iq] > 0.0:
+        levels.pairwg[sl] = pwn
+    if forces.state_cutoff[iq] > 0.0:
+        levels.wstates[sl] = wsn
+    return levels
+
+
+def hfb_natural(forces, grids, levels, meanfield, params, static, pairs, isospin, iternat):
+    """Gradient sub-iterations of the canonical basis in configuration space
+    (static.py:920-1070) for the npsi states of one isospin (the reference
+    slices by nneut/nprot and passes no pairing data, Q3): damped step along
+    symcond, QR re-orthonormalisation (numpy.linalg.qr, LAPACK Householder
+    as jnp.linalg.qr), transformed matrices, s.p. energies and gaps, pairing
+    (a one-isospin sub-solve, the full pair at the last sub-iteration), new
+    symcond; then psi <- psi cm, lagrange <- psi_new htran (the reference
+    passes htran where its helper expects lambda), hmatrix, gapmatrix,
+    lambda_save."""
+    sl = levels.block(isospin)
+    nst = sl.stop - sl.start
+    cm = np.eye(nst, dtype=np.complex128)
+    e = levels.sp_energy[sl]
+    if static.e0bas == 0.0:
+        dampstep = np.full(nst, static.delstepbas / (np.max(e) - np.min(e)))
+    else:
+        dampstep = static.delstepbas / (e - levels.sp_energy[sl.start] + static.e0bas)
+    hmat = static.hmatrix[isospin, :nst, :nst].copy()
+    gapmat = static.gapmatrix[isospin, :nst, :nst].copy()
+    htran = gaptran = None
+    for it in range(iternat):
+        cm = cm - dampstep[:, None] * static.symcond[isospin, :nst, :nst]
+        cm, _ = np.linalg.qr(cm)
+        htran = np.conj(cm.T) @ hmat @ cm
+        gaptran = np.conj(cm.T) @ gapmat @ cm
+        hcm = hmat @ cm
+        gapcm = gapmat @ cm
+        last = it == iternat - 1
+        if forces.ipair != 0 or last:
+            levels.sp_energy[sl] = np.real(np.diag(htran))
+            if forces.ipair != 0:
+                levels.deltaf[sl] = np.real(np.diag(gaptran)) * levels.pairwg[sl]
+                if not last:
+                    pair_subiter(levels, forces, pairs, isospin)
+                else:
+                    pair(levels, meanfield, forces, params, grids, pairs)
+        if not last:
+            lam = lambda_matrix(htran, gaptran, levels, sl)
+            w = levels.wocc[sl] * levels.wstates[sl]
+            wuv = levels.wguv[sl] * levels.pairwg[sl] * levels.wstates[sl]
+            static.symcond[isospin, :nst, :nst] = w[None, :] * hcm - wuv[None, :] * gapcm - cm @ lam
+    shape = levels.psi[sl].shape
+    psi_new = as2d(levels.psi[sl]) @ cm
+    levels.psi[sl] = from2d(psi_new, shape)
+    levels.lagrange[sl] = from2d(psi_new @ htran, shape)
+    static.hmatrix[isospin, :nst, :nst] = htran
+    static.gapmatrix[isospin, :nst, :nst] = gaptran
+    static.lambda_save[isospin, :nst, :nst] = lambda_matrix(htran, gaptran, levels, sl)
+    return levels, static
+
+
+def lastdiag(forces, grids, levels, params, static, pairs, isospin):
+    """Quasi-particle energies and lower-component norms of one isospin
+    (static.py:1072-1168) over its npsi states (Q3): the matrix
+    (h - eferm) xi + Delta eta in the canonical basis, xi_ij =
+    -sqrt(v_i v_j) + sqrt((1-v_i)(1-v_j)), eta_ij = sqrt(v_i (1-v_j)) +
+    sqrt(v_j (1-v_i)), v = wocc; eigh; the lower components from the
+    coefficient matrix of time-reversed pairs (global index parity)."""
+    sl = levels.block(isospin)
+    nst = sl.stop - sl.start
+    v = levels.wocc[sl]
+    xi = -np.sqrt(np.outer(v, v)) + np.sqrt(np.outer(1.0 - v, 1.0 - v))
+    eta = np.sqrt(np.outer(v, 1.0 - v)) + np.sqrt(np.outer(1.0 - v, v))
+    h = static.hmatrix[isospin, :nst, :nst] - pairs.eferm[isospin] * np.eye(nst)
+    H = h * xi + static.gapmatrix[isospin, :nst, :nst] * eta
+    ev, U = np.linalg.eigh(0.5 * (H + np.conj(H.T)))
+    gi = sl.start + np.arange(nst)
+    coeff = np.zeros((nst, nst), dtype=np.complex128)
+    for i in range(nst):
+        for j in range(nst):
+            if gi[i] - gi[j] == 1 and gi[i] % 2 == 0:
+                coeff[i, j] = -np.sqrt(levels.wocc[gi[i]])
+            if gi[j] - gi[i] == 1 and gi[i] % 2 == 1:
+                coeff[i, j] = np.sqrt(levels.wocc[gi[i]])
+    low = coeff @ U
+    norm = np.sqrt(np.sum(np.abs(low) ** 2, axis=0))
+    order = np.argsort(ev, kind="stable")
+    return ev[order], norm[order]
 
 
 # ---------------------------------------------------------------------------
@@ -1464,6 +1609,9 @@ def iteration(st: HFState, i: int):
     diagstep(st.energies, f, st.grids, st.levels, s, s.tdiag, True)
     if f.ipair != 0:
         pair(st.levels, st.meanfield, f, st.params, st.grids, st.pairs)
+    if f.ipair != 0 and s.iternat > 0 and i > s.iternat_start and not f.tbcs:  # static.py:1795-1797
+        for iq in range(2):
+            hfb_natural(f, st.grids, st.levels, st.meanfield, st.params, s, st.pairs, iq, s.iternat)
     d = st.densities
     old_rho, old_tau, old_chi = d.rho.copy(), d.tau.copy(), d.chi.copy()
     add_density(d, st.grids, st.levels)
