@@ -484,6 +484,34 @@ int hfb_sum_energy(int nstates, const double* wocc, const double* wstates, const
                    void* stream);
 
 /* ---------------------------------------------------------------------
+ * Constraints on the quadrupole moments and the principal axes
+ * (constraint.py:199-344).  Fields: (nc, nvox) doubles, isoscalar (one
+ * field for both isospins), nc <= 8.
+ * --------------------------------------------------------------------- */
+
+/* upot (2, nvox) -= sum_i lambda_i field_i for both isospins
+ * (add_constraint_potential, constraint.py:199-213, as called from skyrme,
+ * meanfield.py:246-248); upot32 (optional): the float32 copy of upot,
+ * refreshed from the result. */
+int hfb_constraint_potential(int nc, long long nvox, const double* fields, const double* lambda, double* upot,
+                             float* upot32, void* stream);
+
+/* Expectation values <Q_i> = wxyz sum field_i (rho_n + rho_p), variances
+ * |<Q_i^2> - <Q_i>^2 / N| and the multiplier update of constraint.py:216-289
+ * in one launch: old[i] <- actual[i] (before_constraint_step), then
+ * lambda[i] += -qeps max(e0act,1)/x0act (<Q_i> - old[i])/(2 var + d0),
+ * qcorr[i] = c0 (<Q_i> - goal[i])/(2 var + d0), actual[i] = <Q_i>,
+ * var[i] = var.  rho: (2, nvox). */
+int hfb_constraint_tune(int nc, long long nvox, double wxyz, const double* fields, const double* rho, double qeps,
+                        double d0, double c0, double e0act, double x0act, double* lambda, const double* goal,
+                        double* actual, double* old, double* var, double* qcorr, void* stream);
+
+/* psi_s (nstates, 2, nvox) *= exp(-wocc_s wstates_s sum_i qcorr_i
+ * field_i) in place (apply_wavefunction_corrections, constraint.py:279-318). */
+int hfb_constraint_correct(int prec, void* psi, int nstates, long long nvox, int nc, const double* fields,
+                           const double* qcorr, const double* wocc, const double* wstates, void* stream);
+
+/* ---------------------------------------------------------------------
  * BCS pairing (pairs.py:25-220)
  * --------------------------------------------------------------------- */
 

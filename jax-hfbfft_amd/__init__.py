@@ -8,7 +8,7 @@ hand-written HIP kernels in ``libhfbhip.so`` (C ABI in include/hfb_hip.h).
 import importlib as _importlib
 
 _SUBMODULES = ("_lib", "kernels", "adjoint", "driver", "static", "meanfield", "densities", "coulomb", "levels", "grids",
-               "forces", "params", "energies", "inout", "pairs", "trivial", "moment", "output", "reader")
+               "forces", "params", "energies", "inout", "pairs", "trivial", "moment", "output", "reader", "constraint")
 
 
 def __getattr__(name):

@@ -84,6 +84,11 @@ _SIGS = {
     "hfb_loewdin": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hfb_hfb_matrices": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int]
                          + [c_void_p] * 15 + [c_void_p]),
+    "hfb_constraint_potential": (c_int, [c_int, ctypes.c_longlong, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "hfb_constraint_tune": (c_int, [c_int, ctypes.c_longlong, c_double, c_void_p, c_void_p] + [c_double] * 5 +
+                            [c_void_p] * 6 + [c_void_p]),
+    "hfb_constraint_correct": (c_int, [c_int, c_void_p, c_int, ctypes.c_longlong, c_int] + [c_void_p] * 4 +
+                               [c_void_p]),
     "hfb_natural": (c_int, [c_int] * 7 + [c_double] * 6 + [c_void_p] * 18 + [c_void_p]),
     "hfb_lastdiag": (c_int, [c_int, c_int, c_int, c_int, c_double] + [c_void_p] * 6 + [c_void_p]),
     "hfb_density_sizes": (c_int, [c_void_p, c_int, POINTER(ctypes.c_longlong), POINTER(ctypes.c_longlong)]),

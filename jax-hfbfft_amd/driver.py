@@ -99,7 +99,7 @@ class HFRun:
     @classmethod
     def from_config(cls, precision="f64", device="cuda:0", name="SV-bas", n=24, d=1.0, nneut=8, nprot=8, npsi=None,
                     b=1.8, beta2=0.0, ipair=0, tbcs=False, v0=None, noise=1e-3, seed=0, mprint=10, psi0=None,
-                    **static_kw):
+                    constraints=None, **static_kw):
         if precision not in ("f64", "mixed"):
             raise ValueError(f"precision must be 'f64' or 'mixed', not {precision!r}")
         dev = torch.device(device)
@@ -116,6 +116,9 @@ class HFRun:
         rx, ry, rz = deformed_radii(b, beta2)
         forces, static = st_mod.init_static(forces, levels, radinx=rx, radiny=ry, radinz=rz, **static_kw)
         coulomb = init_coulomb(grids, dev)
+        if constraints:
+            from .constraint import init_constraints
+            static.constraints = init_constraints(constraints, nneut + nprot, grids, dev)
         if psi0 is None:
             psi0 = st_mod.harmosc_host(grids, levels.npsi, nneut, nprot, (rx, ry, rz))
             if noise:

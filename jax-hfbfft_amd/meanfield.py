@@ -134,8 +134,6 @@ def skyrme(coulomb, densities, forces, grids, meanfield, params, static, constra
     and the v_pair maxima to a later kernels.energy_tail; ``with_energy``
     (with finish=False) also leaves the partials of integ_energy for it,
     from the same launch."""
-    if constraints is not None and getattr(constraints, "tconstraint", False):
-        raise NotImplementedError("constraint potentials are not part of the static path")
     if static is not None and getattr(static, "outertype", "N") != "N":
         raise NotImplementedError(f"outertype '{static.outertype}' not implemented.")
     ctx = grids.ctx
@@ -154,6 +152,9 @@ def skyrme(coulomb, densities, forces, grids, meanfield, params, static, constra
         kernels.skyrme_energy(ctx, fc, densities, derivs, coulomb.wcoul, meanfield, meanfield.f32 or None)
     else:
         kernels.skyrme(ctx, fc, densities, derivs, coulomb.wcoul, meanfield, E, meanfield.f32 or None, finish=finish)
+    if constraints is not None and constraints.tconstraint:  # meanfield.py:246-248
+        from .constraint import add_constraint_potential
+        add_constraint_potential(meanfield.upot, constraints, (meanfield.f32 or {}).get("upot"))
     meanfield.derivs_ctx = ctx
     meanfield.ecorrp_dev = E[kernels.EN["ECORRP"]]
     meanfield.vpair_max = E[kernels.EN["VMAX"]:kernels.EN["VMAX"] + 2]

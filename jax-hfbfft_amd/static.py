@@ -53,6 +53,7 @@ class Static:
     lambda_save: torch.Tensor = None
     acc: torch.Tensor = None  # device (sumflu, delesum)
     qp: list = None  # [(quasi-particle energies, lower norms)] per isospin after the loop (lastdiag)
+    constraints: object = None  # constraint.Constraints of a constrained run
 
     @property
     def sumflu(self):
@@ -397,8 +398,8 @@ def prelude(coulomb, densities, energies, forces, grids, levels, meanfield, mome
         diagstep(energies, forces, grids, levels, static, False, True)
     add_density(densities, grids, levels)
     derivs = density_derivs(grids, densities)
-    skyrme(coulomb, densities, forces, grids, meanfield, params, static, derivs=derivs, E=energies.vec,
-           mass_number=levels.mass_number)
+    skyrme(coulomb, densities, forces, grids, meanfield, params, static, constraints=static.constraints,
+           derivs=derivs, E=energies.vec, mass_number=levels.mass_number)
     grstep(forces, grids, levels, meanfield, params, static)
     if forces.ipair != 0:
         pair(levels, meanfield, forces, params, grids, pairs, E=energies.vec, check_status=False)
@@ -430,6 +431,11 @@ def iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, m
         # configuration-space sub-iterations (static.py:1795-1797)
         for iq in range(2):
             hfb_natural(forces, grids, levels, meanfield, params, static, iq, static.iternat, pairs=pairs)
+    c = static.constraints
+    if c is not None and c.tconstraint:
+        # multipliers and the correction of the states (constraint.py:328-344)
+        from .constraint import tune_constraints
+        tune_constraints(c, densities, levels, grids, static.e0dmp, static.x0dmp)
     # density with the 0.2/0.8 mixing of rho, tau, chi and the per-state
     # properties from the same transforms (static.py:1800-1833)
     # (sp_properties, static.py:1831, is finished in the same launch)
@@ -440,8 +446,9 @@ def iterate(i, coulomb, densities, energies, forces, grids, levels, meanfield, m
     # skyrme, sp_properties, integ_energy and sum_energy (static.py:1810-1833)
     # with the scalar finishing steps of the first, third and fourth in one
     # launch at the end
-    skyrme(coulomb, densities, forces, grids, meanfield, params, static, derivs=derivs, E=energies.vec,
-           mass_number=levels.mass_number, finish=False, with_energy=True, wcoul_ready=bool(params.tcoul))
+    skyrme(coulomb, densities, forces, grids, meanfield, params, static, constraints=c, derivs=derivs,
+           E=energies.vec, mass_number=levels.mass_number, finish=False, with_energy=True,
+           wcoul_ready=bool(params.tcoul))
     if static.tsort:  # static.py:1829-1831
         for iq in range(2):
             sort_states(forces, grids, levels, iq)

@@ -581,8 +581,10 @@ def lap3(grids, f):
     return apply_x(grids.der2x, f) + apply_y(grids.der2y, f) + apply_z(grids.der2z, f)
 
 
-def skyrme(coulomb, densities, forces, grids, meanfield, tcoul=True, outertype="N"):
-    """Mean-field potentials from the densities (meanfield.py:163-391)."""
+def skyrme(coulomb, densities, forces, grids, meanfield, tcoul=True, outertype="N", constraints=None):
+    """Mean-field potentials from the densities (meanfield.py:163-391); with
+    active constraints U_q -= sum_i lambda_i Q_i (meanfield.py:246-248,
+    constraint.py:199-213, the same field for both isospins)."""
     f = forces
     rho = densities.rho
     eps = 1.0e-25
@@ -635,6 +637,9 @@ def skyrme(coulomb, densities, forces, grids, meanfield, tcoul=True, outertype="
             v_pair[iq] = v0act * densities.chi[iq]
     if outertype != "N":
         raise NotImplementedError(f"outertype '{outertype}' not implemented.")
+    if constraints is not None and constraints.tconstraint:
+        cpot = np.tensordot(constraints.lambda_crank, constraints.constr_field, axes=(0, 0))
+        upot = upot - cpot[None]
     meanfield.upot, meanfield.bmass, meanfield.wlspot = upot, bmass, wl
     meanfield.aq, meanfield.spot, meanfield.divaq, meanfield.dbmass = aq, spot, divaq, dbmass
     meanfield.v_pair = v_pair
@@ -1210,6 +1215,105 @@ def lastdiag(forces, grids, levels, params, static, pairs, isospin):
 
 
 # ---------------------------------------------------------------------------
+# Constraints on the quadrupole moments and the principal axes
+#                                                         constraint.py:1-430
+# ---------------------------------------------------------------------------
+@dataclass
+class Constraints:
+    tconstraint: bool = False
+    tq_prin_axes: bool = False
+    alpha20_wanted: float = -1e99
+    alpha22_wanted: float = -1e99
+    c0constr: float = 0.8
+    d0constr: float = 1e-4
+    qepsconstr: float = 0.3
+    dampgamma: float = 1.0
+    damprad: float = 6.0
+    r0rms: float = 0.93
+    numconstraint: int = 0
+    constr_field: np.ndarray = None   # (numconstraint, nx, ny, nz), the same for both isospins
+    lambda_crank: np.ndarray = None
+    goal_crank: np.ndarray = None
+    actual_crank: np.ndarray = None
+    old_crank: np.ndarray = None
+    actual_crank2: np.ndarray = None
+    qcorr: np.ndarray = None
+
+
+def init_constraints(params, mass_number, grids):
+    """Constraint fields with the reference's normalisations and masking
+    (constraint.py:49-196): Q20 ~ 2z^2-x^2-y^2, Q22 ~ x^2-y^2, and with
+    tq_prin_axes xy, xz, yz, x, y, z (goal 0), each times
+    1/(1+exp((r-damprad)/dampgamma)); multipliers start at -0.2."""
+    c = Constraints()
+    for k in ("alpha20_wanted", "alpha22_wanted", "tq_prin_axes", "c0constr", "d0constr", "qepsconstr", "dampgamma",
+              "damprad", "r0rms"):
+        if k in params and (not k.startswith("alpha") or params[k] > -1e90):
+            setattr(c, k, params[k])
+    X = grids.x[:, None, None]
+    Y = grids.y[None, :, None]
+    Z = grids.z[None, None, :]
+    mask = 1.0 / (1.0 + np.exp((np.sqrt(X ** 2 + Y ** 2 + Z ** 2) - c.damprad) / c.dampgamma))
+    norm = c.r0rms ** 2 * mass_number ** (5.0 / 3.0)
+    fields, goals = [], []
+    if c.alpha20_wanted > -1e90:
+        fields.append(math.sqrt(math.pi / 5.0) / norm * (2.0 * Z ** 2 - X ** 2 - Y ** 2) * mask)
+        goals.append(c.alpha20_wanted)
+    if c.alpha22_wanted > -1e90:
+        fields.append(math.sqrt(1.2 * math.pi) / norm * (X ** 2 - Y ** 2) * mask)
+        goals.append(c.alpha22_wanted)
+    if c.tq_prin_axes:
+        fxy = 0.5 * math.sqrt(math.pi / 5.0) / norm
+        for f in (X * Y, X * Z, Y * Z, X, Y, Z):
+            fields.append(fxy * f * mask * np.ones((grids.nx, grids.ny, grids.nz)))
+            goals.append(0.0)
+    c.numconstraint = len(fields)
+    if not fields:
+        return c
+    c.tconstraint = True
+    c.constr_field = np.stack([np.broadcast_to(f, (grids.nx, grids.ny, grids.nz)) for f in fields])
+    n = c.numconstraint
+    c.lambda_crank = np.full(n, -0.2)
+    c.goal_crank = np.array(goals, dtype=float)
+    c.actual_crank = np.zeros(n)
+    c.old_crank = np.zeros(n)
+    c.actual_crank2 = np.zeros(n)
+    c.qcorr = np.zeros(n)
+    return c
+
+
+def constraint_expectation_values(densities, constraints, grids):
+    """<Q_i> = wxyz sum Q_i (rho_n + rho_p) and the variances
+    |<Q_i^2> - <Q_i>^2 / N| (constraint.py:216-245)."""
+    rho = densities.rho[0] + densities.rho[1]
+    numb = grids.wxyz * np.sum(rho)
+    f = constraints.constr_field
+    ex = grids.wxyz * np.sum(f * rho[None], axis=(1, 2, 3))
+    m2 = grids.wxyz * np.sum(f ** 2 * rho[None], axis=(1, 2, 3))
+    return ex, np.abs(m2 - ex ** 2 / numb)
+
+
+def tune_constraints(constraints, densities, levels, grids, e0act, x0act, apply_corrections=True):
+    """Multiplier update and wave-function correction (constraint.py:248-344):
+    lambda_i += -qeps max(e0,1)/x0 (<Q_i> - old_i)/(2 var_i + d0), qcorr_i =
+    c0 (<Q_i> - goal_i)/(2 var_i + d0), psi_s *= exp(-wocc_s wstates_s sum_i
+    qcorr_i Q_i) (the reference indexes the isospin as isospin - 1, which
+    for 0/1 labels picks the wrong field; the field is isoscalar, so the
+    isospin does not matter here)."""
+    c = constraints
+    ex, var = constraint_expectation_values(densities, c, grids)
+    c.lambda_crank = c.lambda_crank - c.qepsconstr * (max(e0act, 1.0) / x0act) * (ex - c.old_crank) / (
+        2.0 * var + c.d0constr)
+    c.qcorr = c.c0constr * (ex - c.goal_crank) / (2.0 * var + c.d0constr)
+    c.actual_crank, c.actual_crank2 = ex, var
+    if apply_corrections:
+        corr = -np.tensordot(c.qcorr, c.constr_field, axes=(0, 0))
+        w = levels.wstates * levels.wocc
+        levels.psi = levels.psi * np.exp(w[:, None, None, None, None] * corr[None, None])
+    return c
+
+
+# ---------------------------------------------------------------------------
 # Single-particle properties                                  inout.py:64-149
 # ---------------------------------------------------------------------------
 def sp_properties(forces, grids, levels, cmtot=(0.0, 0.0, 0.0)):
@@ -1556,10 +1660,11 @@ class HFState:
     params: Params
     energies: Energies
     pairs: Pairs
+    constraints: Constraints = None
 
 
 def setup(name="SLy4", n=24, d=1.0, nneut=8, nprot=8, npsi=None, b=1.8, beta2=0.0,
-          ipair=0, tbcs=False, v0=None, noise=1e-3, seed=0, mprint=10, **static_kw) -> HFState:
+          ipair=0, tbcs=False, v0=None, noise=1e-3, seed=0, mprint=10, constraints=None, **static_kw) -> HFState:
     """Build every state container and the initial wave functions
     (init sequence of test.py:145-163)."""
     forces = init_forces(name, ipair=ipair, tbcs=tbcs,
@@ -1573,7 +1678,8 @@ def setup(name="SLy4", n=24, d=1.0, nneut=8, nprot=8, npsi=None, b=1.8, beta2=0.
     if noise:
         levels.psi = add_noise(levels.psi, noise, seed)
     return HFState(forces, grids, levels, init_meanfield(grids), init_densities(grids), coulomb, static,
-                   Params(mprint=mprint), Energies(), Pairs())
+                   Params(mprint=mprint), Energies(), Pairs(),
+                   init_constraints(constraints, nneut + nprot, grids) if constraints else None)
 
 
 def prelude(st: HFState, restart=False):
@@ -1584,7 +1690,7 @@ def prelude(st: HFState, restart=False):
         st.params.iteration = 0
         diagstep(st.energies, st.forces, st.grids, st.levels, st.static, False, True)
     add_density(st.densities, st.grids, st.levels)
-    skyrme(st.coulomb, st.densities, st.forces, st.grids, st.meanfield)
+    skyrme(st.coulomb, st.densities, st.forces, st.grids, st.meanfield, constraints=getattr(st, "constraints", None))
     grstep(st.forces, st.grids, st.levels, st.meanfield, st.params, st.static)
     if st.forces.ipair != 0:
         pair(st.levels, st.meanfield, st.forces, st.params, st.grids, st.pairs)
@@ -1600,6 +1706,9 @@ def prelude(st: HFState, restart=False):
 def iteration(st: HFState, i: int):
     """One SCF iteration, the loop body of static.py:1755-1882 without output."""
     f, s = st.forces, st.static
+    c = getattr(st, "constraints", None)
+    if c is not None and c.tconstraint:  # constraint.py:321-325
+        c.old_crank = c.actual_crank.copy()
     st.params.iteration = i
     f.tbcs = True if i <= s.inibcs else st._tbcssav
     s.tdiag = not (i > s.inidiag)
@@ -1612,13 +1721,15 @@ def iteration(st: HFState, i: int):
     if f.ipair != 0 and s.iternat > 0 and i > s.iternat_start and not f.tbcs:  # static.py:1795-1797
         for iq in range(2):
             hfb_natural(f, st.grids, st.levels, st.meanfield, st.params, s, st.pairs, iq, s.iternat)
+    if c is not None and c.tconstraint:
+        tune_constraints(c, st.densities, st.levels, st.grids, s.e0dmp, s.x0dmp)
     d = st.densities
     old_rho, old_tau, old_chi = d.rho.copy(), d.tau.copy(), d.chi.copy()
     add_density(d, st.grids, st.levels)
     d.rho = 0.2 * d.rho + 0.8 * old_rho
     d.tau = 0.2 * d.tau + 0.8 * old_tau
     d.chi = 0.2 * d.chi + 0.8 * old_chi
-    skyrme(st.coulomb, d, f, st.grids, st.meanfield)
+    skyrme(st.coulomb, d, f, st.grids, st.meanfield, constraints=c)
     if s.tsort:  # static.py:1829-1831
         for iq in range(2):
             sort_states(f, st.grids, st.levels, iq)
