@@ -339,12 +339,8 @@ __global__ void __launch_bounds__(256) rotate_kernel(RotArgs ra) {
 // input of its points before their first store, so out1 may alias psi.
 typedef float rv4f __attribute__((ext_vector_type(4)));
 constexpr int ROT_MFMA_NMAX = 24;
-#ifndef HFB_ROT_TPW
-#define HFB_ROT_TPW 8  // tiles of 16 points a wavefront loads at once
-#endif
-#ifndef HFB_ROT_BPC
-#define HFB_ROT_BPC 1  // workgroups per CU
-#endif
+constexpr int ROT_TPW = 8;  // tiles of 16 points a wavefront loads at once
+constexpr int ROT_BPC = 1;  // workgroups per CU
 
 template <int KS, int MT, int TPW>
 __global__ void __launch_bounds__(256) rotate_mfma_kernel(RotArgs ra) {
@@ -551,17 +547,17 @@ int launch_rotate(int prec, RotArgs ra, int niso, hipStream_t st) {
   if (nmax == 0) return HFB_OK;
   if (prec == HFB_F32 && nmax <= ROT_MFMA_NMAX && ra.K % 16 == 0) {
     // matrix cores: KS k-steps over the 2 nmax input columns, MT tiles over
-    // the (up to) 4 nmax out columns; HFB_ROT_BPC workgroups per CU, each
-    // wavefront loading HFB_ROT_TPW consecutive tiles at a time (measured at
+    // the (up to) 4 nmax out columns; ROT_BPC workgroups per CU, each
+    // wavefront loading ROT_TPW consecutive tiles at a time (measured at
     // Ca40: 8 tiles, one workgroup per CU 20.3 us; 2 tiles, three 24.3 us)
     const int ks = (2 * nmax + 3) / 4;
     bool two = false;
     for (int iq = 0; iq < niso; ++iq) two = two || (ra.iso[iq].nst > 0 && ra.iso[iq].U2);
     const int mt = ((two ? 4 : 2) * nmax + 15) / 16;
     const int64_t ntile = ra.K / 16;
-    constexpr int tpw = HFB_ROT_TPW;
+    constexpr int tpw = ROT_TPW;
     int64_t nb = (ntile + 4 * tpw - 1) / (4 * tpw);
-    const int64_t cap = (HFB_ROT_BPC * 256 + niso - 1) / niso;
+    const int64_t cap = (ROT_BPC * 256 + niso - 1) / niso;
     if (nb > cap) nb = cap;
     const dim3 grid((unsigned)nb, niso);
 #define HFB_ROTM(KSV, MTV)                                                                     \

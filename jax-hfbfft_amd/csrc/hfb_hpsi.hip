@@ -591,126 +591,6 @@ __global__ void __launch_bounds__(256, 3) hpsi_mfma_x_kernel(const cx<float>* __
   }
 }
 
-// y-z plane pass for 32 x 32 planes (x fixed, one state per workgroup):
-// the spinor plane goes to LDS; wavefronts 0, 1 compute the y terms of the
-// 16-column z slices, wavefronts 2, 3 the z terms of the 16-row y slices,
-// both into LDS; then every thread finishes its points: h psi = x pass +
-// y terms + z terms, v_pair psi, pout = w h psi - wuv v_pair psi
-// (meanfield.py:150-157) and the overlap partial sums of the plane.
-constexpr int MP = 33;  // LDS row pitch of a 32 x 32 plane (complex)
-
-__global__ void __launch_bounds__(256, 3) hpsi_mfma_plane_kernel(const cx<float>* __restrict__ psi,
-                                                                 cx<float>* __restrict__ pmf,
-                                                                 cx<float>* __restrict__ pout,
-                                                                 cx<float>* __restrict__ pdel, MF<float> mf,
-                                                                 const double* __restrict__ w,
-                                                                 const double* __restrict__ wuv, int npsi0, int nx,
-                                                                 float dy, float dz, const float* __restrict__ Dy,
-                                                                 const float* __restrict__ Dz,
-                                                                 double* __restrict__ partial) {
-  constexpr int N = 32, NP = N * MP;
-  extern __shared__ unsigned char mp_lds[];
-  cx<float>* P = (cx<float>*)mp_lds;  // [spin][y][z]: spinor plane
-  cx<float>* H = P + 2 * NP;          // [axis y,z][spin][y][z]: axis terms
-  __shared__ double red[4 * 2];
-  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int q = lane >> 4, c = lane & 15;
-  const int x = blockIdx.x, s = blockIdx.y;
-  const int iq = s >= npsi0 ? 1 : 0;
-  const int64_t nvox = (int64_t)nx * N * N;
-  const cx<float>* pa = psi + (int64_t)(2 * s) * nvox + (int64_t)x * N * N;  // plane of spin up
-  const cx<float>* pb = pa + nvox;
-  const int64_t fo = (int64_t)iq * nvox + (int64_t)x * N * N;  // plane in an isospin field
-  for (int e = threadIdx.x; e < 2 * N * N; e += 256) {
-    const int sp = e / (N * N), yz = e % (N * N);
-    P[sp * NP + (yz / N) * MP + yz % N] = (sp ? pb : pa)[yz];
-  }
-  __syncthreads();
-  {
-    const int axis = wave < 2 ? 1 : 2;  // wavefront-uniform
-    const int sl = (wave & 1) * 16;     // z slice (y pass) or y slice (z pass)
-    float Af[2][8];
-    load_dfrag(axis == 1 ? Dy : Dz, q, c, Af);
-    float ar[8], ai[8], br[8], bi[8], Bv[8], Av[8], wa[8], wb[8];
-    const float* Bq = mf.B + fo;
-    const float* Aa = mf.A + fo + (int64_t)(iq * 2 + axis) * nvox;  // aq[iq][axis]
-    const float* W0 = mf.W + fo + (int64_t)(iq * 2) * nvox;         // wlspot[iq][0]
-    const float* Wb = mf.W + fo + (int64_t)(iq * 2 + (axis == 1 ? 2 : 1)) * nvox;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const int i = 4 * m + q;  // position along the axis
-      const int yy = axis == 1 ? i : sl + c, zz = axis == 1 ? sl + c : i;
-      const cx<float> va = P[yy * MP + zz], vb = P[NP + yy * MP + zz];
-      ar[m] = va.x;
-      ai[m] = va.y;
-      br[m] = vb.x;
-      bi[m] = vb.y;
-      const unsigned off = (unsigned)(yy * N + zz);
-      Bv[m] = ldb(Bq, 4u * off);
-      Av[m] = ldb(Aa, 4u * off);
-      wa[m] = ldb(W0, 4u * off);
-      wb[m] = ldb(Wb, 4u * off);
-    }
-    v4f h[4][2];
-    const float d = axis == 1 ? dy : dz;
-    auto none = [](int, float (&)[4]) {};
-    if (axis == 1)
-      mfma_axis<1>(Af, ar, ai, br, bi, Bv, Av, wa, wb, d, q, none, h);
-    else
-      mfma_axis<2>(Af, ar, ai, br, bi, Bv, Av, wa, wb, d, q, none, h);
-    cx<float>* Ha = H + (axis - 1) * 2 * NP;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const int b = m / 4, r = m % 4;
-      const int i = 4 * m + q;
-      const int yy = axis == 1 ? i : sl + c, zz = axis == 1 ? sl + c : i;
-      Ha[yy * MP + zz] = cx<float>{h[0][b][r], h[1][b][r]};
-      Ha[NP + yy * MP + zz] = cx<float>{h[2][b][r], h[3][b][r]};
-    }
-  }
-  __syncthreads();
-  // finish: 4 points of both spins per thread, consecutive z across lanes
-  const double ws = w[s], wu = wuv[s];
-  const float* V = mf.V + fo;
-  cx<float>* ma = pmf + (int64_t)(2 * s) * nvox + (int64_t)x * N * N;
-  cx<float>* mb = ma + nvox;
-  double acc[2] = {0.0, 0.0};
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int yz = threadIdx.x + 256 * j;
-    const int li = (yz / N) * MP + yz % N;
-    const cx<float> av = P[li], bv = P[NP + li];
-    const cx<float> xa = ma[yz], xb = mb[yz];
-    const cx<float> h0 = cadd(cadd(xa, H[li]), H[2 * NP + li]);
-    const cx<float> h1 = cadd(cadd(xb, H[NP + li]), H[3 * NP + li]);
-    ma[yz] = h0;
-    mb[yz] = h1;
-    const float v = V[yz];
-    const cx<float> e0 = cscale(av, v), e1 = cscale(bv, v);
-    if (pdel) {
-      pdel[(int64_t)(2 * s) * nvox + (int64_t)x * N * N + yz] = e0;
-      pdel[(int64_t)(2 * s + 1) * nvox + (int64_t)x * N * N + yz] = e1;
-    }
-    const cx<float> r0 = csub(cscale(h0, float(ws)), cscale(e0, float(wu)));
-    const cx<float> r1 = csub(cscale(h1, float(ws)), cscale(e1, float(wu)));
-    if (pout) {
-      pout[(int64_t)(2 * s) * nvox + (int64_t)x * N * N + yz] = r0;
-      pout[(int64_t)(2 * s + 1) * nvox + (int64_t)x * N * N + yz] = r1;
-    }
-    acc[0] += double(av.x) * double(h0.x) + double(av.y) * double(h0.y) + double(bv.x) * double(h1.x) +
-              double(bv.y) * double(h1.y);
-    acc[1] += double(av.x) * double(r0.x) + double(av.y) * double(r0.y) + double(bv.x) * double(r1.x) +
-              double(bv.y) * double(r1.y);
-  }
-  if (partial) {
-    block_sum<2>(acc, red);
-    if (threadIdx.x == 0) {
-      partial[((int64_t)s * nx + x) * 2 + 0] = acc[0];
-      partial[((int64_t)s * nx + x) * 2 + 1] = acc[1];
-    }
-  }
-}
-
 // The y and z axes of h psi for one y-z plane (x fixed) of one state, when
 // ny == nz: the plane of both spin components is read once along z pencils
 // and kept in LDS; the y pencils take it from there, compute the y terms
@@ -893,17 +773,10 @@ int launch_hpsi_plane(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T
 }
 
 // FP32 spinors on a 32^3 grid: the x pass on the matrix cores, then the
-// y-z plane pass (LaneFFT, or the matrix-core plane kernel)
+// y-z plane pass (LaneFFT)
 inline bool hpsi_mfma_path(const hfb_ctx* c, int prec) {
   return prec == HFB_F32 && c->nx == 32 && c->ny == 32 && c->nz == 32;
 }
-
-#ifndef HFB_HPSI_MFMA_PLANE
-#define HFB_HPSI_MFMA_PLANE 0
-#endif
-#ifndef HFB_TAIL_IN_PLANE
-#define HFB_TAIL_IN_PLANE 1
-#endif
 
 template <typename T>
 int launch_hpsi_plane(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<T> mf, const double* w,
@@ -914,24 +787,15 @@ int hpsi_mfma(const hfb_ctx* c, const void* psi, int ns, int npsi0, MF<float> mf
               const double* wuv, void* pout, void* pmf, void* pdel, double* partial, hipStream_t st,
               const WJob& job) {
   // the weights in the x launch; the energy tail in the plane launch, which
-  // runs longer (the matrix-core plane kernel takes no job)
-  const bool split = !HFB_HPSI_MFMA_PLANE && HFB_TAIL_IN_PLANE;
-  const WJob xj = split ? weights_part(job) : job;
+  // runs longer
+  const WJob xj = weights_part(job);
   const int rows = ns + job_rows(xj);
   hipLaunchKernelGGL(hpsi_mfma_x_kernel, dim3(c->ny * (c->nz / 16) / 4, rows), dim3(256), 0, st,
                      (const cx<float>*)psi, (cx<float>*)pmf, mf, npsi0, c->ny, c->nz, float(c->dx), c->der1f[0],
                      xj);
   int rc = launch_status();
   if (rc != HFB_OK) return rc;
-  if (!HFB_HPSI_MFMA_PLANE)
-    return launch_hpsi_plane<float>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st,
-                                    split ? tail_part(job) : WJob{});
-  const size_t lds = 6 * sizeof(cx<float>) * 32 * MP;
-  allow_full_lds((const void*)hpsi_mfma_plane_kernel);
-  hipLaunchKernelGGL(hpsi_mfma_plane_kernel, dim3(c->nx, ns), dim3(256), lds, st, (const cx<float>*)psi,
-                     (cx<float>*)pmf, (cx<float>*)pout, (cx<float>*)pdel, mf, w, wuv, npsi0, c->nx, float(c->dy),
-                     float(c->dz), c->der1f[1], c->der1f[2], partial);
-  return launch_status();
+  return launch_hpsi_plane<float>(c, psi, ns, npsi0, mf, w, wuv, pout, pmf, pdel, partial, st, tail_part(job));
 }
 
 template <typename T>
