@@ -8,9 +8,10 @@
 // Rotation: out_j = sum_i psi_i U[i][j] for two matrices at once (psi U and
 // psi (U lambda+), i.e. the rotated states and the Lagrange field).
 //
-// On gfx950 the FP64 matrix cores and the FP64 vector units have the same
-// peak rate, and 16x16 matrix tiles padded from 20 or 36 states do 1.6-2.9x
-// the work, so both run as register-tiled FMAs.
+// FP32 blocks (mixed precision) run on the FP32 matrix cores: the Gram on
+// 4x4 tiles (no padding beyond a multiple of 4 states), the rotation on
+// 16x16x4 tiles.  FP64 blocks run as register-tiled FMAs: on gfx950 the
+// FP64 matrix cores and vector units have the same peak rate.
 #include <math.h>
 
 #include "hfb_ctx.h"
@@ -224,6 +225,248 @@ __global__ void __launch_bounds__(256) gram_vec_reduce_kernel(GramRedIso r0, Gra
   }
 }
 
+// ---------------------------------------------------------------------------
+// Gram of FP32 blocks on the matrix cores (v_mfma_f32_4x4x1_16b_f32: 16
+// independent 4x4x1 outer products, "blocks", per instruction).  In real form
+// conj(a) b = (ar br + ai bi) + i (ar bi - ai br) over the interleaved
+// (re, im) floats of a state row.  A wavefront owns up to GM_BT x GM_BT
+// tiles of 4 x 4 entries of one C_m (rows of A: tiles ti0.., of B: tj0..) and
+// a chunk of the row; per group of 64 floats (32 complex points) block b
+// takes floats 4b..4b+3: lane l = 4 b + r loads that float4 of row 4 t + r
+// (16 lanes read one 256-byte row segment) and its four components enter
+// four products of the block, (re, im, re, im) against the same of B for the
+// real part and against (im, -re, im, -re) for the imaginary part.  Every
+// block accumulates its slices of the chunk in FP32 (at most GM_GMAX groups,
+// 4 GM_GMAX products); at the end the 16 block sums of each entry are added
+// through LDS (one row of tiles per pass) and written as FP64 chunk partials
+// in gram_vec_kernel's layout, summed by gram_vec_reduce_kernel.
+// Operand layout of the instruction: A lane 4b+i = A_b[i], B lane 4b+j =
+// B_b[j], result register i of lane 4b+j = D_b[i][j].
+// ---------------------------------------------------------------------------
+typedef float gm_v4 __attribute__((ext_vector_type(4)));
+constexpr int GM_BT = 5;     // tiles per block edge at most (2 x 25 accumulators fit the AGPRs)
+constexpr int GM_GMAX = 64;  // 64-float groups per chunk at most (FP32 sums of 256 products)
+constexpr int GM_GMIN = 8;   // and at least, where the row allows
+constexpr int GM_WAVES = 2048;  // wavefronts a launch aims for
+
+struct GramMfmaIso {
+  int nst, nt, nb;  // states, 4-row tiles, blocks per edge
+  int nsub;         // (matrix, block) items per chunk
+};
+
+struct GramMfmaArgs {
+  GramIso iso[2];
+  GramMfmaIso mi[2];
+  int nm, bsame;
+  int64_t Kr;    // floats per state row
+  int gpc;       // groups per chunk
+  int ngroup;    // groups per row
+  int nchunk;
+  int64_t nitem; // nchunk * (nsub0 + nsub1)
+};
+
+// Tiles [lo, hi) of block i of nb over nt tiles (balanced).
+__device__ __forceinline__ void gm_span(int i, int nb, int nt, int& lo, int& cnt) {
+  lo = i * nt / nb;
+  cnt = (i + 1) * nt / nb - lo;
+}
+
+// (matrix, block row, block column) of sub-item s: matrices in order; for a
+// matrix with B == A only blocks bi <= bj.
+__device__ __forceinline__ void gm_sub(int s, int nb, int nm, int bsame, int& m, int& bi, int& bj) {
+  m = 0;
+  for (; m < nm; ++m) {
+    const int cnt = ((bsame >> m) & 1) ? nb * (nb + 1) / 2 : nb * nb;
+    if (s < cnt) break;
+    s -= cnt;
+  }
+  if (!((bsame >> m) & 1)) {
+    bi = s / nb;
+    bj = s % nb;
+    return;
+  }
+  bi = 0;
+  while (s >= nb - bi) {
+    s -= nb - bi;
+    ++bi;
+  }
+  bj = bi + s;
+}
+
+// One 64-float group of the rows of this wavefront's tiles: the group's
+// base (uniform) plus fixed 32-bit byte offsets per lane.  The loads are
+// unconditional (rows past the block or past nst read clamped, valid rows;
+// B == A reads the same addresses), so the compiler can count the loads in
+// flight and wait for one group only.
+template <int BT>
+__device__ __forceinline__ void gm_load(gm_v4 (&a)[BT], gm_v4 (&b)[BT], const char* pa, const char* pb,
+                                        const uint32_t (&aoff)[BT], const uint32_t (&boff)[BT]) {
+#pragma unroll
+  for (int t = 0; t < BT; ++t) a[t] = *(const gm_v4*)(pa + aoff[t]);
+#pragma unroll
+  for (int u = 0; u < BT; ++u) b[u] = *(const gm_v4*)(pb + boff[u]);
+}
+
+// The eight products of every tile for one group, component by component
+// over all tiles, so that an accumulator's next product comes 2 BT^2
+// instructions later (no waiting on the matrix pipeline).  Tiles of a
+// partial block read clamped rows: computed, never summed.
+template <int BT>
+__device__ __forceinline__ void gm_mma(gm_v4 (&accr)[BT][BT], gm_v4 (&acci)[BT][BT], const gm_v4 (&a)[BT],
+                                       const gm_v4 (&b)[BT]) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+#pragma unroll
+    for (int u = 0; u < BT; ++u) {
+      // real part: (re, im) x (re, im); imaginary part: x (im, -re)
+      const float br = b[u][c];
+      const float bi = (c & 1) ? -b[u][c - 1] : b[u][c + 1];
+#pragma unroll
+      for (int t = 0; t < BT; ++t) {
+        accr[t][u] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[t][c], br, accr[t][u], 0, 0, 0);
+        acci[t][u] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[t][c], bi, acci[t][u], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// BT: tiles per block edge of this launch (at most GM_BT).  Groups are
+// loaded three ahead of their products (a ring of three register sets), so
+// the one wavefront per SIMD has two groups of products to cover a load.
+template <int BT>
+__global__ void __launch_bounds__(256, 1) gram_mfma_kernel(GramMfmaArgs ga, hfb_finish_job fjob) {
+  if (fjob.nstates > 0 && blockIdx.x == gridDim.x - 1) {  // the extra workgroup: the step's scalar tail
+    grstep_finish_body(fjob);
+    return;
+  }
+  // per wavefront: one row of tiles, (re, im) x BT tiles x 4 registers x 64 lanes
+  __shared__ gm_v4 red[4][2 * BT * 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t item = (int64_t)blockIdx.x * 4 + wv;
+  const bool live = item < ga.nitem;
+  int iq = 0, m = 0, ti0 = 0, nti = 0, tj0 = 0, ntj = 0, chunk = 0;
+  if (live) {
+    const int nsub = ga.mi[0].nsub + ga.mi[1].nsub;
+    chunk = (int)(item / nsub);
+    int s = (int)(item % nsub);
+    if (s >= ga.mi[0].nsub) {
+      s -= ga.mi[0].nsub;
+      iq = 1;
+    }
+    int bi, bj;
+    gm_sub(s, ga.mi[iq].nb, ga.nm, ga.bsame, m, bi, bj);
+    gm_span(bi, ga.mi[iq].nb, ga.mi[iq].nt, ti0, nti);
+    gm_span(bj, ga.mi[iq].nb, ga.mi[iq].nt, tj0, ntj);
+  }
+  iq = __builtin_amdgcn_readfirstlane(iq);
+  m = __builtin_amdgcn_readfirstlane(m);
+  ti0 = __builtin_amdgcn_readfirstlane(ti0);
+  nti = __builtin_amdgcn_readfirstlane(nti);
+  tj0 = __builtin_amdgcn_readfirstlane(tj0);
+  ntj = __builtin_amdgcn_readfirstlane(ntj);
+  chunk = __builtin_amdgcn_readfirstlane(chunk);
+  const GramIso& g = ga.iso[iq];
+  const int nst = ga.mi[iq].nst;
+  const bool sym = (ga.bsame >> m) & 1;
+  const char* A = (const char*)g.A;
+  const char* B = sym ? A : (const char*)g.B[m];
+  const int r = lane & 3, blk = lane >> 2;
+  // byte offsets of this lane's rows (rows past nst clamped: computed, never
+  // summed); a block is below 4 GiB (checked by the host)
+  uint32_t aoff[BT], boff[BT];
+#pragma unroll
+  for (int t = 0; t < BT; ++t) {
+    aoff[t] = (uint32_t)(((int64_t)min(4 * (ti0 + t) + r, nst - 1) * ga.Kr + 4 * blk) * 4);
+    boff[t] = (uint32_t)(((int64_t)min(4 * (tj0 + t) + r, nst - 1) * ga.Kr + 4 * blk) * 4);
+  }
+  gm_v4 accr[BT][BT], acci[BT][BT];
+#pragma unroll
+  for (int t = 0; t < BT; ++t)
+#pragma unroll
+    for (int u = 0; u < BT; ++u) accr[t][u] = acci[t][u] = gm_v4{0.f, 0.f, 0.f, 0.f};
+  if (live) {
+    const int g0 = chunk * ga.gpc, g1 = min(ga.ngroup, g0 + ga.gpc);
+    // groups past the chunk's end load the last group again (never used)
+    auto ld = [&](gm_v4(&a)[BT], gm_v4(&b)[BT], int gg) {
+      const int64_t go = (int64_t)min(gg, g1 - 1) * 256;
+      gm_load<BT>(a, b, A + go, B + go, aoff, boff);
+    };
+    gm_v4 a0[BT], b0[BT], a1[BT], b1[BT], a2[BT], b2[BT];
+    ld(a0, b0, g0);
+    ld(a1, b1, g0 + 1);
+    ld(a2, b2, g0 + 2);
+#pragma unroll 1
+    for (int gi = g0; gi < g1; gi += 3) {
+      // the scheduling barriers keep each reload after the products that read
+      // its registers (hoisting it would need a fourth register set)
+      gm_mma<BT>(accr, acci, a0, b0);
+      __builtin_amdgcn_sched_barrier(0);
+      ld(a0, b0, gi + 3);
+      __builtin_amdgcn_sched_barrier(0);
+      if (gi + 1 < g1) gm_mma<BT>(accr, acci, a1, b1);
+      __builtin_amdgcn_sched_barrier(0);
+      ld(a1, b1, gi + 4);
+      __builtin_amdgcn_sched_barrier(0);
+      if (gi + 2 < g1) gm_mma<BT>(accr, acci, a2, b2);
+      __builtin_amdgcn_sched_barrier(0);
+      ld(a2, b2, gi + 5);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // block sums, one row of tiles per pass: lane 4b+j stores registers 0..3
+  // of tile u at red[b][(part, u, j)] (a float4 per tile); lane (u, j) then
+  // adds the 16 float4 of each part (consecutive lanes read consecutive
+  // float4: no bank conflicts) and writes entries (4 t + i, 4 u + j).
+  gm_v4* R = red[wv];
+  const int j = lane & 3;
+  const int np = g.np;
+#pragma unroll
+  for (int t = 0; t < BT; ++t) {
+#pragma unroll
+    for (int u = 0; u < BT; ++u) {
+      R[blk * (8 * BT) + (0 * BT + u) * 4 + j] = accr[t][u];
+      R[blk * (8 * BT) + (1 * BT + u) * 4 + j] = acci[t][u];
+    }
+    __syncthreads();
+    if (live && t < nti && lane < 4 * ntj) {
+      const int u = lane >> 2;
+      gm_v4 sr = R[(0 * BT + u) * 4 + j];
+      gm_v4 si = R[(1 * BT + u) * 4 + j];
+#pragma unroll
+      for (int bb = 1; bb < 16; ++bb) {
+        sr += R[bb * (8 * BT) + (0 * BT + u) * 4 + j];
+        si += R[bb * (8 * BT) + (1 * BT + u) * 4 + j];
+      }
+      const int col = 4 * (tj0 + u) + j;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = 4 * (ti0 + t) + i;
+        double* dst = g.part + 2 * ((((int64_t)m * np + row) * np + col) * ga.nchunk + chunk);
+        dst[0] = (double)sr[i];
+        dst[1] = (double)si[i];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Chunks of the matrix-core Gram: at most GM_GMAX groups each, at least
+// GM_GMIN where the row allows, otherwise enough for about GM_WAVES
+// wavefronts over nsub (matrix, block) items.
+int gram_mfma_nchunk(int64_t ngroup, int nsub) {
+  int64_t nc = (GM_WAVES + nsub - 1) / (nsub > 0 ? nsub : 1);
+  const int64_t most = ngroup / GM_GMIN > 0 ? ngroup / GM_GMIN : 1;
+  if (nc > most) nc = most;
+  const int64_t least = (ngroup + GM_GMAX - 1) / GM_GMAX;
+  if (nc < least) nc = least;
+  return (int)nc;
+}
+
+int gram_mfma_blocks(int nst) {
+  const int nt = (nst + 3) / 4;
+  return (nt + GM_BT - 1) / GM_BT;
+}
+
 // Rotation out_j[k] = sum_i psi_i[k] U[i][j] (two matrices U1, U2 at
 // once).  A workgroup stages psi_i[k] for every i and its 64*KPL
 // consecutive k in LDS; each wavefront then produces outputs j in groups of
@@ -430,12 +673,41 @@ __global__ void __launch_bounds__(256) rotate_mfma_kernel(RotArgs ra) {
 
 extern "C" int hfb_gram_workspace(int nst, long long K) {
   const int np4 = (nst + GV_T - 1) / GV_T * GV_T;
-  return gram_vec_nchunk(nst, K) * 4 * np4 * np4 * 2;  // doubles for up to four matrices
+  int nc = gram_vec_nchunk(nst, K);
+  if (K % 32 == 0) {  // the matrix-core path: fewest items (one Hermitian matrix) give most chunks
+    const int nb = gram_mfma_blocks(nst);
+    const int mc = gram_mfma_nchunk(2 * K / 64, nb * (nb + 1) / 2);
+    nc = mc > nc ? mc : nc;
+  }
+  return nc * 4 * np4 * np4 * 2;  // doubles for up to four matrices
 }
 
 // Both isospin blocks in one launch: A[iq], B[iq*4 + m], out[iq] (nm, nst_iq,
 // nst_iq) complex, part[iq] workspaces of hfb_gram_workspace(nst_iq, K).
 // A block with nst 0 is skipped.
+namespace {
+// Chunk sums of both isospin blocks (gram_vec_reduce_kernel).
+int gram_reduce(const GramVecArgs& v, int nm, const int* nst, double* const* part, double* const* out, int nchunk,
+                double scale, hipStream_t st) {
+  GramRedIso rr[2] = {};
+  int64_t tot = 0;
+  for (int iq = 0; iq < 2; ++iq) {
+    rr[iq].nst = nst[iq];
+    if (nst[iq] <= 0) continue;
+    rr[iq].part = part[iq];
+    rr[iq].out = out[iq];
+    rr[iq].np = (nst[iq] + GV_T - 1) / GV_T * GV_T;
+    const int64_t t = (int64_t)nm * nst[iq] * nst[iq];
+    tot = t > tot ? t : tot;
+  }
+  int nb = (int)((tot * 64 + 255) / 256);
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(gram_vec_reduce_kernel, dim3(nb < 1 ? 1 : nb, 2), dim3(256), 0, st, rr[0], rr[1], nchunk, nm,
+                     v.bsame, scale);
+  return launch_status();
+}
+}  // namespace
+
 extern "C" int hfb_gram2_ex(int prec, const void* const* A, const void* const* B, int nm, const int* nst,
                             long long K, double scale, double* const* part, double* const* out,
                             const hfb_finish_job* job, void* stream) {
@@ -473,6 +745,49 @@ extern "C" int hfb_gram2_ex(int prec, const void* const* A, const void* const* B
   }
   if (prec != HFB_F64 && prec != HFB_F32) return HFB_E_ARG;
   hipStream_t st = (hipStream_t)stream;
+  int nsmax = nst[0] > nst[1] ? nst[0] : nst[1];
+  if (prec == HFB_F32 && K % 32 == 0 && (int64_t)nsmax * K * 8 < (1LL << 32)) {  // matrix cores (gram_mfma_kernel)
+    GramMfmaArgs ma{};
+    ma.iso[0] = v.iso[0];
+    ma.iso[1] = v.iso[1];
+    ma.nm = nm;
+    ma.bsame = v.bsame;
+    ma.Kr = 2 * K;
+    ma.ngroup = (int)(2 * K / 64);
+    int nsub = 0;
+    for (int iq = 0; iq < 2; ++iq) {
+      GramMfmaIso& mi = ma.mi[iq];
+      mi.nst = nst[iq];
+      if (nst[iq] <= 0) continue;
+      mi.nt = (nst[iq] + 3) / 4;
+      mi.nb = gram_mfma_blocks(nst[iq]);
+      for (int m = 0; m < nm; ++m) mi.nsub += ((v.bsame >> m) & 1) ? mi.nb * (mi.nb + 1) / 2 : mi.nb * mi.nb;
+      nsub += mi.nsub;
+    }
+    const int nc0 = gram_mfma_nchunk(ma.ngroup, nsub);
+    ma.gpc = (ma.ngroup + nc0 - 1) / nc0;
+    ma.nchunk = (ma.ngroup + ma.gpc - 1) / ma.gpc;
+    ma.nitem = (int64_t)ma.nchunk * nsub;
+    const int64_t gx = (ma.nitem + 3) / 4 + (job ? 1 : 0);
+    int bt = 1;  // the largest block edge of this launch
+    for (int iq = 0; iq < 2; ++iq)
+      if (ma.mi[iq].nst > 0) {
+        const int e = (ma.mi[iq].nt + ma.mi[iq].nb - 1) / ma.mi[iq].nb;
+        bt = e > bt ? e : bt;
+      }
+    const dim3 grid((unsigned)gx);
+    if (bt <= 2)
+      hipLaunchKernelGGL(gram_mfma_kernel<2>, grid, dim3(256), 0, st, ma, fj);
+    else if (bt <= 3)
+      hipLaunchKernelGGL(gram_mfma_kernel<3>, grid, dim3(256), 0, st, ma, fj);
+    else if (bt <= 4)
+      hipLaunchKernelGGL(gram_mfma_kernel<4>, grid, dim3(256), 0, st, ma, fj);
+    else
+      hipLaunchKernelGGL(gram_mfma_kernel<GM_BT>, grid, dim3(256), 0, st, ma, fj);
+    int rc = launch_status();
+    if (rc != HFB_OK) return rc;
+    return gram_reduce(v, nm, nst, part, out, ma.nchunk, scale, st);
+  }
   {  // register-tiled path (gram_vec_kernel)
     v.nm = nm;
     v.K = K;
@@ -495,23 +810,7 @@ extern "C" int hfb_gram2_ex(int prec, const void* const* A, const void* const* B
       return HFB_E_ARG;
     int rc = launch_status();
     if (rc != HFB_OK) return rc;
-    GramRedIso rr[2] = {};
-    int64_t tot = 0;
-    for (int iq = 0; iq < 2; ++iq) {
-      rr[iq].nst = nst[iq];
-      if (nst[iq] <= 0) continue;
-      rr[iq].part = part[iq];
-      rr[iq].out = out[iq];
-      rr[iq].np = (nst[iq] + GV_T - 1) / GV_T * GV_T;
-      const int64_t t = (int64_t)nm * nst[iq] * nst[iq];
-      tot = t > tot ? t : tot;
-    }
-    int nb = (int)((tot * 64 + 255) / 256);
-    if (nb > 4096) nb = 4096;
-    hipLaunchKernelGGL(gram_vec_reduce_kernel, dim3(nb < 1 ? 1 : nb, 2), dim3(256), 0, st, rr[0], rr[1], v.nchunk, nm,
-                       v.bsame, scale);
-    rc = launch_status();
-    return rc;
+    return gram_reduce(v, nm, nst, part, out, v.nchunk, scale, st);
   }
 }
 
