@@ -1,5 +1,4 @@
-// Subspace matrices and rotations of diagstep (static.py:261-577), FP64
-// arithmetic on the vector units.
+// Subspace matrices and rotations of diagstep (static.py:261-577).
 //
 // Gram: C_m = A^H B_m for up to four right-hand blocks sharing A = psi
 // (overlap psi^H psi, H_old = psi^H hmfpsi, lambda_lin = psi^H hampsi,
@@ -10,8 +9,9 @@
 //
 // FP32 blocks (mixed precision) run on the FP32 matrix cores: the Gram on
 // 4x4 tiles (no padding beyond a multiple of 4 states), the rotation on
-// 16x16x4 tiles.  FP64 blocks run as register-tiled FMAs: on gfx950 the
-// FP64 matrix cores and vector units have the same peak rate.
+// 16x16 tiles (up to 24 states with the whole coefficient matrix in
+// registers, above that one 16-row tile per wavefront and k-step).  FP64
+// blocks run as register-tiled FMAs on the vector units.
 #include <math.h>
 
 #include "hfb_ctx.h"
@@ -669,6 +669,109 @@ __global__ void __launch_bounds__(256) rotate_mfma_kernel(RotArgs ra) {
   }
 }
 
+// FP32 rotation of larger blocks on the matrix cores (v_mfma_f32_16x16x4f32),
+// in the same real form out = C psi: rows M = (matrix m, state j, re/im) of
+// the result, columns k = (state i, re/im) of the input.  A workgroup takes
+// RB_PTS consecutive grid points and stages every input column of them in
+// LDS ([k][point], all loaded before the first store, so out1 may alias
+// psi); its four wavefronts take the 16-row tiles mt = w, w + 4, ... (at most
+// MTW each) over the four 16-point subtiles.  Per k-step of four input
+// columns a lane needs C[16 mt + l % 16][4 ks + l / 16], one FP64 entry of
+// U_m (re or im, sign by the 2 x 2 block) read straight from U (L2
+// resident) one k-step ahead of its products.  Result lane l, register r =
+// row 16 mt + 4 (l / 16) + r at point 16 s + l % 16: registers (0, 1) and
+// (2, 3) are the (re, im) of one state, stored as one 8-byte value.
+constexpr int RB_PTS = 64;
+constexpr int RB_MTW_MAX = 8;
+
+template <int MTW>
+__global__ void __launch_bounds__(256) rotate_mfma_big_kernel(RotArgs ra) {
+  extern __shared__ float rb_lds[];  // [4 ks_n][RB_PTS]
+  const RotIso& ri = ra.iso[blockIdx.y];
+  const int n = ri.nst;
+  const int64_t K = ra.K;
+  const int64_t p0 = (int64_t)blockIdx.x * RB_PTS;
+  if (n <= 0 || p0 >= K) return;  // uniform over the workgroup
+  const bool two = ri.U2 != nullptr;
+  const int kin = 2 * n, ks_n = (kin + 3) / 4, kpad = 4 * ks_n;
+  const int mtot = (two ? 4 : 2) * n, mt_n = (mtot + 15) / 16;
+  // stage the input: complex (state i, point p) -> rows 2i (re), 2i + 1 (im)
+  const float2* psi = (const float2*)ri.psi;
+  for (int e = threadIdx.x; e < (kpad / 2) * RB_PTS; e += 256) {
+    const int i = e / RB_PTS, p = e % RB_PTS;
+    float2 v = {0.f, 0.f};
+    if (i < n && p0 + p < K) v = psi[(int64_t)i * K + p0 + p];
+    rb_lds[(2 * i) * RB_PTS + p] = v.x;
+    rb_lds[(2 * i + 1) * RB_PTS + p] = v.y;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = (lane >> 4) & 1;  // input re/im of this lane (k = 4 ks + lane / 16)
+  // per tile q: this lane's row -> U_m element offset of (i = 0, j), which
+  // double (re or im) and the sign; rows past mtot give zero
+  const double* ub[MTW];
+  double sg[MTW];
+#pragma unroll
+  for (int q = 0; q < MTW; ++q) {
+    const int mt = w + 4 * q;
+    const int M = 16 * mt + (lane & 15);
+    const bool ok = mt < mt_n && M < mtot;
+    const int Mc = ok ? M : 0;
+    const int m = Mc / kin, jj = Mc % kin, j = jj >> 1, co = jj & 1;
+    const double* U = m == 0 ? ri.U1 : ri.U2;
+    ub[q] = U + 2 * j + (co == c ? 0 : 1);
+    sg[q] = ok ? ((co == 0 && c == 1) ? -1.0 : 1.0) : 0.0;
+  }
+  rv4f acc[MTW][4];
+#pragma unroll
+  for (int q = 0; q < MTW; ++q)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) acc[q][s] = rv4f{0.f, 0.f, 0.f, 0.f};
+  // input state of this lane at k-step ks: i = 2 ks + lane / 32 (clamped; a
+  // padded column has a zero B row, so its A value is irrelevant)
+  auto uload = [&](double (&v)[MTW], int ks) {
+    const int i = min(2 * ks + (lane >> 5), n - 1);
+#pragma unroll
+    for (int q = 0; q < MTW; ++q) v[q] = ub[q][2 * (int64_t)i * n];
+  };
+  double vn[MTW];
+  uload(vn, 0);
+  for (int ks = 0; ks < ks_n; ++ks) {
+    float a[MTW];
+#pragma unroll
+    for (int q = 0; q < MTW; ++q) a[q] = (float)(sg[q] * vn[q]);
+    if (ks + 1 < ks_n) uload(vn, ks + 1);
+    float b[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) b[s] = rb_lds[(4 * ks + (lane >> 4)) * RB_PTS + 16 * s + (lane & 15)];
+#pragma unroll
+    for (int q = 0; q < MTW; ++q) {
+      if (w + 4 * q >= mt_n) continue;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc[q][s] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q], b[s], acc[q][s], 0, 0, 0);
+    }
+  }
+  float2* o1 = (float2*)ri.out1;
+  float2* o2 = (float2*)ri.out2;
+#pragma unroll
+  for (int q = 0; q < MTW; ++q) {
+    const int mt = w + 4 * q;
+    if (mt >= mt_n) continue;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int M = 16 * mt + 4 * (lane >> 4) + 2 * h;
+      if (M >= mtot) continue;
+      const int m = M / kin, j = (M % kin) >> 1;
+      float2* o = m == 0 ? o1 : o2;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int64_t p = p0 + 16 * s + (lane & 15);
+        if (p < K) o[(int64_t)j * K + p] = float2{acc[q][s][2 * h], acc[q][s][2 * h + 1]};
+      }
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" int hfb_gram_workspace(int nst, long long K) {
@@ -870,6 +973,28 @@ int launch_rotate(int prec, RotArgs ra, int niso, hipStream_t st) {
     HFB_ROTM(10, 5)
     HFB_ROTM(12, 6)
 #undef HFB_ROTM
+  }
+  bool two_any = false;
+  for (int iq = 0; iq < niso; ++iq) two_any = two_any || (ra.iso[iq].nst > 0 && ra.iso[iq].U2);
+  const int mt_max = ((two_any ? 4 : 2) * nmax + 15) / 16;
+  if (prec == HFB_F32 && mt_max <= 4 * RB_MTW_MAX) {
+    // matrix cores, larger blocks: RB_PTS points per workgroup, every input
+    // column staged in LDS
+    const int mtw = (mt_max + 3) / 4;
+    const size_t lds = sizeof(float) * (size_t)(4 * ((2 * nmax + 3) / 4)) * RB_PTS;
+    const dim3 grid((unsigned)((ra.K + RB_PTS - 1) / RB_PTS), niso);
+#define HFB_ROTB(MW)                                                          \
+  if (mtw <= MW) {                                                            \
+    auto kern = rotate_mfma_big_kernel<MW>;                                   \
+    if (lds > 32 * 1024) allow_full_lds((const void*)kern);                   \
+    hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, ra);                   \
+    return launch_status();                                                   \
+  }
+    HFB_ROTB(2)
+    HFB_ROTB(4)
+    HFB_ROTB(6)
+    HFB_ROTB(8)
+#undef HFB_ROTB
   }
   const size_t csz = prec == HFB_F64 ? sizeof(cx<double>) : sizeof(cx<float>);
   // two k per lane while the staged tile fits in LDS, else one
