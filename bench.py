@@ -16,6 +16,10 @@ and Fermi energies are compared there ("unrefined"), then the mixed run is
 promoted to FP64 and both runs do `--refine` more FP64 iterations before the
 reported comparison.
 
+GEMM efficiency: `gemm` times the loop's Gram and rotation calls on the
+states of each run (HIP events around a replayed graph) and reports
+F_gemm / (t * peak) with F_gemm = 4 * 8 * K * sum_q nst_q^2.
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -154,6 +158,52 @@ def hpsi_burst(pkg, run, reps=20):
     return a.elapsed_time(b) / reps
 
 
+def gemm_burst(pkg, run, reps=10):
+    """Average time of the loop's GEMM-shaped work, the four kinds of
+    F_gemm (static.py:301-303, 466, 496, 562): the Gram matrices
+    psi^H psi and psi^H hmfpsi of both isospin blocks (one hfb_gram2 call)
+    and the rotations psi U and psi (U lambda+) (one hfb_rotate2 call, into
+    scratch so the states stay unchanged), on the run's own states, over
+    `reps` repetitions captured in one HIP graph and timed with HIP events
+    around its replay.  Returns (ms per repetition, F_gemm)."""
+    import torch
+    kernels = pkg.kernels
+    ctx, lv = run.grids.ctx, run.levels
+    n0, n1 = int(lv.npsi[0]), int(lv.npsi[1])
+    sl = [slice(0, n0), slice(n0, n0 + n1)]
+    blocks = [(lv.psi[s], [lv.psi[s], lv.hmfpsi[s]]) for s in sl]
+    outs = [torch.empty((2, n, n), dtype=torch.complex128, device=lv.psi.device) for n in (n0, n1)]
+    U = [torch.eye(n, dtype=torch.complex128, device=lv.psi.device) for n in (n0, n1)]
+    o1, o2 = torch.empty_like(lv.psi), torch.empty_like(lv.psi)
+    rot = [(lv.psi[s], u, u, o1[s], o2[s]) for s, u in zip(sl, U)]
+    K = lv.psi[0].numel()
+    scale = float(run.grids.wxyz)
+
+    def calls():
+        for _ in range(reps):
+            kernels.gram2(ctx, blocks, scale, outs)
+            kernels.rotate2(rot, K)
+    calls()  # workspaces exist before capture
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        g.capture_begin()
+        calls()
+        g.capture_end()
+    torch.cuda.current_stream().wait_stream(side)
+    g.replay()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    g.replay()
+    b.record()
+    torch.cuda.synchronize()
+    flops = 4 * 8 * K * (n0 * n0 + n1 * n1)
+    return a.elapsed_time(b) / reps, flops
+
+
 def snapshot(run):
     """E_B and the observables of a run (output.py:177-293, pairs.py:125-127)."""
     e = run.energies_host()
@@ -255,6 +305,14 @@ def main():
         timing[prec] = time.perf_counter() - t0
 
     burst_ms = hpsi_burst(pkg, runs["mixed"])
+    gemm = {}
+    for prec, peak in (("mixed", 157.3e12), ("f64", 78.6e12)):
+        ms, flops = gemm_burst(pkg, runs[prec])
+        gemm[prec] = {"avg_ms": ms, "frac": flops / (ms * 1e-3 * peak), "peak_tflops": peak / 1e12}
+    gemm["flops"] = flops
+    gemm["kinds"] = ("psi^H psi, psi^H hmfpsi (hfb_gram2), psi U, psi U lambda+ (hfb_rotate2); "
+                     "F_gemm = 4 * 8 * K * sum_q nst_q^2, frac = F_gemm / (t * peak); mixed: FP32 matrix cores, "
+                     "f64: FP64 peak")
 
     # accuracy: both runs continue (the mixed run in mixed precision) to the
     # iteration where the FP64 run has converged; then the mixed run is
@@ -309,7 +367,7 @@ def main():
         "ehf_mixed": e_mix["ehf"], "ehf_f64": e_64["ehf"], "accuracy": accuracy,
         "hbm_frac_iter_mixed": window_bytes(runs["f64"], args, 8) * mixed_ips / args.steps / HBM_PEAK,
         "hbm_frac_iter_f64": window_bytes(runs["f64"], args, 16) * f64_ips / args.steps / HBM_PEAK,
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "gemm": gemm, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

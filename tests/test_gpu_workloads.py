@@ -107,12 +107,32 @@ def test_pb208_energy_falls(pb208):
     assert -2500.0 < e[-1] < -1000.0, e[-1]
 
 
-def test_pb208_mixed_follows_f64(pb208):
-    _, h64, _ = pb208["f64"]
-    _, hmx, _ = pb208["mixed"]
-    d = abs(h64[-1]["ehf"] - hmx[-1]["ehf"])
-    print("Pb208 |E_mixed - E_f64| after 12 iterations [MeV]:", d)
-    assert d < 0.05, d
+def test_pb208_mixed_precision_at_convergence(pkg, dev):
+    """Pb208 (126+82 states, 48^3) run to convergence (1500 iterations; the
+    FP64 energy then changes by < 1e-8 MeV per iteration) in mixed precision
+    and in FP64 from the same start: E_B within 1 keV and rms radii within
+    1e-5 fm before any FP64 refinement, and E_B within 1 keV after 50 FP64
+    iterations of the promoted mixed run."""
+    cfg = dict(pkg.driver.CONFIGS["Pb208"])
+    runs = {}
+    for prec in ("mixed", "f64"):
+        r = pkg.driver.HFRun.from_config(precision=prec, **cfg)
+        r.prelude()
+        r.enable_graphs(True)
+        r.run(1500, start=1, check_every=0)
+        runs[prec] = r
+        print("Pb208", prec, "after 1500 iterations", r.energies_host()["ehf"], flush=True)
+    e = {p: r.energies_host()["ehf"] for p, r in runs.items()}
+    ob = {p: r.observables() for p, r in runs.items()}
+    print("Pb208 unrefined |dE| keV", abs(e["mixed"] - e["f64"]) * 1e3, flush=True)
+    assert abs(e["mixed"] - e["f64"]) < 1e-3
+    assert np.max(np.abs(ob["mixed"]["rms"] - ob["f64"]["rms"])) < 1e-5
+    runs["mixed"].promote()
+    for r in runs.values():
+        r.run(50, start=1501, check_every=0)
+    e = {p: r.energies_host()["ehf"] for p, r in runs.items()}
+    print("Pb208 refined |dE| keV", abs(e["mixed"] - e["f64"]) * 1e3, flush=True)
+    assert abs(e["mixed"] - e["f64"]) < 1e-3
 
 
 def test_ca40_mixed_precision_at_convergence(pkg, dev):
