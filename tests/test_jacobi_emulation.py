@@ -79,7 +79,7 @@ def jacobi_emulated(M, sort=True, max_sweeps=40):
     for sweep in range(max_sweeps):
         dia = np.sum(np.abs(A[i == j]) ** 2)
         off = 2.0 * np.sum(np.abs(A[i != j]) ** 2)
-        if off <= 1e-30 * dia or off == 0.0:
+        if off <= 1e-26 * dia or off == 0.0:  # the kernel's JACOBI_TOL2
             break
         sweeps += 1
         for t in range(m - 1):
