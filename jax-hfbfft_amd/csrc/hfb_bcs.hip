@@ -4,10 +4,11 @@
 // on [-100, 100] (pairs.py:47-61, brentq with xtol 1e-14).  The particle
 // number is monotone in lambda, so the root is bracketed by a parallel
 // bisection: each round evaluates the sum at one trial point per thread and
-// keeps the sub-interval where the sign changes, shrinking the bracket by
-// the thread count per round until it is below 1e-15 (brentq's own
-// tolerance is 1e-14 + 4 eps |lambda|).  A bracket without a sign change
-// sets status = 1 (brentq raises ValueError there).
+// keeps the sub-interval where the sign changes (the first trial point past
+// it found by a wavefront ballot), shrinking the bracket by the thread
+// count per round until it is below 1e-15 (brentq's own tolerance is
+// 1e-14 + 4 eps |lambda|).  A bracket without a sign change sets
+// status = 1 (brentq raises ValueError there).
 #include <math.h>
 
 #include "hfb_ctx.h"
@@ -43,19 +44,40 @@ struct BcsArgs {
   int* status;
 };
 
-__device__ double particle_number(double lam, const double* e, const double* d, const double* ws, int n) {
+// The part of sum_k clip(v2_k(lambda)) ws_k over the levels k = sub,
+// sub + BCS_SPLIT, ...; v2 = (1 - edif / sqrt(edif^2 + d^2)) / 2 with the
+// reference's 1e-20 floor on the square root, 1/sqrt by the refined hardware
+// estimate (a few ulp).
+constexpr int BCS_NT = 1024;  // threads: BCS_SPLIT per trial point
+constexpr int BCS_SPLIT = 4;
+__device__ __forceinline__ double rsq_refined(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * fma(-0.5 * x * y, y, 1.5);
+  return y * fma(-0.5 * x * y, y, 1.5);
+}
+__device__ double particle_number_part(double lam, const double* e, const double* d, const double* ws, int n,
+                                       int sub) {
   double s = 0.0;
-  for (int k = 0; k < n; ++k) {
+#pragma unroll 4
+  for (int k = sub; k < n; k += BCS_SPLIT) {
     const double edif = e[k] - lam;
-    const double eq = fmax(sqrt(edif * edif + d[k] * d[k]), 1.0e-20);
-    double v2 = 0.5 * (1.0 - edif / eq);
+    const double q2 = fmax(fma(edif, edif, d[k] * d[k]), 1.0e-40);
+    double v2 = 0.5 - 0.5 * edif * rsq_refined(q2);
     v2 = fmin(fmax(v2, 1.0e-10), 1.0 - 1.0e-10);
-    s += v2 * ws[k];
+    s = fma(v2, ws[k], s);
   }
   return s;
 }
+// the sum over the BCS_SPLIT consecutive lanes of a trial point
+__device__ __forceinline__ double particle_number_lanes(double lam, const double* e, const double* d,
+                                                        const double* ws, int n) {
+  double s = particle_number_part(lam, e, d, ws, n, threadIdx.x & (BCS_SPLIT - 1));
+  s += lane_xor<1>(s);
+  s += lane_xor<2>(s);
+  return s;
+}
 
-__global__ void __launch_bounds__(256) bcs_kernel(BcsArgs a) {
+__global__ void __launch_bounds__(BCS_NT) bcs_kernel(BcsArgs a) {
   const int iq = blockIdx.x;
   const BcsIso b = a.iso[iq];
   const int n = b.nst;
@@ -63,7 +85,6 @@ __global__ void __launch_bounds__(256) bcs_kernel(BcsArgs a) {
   // the levels of a block up to BCS_LDS states are staged in LDS, larger
   // blocks are read from global memory (L2) by every trial evaluation
   __shared__ double e_l[BCS_LDS], dl_l[BCS_LDS], ws_l[BCS_LDS];
-  __shared__ double fval[256];
   __shared__ double lo, hi, flo;
   __shared__ int bad;
   const bool staged = n <= BCS_LDS;
@@ -77,47 +98,58 @@ __global__ void __launch_bounds__(256) bcs_kernel(BcsArgs a) {
       ws_l[k] = a.wstates[b.start + k];
     }
   }
+  __syncthreads();
+  __shared__ double fend[2];
+  __shared__ int wfirst[BCS_NT / 64];
+  // the bracket's end points (lanes 0-3 and 4-7)
+  if (tid < 2 * BCS_SPLIT) {
+    const double fe = particle_number_lanes(tid < BCS_SPLIT ? -100.0 : 100.0, e, dl, ws, n) - b.target;
+    if ((tid & (BCS_SPLIT - 1)) == 0) fend[tid / BCS_SPLIT] = fe;
+  }
+  __syncthreads();
   if (tid == 0) {
     lo = -100.0;
     hi = 100.0;
     bad = 0;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    const double fa = particle_number(lo, e, dl, ws, n) - b.target;
-    const double fb = particle_number(hi, e, dl, ws, n) - b.target;
+    const double fa = fend[0], fb = fend[1];
     flo = fa;
     if (fa * fb > 0.0) bad = 1;
     if (fa == 0.0) hi = lo;
     if (fb == 0.0) lo = hi;
   }
   __syncthreads();
-  const int nt = blockDim.x;
+  constexpr int npt = BCS_NT / BCS_SPLIT;  // trial points per round
+  const int pt = tid / BCS_SPLIT;
   for (int round = 0; round < 12 && !bad; ++round) {
     const double l0 = lo, h0 = hi;
     if (h0 - l0 <= 1.0e-15 * fmax(1.0, fabs(l0))) break;
-    // trial points strictly inside (l0, h0)
-    const double x = l0 + (h0 - l0) * double(tid + 1) / double(nt + 1);
-    fval[tid] = particle_number(x, e, dl, ws, n) - b.target;
+    // trial points strictly inside (l0, h0), increasing with the index
+    const double x = l0 + (h0 - l0) * double(pt + 1) / double(npt + 1);
+    const double f = particle_number_lanes(x, e, dl, ws, n) - b.target;
+    // the first trial point that is a root or lies past the sign change
+    // (the particle number grows with lambda): a ballot per wavefront over
+    // the first lane of each point, the smallest index over the wavefronts
+    const bool past = f == 0.0 || ((f < 0.0) != (flo < 0.0));
+    const unsigned long long m = __ballot(past) & 0x1111111111111111ull;
+    if ((tid & 63) == 0) wfirst[tid >> 6] = m ? (tid >> 6) * (64 / BCS_SPLIT) + __builtin_ctzll(m) / BCS_SPLIT : npt;
     __syncthreads();
-    if (tid == 0) {
-      double nl = l0, nh = h0;
-      const bool neg_lo = flo < 0.0;
-      for (int t = 0; t < nt; ++t) {
-        const double xt = l0 + (h0 - l0) * double(t + 1) / double(nt + 1);
-        if (fval[t] == 0.0) {
-          nl = nh = xt;
-          break;
-        }
-        if ((fval[t] < 0.0) == neg_lo) {
-          nl = xt;
-        } else {
-          nh = xt;
-          break;
-        }
+    int t = npt;
+    for (int w = 0; w < BCS_NT / 64; ++w) t = min(t, wfirst[w]);
+    // new bracket (the in-order scan's result): a root at t closes it
+    // there; else lo = the trial point before t (l0 for t = 0), hi = the
+    // point at t (h0 when every point lies before the sign change).  One
+    // thread writes it; x_{t-1} is recomputed with the same expression.
+    if (tid == (t < npt ? t : 0) * BCS_SPLIT) {
+      if (t == npt) {
+        lo = l0 + (h0 - l0) * double(npt) / double(npt + 1);
+        hi = h0;
+      } else if (f == 0.0) {
+        lo = x;
+        hi = x;
+      } else {
+        lo = t > 0 ? l0 + (h0 - l0) * double(t) / double(npt + 1) : l0;
+        hi = x;
       }
-      lo = nl;
-      hi = nh;
     }
     __syncthreads();
   }
@@ -152,7 +184,7 @@ __global__ void __launch_bounds__(256) bcs_kernel(BcsArgs a) {
     v[2] += wo * wsc;
     v[3] += dl[k] * wo * wsc;
   }
-  __shared__ double red[4 * 4];
+  __shared__ double red[4 * (BCS_NT / 64)];
   block_sum<4>(v, red);
   if (tid == 0) {
     const double suv = fmax(v[0], 1.0e-20), sv2 = fmax(v[2], 1.0e-20);
@@ -205,6 +237,6 @@ extern "C" int hfb_bcs(int nst0, int nst1, double nneut, double nprot, const dou
   a.avg = avg;
   a.E = E;
   a.status = status;
-  hipLaunchKernelGGL(bcs_kernel, dim3(2), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(bcs_kernel, dim3(2), dim3(BCS_NT), 0, st, a);
   return launch_status();
 }

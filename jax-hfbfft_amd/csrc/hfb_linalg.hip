@@ -11,6 +11,9 @@
 // returned in ascending order with matching eigenvector columns.
 #include <math.h>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "hfb_ctx.h"
 
 using namespace hfb;
@@ -18,6 +21,8 @@ using namespace hfb;
 namespace {
 
 constexpr int NMAXJ = 256;
+constexpr int NDIAG_LDS = 36;   // blocks up to this size keep the five work matrices in LDS
+constexpr int NDIAG_MAX = 256;  // largest block hfb_diag_small accepts
 
 // Jacobi stops when the squared off-diagonal Frobenius norm is below
 // JACOBI_TOL2 times the squared diagonal one (relative off-diagonal norm
@@ -224,170 +229,355 @@ struct JacobiShared {
   double ev[NMAXJ];
 };
 
-// Rotations of two consecutive rounds (buffer t & 1) for jacobi_eigh.
-struct JacobiShared2 {
-  int p[2][NMAXJ / 2], q[2][NMAXJ / 2];
-  double c[2][NMAXJ / 2], s[2][NMAXJ / 2];
-  cx<double> e[2][NMAXJ / 2];
-  double red[32];
-  int perm[NMAXJ];
-  double ev[NMAXJ];
+// Eigenvalues ev (n) into ascending order with the columns of V (n x n,
+// leading dimension n) following them; T is n x n scratch.  Stable order
+// by rank: index i goes to the number of eigenvalues below it, ties kept in
+// index order (perm starts as the identity, so that every entry stays an
+// index below n whatever the values).
+template <int NP, int NT>
+__device__ __forceinline__ void sort_eigen(cx<double>* V, cx<double>* T, int n, double* ev, int* perm) {
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n; i += NT) perm[i] = i;
+  __syncthreads();
+  for (int i = tid; i < n; i += NT) {
+    const double e = ev[i];
+    int r = 0;
+    for (int j = 0; j < n; ++j) {
+      const double f = ev[j];
+      r += (f < e || (f == e && j < i)) ? 1 : 0;
+    }
+    perm[r] = i;
+  }
+  __syncthreads();
+  tile_for<NP, NT>(n, [&](int i, int j) { T[i * n + j] = V[i * n + perm[j]]; });
+  __syncthreads();
+  tile_for<NP, NT>(n, [&](int i, int j) { V[i * n + j] = T[i * n + j]; });
+  double evs = 0.0;
+  if (tid < n) evs = ev[perm[tid]];
+  __syncthreads();
+  if (tid < n) ev[tid] = evs;
+  __syncthreads();
+}
+
+// LDS state of the wavefront Jacobi (blocks up to NDIAG_LDS states).
+struct JacobiSmall {
+  double ev[NDIAG_LDS];
+  int perm[NDIAG_LDS];
+  double part[2][3][2];  // per sweep parity and rounds wavefront (at most 3): off, dia
+  unsigned int cnt;      // rounds wavefronts' completed rounds
 };
 
-// Eigen-decomposition of the Hermitian A (n x n, dense; destroyed) for
-// blocks whose work matrices live in LDS; V gets the
-// eigenvectors as columns, js.ev the eigenvalues; sorted ascending when
-// `sort` (T is scratch for the column permutation).  Parallel cyclic Jacobi:
-// in each round the n/2 disjoint pairs (p_k, q_k) of a round-robin ordering
-// are rotated together, A <- R^H A R with R block-diagonal over the pairs.
-// A round computes the rotations (one thread per pair), then every 2x2
-// block (pair a, pair b) of the new matrix, R_a^H A[a,b] R_b, into the
-// second buffer A2 and V <- V R in place; the buffers then swap.  For odd n
-// one index per round has no partner and is carried unrotated.
+// Pair k of round t of the round-robin ordering over m (even) indices:
+// index 0 stays, the others move by one place per round; p < q.
+__device__ __forceinline__ void rr_pair(int k, int t, int m, int& p, int& q) {
+  const int m1 = m - 1;
+  int xa = k - 1 + t;
+  xa = xa >= m1 ? xa - m1 : xa;
+  int xb = m1 - k - 1 + t;
+  xb = xb >= m1 ? xb - m1 : xb;
+  const int a0 = k == 0 ? 0 : xa + 1, b0 = xb + 1;
+  p = a0 < b0 ? a0 : b0;
+  q = a0 < b0 ? b0 : a0;
+}
+
+// Rotation R = [[c, sg], [-conj(sg), c]] (c real, c^2 + |sg|^2 = 1) that
+// zeroes entry (p, q) of R^H [[app, apq], [conj(apq), aqq]] R, the smaller
+// of the two angles: with d = (aqq - app)/2, rho = sqrt(d^2 + |apq|^2) and
+// w = rho + |d|, tan(theta) = sign(d) |apq| / w, so c = w / sqrt(2 rho w)
+// and sg = sign(d) apq / sqrt(2 rho w) (since w^2 + |apq|^2 = 2 rho w).
+// Two inverse square roots and no division.  Entries negligible against
+// the diagonal (below 1e-18 relative) are not rotated.  Branch-free (the
+// unrotated case runs the same arithmetic on a safe operand and is selected
+// away), so that the loads of a round issue together and the rotations of
+// several pairs interleave.
+__device__ __forceinline__ void jw_rot(double app, double aqq, cx<double> apq, double& c, cx<double>& sg) {
+  const double r2 = cabs2(apq), den = fabs(app) + fabs(aqq);
+  const bool rot = r2 > 0.0 && r2 > 1e-36 * den * den;
+  const double d = 0.5 * (aqq - app);
+  const double rho2 = rot ? fma(d, d, r2) : 1.0;
+  const double rho = rho2 * rsq_full(rho2);
+  const double w = rho + fabs(d);
+  const double f = rsq_full(2.0 * rho * w);
+  const double g = d >= 0.0 ? f : -f;
+  c = rot ? w * f : 1.0;
+  sg = rot ? cx<double>{apq.x * g, apq.y * g} : cx<double>{0.0, 0.0};
+}
+
+__device__ __forceinline__ void wave_sync() {
+  // orders this wavefront's LDS accesses for the compiler; the hardware
+  // executes one wavefront's LDS instructions in order
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS of the wavefront Jacobi after the five n x n work matrices: the
+// rotation log, two sweeps (double buffer) of (m - 1) rounds x m/2 pairs,
+// cosines then the complex sines; for odd n also the second matrix buffer
+// (m x m; for even n it is B0).
+__host__ __device__ __forceinline__ int jw_m(int n) { return n + (n & 1); }
+__host__ __device__ __forceinline__ size_t jw_log_bytes(int n) {
+  const int m = jw_m(n);
+  return (size_t)2 * (m - 1) * (m / 2) * (sizeof(double) + 2 * sizeof(double));
+}
+__host__ __device__ __forceinline__ size_t jw_extra_bytes(int n) {
+  const int m = jw_m(n);
+  return jw_log_bytes(n) + ((n & 1) ? (size_t)m * m * 2 * sizeof(double) : 0);
+}
+// wavefronts that run the rounds: one 2x2 block per lane
+__host__ __device__ __forceinline__ int jw_nrw(int n) {
+  const int np = jw_m(n) / 2;
+  return (np * (np + 1) / 2 + 63) / 64;
+}
+
+// Rounds R0 .. R0 + m - 2 (one sweep) of the wavefront Jacobi in the
+// calling wavefront, one of NA that run the rounds; the lane owns block
+// (ka, kb) of the lower block triangle (ka = -1: none; the lane then works
+// on block (0, 0) and writes nothing).  Round R reads the matrix in buffer
+// R & 1 and writes the rotated matrix into the other buffer (a round
+// rewrites every entry once), so the wavefronts of one round never write
+// what another still reads; a wavefront starts round R when all NA have
+// finished round R - 1 (LDS counter cnt, one increment per wavefront and
+// round; with NA = 1 the in-order LDS accesses of one wavefront suffice).
+// A round issues all its loads first, computes the next round's pairs in
+// their shadow, then the rotations and the block, then the writes; no
+// branches inside a round.  Rotations of the diagonal blocks go to the log
+// (lcb, lsb); off/dia receive this lane's share of the norms of the sweep's
+// last matrix.
+template <int NA>
+__device__ __forceinline__ void jw_sweep(cx<double>* __restrict__ Ab0, cx<double>* __restrict__ Ab1,
+                                         double* __restrict__ lcb, cx<double>* __restrict__ lsb, int m, int R0,
+                                         int ka, int kb, unsigned int* cnt, double& off, double& dia) {
+  const int nrnd = m - 1, npair = m >> 1;
+  const bool own = ka >= 0;
+  const int ia = own ? ka : 0, ib = own ? kb : 0;
+  const bool dg = own && ia == ib;
+  int pa, qa, pb, qb;
+  rr_pair(ia, 0, m, pa, qa);
+  rr_pair(ib, 0, m, pb, qb);
+  for (int t = 0; t < nrnd; ++t) {
+    const int R = R0 + t;
+    if constexpr (NA > 1) {
+      // every rounds wavefront has finished round R - 1
+      while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < (unsigned)(NA * R))
+        __builtin_amdgcn_s_sleep(1);
+    }
+    const cx<double>* const A = (R & 1) ? Ab1 : Ab0;
+    cx<double>* const An = (R & 1) ? Ab0 : Ab1;
+    const cx<double> x00 = A[pa * m + pb], x01 = A[pa * m + qb];
+    const cx<double> x10 = A[qa * m + pb], x11 = A[qa * m + qb];
+    const double app = A[pa * m + pa].x, aqq = A[qa * m + qa].x;
+    const cx<double> apq = A[pa * m + qa];
+    const double bpp = A[pb * m + pb].x, bqq = A[qb * m + qb].x;
+    const cx<double> bpq = A[pb * m + qb];
+    // next round's pairs, in the loads' shadow
+    int npa, nqa, npb, nqb;
+    rr_pair(ia, t + 1, m, npa, nqa);
+    rr_pair(ib, t + 1, m, npb, nqb);
+    double ca, cb;
+    cx<double> sa, sb;
+    jw_rot(app, aqq, apq, ca, sa);
+    jw_rot(bpp, bqq, bpq, cb, sb);
+    // columns: [Y_p Y_q] = [X_p X_q] R_b
+    const cx<double> y00 = csub(cscale(x00, cb), cmul(x01, cconj(sb)));
+    const cx<double> y01 = cadd(cmul(x00, sb), cscale(x01, cb));
+    const cx<double> y10 = csub(cscale(x10, cb), cmul(x11, cconj(sb)));
+    const cx<double> y11 = cadd(cmul(x10, sb), cscale(x11, cb));
+    // rows: R_a^H = [[c, -sg], [conj(sg), c]]
+    cx<double> n00 = csub(cscale(y00, ca), cmul(sa, y10));
+    cx<double> n01 = csub(cscale(y01, ca), cmul(sa, y11));
+    cx<double> n10 = cadd(cmul(cconj(sa), y00), cscale(y10, ca));
+    cx<double> n11 = cadd(cmul(cconj(sa), y01), cscale(y11, ca));
+    // the rotated pair's diagonal block is exactly diagonal
+    const bool zap = dg && (sa.x != 0.0 || sa.y != 0.0);
+    n01 = zap ? cx<double>{0.0, 0.0} : n01;
+    n10 = zap ? cx<double>{0.0, 0.0} : n10;
+    n00.y = zap ? 0.0 : n00.y;
+    n11.y = zap ? 0.0 : n11.y;
+    if (t == nrnd - 1) {  // entries on and off the diagonal summed apart (no cancellation)
+      const double d0 = cabs2(n00) + cabs2(n11), o0 = cabs2(n01) + cabs2(n10);
+      off = own ? (dg ? o0 : 2.0 * (d0 + o0)) : 0.0;
+      dia = dg ? d0 : 0.0;
+    }
+    if (own) {
+      An[pa * m + pb] = n00;
+      An[pa * m + qb] = n01;
+      An[qa * m + pb] = n10;
+      An[qa * m + qb] = n11;
+    }
+    if (own && !dg) {
+      An[pb * m + pa] = cconj(n00);
+      An[qb * m + pa] = cconj(n01);
+      An[pb * m + qa] = cconj(n10);
+      An[qb * m + qa] = cconj(n11);
+    }
+    if (dg) {
+      lcb[t * npair + ia] = ca;
+      lsb[t * npair + ia] = sa;
+    }
+    pa = npa;
+    qa = nqa;
+    pb = npb;
+    qb = nqb;
+    if constexpr (NA > 1) {
+      // this wavefront's writes are complete before its increment
+      __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+      if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    wave_sync();
+  }
+}
+
+// Eigen-decomposition of the Hermitian matrix whose lower triangle is M
+// (global, n x n, leading dimension n) for blocks of up to NDIAG_LDS states
+// with their work matrices in LDS: V (n x n) gets the eigenvectors as
+// columns, js.ev the eigenvalues, in ascending order when `sort` (T, n x n,
+// is scratch).  A0 and A1 (LDS, m x m with m = n rounded up to even; the
+// extra index is a zero row and column, whose rotations are the identity)
+// hold the whole Hermitian matrix, alternately.  Parallel cyclic Jacobi
+// with the pair ordering of rr_pair, A <- R^H A R per round with R
+// block-diagonal over the m/2 pairs.  The rounds run in the first
+// jw_nrw(n) wavefronts without workgroup barriers (jw_sweep): their lanes
+// own the 2x2 blocks (pair a >= pair b) of a round, and each lane reads
+// its block and the diagonal blocks of its two pairs, computes both
+// rotations itself (jw_rot) and writes R_a^H X R_b and its mirror.  The
+// other wavefronts apply the logged rotations of the previous sweep to V
+// (a row of V stays in one wavefront, whose LDS accesses execute in order)
+// while the first ones run the next sweep; a barrier separates sweeps.  The
+// off-diagonal norm of a sweep's last round decides whether another sweep
+// follows.
 template <int NP, int NT>
-__device__ __forceinline__ int jacobi_eigh(cx<double>* A, cx<double>* A2, cx<double>* V, int n, JacobiShared2& js,
-                                           bool sort, cx<double>* T) {
-  const int tid = threadIdx.x;
+__device__ __forceinline__ int jacobi_wave(const cx<double>* __restrict__ M, cx<double>* __restrict__ A0,
+                                           cx<double>* __restrict__ A1, cx<double>* __restrict__ V,
+                                           double* __restrict__ lc, cx<double>* __restrict__ ls, int n,
+                                           JacobiSmall& js, bool sort, cx<double>* T) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (n == 1) {
+    if (tid == 0) {
+      js.ev[0] = M[0].x;
+      V[0] = {1.0, 0.0};
+    }
+    __syncthreads();
+    return 0;
+  }
+  const int m = jw_m(n), npair = m >> 1, nrnd = m - 1, nblk = npair * (npair + 1) / 2;
+  const int na = jw_nrw(n);  // rounds wavefronts (the host checks na < NT / 64)
+  const int lsz = nrnd * npair;  // log entries per sweep
+  for (int e = tid; e < m * m; e += NT) {
+    const int i = e / m, j = e - i * m;
+    cx<double> v = {0.0, 0.0};
+    if (i < n && j < n) {
+      if (i > j)
+        v = M[i * n + j];
+      else if (i < j)
+        v = cconj(M[j * n + i]);
+      else
+        v = {M[i * n + i].x, 0.0};
+    }
+    A0[e] = v;
+  }
   tile_for<NP, NT>(n, [&](int i, int j) { V[i * n + j] = cx<double>{i == j ? 1.0 : 0.0, 0.0}; });
-  const int m = n + (n & 1);
-  const int npair = m / 2;
+  if (tid == 0) js.cnt = 0;
+  __syncthreads();
+
+  // this lane's block (ka >= kb) of the lower block triangle
+  int ka = -1, kb = 0;
+  if (wave < na) {
+    const int blk = lane + 64 * wave;
+    int a = (int)((sqrtf(8.0f * blk + 1.0f) - 1.0f) * 0.5f);
+    while (a * (a + 1) / 2 > blk) --a;
+    while ((a + 1) * (a + 2) / 2 <= blk) ++a;
+    if (blk < nblk) {
+      ka = a;
+      kb = blk - a * (a + 1) / 2;
+    }
+    // norms of the initial matrix
+    double off = 0.0, dia = 0.0;
+    if (ka >= 0) {
+      int pa, qa, pb, qb;
+      rr_pair(ka, 0, m, pa, qa);
+      rr_pair(kb, 0, m, pb, qb);
+      const double d0 = cabs2(A0[pa * m + pb]) + cabs2(A0[qa * m + qb]);
+      const double o0 = cabs2(A0[pa * m + qb]) + cabs2(A0[qa * m + pb]);
+      off = ka != kb ? 2.0 * (d0 + o0) : o0;
+      dia = ka != kb ? 0.0 : d0;
+    }
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1) {
+      off += __shfl_xor(off, h, 64);
+      dia += __shfl_xor(dia, h, 64);
+    }
+    if (lane == 0) {
+      js.part[0][wave][0] = off;
+      js.part[0][wave][1] = dia;
+    }
+  }
   __syncthreads();
   int sweep = 0;
-  for (; sweep < 40; ++sweep) {
+  for (;; ++sweep) {
+    const int buf = sweep & 1;
+    // the norms of the current matrix (the previous sweep's last round):
+    // every wavefront takes the same decision
     double off = 0.0, dia = 0.0;
-    HFB_TILE_LOOP(NP, NT, n, i, j) {
-      if (i == j)
-        dia += cabs2(A[i * n + j]);
-      else
-        off += cabs2(A[i * n + j]);
+    for (int w = 0; w < na; ++w) {
+      off += js.part[buf][w][0];
+      dia += js.part[buf][w][1];
     }
-    block_sum2_all<NT>(off, dia, js.red);
-    if (off <= JACOBI_TOL2 * dia || off == 0.0) break;
-    // rotation angles of round t into buffer b (threads < npair)
-    auto rotations = [&](int t, int b) {
-      if (tid < npair) {
-        // round-robin pairs: index 0 fixed, the others rotate by t
-        const int k = tid;
-        const int m1 = m - 1;
-        int xa = k - 1 + t;  // < 2 (m - 1)
-        xa = xa >= m1 ? xa - m1 : xa;
-        int xb = m1 - k - 1 + t;
-        xb = xb >= m1 ? xb - m1 : xb;
-        const int a0 = (k == 0) ? 0 : xa + 1;
-        const int b0 = xb + 1;
-        int p = a0 < b0 ? a0 : b0, q = a0 < b0 ? b0 : a0;
-        double c = 1.0, sn = 0.0;
-        cx<double> ea = {1.0, 0.0};
-        if (q < n) {
-          const double app = A[p * n + p].x, aqq = A[q * n + q].x;
-          const cx<double> apq = A[p * n + q];
-          const double r2 = cabs2(apq), den = fabs(app) + fabs(aqq);
-          if (r2 > 0.0 && r2 > 1e-36 * den * den) {
-            const double ri = rsq_full(r2);  // 1 / |a_pq|
-            ea = {apq.x * ri, apq.y * ri};
-            const double th = 0.5 * (aqq - app) * ri;
-            const double h = 1.0 + th * th;
-            const double rt = h * rsq_full(h);  // sqrt(1 + th^2)
-            const double tt = (th >= 0.0 ? 1.0 : -1.0) * rcp_full(fabs(th) + rt);
-            c = rsq_full(1.0 + tt * tt);
-            sn = tt * c;
+    const bool stop = sweep >= 40 || off <= JACOBI_TOL2 * dia || off == 0.0;
+    if (wave < na) {
+      if (!stop) {
+        double o = 0.0, d = 0.0;
+        cx<double>* const A1b = A1;
+        if (na == 1)
+          jw_sweep<1>(A0, A1b, lc + buf * lsz, ls + buf * lsz, m, sweep * nrnd, ka, kb, &js.cnt, o, d);
+        else if (na == 2)
+          jw_sweep<2>(A0, A1b, lc + buf * lsz, ls + buf * lsz, m, sweep * nrnd, ka, kb, &js.cnt, o, d);
+        else  // at most 3 for blocks up to NDIAG_LDS states
+          jw_sweep<3>(A0, A1b, lc + buf * lsz, ls + buf * lsz, m, sweep * nrnd, ka, kb, &js.cnt, o, d);
+#pragma unroll
+        for (int h = 1; h < 64; h <<= 1) {
+          o += __shfl_xor(o, h, 64);
+          d += __shfl_xor(d, h, 64);
+        }
+        if (lane == 0) {
+          js.part[buf ^ 1][wave][0] = o;
+          js.part[buf ^ 1][wave][1] = d;
+        }
+      }
+    } else if (sweep > 0) {
+      // V <- V R for every round of the previous sweep; row r belongs to
+      // wavefront na + r % (NT/64 - na)
+      const int nvw = NT / 64 - na;
+      const int w = wave - na;
+      const int nrow = w < n ? (n - w + nvw - 1) / nvw : 0;
+      const int nitem = nrow * npair;
+      const double* const lcb = lc + (buf ^ 1) * lsz;
+      const cx<double>* const lsb = ls + (buf ^ 1) * lsz;
+      for (int t = 0; t < nrnd; ++t) {
+        for (int it = lane; it < nitem; it += 64) {
+          const int lr = it / npair, k = it - lr * npair;
+          const int r = w + nvw * lr;
+          int p, q;
+          rr_pair(k, t, m, p, q);
+          if (q < n) {
+            const double c = lcb[t * npair + k];
+            const cx<double> sg = lsb[t * npair + k];
+            const cx<double> vp = V[r * n + p], vq = V[r * n + q];
+            V[r * n + p] = csub(cscale(vp, c), cmul(vq, cconj(sg)));
+            V[r * n + q] = cadd(cmul(vp, sg), cscale(vq, c));
           }
-        } else {
-          q = -1;  // p has no partner this round (odd n)
         }
-        js.p[b][k] = p;
-        js.q[b][k] = q;
-        js.c[b][k] = c;
-        js.s[b][k] = sn;
-        js.e[b][k] = ea;
+        wave_sync();
       }
-    };
-    rotations(0, 0);
-    __syncthreads();
-    for (int t = 0; t < m - 1; ++t) {
-      const int b = t & 1;
-      // every 2x2 block (pair ka, pair kb) of R^H A R into A2
-      const int nblk = npair * npair;
-      for (int it = tid; it < nblk; it += NT) {
-        const int ka = it / npair, kb = it % npair;
-        const int pa = js.p[b][ka], qa = js.q[b][ka], pb = js.p[b][kb], qb = js.q[b][kb];
-        const double ca = js.c[b][ka], sa = js.s[b][ka], cb = js.c[b][kb], sb = js.s[b][kb];
-        const cx<double> eca = cscale(js.e[b][ka], ca), esa = cscale(js.e[b][ka], sa);
-        const cx<double> ecb = cscale(js.e[b][kb], cb), esb = cscale(js.e[b][kb], sb);
-        const cx<double> z = {0.0, 0.0};
-        // rows pa, qa of A, columns pb, qb (absent partners read as zero)
-        const cx<double> x00 = A[pa * n + pb];
-        const cx<double> x01 = qb >= 0 ? A[pa * n + qb] : z;
-        const cx<double> x10 = qa >= 0 ? A[qa * n + pb] : z;
-        const cx<double> x11 = qa >= 0 && qb >= 0 ? A[qa * n + qb] : z;
-        // columns: [xp xq] R_b
-        const cx<double> y00 = csub(cmul(x00, ecb), cscale(x01, sb));
-        const cx<double> y01 = cadd(cmul(x00, esb), cscale(x01, cb));
-        const cx<double> y10 = csub(cmul(x10, ecb), cscale(x11, sb));
-        const cx<double> y11 = cadd(cmul(x10, esb), cscale(x11, cb));
-        // rows: R_a^H
-        cx<double> n00 = csub(cmul(cconj(eca), y00), cscale(y10, sa));
-        cx<double> n01 = csub(cmul(cconj(eca), y01), cscale(y11, sa));
-        cx<double> n10 = cadd(cmul(cconj(esa), y00), cscale(y10, ca));
-        cx<double> n11 = cadd(cmul(cconj(esa), y01), cscale(y11, ca));
-        if (ka == kb && qa >= 0 && sa != 0.0) {  // the rotated pair: diagonal 2x2 block
-          n01 = z;
-          n10 = z;
-          n00.y = 0.0;
-          n11.y = 0.0;
-        }
-        A2[pa * n + pb] = n00;
-        if (qb >= 0) A2[pa * n + qb] = n01;
-        if (qa >= 0) A2[qa * n + pb] = n10;
-        if (qa >= 0 && qb >= 0) A2[qa * n + qb] = n11;
-      }
-      __syncthreads();
-      cx<double>* sw = A;
-      A = A2;
-      A2 = sw;
-      // the next round's angles (first wavefront) beside V <- V R of this
-      // round (the other wavefronts): different buffers, one barrier
-      if (t + 1 < m - 1) rotations(t + 1, b ^ 1);
-      for (int it = tid - 64; it >= 0 && it < n * npair; it += NT - 64) {
-        const int r = it / npair, k = it % npair;
-        const int p = js.p[b][k], q = js.q[b][k];
-        if (q >= 0 && js.s[b][k] != 0.0) {
-          const cx<double> xp = V[r * n + p], xq = V[r * n + q];
-          const cx<double> ec = cscale(js.e[b][k], js.c[b][k]), es = cscale(js.e[b][k], js.s[b][k]);
-          V[r * n + p] = csub(cmul(xp, ec), cscale(xq, js.s[b][k]));
-          V[r * n + q] = cadd(cmul(xp, es), cscale(xq, js.c[b][k]));
-        }
-      }
-      __syncthreads();
     }
+    __syncthreads();
+    if (stop) break;
   }
-  for (int i = tid; i < n; i += NT) js.ev[i] = A[i * n + i].x;
+  // the final matrix: buffer (sweeps * rounds) & 1
+  const cx<double>* const Af = ((sweep * nrnd) & 1) ? A1 : A0;
+  for (int i = tid; i < n; i += NT) js.ev[i] = Af[i * m + i].x;
   __syncthreads();
-  if (sort) {
-    // stable ascending order by rank: index i goes to the number of
-    // eigenvalues below it, ties kept in index order (the identity first, so
-    // that every entry stays an index below n whatever the values)
-    for (int i = tid; i < n; i += NT) js.perm[i] = i;
-    __syncthreads();
-    for (int i = tid; i < n; i += NT) {
-      const double e = js.ev[i];
-      int r = 0;
-      for (int j = 0; j < n; ++j) {
-        const double f = js.ev[j];
-        r += (f < e || (f == e && j < i)) ? 1 : 0;
-      }
-      js.perm[r] = i;
-    }
-    __syncthreads();
-    tile_for<NP, NT>(n, [&](int i, int j) { T[i * n + j] = V[i * n + js.perm[j]]; });
-    __syncthreads();
-    tile_for<NP, NT>(n, [&](int i, int j) { V[i * n + j] = T[i * n + j]; });
-    double evs = 0.0;
-    if (tid < n) evs = js.ev[js.perm[tid]];
-    __syncthreads();
-    if (tid < n) js.ev[tid] = evs;
-    __syncthreads();
-  }
+  if (sort) sort_eigen<NP, NT>(V, T, n, js.ev, js.perm);
   return sweep;
 }
 
@@ -590,8 +780,6 @@ __device__ __forceinline__ int jacobi_packed(cx<double>* __restrict__ A, cx<doub
   return sweep;
 }
 
-constexpr int NDIAG_LDS = 36;   // blocks up to this size keep the five work matrices in LDS
-constexpr int NDIAG_MAX = 256;  // largest block hfb_diag_small accepts
 
 // Inverse square root of the Hermitian positive matrix in Y (scaled copy of
 // S) by the coupled Newton-Schulz iteration Y <- Y T, Z <- T Z,
@@ -652,9 +840,11 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
   extern __shared__ double lds_raw[];
   // one LDS area for the eigensolver's rotation tables: dense-matrix rounds
   // (work matrices in LDS) or packed-matrix rounds (the larger blocks)
-  __shared__ JacobiShared2 js2;
-  JacobiShared& js = reinterpret_cast<JacobiShared&>(js2);
-  double* const jev = WM == WM_LDS ? js2.ev : js.ev;
+  // the eigensolver's LDS state: the wavefront Jacobi (blocks in LDS) or
+  // the packed-triangle rounds (the larger blocks)
+  using JS = std::conditional_t<WM == WM_LDS, JacobiSmall, JacobiShared>;
+  __shared__ JS jsh;
+  double* const jev = jsh.ev;
   __shared__ double red[32];
   const int iq = blockIdx.x;
   const DiagIso d = a.iso[iq];
@@ -690,6 +880,12 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
     Vt = B3;
   }
 
+  // the wavefront Jacobi (blocks in LDS): matrix buffers in B2..B3
+  // (m^2 <= 2 n^2 for n >= 2) and in B0 (even n) or after the rotation log
+  // (odd n), V in B1, the log after B4
+  double* const jlc = (double*)(B4 + n * n);
+  cx<double>* const jls = (cx<double>*)(jlc + 2 * (jw_m(n) - 1) * (jw_m(n) / 2));
+  cx<double>* const jA1 = (n & 1) ? (cx<double>*)((char*)jlc + jw_log_bytes(n)) : B0;
   cx<double>* Urho = B4;
   cx<double>* U = Urho;
   if (a.mode == DIAG_GIVEN_U) {
@@ -758,11 +954,10 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
       __syncthreads();
     } else {
       if constexpr (WM == WM_LDS) {
-        load_lower_herm<NP, NT>(B0, d.S, n);
-        jacobi_eigh<NP, NT>(B0, B3, B1, n, js2, false, B2);
+        jacobi_wave<NP, NT>(d.S, B2, jA1, B1, jlc, jls, n, jsh, false, B2);
       } else {
         load_lower_packed<NT>(Ap, d.S, n);
-        jacobi_packed<NT>(Ap, Vt, B1, n, js, false);
+        jacobi_packed<NT>(Ap, Vt, B1, n, jsh, false);
       }
       __shared__ double fac[NMAXJ];
       if (tid == 0) {
@@ -789,11 +984,10 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
     if (a.tdiag && d.Lam) {
       int sw;
       if constexpr (WM == WM_LDS) {
-        load_lower_herm<NP, NT>(B0, d.Lam, n);
-        sw = jacobi_eigh<NP, NT>(B0, B3, B1, n, js2, true, B2);
+        sw = jacobi_wave<NP, NT>(d.Lam, B2, jA1, B1, jlc, jls, n, jsh, true, B2);
       } else {
         load_lower_packed<NT>(Ap, d.Lam, n);
-        sw = jacobi_packed<NT>(Ap, Vt, B1, n, js, true);
+        sw = jacobi_packed<NT>(Ap, Vt, B1, n, jsh, true);
       }
       if (a.stats && tid == 0) a.stats[iq] = sw;
       mm<NP, NT, false>(B3, Urho, B1, n);
@@ -1188,7 +1382,12 @@ static int launch_diag(const DiagArgs& a, int nbig, hipStream_t st) {
   const size_t cz = sizeof(cx<double>);
   const size_t tri = (size_t)nbig * (nbig + 1) / 2;
   if (nbig <= NDIAG_LDS) {
-    const size_t lds = cz * 5 * (size_t)nbig * nbig;
+    // each block's own layout; the allocation covers both
+    size_t lds = 0;
+    for (int iq = 0; iq < 2; ++iq) {
+      const int nq = a.iso[iq].nst;
+      if (nq > 0) lds = std::max(lds, cz * 5 * (size_t)nq * nq + jw_extra_bytes(nq));
+    }
     if (nbig <= 16)
       HFB_DIAG(16, 256, WM_LDS, lds)
     else if (nbig <= 32)

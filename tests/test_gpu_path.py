@@ -61,7 +61,8 @@ _FAULTED = pytest.mark.skip(reason="this case faulted the device with a since-re
                                    "in hfb_diag_small and the cause is not found; it stays off the device")
 
 
-@pytest.mark.parametrize("npsi", [[12, 10], [21, 15], [36, 30], pytest.param([40, 33], marks=_FAULTED),
+@pytest.mark.parametrize("npsi", [[12, 10], [9, 9], [20, 19], [17, 13], [33, 31], [21, 15], [36, 30],
+                                  pytest.param([40, 33], marks=_FAULTED),
                                   pytest.param([64, 50], marks=_FAULTED), [56, 47], [90, 60], [91, 61], [100, 70],
                                   [126, 82]])
 def test_diagstep_matches_oracle(pkg, O, dev, npsi):

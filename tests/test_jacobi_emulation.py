@@ -198,3 +198,236 @@ def test_jacobi_work_split_and_result(n):
     assert np.allclose(V.conj().T @ V, np.eye(n), atol=1e-12)
     assert np.allclose(H @ V, V * ev[None, :], atol=1e-9 * max(1.0, np.max(np.abs(w))))
     assert sweeps < 40
+
+
+# ---------------------------------------------------------------------------
+# jacobi_wave: the rounds in the first jw_nrw(n) wavefronts (one 2x2 block of
+# the lower block triangle per lane, the matrix double-buffered between
+# rounds), the eigenvector rotations in the other wavefronts one sweep
+# behind, from a double-buffered rotation log.
+
+NDIAG_LDS = 36
+LDS_BYTES = 160 * 1024
+DIAG_STATIC_LDS = 2848  # diag_kernel's static LDS for blocks in LDS (compiler resource report)
+
+
+def rr_pair(k, t, m):
+    m1 = m - 1
+    xa = k - 1 + t
+    xa = np.where(xa >= m1, xa - m1, xa)
+    xb = m1 - k - 1 + t
+    xb = np.where(xb >= m1, xb - m1, xb)
+    a0 = np.where(k == 0, 0, xa + 1)
+    b0 = xb + 1
+    return np.minimum(a0, b0), np.maximum(a0, b0)
+
+
+def instance(n):
+    """(NP, NT) of the diag_kernel instance a block of n states runs in."""
+    if n <= 16:
+        return 16, 256
+    if n <= 32:
+        return 32, 512
+    return 64, 512
+
+
+def jw_m(n):
+    return n + (n & 1)
+
+
+def jw_log_bytes(n):
+    m = jw_m(n)
+    return 2 * (m - 1) * (m // 2) * 24
+
+
+def jw_extra_bytes(n):
+    m = jw_m(n)
+    return jw_log_bytes(n) + (m * m * 16 if n & 1 else 0)
+
+
+def jw_nrw(n):
+    npair = jw_m(n) // 2
+    return (npair * (npair + 1) // 2 + 63) // 64
+
+
+def jw_rot(app, aqq, apq):
+    r2 = abs(apq) ** 2
+    den = abs(app) + abs(aqq)
+    if r2 > 0.0 and r2 > 1e-36 * den * den:
+        d = 0.5 * (aqq - app)
+        rho2 = d * d + r2
+        rho = rho2 * rsq(rho2)
+        w = rho + abs(d)
+        f = rsq(2.0 * rho * w)
+        g = f if d >= 0.0 else -f
+        return w * f, apq * g
+    return 1.0, 0.0j
+
+
+def jacobi_wave_emulated(M, max_sweeps=40):
+    n = M.shape[0]
+    NP, NT = instance(n)
+    if n == 1:
+        return np.array([M[0, 0].real]), np.ones((1, 1), complex), 0
+    m = n + (n & 1)
+    npair, nrnd = m // 2, m - 1
+    nblk = npair * (npair + 1) // 2
+    na = jw_nrw(n)
+    assert na <= 3 and na < NT // 64  # jw_sweep instances 1..3, at least one eigenvector wavefront
+    lsz = nrnd * npair
+    # LDS: five n x n work matrices (B0..B4), the log after B4, for odd n the
+    # second matrix buffer after the log (for even n it is B0); the first
+    # buffer spans B2..B3
+    assert m * m <= 2 * n * n
+    if n % 2 == 0:
+        assert m * m <= n * n
+    assert 16 * 5 * n * n + jw_extra_bytes(n) + DIAG_STATIC_LDS <= LDS_BYTES
+    A = [np.zeros((m, m), dtype=complex), np.full((m, m), np.nan, dtype=complex)]
+    il, jl = np.tril_indices(n)
+    A[0][il, jl] = M[il, jl]
+    A[0][jl, il] = np.conj(M[il, jl])
+    A[0][np.arange(n), np.arange(n)] = np.real(np.diag(M))
+    V = np.eye(n, dtype=complex)
+    lc = np.full(2 * lsz, np.nan)
+    ls = np.full(2 * lsz, np.nan, dtype=complex)
+    # (wavefront, lane) -> block, the kernel's decode
+    blocks = []
+    for wave in range(na):
+        for lane in range(64):
+            blk = lane + 64 * wave
+            a = int((math.sqrt(8.0 * blk + 1.0) - 1.0) * 0.5)
+            while a * (a + 1) // 2 > blk:
+                a -= 1
+            while (a + 1) * (a + 2) // 2 <= blk:
+                a += 1
+            if blk < nblk:
+                blocks.append(((wave, lane), a, blk - a * (a + 1) // 2))
+    assert sorted((a, b) for _, a, b in blocks) == [(a, b) for a in range(npair) for b in range(a + 1)]
+    # eigenvector items: row r in wavefront na + r % (NT/64 - na)
+    nvw = NT // 64 - na
+    vitems = {}
+    for w in range(nvw):
+        nrow = (n - w + nvw - 1) // nvw if w < n else 0
+        items = []
+        for it in range(nrow * npair):
+            lr, k = divmod(it, npair)
+            items.append((it % 64, w + nvw * lr, k))
+        vitems[w] = items
+    assert sorted((r, k) for w in vitems for _, r, k in vitems[w]) == [(r, k) for r in range(n) for k in range(npair)]
+
+    def norms(Acur):  # as the kernel sums them: off-diagonal entries apart from the diagonal
+        od = Acur - np.diag(np.diag(Acur))
+        return np.sum(np.abs(od) ** 2), np.sum(np.abs(np.diag(Acur)) ** 2)
+
+    def apply_v(buf):
+        for t in range(nrnd):
+            for w in range(nvw):
+                touched = set()
+                for _, r, k in vitems[w]:
+                    p, q = (int(x) for x in rr_pair(k, t, m))
+                    if q < n:
+                        idx = buf * lsz + t * npair + k
+                        assert 0 <= idx < 2 * lsz and not np.isnan(lc[idx])
+                        assert 0 <= r * n + p < n * n and 0 <= r * n + q < n * n
+                        assert (r, p) not in touched and (r, q) not in touched
+                        touched |= {(r, p), (r, q)}
+                        c, sg = lc[idx], ls[idx]
+                        vp, vq = V[r, p], V[r, q]
+                        V[r, p] = vp * c - vq * np.conj(sg)
+                        V[r, q] = vp * sg + vq * c
+
+    off, dia = norms(A[0])
+    sweep = 0
+    R = 0
+    while True:
+        buf = sweep & 1
+        stop = sweep >= max_sweeps or off <= 1e-26 * dia or off == 0.0
+        if not stop:
+            lc[buf * lsz:(buf + 1) * lsz] = np.nan
+            for t in range(nrnd):
+                cur, nxt = A[R & 1], A[(R + 1) & 1]
+                nxt[:] = np.nan
+                writes = {}
+                for (wl, a, b) in blocks:
+                    pa, qa = (int(x) for x in rr_pair(a, t, m))
+                    pb, qb = (int(x) for x in rr_pair(b, t, m))
+                    assert 0 <= pa < qa < m and 0 <= pb < qb < m
+                    X = cur[np.ix_([pa, qa], [pb, qb])]
+                    ca, sa = jw_rot(cur[pa, pa].real, cur[qa, qa].real, cur[pa, qa])
+                    cb, sb = jw_rot(cur[pb, pb].real, cur[qb, qb].real, cur[pb, qb])
+                    assert not np.isnan(X).any()
+                    Y = np.empty((2, 2), complex)
+                    Y[:, 0] = X[:, 0] * cb - X[:, 1] * np.conj(sb)
+                    Y[:, 1] = X[:, 0] * sb + X[:, 1] * cb
+                    N = np.empty((2, 2), complex)
+                    N[0] = Y[0] * ca - sa * Y[1]
+                    N[1] = np.conj(sa) * Y[0] + Y[1] * ca
+                    if a == b:
+                        if sa != 0.0:
+                            N[0, 1] = N[1, 0] = 0.0
+                            N[0, 0] = N[0, 0].real
+                            N[1, 1] = N[1, 1].real
+                        idx = buf * lsz + t * npair + a
+                        assert 0 <= idx < 2 * lsz
+                        lc[idx], ls[idx] = ca, sa
+                    ent = [((pa, pb), N[0, 0]), ((pa, qb), N[0, 1]), ((qa, pb), N[1, 0]), ((qa, qb), N[1, 1])]
+                    if a != b:
+                        ent += [((y, x), np.conj(v)) for (x, y), v in ent]
+                    for (x, y), v in ent:
+                        assert 0 <= x < m and 0 <= y < m
+                        assert writes.setdefault((x, y), wl) == wl, "two lanes write one entry"
+                        nxt[x, y] = v
+                assert len(writes) == m * m  # a round rewrites the whole matrix once
+                R += 1
+            off, dia = norms(A[R & 1])
+        if sweep > 0:
+            apply_v(buf ^ 1)
+        if stop:
+            break
+        sweep += 1
+    Af = A[R & 1]
+    assert R == sweep * nrnd
+    if n & 1:  # the extra index stays a zero row and column
+        assert np.all(Af[n, :] == 0) and np.all(Af[:, n] == 0)
+    ev = np.real(np.diag(Af))[:n]
+    perm = np.empty(n, dtype=int)
+    for x in range(n):
+        perm[sum(1 for y in range(n) if ev[y] < ev[x] or (ev[y] == ev[x] and y < x))] = x
+    return ev[perm], V[:, perm], sweep
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 7, 8, 12, 16, 17, 20, 21, 24, 25, 31, 32, 33, 35, 36])
+def test_wave_jacobi_work_split_and_result(n):
+    rng = np.random.default_rng(100 + n)
+    H = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = H + H.conj().T
+    if n >= 8:  # a degenerate cluster, as in spherical shells
+        H[:4, :4] = np.diag([3.0, 3.0, 3.0, 3.0])
+        H[:4, 4:] *= 1e-3
+        H[4:, :4] *= 1e-3
+    lo = np.tril(H) + np.triu(rng.standard_normal((n, n)), 1) * 7.0  # upper triangle must not be read
+    ev, V, sweeps = jacobi_wave_emulated(lo)
+    w = np.linalg.eigvalsh(H, UPLO="L")
+    sc = max(1.0, np.max(np.abs(w)))
+    assert np.allclose(ev, w, atol=1e-12 * sc)
+    assert np.allclose(V.conj().T @ V, np.eye(n), atol=1e-13)
+    assert np.allclose(H @ V, V * ev[None, :], atol=1e-11 * sc)
+    assert (n == 1 or 0 < sweeps) and sweeps < 40
+
+
+def test_wave_jacobi_nearly_diagonal_sweeps():
+    """Near convergence of the SCF iteration lambda_lin is nearly diagonal in
+    the previous eigenbasis, with a degenerate shell: the wavefront rounds
+    (rotation of jw_rot) need no more sweeps than the packed-triangle rounds
+    with the same pair ordering and stopping rule."""
+    n = 20
+    rng = np.random.default_rng(7)
+    d = np.sort(rng.uniform(-50.0, 0.0, n))
+    d[4:10] = d[4]  # a six-fold shell
+    E = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    for eps in (1e-5, 1e-7):
+        H = np.diag(d) + eps * (E + E.conj().T)
+        ev, V, sweeps = jacobi_wave_emulated(np.tril(H))
+        _, _, sweeps_packed = jacobi_emulated(np.tril(H))
+        assert sweeps <= sweeps_packed
+        assert np.allclose(H @ V, V * ev[None, :], atol=1e-11 * 50)
