@@ -247,7 +247,6 @@ typedef float gm_v4 __attribute__((ext_vector_type(4)));
 constexpr int GM_BT = 5;     // tiles per block edge at most (2 x 25 accumulators fit the AGPRs)
 constexpr int GM_GMAX = 64;  // 64-float groups per chunk at most (FP32 sums of 256 products)
 constexpr int GM_GMIN = 8;   // and at least, where the row allows
-constexpr int GM_WAVES = 2048;  // wavefronts a launch aims for
 
 struct GramMfmaIso {
   int nst, nt, nb;  // states, 4-row tiles, blocks per edge
@@ -450,11 +449,30 @@ __global__ void __launch_bounds__(256, 1) gram_mfma_kernel(GramMfmaArgs ga, hfb_
   }
 }
 
+// Compute units of the current device (cached per device).
+int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] <= 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    cus[dev] = v;
+  }
+  return cus[dev];
+}
+
 // Chunks of the matrix-core Gram: at most GM_GMAX groups each, at least
-// GM_GMIN where the row allows, otherwise enough for about GM_WAVES
-// wavefronts over nsub (matrix, block) items.
+// GM_GMIN where the row allows, otherwise as many as fit one round of the
+// device: the kernel runs one workgroup of four wavefronts per compute unit,
+// so at most 4 (CUs - 1) wavefronts over nsub (matrix, block) items, one
+// compute unit left for the extra workgroup of the step's tail (measured at
+// Ca40: 33 us for the 950 wavefronts of one round against 45 us for 1536,
+// and 44 us when two workgroups spill into a second round).
 int gram_mfma_nchunk(int64_t ngroup, int nsub) {
-  int64_t nc = (GM_WAVES + nsub - 1) / (nsub > 0 ? nsub : 1);
+  const int64_t waves = 4 * (int64_t)(device_cus() - 1);
+  int64_t nc = waves / (nsub > 0 ? nsub : 1);
+  if (nc < 1) nc = 1;
   const int64_t most = ngroup / GM_GMIN > 0 ? ngroup / GM_GMIN : 1;
   if (nc > most) nc = most;
   const int64_t least = (ngroup + GM_GMAX - 1) / GM_GMAX;

@@ -160,60 +160,57 @@ __device__ void load_lower_herm(cx<double>* A, const cx<double>* M, int n) {
   __syncthreads();
 }
 
-// C = op(A) B for n x n matrices (C distinct from A and B); op(A) = A or A^H.
-// Each thread owns a 2x2 block of C and keeps it in registers; per k it
-// reads two entries of op(A) and two of B (four complex loads for four
-// complex multiply-adds, which keeps the LDS data path from being the
-// limit), operands of MMK consecutive k at a time.  Odd n: the last block
-// row/column reads row/column n-1 twice and stores only what exists.
+// C = op(A) B for n x n complex matrices (leading dimension n; C distinct
+// from A and B); op(A) = A or A^H; LDS or global (L2-resident) operands.
+// On the FP64 matrix cores: each wavefront takes 16 x 16 output tiles in
+// turn and, per k-step of 4, issues four real products into the real and
+// imaginary accumulators, Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br
+// (v_mfma_f64_16x16x4f64: lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16] and holds D[l / 16 + 4 r][l % 16], r = 0..3).  Entries
+// beyond n enter as zeros; the next k-step's operands load during this
+// step's products.  Each operand entry is one 16-byte load per lane and
+// k-step, so the products are not bound by LDS bandwidth.
 __device__ __forceinline__ void mac(cx<double>& c, const cx<double> a, const cx<double> b) {
   c.x = fma(a.x, b.x, fma(-a.y, b.y, c.x));
   c.y = fma(a.x, b.y, fma(a.y, b.x, c.y));
 }
+typedef double mm_d4 __attribute__((ext_vector_type(4)));
 template <bool CONJ_A>
-__device__ __forceinline__ cx<double> mm_opa(const cx<double>* A, int n, int i, int k) {
-  return CONJ_A ? cconj(A[k * n + i]) : A[i * n + k];
+__device__ __forceinline__ void mm_operands(const cx<double>* A, const cx<double>* B, int n, int i, int j, int k,
+                                            cx<double>& a, cx<double>& b) {
+  const bool ka = i < n && k < n, kb = k < n && j < n;
+  const int ic = ka ? i : 0, kc = k < n ? k : 0, jc = kb ? j : 0;
+  const cx<double> av = CONJ_A ? cconj(A[kc * n + ic]) : A[ic * n + kc];
+  const cx<double> bv = B[kc * n + jc];
+  a = ka ? av : cx<double>{0.0, 0.0};
+  b = kb ? bv : cx<double>{0.0, 0.0};
 }
 
 template <int NP, int NT, bool CONJ_A>
 __device__ __forceinline__ void mm(cx<double>* __restrict__ C, const cx<double>* A, const cx<double>* B, int n) {
-  constexpr int MMK = NT >= 1024 ? 1 : 4;  // (1024-thread blocks have 128 registers)
-  const int nh = (n + 1) >> 1;
-  for (int t = threadIdx.x; t < nh * nh; t += NT) {
-    const int i0 = 2 * (t / nh), j0 = 2 * (t % nh);
-    const int i1 = min(i0 + 1, n - 1), j1 = min(j0 + 1, n - 1);
-    cx<double> c00 = {0.0, 0.0}, c01 = {0.0, 0.0}, c10 = {0.0, 0.0}, c11 = {0.0, 0.0};
-    int k = 0;
-    for (; k + MMK <= n; k += MMK) {
-      cx<double> a0[MMK], a1[MMK], b0[MMK], b1[MMK];
-#pragma unroll
-      for (int c = 0; c < MMK; ++c) {
-        a0[c] = mm_opa<CONJ_A>(A, n, i0, k + c);
-        a1[c] = mm_opa<CONJ_A>(A, n, i1, k + c);
-        b0[c] = B[(k + c) * n + j0];
-        b1[c] = B[(k + c) * n + j1];
-      }
-#pragma unroll
-      for (int c = 0; c < MMK; ++c) {
-        mac(c00, a0[c], b0[c]);
-        mac(c01, a0[c], b1[c]);
-        mac(c10, a1[c], b0[c]);
-        mac(c11, a1[c], b1[c]);
-      }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nt = (n + 15) >> 4;
+  for (int tile = wave; tile < nt * nt; tile += NT / 64) {
+    const int ti = tile / nt, tj = tile - ti * nt;
+    const int i = ti * 16 + li, j = tj * 16 + li;
+    mm_d4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+    cx<double> a, b;
+    mm_operands<CONJ_A>(A, B, n, i, j, lk, a, b);
+    for (int k0 = 0; k0 < n; k0 += 4) {
+      cx<double> an, bn;
+      mm_operands<CONJ_A>(A, B, n, i, j, k0 + 4 + lk, an, bn);
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, ci, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, ci, 0, 0, 0);
+      a = an;
+      b = bn;
     }
-    for (; k < n; ++k) {
-      const cx<double> a0 = mm_opa<CONJ_A>(A, n, i0, k), a1 = mm_opa<CONJ_A>(A, n, i1, k);
-      const cx<double> b0 = B[k * n + j0], b1 = B[k * n + j1];
-      mac(c00, a0, b0);
-      mac(c01, a0, b1);
-      mac(c10, a1, b0);
-      mac(c11, a1, b1);
-    }
-    C[i0 * n + j0] = c00;
-    if (j0 + 1 < n) C[i0 * n + j0 + 1] = c01;
-    if (i0 + 1 < n) {
-      C[(i0 + 1) * n + j0] = c10;
-      if (j0 + 1 < n) C[(i0 + 1) * n + j0 + 1] = c11;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = ti * 16 + lk + 4 * r;
+      if (row < n && j < n) C[row * n + j] = cx<double>{cr[r], ci[r]};
     }
   }
   __syncthreads();
