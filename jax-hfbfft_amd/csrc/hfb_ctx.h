@@ -31,6 +31,7 @@ struct hfb_ctx {
 // of a few workgroups (it costs more than a separate launch for wide ones).
 enum : int {
   HFB_SYNC_DIAG = 0,      // diag_kernel: one counter for its two isospin blocks
+  HFB_SYNC_DIAG_U = 1,    // diag_kernel split calls: U_rho handoff flag per isospin (two slots)
   HFB_SYNC_COUNT = 64
 };
 

@@ -65,6 +65,12 @@ struct DiagArgs {
   unsigned int* sync;  // arrival counter of the two isospin blocks
   double* stats;  // optional: [Jacobi sweeps (2), Newton-Schulz iterations (2, -1 = fell back)]
   int tdiag, ipair, tbcs, construct;
+  // a subspace-diagonalising call with its blocks in LDS on two workgroups
+  // per isospin: blockIdx.x = iq + 2 role; role 1 computes U_rho (Loewdin)
+  // and hands it over through U, role 0 runs the eigensolver of
+  // lambda_lin (independent of U_rho, static.py:295-302), then takes U_rho,
+  // forms U = U_rho U_lam and constructs the HFB matrices
+  int split;
   int mode;  // DIAG_FULL, DIAG_LOEWDIN (S -> U only) or DIAG_GIVEN_U (construct from U in d.U)
 };
 
@@ -818,7 +824,7 @@ __device__ __forceinline__ bool newton_schulz(cx<double>*& Y, cx<double>*& Z, cx
 // efluct1 = max over the isospins, efluct2 = their mean (static.py:571-576),
 // by the second of the two isospin blocks to finish.
 __device__ __forceinline__ void diag_finish_arrive(const DiagArgs& a) {
-  if (block_arrive(a.sync, gridDim.x) && threadIdx.x == 0) {
+  if (block_arrive(a.sync, 2u) && threadIdx.x == 0) {
     a.efl1[0] = fmax(a.efl1q[0], a.efl1q[1]);
     a.efl2[0] = 0.5 * (a.efl2q[0] + a.efl2q[1]);
   }
@@ -843,11 +849,12 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
   __shared__ JS jsh;
   double* const jev = jsh.ev;
   __shared__ double red[32];
-  const int iq = blockIdx.x;
+  const int iq = blockIdx.x & 1;
+  const int role = blockIdx.x >> 1;  // 1: the Loewdin workgroup of a split call
   const DiagIso d = a.iso[iq];
   const int n = d.nst;
   if (n <= 0) {
-    if (a.construct) diag_finish_arrive(a);
+    if (a.construct && role == 0) diag_finish_arrive(a);
     return;
   }
   const int tid = threadIdx.x;
@@ -890,94 +897,105 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
     tile_for<NP, NT>(n, [&](int i, int j) { Urho[i * n + j] = d.U[i * n + j]; });
     __syncthreads();
   } else {
-    // Loewdin S^{-1/2} (static.py:463-482)
-    for (int i = tid; i < n; i += NT) a.sp_norm[d.start + i] = d.S[i * n + i].x;
-    load_lower_herm<NP, NT>(B0, d.S, n);
-    cx<double>*Y = B0, *Z = B1, *Y2 = B3, *Z2 = B4;
-    int ns_iters = 0;
-    double cs = 1.0;
-    // Near convergence the overlap is S = I + E with a small E: the binomial
-    // series (I + E)^{-1/2} = sum_k c_k E^k, c_k = (-1)^k C(2k,k)/4^k, is then
-    // exact to round-off in fewer products than the Newton-Schulz iteration
-    // (three per step): to degree 4 for |E|_F < 1e-3 (first omitted term below
-    // 0.25e-15) in 3 products, to degree 7 for |E|_F < 1e-2 (below 0.2e-16) in
-    // 6.  Horner: T = c_{d-1} I + c_d E, then T <- c_k I + E T.  The series
-    // works on S itself; Newton-Schulz on S scaled by a Gershgorin bound.
-    bool ok;
-    {
-      double fr = 0.0, dummy = 0.0;
-      HFB_TILE_LOOP(NP, NT, n, i, j) {
-        const cx<double> v = B0[i * n + j];
-        const double e = i == j ? v.x - 1.0 : v.x;
-        fr += e * e + v.y * v.y;
-      }
-      block_sum2_all<NT>(fr, dummy, red);
-      const int deg = fr < 1e-6 ? 4 : (fr < 1e-4 ? 7 : 0);
-      if (deg > 0) {
-        constexpr double ck[8] = {1.0, -0.5, 0.375, -0.3125, 0.2734375, -0.24609375, 0.2255859375, -0.20947265625};
-        // E in B0, Horner terms in B2 (products in B3), result Z in B1
-        tile_for<NP, NT>(n, [&](int i, int j) {
-          if (i == j) B0[i * n + j].x -= 1.0;
-          const cx<double> e = B0[i * n + j];
-          B2[i * n + j] = cx<double>{(i == j ? ck[deg - 1] : 0.0) + ck[deg] * e.x, ck[deg] * e.y};
-        });
-        __syncthreads();
-        for (int k = deg - 2; k >= 0; --k) {
-          mm<NP, NT, false>(B3, B0, B2, n);
-          cx<double>* dst = k == 0 ? B1 : B2;
+    if (!a.split || role == 1) {
+      // Loewdin S^{-1/2} (static.py:463-482)
+      for (int i = tid; i < n; i += NT) a.sp_norm[d.start + i] = d.S[i * n + i].x;
+      load_lower_herm<NP, NT>(B0, d.S, n);
+      cx<double>*Y = B0, *Z = B1, *Y2 = B3, *Z2 = B4;
+      int ns_iters = 0;
+      double cs = 1.0;
+      // Near convergence the overlap is S = I + E with a small E: the binomial
+      // series (I + E)^{-1/2} = sum_k c_k E^k, c_k = (-1)^k C(2k,k)/4^k, is then
+      // exact to round-off in fewer products than the Newton-Schulz iteration
+      // (three per step): to degree 4 for |E|_F < 1e-3 (first omitted term below
+      // 0.25e-15) in 3 products, to degree 7 for |E|_F < 1e-2 (below 0.2e-16) in
+      // 6.  Horner: T = c_{d-1} I + c_d E, then T <- c_k I + E T.  The series
+      // works on S itself; Newton-Schulz on S scaled by a Gershgorin bound.
+      bool ok;
+      {
+        double fr = 0.0, dummy = 0.0;
+        HFB_TILE_LOOP(NP, NT, n, i, j) {
+          const cx<double> v = B0[i * n + j];
+          const double e = i == j ? v.x - 1.0 : v.x;
+          fr += e * e + v.y * v.y;
+        }
+        block_sum2_all<NT>(fr, dummy, red);
+        const int deg = fr < 1e-6 ? 4 : (fr < 1e-4 ? 7 : 0);
+        if (deg > 0) {
+          constexpr double ck[8] = {1.0, -0.5, 0.375, -0.3125, 0.2734375, -0.24609375, 0.2255859375, -0.20947265625};
+          // E in B0, Horner terms in B2 (products in B3), result Z in B1
           tile_for<NP, NT>(n, [&](int i, int j) {
-            const cx<double> v = B3[i * n + j];
-            dst[i * n + j] = cx<double>{v.x + (i == j ? ck[k] : 0.0), v.y};
+            if (i == j) B0[i * n + j].x -= 1.0;
+            const cx<double> e = B0[i * n + j];
+            B2[i * n + j] = cx<double>{(i == j ? ck[deg - 1] : 0.0) + ck[deg] * e.x, ck[deg] * e.y};
           });
           __syncthreads();
+          for (int k = deg - 2; k >= 0; --k) {
+            mm<NP, NT, false>(B3, B0, B2, n);
+            cx<double>* dst = k == 0 ? B1 : B2;
+            tile_for<NP, NT>(n, [&](int i, int j) {
+              const cx<double> v = B3[i * n + j];
+              dst[i * n + j] = cx<double>{v.x + (i == j ? ck[k] : 0.0), v.y};
+            });
+            __syncthreads();
+          }
+          Z = B1;
+          ok = true;
+        } else {
+          double rowsum = 0.0;  // Gershgorin bound on the largest eigenvalue
+          if (tid < n)
+            for (int j = 0; j < n; ++j) rowsum += sqrt(cabs2(B0[tid * n + j]));
+          cs = block_max_all<NT>(rowsum, red);
+          if (!(cs > 0.0)) cs = 1.0;
+          tile_for<NP, NT>(n, [&](int i, int j) { B0[i * n + j] = cscale(B0[i * n + j], 1.0 / cs); });
+          __syncthreads();
+          ok = newton_schulz<NP, NT>(Y, Z, B2, Y2, Z2, n, red, ns_iters);
         }
-        Z = B1;
-        ok = true;
-      } else {
-        double rowsum = 0.0;  // Gershgorin bound on the largest eigenvalue
-        if (tid < n)
-          for (int j = 0; j < n; ++j) rowsum += sqrt(cabs2(B0[tid * n + j]));
-        cs = block_max_all<NT>(rowsum, red);
-        if (!(cs > 0.0)) cs = 1.0;
-        tile_for<NP, NT>(n, [&](int i, int j) { B0[i * n + j] = cscale(B0[i * n + j], 1.0 / cs); });
+      }
+      if (ok) {
+        const double f = 1.0 / sqrt(cs);
+        // Z may live in B4: the scaling is elementwise, so in place is safe
+        tile_for<NP, NT>(n, [&](int i, int j) { Urho[i * n + j] = cscale(Z[i * n + j], f); });
         __syncthreads();
-        ok = newton_schulz<NP, NT>(Y, Z, B2, Y2, Z2, n, red, ns_iters);
-      }
-    }
-    if (ok) {
-      const double f = 1.0 / sqrt(cs);
-      // Z may live in B4: the scaling is elementwise, so in place is safe
-      tile_for<NP, NT>(n, [&](int i, int j) { Urho[i * n + j] = cscale(Z[i * n + j], f); });
-      __syncthreads();
-    } else {
-      if constexpr (WM == WM_LDS) {
-        jacobi_wave<NP, NT>(d.S, B2, jA1, B1, jlc, jls, n, jsh, false, B2);
       } else {
-        load_lower_packed<NT>(Ap, d.S, n);
-        jacobi_packed<NT>(Ap, Vt, B1, n, jsh, false);
+        if constexpr (WM == WM_LDS) {
+          jacobi_wave<NP, NT>(d.S, B2, jA1, B1, jlc, jls, n, jsh, false, B2);
+        } else {
+          load_lower_packed<NT>(Ap, d.S, n);
+          jacobi_packed<NT>(Ap, Vt, B1, n, jsh, false);
+        }
+        __shared__ double fac[NMAXJ];
+        if (tid == 0) {
+          double wmax = jev[0];
+          for (int i = 1; i < n; ++i) wmax = fmax(wmax, jev[i]);
+          const double thr = fmax(1e-12, 2.220446049250313e-16 * wmax * n);
+          for (int i = 0; i < n; ++i) fac[i] = 1.0 / sqrt(fmax(jev[i], thr));
+        }
+        __syncthreads();
+        const cx<double>* V = B1;
+        tile_for<NP, NT>(n, [&](int i, int j) {  // U = V diag(fac) V^H
+          cx<double> acc = {0.0, 0.0};
+          for (int k = 0; k < n; ++k) acc = cadd(acc, cscale(cmul(V[i * n + k], cconj(V[j * n + k])), fac[k]));
+          Urho[i * n + j] = acc;
+        });
+        __syncthreads();
       }
-      __shared__ double fac[NMAXJ];
-      if (tid == 0) {
-        double wmax = jev[0];
-        for (int i = 1; i < n; ++i) wmax = fmax(wmax, jev[i]);
-        const double thr = fmax(1e-12, 2.220446049250313e-16 * wmax * n);
-        for (int i = 0; i < n; ++i) fac[i] = 1.0 / sqrt(fmax(jev[i], thr));
-      }
-      __syncthreads();
-      const cx<double>* V = B1;
-      tile_for<NP, NT>(n, [&](int i, int j) {  // U = V diag(fac) V^H
-        cx<double> acc = {0.0, 0.0};
-        for (int k = 0; k < n; ++k) acc = cadd(acc, cscale(cmul(V[i * n + k], cconj(V[j * n + k])), fac[k]));
-        Urho[i * n + j] = acc;
-      });
-      __syncthreads();
-    }
 
-    // subspace diagonalisation (static.py:300-303): U <- U_rho U_lam
-    if (a.stats && tid == 0) {
-      a.stats[2 + iq] = ok ? ns_iters : -1;
-      a.stats[iq] = 0;
+      if (a.stats && tid == 0) {
+        a.stats[2 + iq] = ok ? ns_iters : -1;
+        if (!a.split) a.stats[iq] = 0;
+      }
     }
+    if (a.split && role == 1) {
+      // hand U_rho over to the eigensolver's workgroup of this isospin: the
+      // stores, then a release at device scope (the other workgroup may sit
+      // on another XCD)
+      tile_for<NP, NT>(n, [&](int i, int j) { d.U[i * n + j] = Urho[i * n + j]; });
+      __syncthreads();
+      if (tid == 0) __hip_atomic_store(&a.sync[1 + iq], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+    // subspace diagonalisation (static.py:300-303): U <- U_rho U_lam
     if (a.tdiag && d.Lam) {
       int sw;
       if constexpr (WM == WM_LDS) {
@@ -987,6 +1005,18 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
         sw = jacobi_packed<NT>(Ap, Vt, B1, n, jsh, true);
       }
       if (a.stats && tid == 0) a.stats[iq] = sw;
+      if (a.split) {
+        // U_rho from the Loewdin workgroup (acquire at device scope; the flag
+        // returns to zero for the next call)
+        if (tid == 0) {
+          while (__hip_atomic_load(&a.sync[1 + iq], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+            __builtin_amdgcn_s_sleep(2);
+          __hip_atomic_store(&a.sync[1 + iq], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        tile_for<NP, NT>(n, [&](int i, int j) { Urho[i * n + j] = d.U[i * n + j]; });
+        __syncthreads();
+      }
       mm<NP, NT, false>(B3, Urho, B1, n);
       U = B3;
     }
@@ -1374,8 +1404,13 @@ static int launch_diag(const DiagArgs& a, int nbig, hipStream_t st) {
 #define HFB_DIAG(NPV, NTV, WMV, LDSV)                                                        \
   {                                                                                         \
     if ((LDSV) > 32 * 1024) allow_full_lds((const void*)diag_kernel<NPV, NTV, WMV>);        \
-    hipLaunchKernelGGL((diag_kernel<NPV, NTV, WMV>), dim3(2), dim3(NTV), (LDSV), st, a);    \
+    hipLaunchKernelGGL((diag_kernel<NPV, NTV, WMV>), dim3(b.split ? 4 : 2), dim3(NTV), (LDSV), st, b); \
   }
+  // blocks in LDS with the subspace diagonalisation: Loewdin and the
+  // eigensolver on separate workgroups (four in all, each its own compute
+  // unit: the eigensolver's workgroup waits for its Loewdin workgroup)
+  DiagArgs b = a;
+  b.split = nbig <= NDIAG_LDS && a.tdiag && a.mode == DIAG_FULL;
   const size_t cz = sizeof(cx<double>);
   const size_t tri = (size_t)nbig * (nbig + 1) / 2;
   if (nbig <= NDIAG_LDS) {
