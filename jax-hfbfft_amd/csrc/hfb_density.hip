@@ -201,19 +201,28 @@ __device__ __forceinline__ void dens_body(const DensArgs& a, const cx<T>* __rest
       }
     }
     if (a.want_sp) {  // the wavefront covers this tile's pencils for state s
-      // lane sums level by level over the eight values
+      // lane sums level by level over the values this pass fills (kin and
+      // two L components in the x and y passes; the z pass adds the spin
+      // sums and the parity, all but L_z); the others stay zero
+      constexpr unsigned used = AXIS == 0 ? 0x0Du : (AXIS == 1 ? 0x0Bu : 0xF7u);
 #pragma unroll
-      for (int k = 0; k < 8; ++k) sp[k] += lane_xor<1>(sp[k]);
+      for (int k = 0; k < 8; ++k)
+        if (used >> k & 1u) sp[k] += lane_xor<1>(sp[k]);
 #pragma unroll
-      for (int k = 0; k < 8; ++k) sp[k] += lane_xor<2>(sp[k]);
+      for (int k = 0; k < 8; ++k)
+        if (used >> k & 1u) sp[k] += lane_xor<2>(sp[k]);
 #pragma unroll
-      for (int k = 0; k < 8; ++k) sp[k] += lane_xor<4>(sp[k]);
+      for (int k = 0; k < 8; ++k)
+        if (used >> k & 1u) sp[k] += lane_xor<4>(sp[k]);
 #pragma unroll
-      for (int k = 0; k < 8; ++k) sp[k] += lane_xor<8>(sp[k]);
+      for (int k = 0; k < 8; ++k)
+        if (used >> k & 1u) sp[k] += lane_xor<8>(sp[k]);
 #pragma unroll
-      for (int k = 0; k < 8; ++k) sp[k] += lane_xor<16>(sp[k]);
+      for (int k = 0; k < 8; ++k)
+        if (used >> k & 1u) sp[k] += lane_xor<16>(sp[k]);
 #pragma unroll
-      for (int k = 0; k < 8; ++k) sp[k] += __shfl_xor(sp[k], 32, 64);
+      for (int k = 0; k < 8; ++k)
+        if (used >> k & 1u) sp[k] += __shfl_xor(sp[k], 32, 64);
       if (lane < 8) {
         double v = sp[0];
 #pragma unroll
