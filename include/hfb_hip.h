@@ -276,7 +276,11 @@ int hfb_gram2_ex(int prec, const void* const* A, const void* const* B, int nm, c
  * doubles): Jacobi sweeps of the lambda_lin eigensolve per isospin, then
  * Newton-Schulz iterations per isospin (0: the overlap was within 1e-3 of
  * the identity in Frobenius norm and the binomial series to fourth order
- * gave S^{-1/2}; -1: fell back to the eigensolve).
+ * gave S^{-1/2}; -1: fell back to the eigensolve).  jacobi_tol: the
+ * relative off-diagonal Frobenius norm at which the lambda_lin eigensolve
+ * stops (0: 1e-13, the accuracy of FP64 subspace matrices; the mixed path
+ * passes 1e-10 for matrices of FP32 states, whose entries carry about 1e-7;
+ * HFB_E_ARG outside [0, 1)).
  * efluct1/2 are combined over the isospins inside the same launch (an
  * arrival counter of the context). */
 int hfb_diag_small(hfb_ctx* c, int nst0, int nst1, int nmax, const double* const* mats, int tdiag, int ipair, int tbcs,
@@ -284,7 +288,7 @@ int hfb_diag_small(hfb_ctx* c, int nst0, int nst1, int nmax, const double* const
                    const double* pairwg, double* sp_norm, double* sp_energy, double* deltaf, double* hmatrix,
                    double* gapmatrix, double* symcond, double* lambda_save, double* efluct1q, double* efluct2q,
                    double* efluct1, double* efluct2, double* const* uo, double* work, double* stats,
-                   void* stream);
+                   double jacobi_tol, void* stream);
 
 /* loewdin_orthonormalize (static.py:463-482) of one block from its overlap
  * S = wxyz psi_2d^H psi_2d (nst x nst complex128, e.g. from hfb_gram): U

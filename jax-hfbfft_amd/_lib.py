@@ -79,7 +79,7 @@ _SIGS = {
     "hfb_diag_small": (c_int, [c_void_p, c_int, c_int, c_int, POINTER(c_void_p), c_int, c_int, c_int, c_int, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_void_p),
-                               c_void_p, c_void_p, c_void_p]),
+                               c_void_p, c_void_p, c_double, c_void_p]),
     "hfb_poisson_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     "hfb_loewdin": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hfb_hfb_matrices": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int]

@@ -25,12 +25,13 @@ constexpr int NDIAG_LDS = 36;   // blocks up to this size keep the five work mat
 constexpr int NDIAG_MAX = 256;  // largest block hfb_diag_small accepts
 
 // Jacobi stops when the squared off-diagonal Frobenius norm is below
-// JACOBI_TOL2 times the squared diagonal one (relative off-diagonal norm
-// 1e-13): the entries of the subspace matrices carry larger errors than that
-// (FP64 sums of K = 2N products, about sqrt(K) eps = 6e-14 relative at 32^3,
-// and FP32 inputs on the mixed path), so further sweeps only polish
-// round-off.  Measured at Ca40: two sweeps reach 1e-28 .. 1e-26 in the
-// steady iterations, the third went on to 1e-36.
+// tol2 times the squared diagonal one.  The default JACOBI_TOL2 (relative
+// off-diagonal norm 1e-13) is the accuracy of FP64 subspace matrices (sums
+// of K = 2N products, about sqrt(K) eps = 6e-14 relative at 32^3), so
+// further sweeps only polish round-off; measured at Ca40, two sweeps reach
+// 1e-28 .. 1e-26 in the steady iterations, the third went on to 1e-36.
+// Callers whose matrices come from FP32 states (entries accurate to about
+// 1e-7) pass a looser bound for the lambda_lin eigensolve.
 constexpr double JACOBI_TOL2 = 1e-26;
 
 struct DiagIso {
@@ -71,6 +72,7 @@ struct DiagArgs {
   // lambda_lin (independent of U_rho, static.py:295-302), then takes U_rho,
   // forms U = U_rho U_lam and constructs the HFB matrices
   int split;
+  double tol2;  // the lambda_lin eigensolve stops at off-diagonal^2 <= tol2 diagonal^2 (Frobenius)
   int mode;  // DIAG_FULL, DIAG_LOEWDIN (S -> U only) or DIAG_GIVEN_U (construct from U in d.U)
 };
 
@@ -453,7 +455,7 @@ template <int NP, int NT>
 __device__ __forceinline__ int jacobi_wave(const cx<double>* __restrict__ M, cx<double>* __restrict__ A0,
                                            cx<double>* __restrict__ A1, cx<double>* __restrict__ V,
                                            double* __restrict__ lc, cx<double>* __restrict__ ls, int n,
-                                           JacobiSmall& js, bool sort, cx<double>* T) {
+                                           JacobiSmall& js, bool sort, cx<double>* T, double tol2 = JACOBI_TOL2) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (n == 1) {
     if (tid == 0) {
@@ -526,7 +528,7 @@ __device__ __forceinline__ int jacobi_wave(const cx<double>* __restrict__ M, cx<
       off += js.part[buf][w][0];
       dia += js.part[buf][w][1];
     }
-    const bool stop = sweep >= 40 || off <= JACOBI_TOL2 * dia || off == 0.0;
+    const bool stop = sweep >= 40 || off <= tol2 * dia || off == 0.0;
     if (wave < na) {
       if (!stop) {
         double o = 0.0, d = 0.0;
@@ -633,7 +635,8 @@ __device__ void load_lower_packed(cx<double>* A, const cx<double>* M, int n) {
 // unrotated.
 template <int NT>
 __device__ __forceinline__ int jacobi_packed(cx<double>* __restrict__ A, cx<double>* __restrict__ Vt,
-                                             cx<double>* __restrict__ Vout, int n, JacobiShared& js, bool sort) {
+                                             cx<double>* __restrict__ Vout, int n, JacobiShared& js, bool sort,
+                                             double tol2 = JACOBI_TOL2) {
   const int tid = threadIdx.x;
   for (int e = tid; e < n * n; e += NT) {
     const int r = e / n, c = e - r * n;
@@ -656,7 +659,7 @@ __device__ __forceinline__ int jacobi_packed(cx<double>* __restrict__ A, cx<doub
         off += 2.0 * v;
     }
     block_sum2_all<NT>(off, dia, js.red);
-    if (off <= JACOBI_TOL2 * dia || off == 0.0) break;
+    if (off <= tol2 * dia || off == 0.0) break;
     for (int t = 0; t < m - 1; ++t) {
       // rotation angles of round t (threads < npair)
       if (tid < npair) {
@@ -999,10 +1002,10 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
     if (a.tdiag && d.Lam) {
       int sw;
       if constexpr (WM == WM_LDS) {
-        sw = jacobi_wave<NP, NT>(d.Lam, B2, jA1, B1, jlc, jls, n, jsh, true, B2);
+        sw = jacobi_wave<NP, NT>(d.Lam, B2, jA1, B1, jlc, jls, n, jsh, true, B2, a.tol2);
       } else {
         load_lower_packed<NT>(Ap, d.Lam, n);
-        sw = jacobi_packed<NT>(Ap, Vt, B1, n, jsh, true);
+        sw = jacobi_packed<NT>(Ap, Vt, B1, n, jsh, true, a.tol2);
       }
       if (a.stats && tid == 0) a.stats[iq] = sw;
       if (a.split) {
@@ -1444,8 +1447,10 @@ extern "C" int hfb_diag_small(hfb_ctx* c, int nst0, int nst1, int nmax, const do
                               const double* pairwg, double* sp_norm, double* sp_energy, double* deltaf,
                               double* hmatrix, double* gapmatrix, double* symcond, double* lambda_save,
                               double* efluct1q, double* efluct2q, double* efluct1, double* efluct2,
-                              double* const* uo, double* work, double* stats, void* stream) {
+                              double* const* uo, double* work, double* stats, double jacobi_tol,
+                              void* stream) {
   if (!c) return HFB_E_NOCTX;
+  if (!(jacobi_tol >= 0.0) || jacobi_tol >= 1.0) return HFB_E_ARG;
   if (!mats || !uo || !wocc || !wstates || !wguv || !pairwg || !sp_norm || !sp_energy || !deltaf ||
       !hmatrix || !gapmatrix || !symcond || !lambda_save || !efluct1q || !efluct2q || !efluct1 || !efluct2)
     return HFB_E_ARG;
@@ -1488,6 +1493,7 @@ extern "C" int hfb_diag_small(hfb_ctx* c, int nst0, int nst1, int nmax, const do
   a.efl2 = efluct2;
   a.sync = c->sync + HFB_SYNC_DIAG;
   a.tdiag = tdiag;
+  a.tol2 = jacobi_tol > 0.0 ? jacobi_tol * jacobi_tol : JACOBI_TOL2;
   a.ipair = ipair;
   a.tbcs = tbcs;
   a.construct = construct;
@@ -1503,6 +1509,7 @@ extern "C" int hfb_loewdin(int nst, const double* S, double* U, double* sp_norm,
   if (nst <= 0 || nst > NDIAG_MAX) return HFB_E_SIZE;
   if (nst > NDIAG_LDS && !work) return HFB_E_ARG;
   DiagArgs a{};
+  a.tol2 = JACOBI_TOL2;
   a.nmax = nst;
   DiagIso& d = a.iso[0];
   d.nst = nst;
@@ -1532,6 +1539,7 @@ extern "C" int hfb_hfb_matrices(hfb_ctx* c, int iq, int start, int nst, int nmax
   if (iq < 0 || iq > 1 || start < 0 || nst <= 0 || nst > nmax || nst > NDIAG_MAX) return HFB_E_SIZE;
   if (nst > NDIAG_LDS && !work) return HFB_E_ARG;
   DiagArgs a{};
+  a.tol2 = JACOBI_TOL2;
   a.nmax = nmax;
   DiagIso& d = a.iso[iq];
   d.nst = nst;

@@ -436,8 +436,11 @@ def rotate2(blocks, K):
     check(lib().hfb_rotate2(prec, *arrs, ns, K, stream_handle()), "rotate2")
 
 
-def diag_small(ctx, nst0, nst1, nmax, mats, tdiag, ipair, tbcs, construct, levels, static, E, uo, work=None):
-    """Loewdin, optional diagonalisation and HFB matrices for both isospins."""
+def diag_small(ctx, nst0, nst1, nmax, mats, tdiag, ipair, tbcs, construct, levels, static, E, uo, work=None,
+               jacobi_tol=0.0):
+    """Loewdin, optional diagonalisation and HFB matrices for both isospins;
+    ``jacobi_tol``: relative off-diagonal norm at which the lambda_lin
+    eigensolve stops (0: the FP64 default 1e-13)."""
     m = (c_void_p * 8)(*[(t.data_ptr() if t is not None else None) for t in mats])
     u = (c_void_p * 4)(*[t.data_ptr() for t in uo])
     Ep = E.data_ptr()
@@ -449,7 +452,7 @@ def diag_small(ctx, nst0, nst1, nmax, mats, tdiag, ipair, tbcs, construct, level
                                ptr(static.lambda_save), c_void_p(Ep + sz * EN["EFL1Q"]),
                                c_void_p(Ep + sz * EN["EFL2Q"]), c_void_p(Ep + sz * EN["EFL1"]),
                                c_void_p(Ep + sz * EN["EFL2"]), u, ptr(work), c_void_p(Ep + sz * EN["JACOBI"]),
-                               stream_handle()), "diag_small")
+                               float(jacobi_tol), stream_handle()), "diag_small")
 
 
 # ---------------------------------------------------------------------------

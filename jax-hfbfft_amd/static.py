@@ -367,8 +367,11 @@ def diagstep(energies, forces, grids, levels, static, diagonalize=False, constru
     kernels.gram2(ctx, blocks, grids.wxyz, outs, finish_job=finish_job)
     # blocks above 36 states keep the subspace work matrices in global memory
     work = ctx.work(("diag_work", nmax), (10 * nmax * nmax,), torch.complex128) if max(n0, n1) > 36 else None
+    # the lambda_lin eigensolve to the accuracy of its entries: 1e-13 for FP64
+    # states, 1e-10 for FP32 states (FP32 Gram entries carry about 1e-7)
+    jtol = 1e-10 if levels.psi.dtype == torch.complex64 else 1e-13
     kernels.diag_small(ctx, n0, n1, nmax, mats, diagonalize, forces.ipair != 0, forces.tbcs, construct, levels, static,
-                       energies.vec, uo, work)
+                       energies.vec, uo, work, jacobi_tol=jtol)
     # psi <- psi U and lagrange <- psi M2 in place (every workgroup reads its
     # columns of psi before it writes them)
     rot = []
