@@ -269,7 +269,6 @@ struct JacobiSmall {
   double ev[NDIAG_LDS];
   int perm[NDIAG_LDS];
   double part[2][3][2];  // per sweep parity and rounds wavefront (at most 3): off, dia
-  unsigned int cnt;      // rounds wavefronts' completed rounds
 };
 
 // Pair k of round t of the round-robin ordering over m (even) indices:
@@ -335,101 +334,110 @@ __host__ __device__ __forceinline__ int jw_nrw(int n) {
   return (np * (np + 1) / 2 + 63) / 64;
 }
 
-// Rounds R0 .. R0 + m - 2 (one sweep) of the wavefront Jacobi in the
-// calling wavefront, one of NA that run the rounds; the lane owns block
-// (ka, kb) of the lower block triangle (ka = -1: none; the lane then works
-// on block (0, 0) and writes nothing).  Round R reads the matrix in buffer
-// R & 1 and writes the rotated matrix into the other buffer (a round
-// rewrites every entry once), so the wavefronts of one round never write
-// what another still reads; a wavefront starts round R when all NA have
-// finished round R - 1 (LDS counter cnt, one increment per wavefront and
-// round; with NA = 1 the in-order LDS accesses of one wavefront suffice).
-// A round issues all its loads first, computes the next round's pairs in
-// their shadow, then the rotations and the block, then the writes; no
-// branches inside a round.  Rotations of the diagonal blocks go to the log
-// (lcb, lsb); off/dia receive this lane's share of the norms of the sweep's
-// last matrix.
-template <int NA>
-__device__ __forceinline__ void jw_sweep(cx<double>* __restrict__ Ab0, cx<double>* __restrict__ Ab1,
-                                         double* __restrict__ lcb, cx<double>* __restrict__ lsb, int m, int R0,
-                                         int ka, int kb, unsigned int* cnt, double& off, double& dia) {
-  const int nrnd = m - 1, npair = m >> 1;
+// The 2x2 block (pair a, pair b) = (ka, kb) of one Jacobi round: R_a^H X
+// R_b of the matrix A (m x m), written with its mirror into An (a round
+// rewrites every entry once, so with A and An distinct the blocks of one
+// round never write what another still reads); the lane of a diagonal
+// block logs its rotation (lcr, lsr: this round's slots).  ka = -1: no
+// block (the lane works on block (0, 0) and writes nothing).  With `last`,
+// off/dia receive the block's norms (entries on and off the diagonal summed
+// apart, no cancellation).  Loads first, no branches before the writes.
+__device__ __forceinline__ void jw_block(const cx<double>* __restrict__ A, cx<double>* __restrict__ An,
+                                         double* __restrict__ lcr, cx<double>* __restrict__ lsr, int m, int t, int ka,
+                                         int kb, bool last, double& off, double& dia) {
   const bool own = ka >= 0;
   const int ia = own ? ka : 0, ib = own ? kb : 0;
   const bool dg = own && ia == ib;
   int pa, qa, pb, qb;
-  rr_pair(ia, 0, m, pa, qa);
-  rr_pair(ib, 0, m, pb, qb);
+  rr_pair(ia, t, m, pa, qa);
+  rr_pair(ib, t, m, pb, qb);
+  const cx<double> x00 = A[pa * m + pb], x01 = A[pa * m + qb];
+  const cx<double> x10 = A[qa * m + pb], x11 = A[qa * m + qb];
+  const double app = A[pa * m + pa].x, aqq = A[qa * m + qa].x;
+  const cx<double> apq = A[pa * m + qa];
+  const double bpp = A[pb * m + pb].x, bqq = A[qb * m + qb].x;
+  const cx<double> bpq = A[pb * m + qb];
+  double ca, cb;
+  cx<double> sa, sb;
+  jw_rot(app, aqq, apq, ca, sa);
+  jw_rot(bpp, bqq, bpq, cb, sb);
+  // columns: [Y_p Y_q] = [X_p X_q] R_b
+  const cx<double> y00 = csub(cscale(x00, cb), cmul(x01, cconj(sb)));
+  const cx<double> y01 = cadd(cmul(x00, sb), cscale(x01, cb));
+  const cx<double> y10 = csub(cscale(x10, cb), cmul(x11, cconj(sb)));
+  const cx<double> y11 = cadd(cmul(x10, sb), cscale(x11, cb));
+  // rows: R_a^H = [[c, -sg], [conj(sg), c]]
+  cx<double> n00 = csub(cscale(y00, ca), cmul(sa, y10));
+  cx<double> n01 = csub(cscale(y01, ca), cmul(sa, y11));
+  cx<double> n10 = cadd(cmul(cconj(sa), y00), cscale(y10, ca));
+  cx<double> n11 = cadd(cmul(cconj(sa), y01), cscale(y11, ca));
+  // the rotated pair's diagonal block is exactly diagonal
+  const bool zap = dg && (sa.x != 0.0 || sa.y != 0.0);
+  n01 = zap ? cx<double>{0.0, 0.0} : n01;
+  n10 = zap ? cx<double>{0.0, 0.0} : n10;
+  n00.y = zap ? 0.0 : n00.y;
+  n11.y = zap ? 0.0 : n11.y;
+  if (last) {
+    const double d0 = cabs2(n00) + cabs2(n11), o0 = cabs2(n01) + cabs2(n10);
+    off = own ? (dg ? o0 : 2.0 * (d0 + o0)) : 0.0;
+    dia = dg ? d0 : 0.0;
+  }
+  if (own) {
+    An[pa * m + pb] = n00;
+    An[pa * m + qb] = n01;
+    An[qa * m + pb] = n10;
+    An[qa * m + qb] = n11;
+  }
+  if (own && !dg) {
+    An[pb * m + pa] = cconj(n00);
+    An[qb * m + pa] = cconj(n01);
+    An[pb * m + qa] = cconj(n10);
+    An[qb * m + qa] = cconj(n11);
+  }
+  if (dg) {
+    lcr[ia] = ca;
+    lsr[ia] = sa;
+  }
+}
+
+// One sweep (rounds R0 .. R0 + m - 2) of the wavefront Jacobi when one
+// wavefront runs the rounds: round R reads buffer R & 1 and writes the
+// other; the wavefront's LDS accesses execute in order, so the next round
+// sees this round's writes without a barrier.  Rotations go to the sweep's
+// log (lcb, lsb: (m - 1) rounds x m/2 pairs); off/dia receive this lane's
+// share of the norms of the sweep's last matrix.
+__device__ __forceinline__ void jw_sweep(cx<double>* __restrict__ Ab0, cx<double>* __restrict__ Ab1,
+                                         double* __restrict__ lcb, cx<double>* __restrict__ lsb, int m, int R0,
+                                         int ka, int kb, double& off, double& dia) {
+  const int nrnd = m - 1, npair = m >> 1;
   for (int t = 0; t < nrnd; ++t) {
     const int R = R0 + t;
-    if constexpr (NA > 1) {
-      // every rounds wavefront has finished round R - 1
-      while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < (unsigned)(NA * R))
-        __builtin_amdgcn_s_sleep(1);
-    }
-    const cx<double>* const A = (R & 1) ? Ab1 : Ab0;
-    cx<double>* const An = (R & 1) ? Ab0 : Ab1;
-    const cx<double> x00 = A[pa * m + pb], x01 = A[pa * m + qb];
-    const cx<double> x10 = A[qa * m + pb], x11 = A[qa * m + qb];
-    const double app = A[pa * m + pa].x, aqq = A[qa * m + qa].x;
-    const cx<double> apq = A[pa * m + qa];
-    const double bpp = A[pb * m + pb].x, bqq = A[qb * m + qb].x;
-    const cx<double> bpq = A[pb * m + qb];
-    // next round's pairs, in the loads' shadow
-    int npa, nqa, npb, nqb;
-    rr_pair(ia, t + 1, m, npa, nqa);
-    rr_pair(ib, t + 1, m, npb, nqb);
-    double ca, cb;
-    cx<double> sa, sb;
-    jw_rot(app, aqq, apq, ca, sa);
-    jw_rot(bpp, bqq, bpq, cb, sb);
-    // columns: [Y_p Y_q] = [X_p X_q] R_b
-    const cx<double> y00 = csub(cscale(x00, cb), cmul(x01, cconj(sb)));
-    const cx<double> y01 = cadd(cmul(x00, sb), cscale(x01, cb));
-    const cx<double> y10 = csub(cscale(x10, cb), cmul(x11, cconj(sb)));
-    const cx<double> y11 = cadd(cmul(x10, sb), cscale(x11, cb));
-    // rows: R_a^H = [[c, -sg], [conj(sg), c]]
-    cx<double> n00 = csub(cscale(y00, ca), cmul(sa, y10));
-    cx<double> n01 = csub(cscale(y01, ca), cmul(sa, y11));
-    cx<double> n10 = cadd(cmul(cconj(sa), y00), cscale(y10, ca));
-    cx<double> n11 = cadd(cmul(cconj(sa), y01), cscale(y11, ca));
-    // the rotated pair's diagonal block is exactly diagonal
-    const bool zap = dg && (sa.x != 0.0 || sa.y != 0.0);
-    n01 = zap ? cx<double>{0.0, 0.0} : n01;
-    n10 = zap ? cx<double>{0.0, 0.0} : n10;
-    n00.y = zap ? 0.0 : n00.y;
-    n11.y = zap ? 0.0 : n11.y;
-    if (t == nrnd - 1) {  // entries on and off the diagonal summed apart (no cancellation)
-      const double d0 = cabs2(n00) + cabs2(n11), o0 = cabs2(n01) + cabs2(n10);
-      off = own ? (dg ? o0 : 2.0 * (d0 + o0)) : 0.0;
-      dia = dg ? d0 : 0.0;
-    }
-    if (own) {
-      An[pa * m + pb] = n00;
-      An[pa * m + qb] = n01;
-      An[qa * m + pb] = n10;
-      An[qa * m + qb] = n11;
-    }
-    if (own && !dg) {
-      An[pb * m + pa] = cconj(n00);
-      An[qb * m + pa] = cconj(n01);
-      An[pb * m + qa] = cconj(n10);
-      An[qb * m + qa] = cconj(n11);
-    }
-    if (dg) {
-      lcb[t * npair + ia] = ca;
-      lsb[t * npair + ia] = sa;
-    }
-    pa = npa;
-    qa = nqa;
-    pb = npb;
-    qb = nqb;
-    if constexpr (NA > 1) {
-      // this wavefront's writes are complete before its increment
-      __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
-      if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
+    jw_block((R & 1) ? Ab1 : Ab0, (R & 1) ? Ab0 : Ab1, lcb + t * npair, lsb + t * npair, m, t, ka, kb,
+             t == nrnd - 1, off, dia);
     wave_sync();
+  }
+}
+
+// V <- V R for one round t (rotations in lcr, lsr); row r of V belongs to
+// eigenvector wavefront w = r % nvw, whose lanes take its (row, pair) items.
+__device__ __forceinline__ void jw_vround(cx<double>* __restrict__ V, const double* __restrict__ lcr,
+                                          const cx<double>* __restrict__ lsr, int n, int m, int t, int w, int nvw,
+                                          int lane) {
+  const int npair = m >> 1;
+  const int nrow = w < n ? (n - w + nvw - 1) / nvw : 0;
+  const int nitem = nrow * npair;
+  for (int it = lane; it < nitem; it += 64) {
+    const int lr = it / npair, k = it - lr * npair;
+    const int r = w + nvw * lr;
+    int p, q;
+    rr_pair(k, t, m, p, q);
+    if (q < n) {
+      const double c = lcr[k];
+      const cx<double> sg = lsr[k];
+      const cx<double> vp = V[r * n + p], vq = V[r * n + q];
+      V[r * n + p] = csub(cscale(vp, c), cmul(vq, cconj(sg)));
+      V[r * n + q] = cadd(cmul(vp, sg), cscale(vq, c));
+    }
   }
 }
 
@@ -482,7 +490,6 @@ __device__ __forceinline__ int jacobi_wave(const cx<double>* __restrict__ M, cx<
     A0[e] = v;
   }
   tile_for<NP, NT>(n, [&](int i, int j) { V[i * n + j] = cx<double>{i == j ? 1.0 : 0.0, 0.0}; });
-  if (tid == 0) js.cnt = 0;
   __syncthreads();
 
   // this lane's block (ka >= kb) of the lower block triangle
@@ -518,61 +525,61 @@ __device__ __forceinline__ int jacobi_wave(const cx<double>* __restrict__ M, cx<
     }
   }
   __syncthreads();
+  const int nvw = NT / 64 - na;  // eigenvector wavefronts
+  const int w = wave - na;
   int sweep = 0;
   for (;; ++sweep) {
     const int buf = sweep & 1;
     // the norms of the current matrix (the previous sweep's last round):
     // every wavefront takes the same decision
     double off = 0.0, dia = 0.0;
-    for (int w = 0; w < na; ++w) {
-      off += js.part[buf][w][0];
-      dia += js.part[buf][w][1];
+    for (int k = 0; k < na; ++k) {
+      off += js.part[buf][k][0];
+      dia += js.part[buf][k][1];
     }
     const bool stop = sweep >= 40 || off <= tol2 * dia || off == 0.0;
-    if (wave < na) {
-      if (!stop) {
-        double o = 0.0, d = 0.0;
-        cx<double>* const A1b = A1;
-        if (na == 1)
-          jw_sweep<1>(A0, A1b, lc + buf * lsz, ls + buf * lsz, m, sweep * nrnd, ka, kb, &js.cnt, o, d);
-        else if (na == 2)
-          jw_sweep<2>(A0, A1b, lc + buf * lsz, ls + buf * lsz, m, sweep * nrnd, ka, kb, &js.cnt, o, d);
-        else  // at most 3 for blocks up to NDIAG_LDS states
-          jw_sweep<3>(A0, A1b, lc + buf * lsz, ls + buf * lsz, m, sweep * nrnd, ka, kb, &js.cnt, o, d);
-#pragma unroll
-        for (int h = 1; h < 64; h <<= 1) {
-          o += __shfl_xor(o, h, 64);
-          d += __shfl_xor(d, h, 64);
-        }
-        if (lane == 0) {
-          js.part[buf ^ 1][wave][0] = o;
-          js.part[buf ^ 1][wave][1] = d;
+    double o = 0.0, d = 0.0;
+    if (na == 1) {
+      // one rounds wavefront, no barriers inside a sweep: the eigenvector
+      // wavefronts apply the previous sweep's logged rotations meanwhile
+      if (wave == 0) {
+        if (!stop) jw_sweep(A0, A1, lc + buf * lsz, ls + buf * lsz, m, sweep * nrnd, ka, kb, o, d);
+      } else if (sweep > 0) {
+        for (int t = 0; t < nrnd; ++t) {
+          jw_vround(V, lc + (buf ^ 1) * lsz + t * npair, ls + (buf ^ 1) * lsz + t * npair, n, m, t, w, nvw, lane);
+          wave_sync();
         }
       }
-    } else if (sweep > 0) {
-      // V <- V R for every round of the previous sweep; row r belongs to
-      // wavefront na + r % (NT/64 - na)
-      const int nvw = NT / 64 - na;
-      const int w = wave - na;
-      const int nrow = w < n ? (n - w + nvw - 1) / nvw : 0;
-      const int nitem = nrow * npair;
-      const double* const lcb = lc + (buf ^ 1) * lsz;
-      const cx<double>* const lsb = ls + (buf ^ 1) * lsz;
+    } else if (!stop) {
+      // several rounds wavefronts: a workgroup barrier after every round;
+      // the eigenvector wavefronts apply the previous round meanwhile (the
+      // first round of a sweep: the previous sweep's last round), from the
+      // two-slot log of the first 2 m/2 entries
       for (int t = 0; t < nrnd; ++t) {
-        for (int it = lane; it < nitem; it += 64) {
-          const int lr = it / npair, k = it - lr * npair;
-          const int r = w + nvw * lr;
-          int p, q;
-          rr_pair(k, t, m, p, q);
-          if (q < n) {
-            const double c = lcb[t * npair + k];
-            const cx<double> sg = lsb[t * npair + k];
-            const cx<double> vp = V[r * n + p], vq = V[r * n + q];
-            V[r * n + p] = csub(cscale(vp, c), cmul(vq, cconj(sg)));
-            V[r * n + q] = cadd(cmul(vp, sg), cscale(vq, c));
-          }
+        const int R = sweep * nrnd + t;
+        if (wave < na) {
+          jw_block((R & 1) ? A1 : A0, (R & 1) ? A0 : A1, lc + (R & 1) * npair, ls + (R & 1) * npair, m, t, ka, kb,
+                   t == nrnd - 1, o, d);
+        } else if (R > 0) {
+          const int Rp = R - 1;
+          jw_vround(V, lc + (Rp & 1) * npair, ls + (Rp & 1) * npair, n, m, Rp % nrnd, w, nvw, lane);
         }
-        wave_sync();
+        __syncthreads();
+      }
+    } else if (sweep > 0 && wave >= na) {
+      // converged: the last round's rotations
+      const int Rp = sweep * nrnd - 1;
+      jw_vround(V, lc + (Rp & 1) * npair, ls + (Rp & 1) * npair, n, m, Rp % nrnd, w, nvw, lane);
+    }
+    if (wave < na && !stop) {
+#pragma unroll
+      for (int h = 1; h < 64; h <<= 1) {
+        o += __shfl_xor(o, h, 64);
+        d += __shfl_xor(d, h, 64);
+      }
+      if (lane == 0) {
+        js.part[buf ^ 1][wave][0] = o;
+        js.part[buf ^ 1][wave][1] = d;
       }
     }
     __syncthreads();

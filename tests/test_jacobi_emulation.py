@@ -319,14 +319,17 @@ def jacobi_wave_emulated(M, max_sweeps=40):
         od = Acur - np.diag(np.diag(Acur))
         return np.sum(np.abs(od) ** 2), np.sum(np.abs(np.diag(Acur)) ** 2)
 
-    def apply_v(buf):
-        for t in range(nrnd):
+    def apply_v(buf, rounds=None):
+        # one rounds wavefront: a whole sweep from the sweep's log slots
+        # (buf * lsz + t * npair + k); several: one round R from the two-slot
+        # log ((R & 1) * npair + k)
+        for t in (range(nrnd) if rounds is None else [rounds % nrnd]):
             for w in range(nvw):
                 touched = set()
                 for _, r, k in vitems[w]:
                     p, q = (int(x) for x in rr_pair(k, t, m))
                     if q < n:
-                        idx = buf * lsz + t * npair + k
+                        idx = buf * lsz + t * npair + k if rounds is None else (rounds & 1) * npair + k
                         assert 0 <= idx < 2 * lsz and not np.isnan(lc[idx])
                         assert 0 <= r * n + p < n * n and 0 <= r * n + q < n * n
                         assert (r, p) not in touched and (r, q) not in touched
@@ -343,7 +346,8 @@ def jacobi_wave_emulated(M, max_sweeps=40):
         buf = sweep & 1
         stop = sweep >= max_sweeps or off <= 1e-26 * dia or off == 0.0
         if not stop:
-            lc[buf * lsz:(buf + 1) * lsz] = np.nan
+            if na == 1:
+                lc[buf * lsz:(buf + 1) * lsz] = np.nan
             for t in range(nrnd):
                 cur, nxt = A[R & 1], A[(R + 1) & 1]
                 nxt[:] = np.nan
@@ -367,7 +371,7 @@ def jacobi_wave_emulated(M, max_sweeps=40):
                             N[0, 1] = N[1, 0] = 0.0
                             N[0, 0] = N[0, 0].real
                             N[1, 1] = N[1, 1].real
-                        idx = buf * lsz + t * npair + a
+                        idx = buf * lsz + t * npair + a if na == 1 else (R & 1) * npair + a
                         assert 0 <= idx < 2 * lsz
                         lc[idx], ls[idx] = ca, sa
                     ent = [((pa, pb), N[0, 0]), ((pa, qb), N[0, 1]), ((qa, pb), N[1, 0]), ((qa, qb), N[1, 1])]
@@ -378,10 +382,15 @@ def jacobi_wave_emulated(M, max_sweeps=40):
                         assert writes.setdefault((x, y), wl) == wl, "two lanes write one entry"
                         nxt[x, y] = v
                 assert len(writes) == m * m  # a round rewrites the whole matrix once
+                if na > 1:  # the eigenvector wavefronts, one round behind (barrier per round)
+                    if R > 0:
+                        apply_v(None, R - 1)
                 R += 1
             off, dia = norms(A[R & 1])
-        if sweep > 0:
+        if na == 1 and sweep > 0:
             apply_v(buf ^ 1)
+        if na > 1 and stop and R > 0:
+            apply_v(None, R - 1)
         if stop:
             break
         sweep += 1
