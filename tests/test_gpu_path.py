@@ -185,6 +185,41 @@ def test_density_sp_skyrme_energy(pkg, O, dev):
         assert abs(e[nm] - r) <= 1e-10 * max(1.0, abs(r)), (nm, e[nm], r)
 
 
+@pytest.mark.parametrize("n", [24, 32])
+def test_density_sp_f32_matches_oracle(pkg, O, dev, n):
+    """FP32 states: the density sweep (on 32^3 the matrix-core derivative
+    passes, otherwise the lane passes) against the FP64 oracle applied to
+    the same FP32-rounded states.  FP32 products and derivatives: 2e-5 of
+    the largest entry."""
+    ref, run = small_case(O, pkg, n=n, ipair=6, npsi=[10, 9], precision="mixed")
+    rng = np.random.default_rng(19)
+    S = ref.levels.nstmax
+    ref.levels.wocc = rng.uniform(0.2, 1.0, S)
+    ref.levels.wguv = rng.uniform(0.0, 0.5, S)
+    ref.levels.wstates = rng.uniform(0.8, 1.0, S)
+    ref.levels.pairwg = rng.uniform(0.5, 1.0, S)
+    for nm in ("wocc", "wguv", "wstates", "pairwg"):
+        getattr(run.levels, nm).copy_(T(getattr(ref.levels, nm)))
+    assert run.levels.psi.dtype == torch.complex64
+    ref.levels.psi = run.levels.psi.cpu().numpy().astype(np.complex128)
+    O.add_density(ref.densities, ref.grids, ref.levels)
+    _, spart = pkg.densities.add_density_sweep(run.densities, run.grids, run.levels, want_sp=True, moment=run.moment)
+    # the spin density and the current nearly cancel for these states: their
+    # errors are measured against the scale of the terms they sum, rho for
+    # the spin density and sqrt(rho tau) for the currents
+    d = ref.densities
+    r0, t0 = np.max(np.abs(d.rho)), np.max(np.abs(d.tau))
+    scale = dict(rho=r0, chi=np.max(np.abs(d.chi)), tau=t0, sdens=r0, current=np.sqrt(r0 * t0),
+                 sodens=np.sqrt(r0 * t0))
+    for nm, sc in scale.items():
+        a, b = getattr(run.densities, nm).cpu().numpy(), getattr(d, nm)
+        assert np.max(np.abs(a - b)) < 2e-5 * max(np.max(np.abs(b)), sc), nm
+    O.sp_properties(ref.forces, ref.grids, ref.levels)
+    pkg.inout.sp_properties(run.forces, run.grids, run.levels, run.moment, spart=spart)
+    for nm in ("sp_kinetic", "sp_orbital", "sp_spin", "sp_parity"):
+        assert rel(getattr(run.levels, nm).cpu().numpy(), getattr(ref.levels, nm)) < 2e-5, nm
+
+
 def test_bcs_matches_oracle(pkg, O, dev):
     ref, run = small_case(O, pkg, ipair=5, tbcs=True, npsi=[12, 10], nneut=8, nprot=6)
     rng = np.random.default_rng(11)
