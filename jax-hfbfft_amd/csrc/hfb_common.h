@@ -153,6 +153,21 @@ __device__ __forceinline__ void load_twiddles(cx<T>* tw_lds, const cx<T>* tw_glo
 // original layout.  Both directions are unnormalised.  tw is the table
 // exp(-2 pi i m / N), m < N, in global memory.
 // ---------------------------------------------------------------------------
+// Wait until at most N vector memory operations of this wavefront are in
+// flight (they complete in order).  The compiler tracks this wait (it adds
+// none of its own for loads older than the N youngest) but not the LDS that
+// global_load_lds writes: the empty asm statements keep every memory access
+// on its side of the wait.
+template <int N>
+__device__ __forceinline__ void vm_wait() {
+  static_assert(N >= 0 && N < 64, "vmcnt field");
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);  // expcnt and lgkmcnt not waited on
+  asm volatile("" ::: "memory");
+}
+
+typedef __attribute__((address_space(3))) void* lds_void;
+
 template <int H>
 __device__ __forceinline__ int lane_xor_i(int v) {
   if constexpr (H == 1)

@@ -330,8 +330,6 @@ constexpr int DM_N = 32;
 constexpr int DM_PAIRS = DM_N;  // workgroups per axis: 64 tiles (32 planes x 2 column halves) in pairs
 constexpr int DM_SLOT = 2 * DM_N * 16;  // complex values of one staged tile (both spins)
 
-typedef __attribute__((address_space(3))) void* lds_void;
-
 __device__ __forceinline__ int dm_row(int blk, int row) { return 16 * blk + 4 * (row % 4) + row / 4; }
 
 // LDS index (complex) of the value at position p of column c of one spin in
@@ -343,18 +341,6 @@ __device__ __forceinline__ int dm_lidx(int spin, int p, int c) {
     return (((spin * 16 + (p >> 1)) * 16 + c) << 1) + (p & 1);
   else
     return (spin * 32 + p) * 16 + c;
-}
-
-// Wait until at most N vector memory operations of this wavefront are in
-// flight.  The compiler tracks the wait (so it adds none of its own for loads
-// older than the N youngest) but not the LDS written by global_load_lds: the
-// empty asm statements keep every memory access on its side of the wait.
-template <int N>
-__device__ __forceinline__ void vm_wait() {
-  static_assert(N >= 0 && N < 16, "vmcnt field");
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_waitcnt(0x0F70 | N);  // vmcnt(N), expcnt and lgkmcnt not waited on
-  asm volatile("" ::: "memory");
 }
 
 // The tile (32 positions x 16 columns, both spins: 8 KB) of one state from
