@@ -292,20 +292,6 @@ __device__ __forceinline__ void gm_sub(int s, int nb, int nm, int bsame, int& m,
   bj = bi + s;
 }
 
-// One 64-float group of the rows of this wavefront's tiles: the group's
-// base (uniform) plus fixed 32-bit byte offsets per lane.  The loads are
-// unconditional (rows past the block or past nst read clamped, valid rows;
-// B == A reads the same addresses), so the compiler can count the loads in
-// flight and wait for one group only.
-template <int BT>
-__device__ __forceinline__ void gm_load(gm_v4 (&a)[BT], gm_v4 (&b)[BT], const char* pa, const char* pb,
-                                        const uint32_t (&aoff)[BT], const uint32_t (&boff)[BT]) {
-#pragma unroll
-  for (int t = 0; t < BT; ++t) a[t] = *(const gm_v4*)(pa + aoff[t]);
-#pragma unroll
-  for (int u = 0; u < BT; ++u) b[u] = *(const gm_v4*)(pb + boff[u]);
-}
-
 // The eight products of every tile for one group, component by component
 // over all tiles, so that an accumulator's next product comes 2 BT^2
 // instructions later (no waiting on the matrix pipeline).  Tiles of a
@@ -329,18 +315,29 @@ __device__ __forceinline__ void gm_mma(gm_v4 (&accr)[BT][BT], gm_v4 (&acci)[BT][
   }
 }
 
-// BT: tiles per block edge of this launch (at most GM_BT).  Groups are
-// loaded three ahead of their products (a ring of three register sets), so
-// the one wavefront per SIMD has two groups of products to cover a load.
+// BT: tiles per block edge of this launch (at most GM_BT).  Groups go from
+// HBM straight into a ring of GM_RING LDS slots per wavefront
+// (global_load_lds, load i of a group fills 64 float4 in lane order, so a
+// lane reads back its own float4), GM_RING - 1 groups ahead of their
+// products: the one wavefront per SIMD has two groups of products to cover
+// a load, and the waits are explicit (vm_wait) rather than the compiler's
+// count across the loop.
+constexpr int GM_RING = 3;
+
+template <int BT>
+constexpr size_t gram_mfma_lds() { return (size_t)4 * GM_RING * 2 * BT * 64 * sizeof(gm_v4); }
+
 template <int BT>
 __global__ void __launch_bounds__(256, 1) gram_mfma_kernel(GramMfmaArgs ga, hfb_finish_job fjob) {
   if (fjob.nstates > 0 && blockIdx.x == gridDim.x - 1) {  // the extra workgroup: the step's scalar tail
     grstep_finish_body(fjob);
     return;
   }
-  // per wavefront: one row of tiles, (re, im) x BT tiles x 4 registers x 64 lanes
-  __shared__ gm_v4 red[4][2 * BT * 64];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  extern __shared__ __attribute__((aligned(16))) unsigned char gm_lds[];
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // this wavefront's ring; after the loop its first 2 BT x 64 float4 hold
+  // one row of tiles of the block sums
+  gm_v4* const ring = reinterpret_cast<gm_v4*>(gm_lds) + (size_t)wv * GM_RING * 2 * BT * 64;
   const int64_t item = (int64_t)blockIdx.x * 4 + wv;
   const bool live = item < ga.nitem;
   int iq = 0, m = 0, ti0 = 0, nti = 0, tj0 = 0, ntj = 0, chunk = 0;
@@ -385,38 +382,50 @@ __global__ void __launch_bounds__(256, 1) gram_mfma_kernel(GramMfmaArgs ga, hfb_
     for (int u = 0; u < BT; ++u) accr[t][u] = acci[t][u] = gm_v4{0.f, 0.f, 0.f, 0.f};
   if (live) {
     const int g0 = chunk * ga.gpc, g1 = min(ga.ngroup, g0 + ga.gpc);
-    // groups past the chunk's end load the last group again (never used)
-    auto ld = [&](gm_v4(&a)[BT], gm_v4(&b)[BT], int gg) {
+    // group gg into ring slot k: 2 BT loads, unconditional (groups past the
+    // chunk's end load the last group again, never used), so every group
+    // is the same number of loads in flight
+    auto stage = [&](int k, int gg) {
       const int64_t go = (int64_t)min(gg, g1 - 1) * 256;
-      gm_load<BT>(a, b, A + go, B + go, aoff, boff);
+      gm_v4* dst = ring + k * 2 * BT * 64;
+#pragma unroll
+      for (int t = 0; t < BT; ++t)
+        __builtin_amdgcn_global_load_lds((const void*)(A + go + aoff[t]), (lds_void)(dst + t * 64), 16, 0, 0);
+#pragma unroll
+      for (int u = 0; u < BT; ++u)
+        __builtin_amdgcn_global_load_lds((const void*)(B + go + boff[u]), (lds_void)(dst + (BT + u) * 64), 16, 0,
+                                         0);
     };
-    gm_v4 a0[BT], b0[BT], a1[BT], b1[BT], a2[BT], b2[BT];
-    ld(a0, b0, g0);
-    ld(a1, b1, g0 + 1);
-    ld(a2, b2, g0 + 2);
+    static_assert(GM_RING == 3, "the loop below runs the ring's three slots");
+    stage(0, g0);
+    stage(1, g0 + 1);
+    stage(2, g0 + 2);
+    auto step = [&](int k, int gi) {
+      vm_wait<(GM_RING - 1) * 2 * BT>();  // slot k has landed; the two younger groups stay in flight
+      const gm_v4* src = ring + k * 2 * BT * 64;
+      gm_v4 a[BT], b[BT];
+#pragma unroll
+      for (int t = 0; t < BT; ++t) a[t] = src[t * 64 + lane];
+#pragma unroll
+      for (int u = 0; u < BT; ++u) b[u] = src[(BT + u) * 64 + lane];
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the slot is read before it is loaded again
+      asm volatile("" ::: "memory");
+      stage(k, gi + GM_RING);
+      if (gi < g1) gm_mma<BT>(accr, acci, a, b);
+    };
 #pragma unroll 1
     for (int gi = g0; gi < g1; gi += 3) {
-      // the scheduling barriers keep each reload after the products that read
-      // its registers (hoisting it would need a fourth register set)
-      gm_mma<BT>(accr, acci, a0, b0);
-      __builtin_amdgcn_sched_barrier(0);
-      ld(a0, b0, gi + 3);
-      __builtin_amdgcn_sched_barrier(0);
-      if (gi + 1 < g1) gm_mma<BT>(accr, acci, a1, b1);
-      __builtin_amdgcn_sched_barrier(0);
-      ld(a1, b1, gi + 4);
-      __builtin_amdgcn_sched_barrier(0);
-      if (gi + 2 < g1) gm_mma<BT>(accr, acci, a2, b2);
-      __builtin_amdgcn_sched_barrier(0);
-      ld(a2, b2, gi + 5);
-      __builtin_amdgcn_sched_barrier(0);
+      step(0, gi);
+      step(1, gi + 1);
+      step(2, gi + 2);
     }
+    vm_wait<0>();  // nothing in flight into the ring before it holds the block sums
   }
   // block sums, one row of tiles per pass: lane 4b+j stores registers 0..3
   // of tile u at red[b][(part, u, j)] (a float4 per tile); lane (u, j) then
   // adds the 16 float4 of each part (consecutive lanes read consecutive
   // float4: no bank conflicts) and writes entries (4 t + i, 4 u + j).
-  gm_v4* R = red[wv];
+  gm_v4* R = ring;
   const int j = lane & 3;
   const int np = g.np;
 #pragma unroll
@@ -897,14 +906,18 @@ extern "C" int hfb_gram2_ex(int prec, const void* const* A, const void* const* B
         bt = e > bt ? e : bt;
       }
     const dim3 grid((unsigned)gx);
+    auto launch = [&](auto kern, size_t lds) {
+      if (lds > 32 * 1024) allow_full_lds((const void*)kern);
+      hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, ma, fj);
+    };
     if (bt <= 2)
-      hipLaunchKernelGGL(gram_mfma_kernel<2>, grid, dim3(256), 0, st, ma, fj);
+      launch(gram_mfma_kernel<2>, gram_mfma_lds<2>());
     else if (bt <= 3)
-      hipLaunchKernelGGL(gram_mfma_kernel<3>, grid, dim3(256), 0, st, ma, fj);
+      launch(gram_mfma_kernel<3>, gram_mfma_lds<3>());
     else if (bt <= 4)
-      hipLaunchKernelGGL(gram_mfma_kernel<4>, grid, dim3(256), 0, st, ma, fj);
+      launch(gram_mfma_kernel<4>, gram_mfma_lds<4>());
     else
-      hipLaunchKernelGGL(gram_mfma_kernel<GM_BT>, grid, dim3(256), 0, st, ma, fj);
+      launch(gram_mfma_kernel<GM_BT>, gram_mfma_lds<GM_BT>());
     int rc = launch_status();
     if (rc != HFB_OK) return rc;
     return gram_reduce(v, nm, nst, part, out, ma.nchunk, scale, st);
