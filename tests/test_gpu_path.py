@@ -185,15 +185,21 @@ def test_density_sp_skyrme_energy(pkg, O, dev):
         assert abs(e[nm] - r) <= 1e-10 * max(1.0, abs(r)), (nm, e[nm], r)
 
 
-@pytest.mark.parametrize("n", [24, 32])
-def test_density_sp_f32_matches_oracle(pkg, O, dev, n):
+@pytest.mark.parametrize("n,npsi", [(24, [10, 9]), (32, [10, 9]), (32, [140, 9])])
+def test_density_sp_f32_matches_oracle(pkg, O, dev, n, npsi):
     """FP32 states: the density sweep (on 32^3 the matrix-core derivative
     passes, otherwise the lane passes) against the FP64 oracle applied to
     the same FP32-rounded states.  FP32 products and derivatives: 2e-5 of
-    the largest entry."""
-    ref, run = small_case(O, pkg, n=n, ipair=6, npsi=[10, 9], precision="mixed")
+    the largest entry.  140 neutron states: state chunks of 70, past the 64
+    weights a wavefront holds at a time (random states, no oscillator
+    basis).  Without the per-state sums the densities are the same bits."""
+    ref, run = small_case(O, pkg, n=n, ipair=6, npsi=npsi, precision="mixed")
     rng = np.random.default_rng(19)
     S = ref.levels.nstmax
+    if npsi[0] > 64:
+        x = rng.standard_normal(run.levels.psi.shape) + 1j * rng.standard_normal(run.levels.psi.shape)
+        x /= np.sqrt(np.sum(np.abs(x) ** 2, axis=(1, 2, 3, 4), keepdims=True) * ref.grids.wxyz)
+        run.levels.psi.copy_(T(x, torch.complex64))
     ref.levels.wocc = rng.uniform(0.2, 1.0, S)
     ref.levels.wguv = rng.uniform(0.0, 0.5, S)
     ref.levels.wstates = rng.uniform(0.8, 1.0, S)
@@ -218,6 +224,11 @@ def test_density_sp_f32_matches_oracle(pkg, O, dev, n):
     pkg.inout.sp_properties(run.forces, run.grids, run.levels, run.moment, spart=spart)
     for nm in ("sp_kinetic", "sp_orbital", "sp_spin", "sp_parity"):
         assert rel(getattr(run.levels, nm).cpu().numpy(), getattr(ref.levels, nm)) < 2e-5, nm
+    first = {nm: getattr(run.densities, nm).clone() for nm in scale}
+    _, none = pkg.densities.add_density_sweep(run.densities, run.grids, run.levels, want_sp=False, moment=run.moment)
+    assert none is None
+    for nm in scale:
+        assert torch.equal(getattr(run.densities, nm), first[nm]), nm
 
 
 def test_bcs_matches_oracle(pkg, O, dev):
