@@ -421,29 +421,39 @@ __global__ void __launch_bounds__(256, 1) gram_mfma_kernel(GramMfmaArgs ga, hfb_
     }
     vm_wait<0>();  // nothing in flight into the ring before it holds the block sums
   }
-  // block sums, one row of tiles per pass: lane 4b+j stores registers 0..3
-  // of tile u at red[b][(part, u, j)] (a float4 per tile); lane (u, j) then
-  // adds the 16 float4 of each part (consecutive lanes read consecutive
-  // float4: no bank conflicts) and writes entries (4 t + i, 4 u + j).
+  // block sums through this wavefront's ring (idle now; only this
+  // wavefront touches it, and its LDS accesses run in order: no barrier),
+  // GM_TP rows of tiles per pass: lane 4b+j stores registers 0..3 of tile
+  // (t, u) of each part at R[b][(t - t0, part, u, j)] (a float4 per tile);
+  // lane (t - t0, u, j) then adds the 16 float4 of both parts (consecutive
+  // lanes read consecutive float4) and writes entries (4 t + i, 4 u + j).
+  constexpr int GM_TP = 3;  // rows of tiles per pass: 16 blocks x 3 x 2 parts x BT x 4 fill the ring
+  static_assert(16 * GM_TP * 2 * BT * 4 <= GM_RING * 2 * BT * 64, "block sums fit the ring");
+  static_assert(GM_TP * BT * 4 <= 64, "one lane per (row, tile, column) of a pass");
+  constexpr int SB = GM_TP * 2 * BT * 4;  // float4 per block
   gm_v4* R = ring;
   const int j = lane & 3;
   const int np = g.np;
 #pragma unroll
-  for (int t = 0; t < BT; ++t) {
+  for (int t0 = 0; t0 < BT; t0 += GM_TP) {
 #pragma unroll
-    for (int u = 0; u < BT; ++u) {
-      R[blk * (8 * BT) + (0 * BT + u) * 4 + j] = accr[t][u];
-      R[blk * (8 * BT) + (1 * BT + u) * 4 + j] = acci[t][u];
+    for (int tt = 0; tt < GM_TP; ++tt) {
+      if (t0 + tt >= BT) break;
+#pragma unroll
+      for (int u = 0; u < BT; ++u) {
+        R[blk * SB + ((tt * 2 + 0) * BT + u) * 4 + j] = accr[t0 + tt][u];
+        R[blk * SB + ((tt * 2 + 1) * BT + u) * 4 + j] = acci[t0 + tt][u];
+      }
     }
-    __syncthreads();
-    if (live && t < nti && lane < 4 * ntj) {
-      const int u = lane >> 2;
-      gm_v4 sr = R[(0 * BT + u) * 4 + j];
-      gm_v4 si = R[(1 * BT + u) * 4 + j];
+    const int tt = lane / (4 * BT), u = (lane >> 2) % BT;
+    const int t = t0 + tt;
+    if (live && lane < GM_TP * BT * 4 && t < BT && t < nti && u < ntj) {
+      gm_v4 sr = R[((tt * 2 + 0) * BT + u) * 4 + j];
+      gm_v4 si = R[((tt * 2 + 1) * BT + u) * 4 + j];
 #pragma unroll
       for (int bb = 1; bb < 16; ++bb) {
-        sr += R[bb * (8 * BT) + (0 * BT + u) * 4 + j];
-        si += R[bb * (8 * BT) + (1 * BT + u) * 4 + j];
+        sr += R[bb * SB + ((tt * 2 + 0) * BT + u) * 4 + j];
+        si += R[bb * SB + ((tt * 2 + 1) * BT + u) * 4 + j];
       }
       const int col = 4 * (tj0 + u) + j;
 #pragma unroll
@@ -454,7 +464,6 @@ __global__ void __launch_bounds__(256, 1) gram_mfma_kernel(GramMfmaArgs ga, hfb_
         dst[1] = (double)si[i];
       }
     }
-    __syncthreads();
   }
 }
 
