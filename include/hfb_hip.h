@@ -365,7 +365,9 @@ int hfb_rotate2(int prec, const void* const* psi, const double* const* U1, const
  * --------------------------------------------------------------------- */
 
 /* Sizes in doubles of the dout scratch of hfb_density and of its per-state
- * partial sums. */
+ * partial sums.  The layout of dout depends on prec and the grid (FP32
+ * states on 32^3 grids fill half of it with FP32 partials); the finish
+ * calls take the same prec. */
 int hfb_density_sizes(hfb_ctx* ctx, int prec, long long* dout_doubles, long long* spart_doubles_per_state);
 
 /* One sweep over the states of both isospins (nst0 + nst1 of them) that
@@ -378,16 +380,17 @@ int hfb_density(hfb_ctx* ctx, int prec, const void* psi, int nst0, int nst1, con
                 const double* wstates, const double* wguv, const double* pairwg, const double* cmtot,
                 int want_dens, int want_sp, double* dout, double* spart, void* stream);
 
-/* dout -> the Densities fields rho, chi, tau (2,n3), current, sdens, sodens
- * (2,3,n3).  mix: x <- addnew*x_new + (1-addnew)*x for rho, tau, chi
- * (static.py:1841-1848); the other fields are replaced. */
-int hfb_density_finish(hfb_ctx* ctx, const double* dout, double* rho, double* chi, double* tau, double* current,
-                       double* sdens, double* sodens, int mix, double addnew, void* stream);
+/* dout (written by hfb_density with the same prec) -> the Densities fields
+ * rho, chi, tau (2,n3), current, sdens, sodens (2,3,n3).  mix: x <-
+ * addnew*x_new + (1-addnew)*x for rho, tau, chi (static.py:1841-1848); the
+ * other fields are replaced. */
+int hfb_density_finish(hfb_ctx* ctx, int prec, const double* dout, double* rho, double* chi, double* tau,
+                       double* current, double* sdens, double* sodens, int mix, double addnew, void* stream);
 
 /* hfb_density_finish and hfb_sp_finish in one launch (the loop's density
  * update and single-particle properties, static.py:1800-1823 with
  * inout.py:64-149); arguments as there. */
-int hfb_density_sp_finish(hfb_ctx* ctx, const double* dout, double* rho, double* chi, double* tau,
+int hfb_density_sp_finish(hfb_ctx* ctx, int prec, const double* dout, double* rho, double* chi, double* tau,
                           double* current, double* sdens, double* sodens, int mix, double addnew,
                           const double* spart, int nst0, int nst1, double h2m0, double h2m1, double* sp_kinetic,
                           double* sp_orbital, double* sp_spin, double* sp_parity, void* stream);

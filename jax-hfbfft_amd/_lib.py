@@ -94,10 +94,10 @@ _SIGS = {
     "hfb_density_sizes": (c_int, [c_void_p, c_int, POINTER(ctypes.c_longlong), POINTER(ctypes.c_longlong)]),
     "hfb_density": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
-    "hfb_density_finish": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                   c_int, c_double, c_void_p]),
-    "hfb_density_sp_finish": (c_int, [c_void_p] * 8 + [c_int, c_double, c_void_p, c_int, c_int, c_double, c_double]
-                              + [c_void_p] * 5),
+    "hfb_density_finish": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_int, c_double, c_void_p]),
+    "hfb_density_sp_finish": (c_int, [c_void_p, c_int] + [c_void_p] * 7 + [c_int, c_double, c_void_p, c_int, c_int,
+                                                                           c_double, c_double] + [c_void_p] * 5),
     "hfb_sp_finish": (c_int, [c_void_p, c_void_p, c_int, c_int, c_double, c_double, c_void_p, c_void_p, c_void_p,
                               c_void_p, c_void_p]),
     "hfb_deriv_workspace": (ctypes.c_longlong, [c_void_p]),

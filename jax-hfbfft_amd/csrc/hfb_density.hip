@@ -15,14 +15,17 @@
 // over the passes to even out their register load: the y pass adds rho and
 // chi, the z pass the spin density, the per-state spin sums and the parity.
 //
-// Density partials (double) go per chunk and axis into slots of
+// Density partials go per chunk and axis into slots of
 // `dout` = [DENS_G chunk][2 iso][17 slot][nvox]:
 //   5 tau_x 6 j_x 7 Jy(x) 8 Jz(x)                              (x pass)
 //   9 tau_y 10 j_y 11 Jx(y) 12 Jz(y)  0 rho 1 chi                (y pass)
 //   13 tau_z 14 j_z 15 Jx(z) 16 Jy(z)  2 sx 3 sy 4 sz            (z pass)
-// and combined by hfb_density_finish (with the linear mixing of
-// static.py:1817-1823 when requested).  Per-state sums go to
-// spart = [3 axis][S][ntile_a][8]: kin, Lx, Ly, Lz, sx, sy, sz, parity.
+// (doubles; the matrix-core sweep of FP32 states on 32^3 writes floats, in
+// half of the scratch, and fills sz from its x pass) and combined by
+// hfb_density_finish / hfb_density_sp_finish, called with the same
+// precision (with the linear mixing of static.py:1817-1823 when
+// requested).  Per-state sums go to spart = [3 axis][S][ntile_a][8]: kin,
+// Lx, Ly, Lz, sx, sy, sz, parity.
 #include <math.h>
 
 #include "hfb_ctx.h"
@@ -602,7 +605,8 @@ __device__ __forceinline__ void densm_body(const DensArgs& a, const float* __res
   if (!a.want_dens) return;
   // all accumulators of the four wavefronts to LDS (the staging slots, idle
   // now; every load into them has completed) in one round, then the sum of
-  // each tile's two wavefronts in FP64 to the partial slots
+  // each tile's two wavefronts to the FP32 partial slots (the chunks are
+  // summed in FP64 by the finish)
   float* comb = reinterpret_cast<float*>(buf);
   auto pidx = [&](int pos, int cc) { return AXIS == 2 ? cc * N + pos : pos * 16 + cc; };
   int thirds[2], halves[2];
@@ -614,7 +618,7 @@ __device__ __forceinline__ void densm_body(const DensArgs& a, const float* __res
 #pragma unroll
     for (int m = 0; m < 8; ++m) comb[(wave * NACC + k) * 16 * N + pidx(4 * m + q, c)] = acc[m][k];
   __syncthreads();
-  double* const d0 = a.dout + ((int64_t)chunk * 2 + iq) * NSLOT * nvox;
+  float* const d0 = reinterpret_cast<float*>(a.dout) + ((int64_t)chunk * 2 + iq) * NSLOT * nvox;
   for (int e = threadIdx.x; e < NACC * 2 * 16 * N; e += 256) {
     const int k = e / (2 * 16 * N), t2 = (e / (16 * N)) & 1, el = e % (16 * N);
     int slot;
@@ -634,7 +638,7 @@ __device__ __forceinline__ void densm_body(const DensArgs& a, const float* __res
     }
     const int pe = pidx(pos, cc);
     const double v = double(comb[(t2 * NACC + k) * 16 * N + pe]) + double(comb[((t2 + 2) * NACC + k) * 16 * N + pe]);
-    d0[(int64_t)slot * nvox + thirds[t2] * st + (halves[t2] * 16 + cc) * sc + pos * sa] = v;
+    d0[(int64_t)slot * nvox + thirds[t2] * st + (halves[t2] * 16 + cc) * sc + pos * sa] = (float)v;
   }
 }
 
@@ -662,6 +666,10 @@ bool dens_mfma_path(const hfb_ctx* c, int prec) {
   return prec == HFB_F32 && c->nx == DM_N && c->ny == DM_N && c->nz == DM_N && c->der1f[0] && c->der1f[1] &&
          c->der1f[2];
 }
+
+// The partial slots in dout are FP32 (half of the scratch) where the
+// matrix-core sweep writes them, FP64 otherwise.
+bool dens_partials_f32(const hfb_ctx* c, int prec) { return dens_mfma_path(c, prec); }
 
 int launch_dens_mfma(const hfb_ctx* c, DensArgs a, double* sp0, double* sp1, double* sp2, hipStream_t st) {
   // the per-state sums keep dens_body's layout: 128 tile slots per axis
@@ -698,7 +706,9 @@ int launch_dens(const hfb_ctx* c, DensArgs a, hipStream_t st) {
 // Sum the axis slots, optional mixing new*addnew + old*(1-addnew) of rho,
 // tau, chi (static.py:1817-1823).
 // (workgroup vb of vg workgroups)
-__device__ __forceinline__ void dens_finish_body(const double* __restrict__ d, int64_t nvox, double* rho,
+// P: the type of the partial slots (FP32 from the matrix-core sweep, else FP64)
+template <typename P>
+__device__ __forceinline__ void dens_finish_body(const P* __restrict__ d, int64_t nvox, double* rho,
                                                  double* chi, double* tau, double* cur, double* sdens,
                                                  double* sodens, int mix, double addnew, int vb, int vg) {
   const int64_t tot = 2 * nvox;
@@ -709,7 +719,7 @@ __device__ __forceinline__ void dens_finish_body(const double* __restrict__ d, i
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k) {
       double acc = 0.0;
-      for (int gch = 0; gch < DENS_G; ++gch) acc += d[(((int64_t)gch * 2 + iq) * NSLOT + k) * nvox + v];
+      for (int gch = 0; gch < DENS_G; ++gch) acc += (double)d[(((int64_t)gch * 2 + iq) * NSLOT + k) * nvox + v];
       sl[k] = acc;
     }
     const double* s = sl;
@@ -737,7 +747,8 @@ __device__ __forceinline__ void dens_finish_body(const double* __restrict__ d, i
   }
 }
 
-__global__ void dens_finish_kernel(const double* __restrict__ d, int64_t nvox, double* rho, double* chi,
+template <typename P>
+__global__ void dens_finish_kernel(const P* __restrict__ d, int64_t nvox, double* rho, double* chi,
                                    double* tau, double* cur, double* sdens, double* sodens, int mix, double addnew) {
   dens_finish_body(d, nvox, rho, chi, tau, cur, sdens, sodens, mix, addnew, blockIdx.x, gridDim.x);
 }
@@ -787,7 +798,8 @@ struct SpFin {
   double *kin, *orb, *spin, *par;
 };
 
-__global__ void __launch_bounds__(256) dens_sp_finish_kernel(const double* __restrict__ d, int64_t nvox, double* rho,
+template <typename P>
+__global__ void __launch_bounds__(256) dens_sp_finish_kernel(const P* __restrict__ d, int64_t nvox, double* rho,
                                                              double* chi, double* tau, double* cur, double* sdens,
                                                              double* sodens, int mix, double addnew, int nfin,
                                                              SpFin f) {
@@ -869,25 +881,31 @@ extern "C" int hfb_density(hfb_ctx* c, int prec, const void* psi, int nst0, int 
   return rc;
 }
 
-extern "C" int hfb_density_finish(hfb_ctx* c, const double* dout, double* rho, double* chi, double* tau,
-                                  double* current, double* sdens, double* sodens, int mix, double addnew,
-                                  void* stream) {
+extern "C" int hfb_density_finish(hfb_ctx* c, int prec, const double* dout, double* rho, double* chi,
+                                  double* tau, double* current, double* sdens, double* sodens, int mix,
+                                  double addnew, void* stream) {
   if (!c) return HFB_E_NOCTX;
   if (!dout || !rho || !chi || !tau || !current || !sdens || !sodens) return HFB_E_ARG;
+  if (prec != HFB_F32 && prec != HFB_F64) return HFB_E_ARG;
   const int64_t nvox = (int64_t)c->nx * c->ny * c->nz;
   const int nb = (int)((2 * nvox + 255) / 256);
-  hipLaunchKernelGGL(dens_finish_kernel, dim3(nb), dim3(256), 0, (hipStream_t)stream, dout, nvox, rho, chi, tau,
-                     current, sdens, sodens, mix, addnew);
+  if (dens_partials_f32(c, prec))
+    hipLaunchKernelGGL(dens_finish_kernel<float>, dim3(nb), dim3(256), 0, (hipStream_t)stream,
+                       (const float*)dout, nvox, rho, chi, tau, current, sdens, sodens, mix, addnew);
+  else
+    hipLaunchKernelGGL(dens_finish_kernel<double>, dim3(nb), dim3(256), 0, (hipStream_t)stream, dout, nvox, rho,
+                       chi, tau, current, sdens, sodens, mix, addnew);
   return launch_status();
 }
 
-extern "C" int hfb_density_sp_finish(hfb_ctx* c, const double* dout, double* rho, double* chi, double* tau,
-                                     double* current, double* sdens, double* sodens, int mix, double addnew,
-                                     const double* spart, int nst0, int nst1, double h2m0, double h2m1,
-                                     double* sp_kinetic, double* sp_orbital, double* sp_spin, double* sp_parity,
-                                     void* stream) {
+extern "C" int hfb_density_sp_finish(hfb_ctx* c, int prec, const double* dout, double* rho, double* chi,
+                                     double* tau, double* current, double* sdens, double* sodens, int mix,
+                                     double addnew, const double* spart, int nst0, int nst1, double h2m0,
+                                     double h2m1, double* sp_kinetic, double* sp_orbital, double* sp_spin,
+                                     double* sp_parity, void* stream) {
   if (!c) return HFB_E_NOCTX;
   if (!dout || !rho || !chi || !tau || !current || !sdens || !sodens) return HFB_E_ARG;
+  if (prec != HFB_F32 && prec != HFB_F64) return HFB_E_ARG;
   if (!spart || !sp_kinetic || !sp_orbital || !sp_spin || !sp_parity) return HFB_E_ARG;
   const int ns = nst0 + nst1;
   if (ns <= 0) return HFB_E_ARG;
@@ -896,8 +914,12 @@ extern "C" int hfb_density_sp_finish(hfb_ctx* c, const double* dout, double* rho
   SpFin f{spart, ns, nst0, pencils_axis(0, c->nx, c->ny, c->nz) / DENS_PW,
           pencils_axis(1, c->nx, c->ny, c->nz) / DENS_PW, pencils_axis(2, c->nx, c->ny, c->nz) / DENS_PW,
           c->dx * c->dy * c->dz, h2m0, h2m1, sp_kinetic, sp_orbital, sp_spin, sp_parity};
-  hipLaunchKernelGGL(dens_sp_finish_kernel, dim3(nfin + ns), dim3(256), 0, (hipStream_t)stream, dout, nvox, rho,
-                     chi, tau, current, sdens, sodens, mix, addnew, nfin, f);
+  if (dens_partials_f32(c, prec))
+    hipLaunchKernelGGL(dens_sp_finish_kernel<float>, dim3(nfin + ns), dim3(256), 0, (hipStream_t)stream,
+                       (const float*)dout, nvox, rho, chi, tau, current, sdens, sodens, mix, addnew, nfin, f);
+  else
+    hipLaunchKernelGGL(dens_sp_finish_kernel<double>, dim3(nfin + ns), dim3(256), 0, (hipStream_t)stream, dout,
+                       nvox, rho, chi, tau, current, sdens, sodens, mix, addnew, nfin, f);
   return launch_status();
 }
 

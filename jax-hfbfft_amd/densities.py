@@ -51,5 +51,5 @@ def add_density_sweep(densities, grids, levels, mix=False, addnew=0.2, moment=No
     if want_sp and sp_h2m is not None:
         kernels.density_sp_finish(ctx, dout, densities, mix, addnew, spart, n0, n1, sp_h2m, levels)
     else:
-        kernels.density_finish(ctx, dout, densities, mix, addnew)
+        kernels.density_finish(ctx, dout, densities, mix, addnew, psi_dtype=levels.psi.dtype)
     return densities, spart

@@ -473,15 +473,19 @@ def density(ctx, psi, nst0, nst1, wocc, wstates, wguv, pairwg, cmtot, want_dens,
     return dout, spart
 
 
-def density_finish(ctx, dout, dens, mix, addnew=0.2):
-    check(lib().hfb_density_finish(ctx.handle, ptr(dout), ptr(dens.rho), ptr(dens.chi), ptr(dens.tau),
+def density_finish(ctx, dout, dens, mix, addnew=0.2, psi_dtype=torch.complex128):
+    """dout as filled by density() for states of psi_dtype (its layout
+    depends on the precision)."""
+    prec = _lib.prec_of(psi_dtype)
+    check(lib().hfb_density_finish(ctx.handle, prec, ptr(dout), ptr(dens.rho), ptr(dens.chi), ptr(dens.tau),
                                    ptr(dens.current), ptr(dens.sdens), ptr(dens.sodens), int(bool(mix)),
                                    float(addnew), stream_handle()), "density_finish")
 
 
 def density_sp_finish(ctx, dout, dens, mix, addnew, spart, nst0, nst1, h2m, levels):
-    """density_finish and sp_finish in one launch."""
-    check(lib().hfb_density_sp_finish(ctx.handle, ptr(dout), ptr(dens.rho), ptr(dens.chi), ptr(dens.tau),
+    """density_finish and sp_finish in one launch (the precision of levels.psi)."""
+    prec = _lib.prec_of(levels.psi.dtype)
+    check(lib().hfb_density_sp_finish(ctx.handle, prec, ptr(dout), ptr(dens.rho), ptr(dens.chi), ptr(dens.tau),
                                       ptr(dens.current), ptr(dens.sdens), ptr(dens.sodens), int(bool(mix)),
                                       float(addnew), ptr(spart), int(nst0), int(nst1), float(h2m[0]), float(h2m[1]),
                                       ptr(levels.sp_kinetic), ptr(levels.sp_orbital), ptr(levels.sp_spin),
