@@ -808,6 +808,115 @@ __global__ void __launch_bounds__(256) rotate_mfma_big_kernel(RotArgs ra) {
   }
 }
 
+// FP64 rotation on the FP64 matrix cores (v_mfma_f64_16x16x4f64), the real
+// form of rotate_mfma_big_kernel: a workgroup stages every input column of
+// RD_PTS consecutive grid points in LDS (FP64 rows (state i, re/im), all
+// loaded before the first store, so out1 may alias psi) and its four
+// wavefronts take the 16-row tiles mt = w, w + 4, ... (at most MTW each)
+// over the RD_SUB 16-point subtiles.  The FP64 product leaves lane l with
+// rows l / 16 + 4 r (r = 0..3) of a tile, so the coefficient rows are
+// permuted: physical row p of a tile carries logical row 4 (p % 4) + p / 4,
+// and lane l then holds logical rows 4 (l / 16) + r, the (re, im) of two
+// output states, stored as two 16-byte values.  Per k-step of four input
+// columns a lane needs one FP64 entry of U_m (re or im, sign by the 2 x 2
+// block of the complex product), read from U (L2 resident) one k-step
+// ahead of its products.
+constexpr int RD_SUB = 2;
+constexpr int RD_PTS = 16 * RD_SUB;
+constexpr int RD_MTW_MAX = 8;
+
+template <int MTW>
+__global__ void __launch_bounds__(256) rotate_mfma_f64_kernel(RotArgs ra) {
+  typedef double rd4 __attribute__((ext_vector_type(4)));
+  extern __shared__ double rd_lds[];  // [4 ks_n][RD_PTS]
+  const RotIso& ri = ra.iso[blockIdx.y];
+  const int n = ri.nst;
+  const int64_t K = ra.K;
+  const int64_t p0 = (int64_t)blockIdx.x * RD_PTS;
+  if (n <= 0 || p0 >= K) return;  // uniform over the workgroup
+  const bool two = ri.U2 != nullptr;
+  const int kin = 2 * n, ks_n = (kin + 3) / 4, kpad = 4 * ks_n;
+  const int mtot = (two ? 4 : 2) * n, mt_n = (mtot + 15) / 16;
+  // stage the input: complex (state i, point p) -> rows 2i (re), 2i + 1 (im)
+  const cx<double>* psi = (const cx<double>*)ri.psi;
+  for (int e = threadIdx.x; e < (kpad / 2) * RD_PTS; e += 256) {
+    const int i = e / RD_PTS, p = e % RD_PTS;
+    cx<double> v = {0.0, 0.0};
+    if (i < n && p0 + p < K) v = psi[(int64_t)i * K + p0 + p];
+    rd_lds[(2 * i) * RD_PTS + p] = v.x;
+    rd_lds[(2 * i + 1) * RD_PTS + p] = v.y;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = (lane >> 4) & 1;  // input re/im of this lane (k = 4 ks + lane / 16)
+  // per tile q: this lane's A row (physical lane % 16, logical 4 (p % 4) +
+  // p / 4) -> U_m element offset of (i = 0, j), which double and the sign;
+  // rows past mtot give zero
+  const double* ub[MTW];
+  double sg[MTW];
+  const int pr = lane & 15, lr = 4 * (pr & 3) + (pr >> 2);
+#pragma unroll
+  for (int q = 0; q < MTW; ++q) {
+    const int mt = w + 4 * q;
+    const int M = 16 * mt + lr;
+    const bool ok = mt < mt_n && M < mtot;
+    const int Mc = ok ? M : 0;
+    const int m = Mc / kin, jj = Mc % kin, j = jj >> 1, co = jj & 1;
+    const double* U = m == 0 ? ri.U1 : ri.U2;
+    ub[q] = U + 2 * j + (co == c ? 0 : 1);
+    sg[q] = ok ? ((co == 0 && c == 1) ? -1.0 : 1.0) : 0.0;
+  }
+  rd4 acc[MTW][RD_SUB];
+#pragma unroll
+  for (int q = 0; q < MTW; ++q)
+#pragma unroll
+    for (int s = 0; s < RD_SUB; ++s) acc[q][s] = rd4{0.0, 0.0, 0.0, 0.0};
+  // input state of this lane at k-step ks: i = 2 ks + lane / 32 (clamped; a
+  // padded column has a zero B row, so its A value is irrelevant)
+  auto uload = [&](double (&v)[MTW], int ks) {
+    const int i = min(2 * ks + (lane >> 5), n - 1);
+#pragma unroll
+    for (int q = 0; q < MTW; ++q) v[q] = ub[q][2 * (int64_t)i * n];
+  };
+  double vn[MTW];
+  uload(vn, 0);
+  for (int ks = 0; ks < ks_n; ++ks) {
+    double a[MTW];
+#pragma unroll
+    for (int q = 0; q < MTW; ++q) a[q] = sg[q] * vn[q];
+    if (ks + 1 < ks_n) uload(vn, ks + 1);
+    double b[RD_SUB];
+#pragma unroll
+    for (int s = 0; s < RD_SUB; ++s) b[s] = rd_lds[(4 * ks + (lane >> 4)) * RD_PTS + 16 * s + (lane & 15)];
+#pragma unroll
+    for (int q = 0; q < MTW; ++q) {
+      if (w + 4 * q >= mt_n) continue;
+#pragma unroll
+      for (int s = 0; s < RD_SUB; ++s)
+        acc[q][s] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], b[s], acc[q][s], 0, 0, 0);
+    }
+  }
+  cx<double>* o1 = (cx<double>*)ri.out1;
+  cx<double>* o2 = (cx<double>*)ri.out2;
+#pragma unroll
+  for (int q = 0; q < MTW; ++q) {
+    const int mt = w + 4 * q;
+    if (mt >= mt_n) continue;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int M = 16 * mt + 4 * (lane >> 4) + 2 * h;
+      if (M >= mtot) continue;
+      const int m = M / kin, j = (M % kin) >> 1;
+      cx<double>* o = m == 0 ? o1 : o2;
+#pragma unroll
+      for (int s = 0; s < RD_SUB; ++s) {
+        const int64_t p = p0 + 16 * s + (lane & 15);
+        if (p < K) o[(int64_t)j * K + p] = cx<double>{acc[q][s][2 * h], acc[q][s][2 * h + 1]};
+      }
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" int hfb_gram_workspace(int nst, long long K) {
@@ -1035,6 +1144,25 @@ int launch_rotate(int prec, RotArgs ra, int niso, hipStream_t st) {
     HFB_ROTB(6)
     HFB_ROTB(8)
 #undef HFB_ROTB
+  }
+  if (prec == HFB_F64 && mt_max <= 4 * RD_MTW_MAX) {
+    // FP64 matrix cores: RD_PTS points per workgroup, every input column
+    // staged in LDS
+    const int mtw = (mt_max + 3) / 4;
+    const size_t lds = sizeof(double) * (size_t)(4 * ((2 * nmax + 3) / 4)) * RD_PTS;
+    const dim3 grid((unsigned)((ra.K + RD_PTS - 1) / RD_PTS), niso);
+#define HFB_ROTD(MW)                                                          \
+  if (mtw <= MW) {                                                            \
+    auto kern = rotate_mfma_f64_kernel<MW>;                                   \
+    if (lds > 32 * 1024) allow_full_lds((const void*)kern);                   \
+    hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, ra);                   \
+    return launch_status();                                                   \
+  }
+    HFB_ROTD(1)
+    HFB_ROTD(2)
+    HFB_ROTD(4)
+    HFB_ROTD(8)
+#undef HFB_ROTD
   }
   const size_t csz = prec == HFB_F64 ? sizeof(cx<double>) : sizeof(cx<float>);
   // two k per lane while the staged tile fits in LDS, else one
