@@ -1,7 +1,10 @@
 """Energy-functional backward (hfb_energy_bwd behind EnergyFunctional).
 
-Checks, on densities of a Ca40 run (32^3, FP64, 15 SCF iterations from the
-oscillator start):
+Checks, on the converged densities of a Ca40 run (32^3, FP64, 1000 SCF
+iterations from the oscillator start, the energy then changing by < 1e-8
+MeV per iteration) and on the densities of a Pb208 run (48^3, 60 mixed-
+precision iterations then 10 FP64 refinement iterations of the promoted
+states):
   * coupling gradients for a batch of 8 seeded +-2% perturbations of the ten
     couplings against central finite differences (h = 1e-5 relative), as
     responses c_k dE/dc_k relative to the set's largest response;
@@ -19,15 +22,28 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def case(pkg):
+@pytest.fixture(scope="module", params=["Ca40", "Pb208"])
+def case(pkg, request):
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
-    cfg = dict(pkg.driver.CONFIGS["Ca40"])
-    run = pkg.driver.HFRun.from_config(precision="f64", **cfg)
-    run.prelude()
-    run.run(15, start=1, check_every=0)
+    cfg = dict(pkg.driver.CONFIGS[request.param])
+    if request.param == "Ca40":  # converged FP64 densities
+        run = pkg.driver.HFRun.from_config(precision="f64", **cfg)
+        run.prelude()
+        run.enable_graphs(True)
+        run.run(1000, start=1, check_every=0)
+        e0 = run.energies_host()["ehf"]
+        run.run(1, start=1001, check_every=0)
+        assert abs(run.energies_host()["ehf"] - e0) < 1e-8
+    else:  # FP32 bulk, then FP64 refinement of the promoted states
+        run = pkg.driver.HFRun.from_config(precision="mixed", **cfg)
+        run.prelude()
+        run.enable_graphs(True)
+        run.run(60, start=1, check_every=0)
+        run.promote()
+        run.run(10, start=61, check_every=0)
     torch.cuda.synchronize()
+    print(request.param, "ehf", run.energies_host()["ehf"], flush=True)
     env = pkg.adjoint.FunctionalEnv(run.grids, run.forces, run.params, run.levels.mass_number)
     return run, env
 
