@@ -20,6 +20,11 @@ GEMM efficiency: `gemm` times the loop's Gram and rotation calls on the
 states of each run (HIP events around a replayed graph) and reports
 F_gemm / (t * peak) with F_gemm = 4 * 8 * K * sum_q nst_q^2.
 
+Energy gradient: `coupling_gradient` times the backward of the energy
+functional w.r.t. the ten Skyrme couplings (a batch of 8 seeded +-2%
+coupling sets) on the refined densities of the mixed run and reports its
+largest deviation from central differences.
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -242,6 +247,48 @@ def compare(unref, ref, acc_it, refine, pairing):
     return out
 
 
+def coupling_gradient(pkg, run, nb=8, reps=5):
+    """Energy gradient w.r.t. the ten Skyrme couplings on the run's densities
+    for nb sets of seeded +-2% perturbations (EnergyFunctional forward and
+    backward, hfb_energy_bwd): average time of one batched evaluation with
+    HIP events on torch's stream, and the largest deviation of the responses
+    c_k dE/dc_k from central differences (h = 1e-5 relative) over all sets,
+    relative to each set's largest response."""
+    import numpy as np
+    import torch
+    A = pkg.adjoint
+    d = run.densities
+    env = A.FunctionalEnv(run.grids, run.forces, run.params, run.levels.mass_number)
+    c0 = A.coupling_vector(env.forces)
+    u = np.random.Generator(np.random.PCG64(7)).uniform(-1.0, 1.0, size=(nb, c0.numel()))
+    C0 = c0[None, :] * (1.0 + 0.02 * torch.from_numpy(u))
+
+    def grad():
+        C = C0.clone().requires_grad_(True)
+        return torch.autograd.grad(A.energy(C, d, env).sum(), [C])[0]
+    g = grad()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(reps):
+        grad()
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / reps
+    fd = torch.zeros_like(C0)
+    for k in range(C0.shape[1]):
+        h = 1e-5 * C0[:, k].abs()
+        cp, cm = C0.clone(), C0.clone()
+        cp[:, k] += h
+        cm[:, k] -= h
+        fd[:, k] = (A.energy(cp, d, env).detach().cpu() - A.energy(cm, d, env).detach().cpu()) / (2.0 * h)
+    rg, rf = g.cpu() * C0, fd * C0
+    err = ((rg - rf).abs().max(dim=1).values / rf.abs().max(dim=1).values).max().item()
+    return {"sets": nb, "avg_ms": ms, "fd_rel_err": err,
+            "densities": "the promoted mixed run after its FP64 refinement iterations",
+            "timing": f"HIP events around {reps} batched forward+backward evaluations"}
+
+
 def cpu_baseline(cfg, iters):
     sys.path.insert(0, str(ROOT))
     from oracle import hfb_oracle as O
@@ -334,6 +381,7 @@ def main():
     ref = {prec: snapshot(r) for prec, r in runs.items()}
     accuracy = compare(unref, ref, acc_it, args.refine, bool(cfg.get("ipair")))
     e_mix, e_64 = ref["mixed"], ref["f64"]
+    cgrad = coupling_gradient(pkg, runs["mixed"])
     err_kev = accuracy["eb_err_kev"]
 
     lv = runs["f64"].levels
@@ -367,7 +415,7 @@ def main():
         "ehf_mixed": e_mix["ehf"], "ehf_f64": e_64["ehf"], "accuracy": accuracy,
         "hbm_frac_iter_mixed": window_bytes(runs["f64"], args, 8) * mixed_ips / args.steps / HBM_PEAK,
         "hbm_frac_iter_f64": window_bytes(runs["f64"], args, 16) * f64_ips / args.steps / HBM_PEAK,
-        "roofline": roof, "gemm": gemm, "cpu_baseline": cpu,
+        "roofline": roof, "gemm": gemm, "coupling_gradient": cgrad, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
