@@ -37,12 +37,12 @@ def main():
 
 
 
-TRAJ_ITER = 60
+TRAJ_ITER = 200
 TVARY_ITER = 12
 
 
 def trajectory():
-    """o16_trajectory.json: iterations 1..60 of O16 with the reference's
+    """o16_trajectory.json: iterations 1..200 of O16 with the reference's
     default switches (inibcs = inidiag = 30), i.e. across the change to the
     Lagrange residual without diagonalisation at iteration 31
     (static.py:1763-1771, 99-104): ehf, ehfint, efluct1/2 and the largest

@@ -2,13 +2,13 @@
 committed fixtures of tests/golden/make_golden.py (the oracle needs about a
 second per O16 iteration on the host, the device about a millisecond).
 
-* O16, 60 iterations with the default switches inibcs = inidiag = 30: the
+* O16, 200 iterations with the default switches inibcs = inidiag = 30: the
   first phase (BCS-branch weights, subspace diagonalisation) and, from
-  iteration 31, the Lagrange residual without diagonalisation
-  (static.py:99-104, 1763-1771).  ehf and ehfint to 1e-7 MeV every
-  iteration, the largest sp_efluct1 of every mprint-th iteration to 1e-6
-  relative, observables (output.py:177-293) and the sorted single-particle
-  energies at iteration 60.
+  iteration 31, 170 iterations of the Lagrange residual without
+  diagonalisation (static.py:99-104, 1763-1771).  ehf and ehfint to 1e-7
+  MeV every iteration, the largest sp_efluct1 of every mprint-th iteration
+  to 1e-6 relative, observables (output.py:177-293) and the sorted
+  single-particle energies at iteration 200.
 * O16 with two empty states per isospin and the adaptive step size
   (tvaryx_0, static.py:1743-1744, 1870-1882), 12 iterations: ehf and the
   x0dmp chosen after every iteration; eager and graph-enabled runs (the
@@ -37,7 +37,7 @@ def _psi0(O, cfg):
     return st.levels.psi
 
 
-def test_o16_sixty_iterations_match_oracle(pkg, O, dev):
+def test_o16_200_iterations_match_oracle(pkg, O, dev):
     ref = json.loads((GOLDEN / "o16_trajectory.json").read_text())
     cfg = ref["config"]
     run = pkg.driver.HFRun.from_config(precision="f64", psi0=_psi0(O, cfg), **cfg)
