@@ -32,6 +32,7 @@ struct hfb_ctx {
 enum : int {
   HFB_SYNC_DIAG = 0,      // diag_kernel: one counter for its two isospin blocks
   HFB_SYNC_DIAG_U = 1,    // diag_kernel split calls: U_rho handoff flag per isospin (two slots)
+  HFB_SYNC_DIAG_CTL = 3,  // wide Loewdin path: per isospin the series degree or 0 = one-workgroup kernel (two slots)
   HFB_SYNC_COUNT = 64
 };
 

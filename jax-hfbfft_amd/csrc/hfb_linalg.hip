@@ -74,6 +74,11 @@ struct DiagArgs {
   int split;
   double tol2;  // the lambda_lin eigensolve stops at off-diagonal^2 <= tol2 diagonal^2 (Frobenius)
   int mode;  // DIAG_FULL, DIAG_LOEWDIN (S -> U only) or DIAG_GIVEN_U (construct from U in d.U)
+  // the wide path (blocks above NDIAG_LDS states, DIAG_FULL without the
+  // diagonalisation): per isospin 4 or 7 = the binomial series of that
+  // degree runs on the wide kernels, 1 = empty block, 0 = diag_kernel takes
+  // the block (Newton-Schulz or eigh); null: diag_kernel takes every block
+  unsigned int* ctl;
 };
 
 enum { DIAG_FULL = 0, DIAG_LOEWDIN = 1, DIAG_GIVEN_U = 2 };
@@ -863,8 +868,9 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
   const int role = blockIdx.x >> 1;  // 1: the Loewdin workgroup of a split call
   const DiagIso d = a.iso[iq];
   const int n = d.nst;
+  if (a.ctl && a.ctl[iq] != 0u) return;  // the wide path has this block
   if (n <= 0) {
-    if (a.construct && role == 0) diag_finish_arrive(a);
+    if (a.construct && role == 0 && !a.ctl) diag_finish_arrive(a);
     return;
   }
   const int tid = threadIdx.x;
@@ -1108,7 +1114,290 @@ __global__ void __launch_bounds__(NT) diag_kernel(DiagArgs a) {
     mm<NP, NT, false>(B0, U, B1, n);
     tile_for<NP, NT>(n, [&](int i, int j) { d.M2[i * n + j] = B0[i * n + j]; });
   }
-  if (a.construct) diag_finish_arrive(a);
+  // with the wide path, its last kernel combines the isospins
+  if (a.construct && !a.ctl) diag_finish_arrive(a);
+}
+
+// ---------------------------------------------------------------------------
+// The Loewdin step and construct_hfb_matrices on many compute units, for
+// blocks above NDIAG_LDS states without the subspace diagonalisation (the
+// steady iterations of the large nuclei, static.py:295-350, 463-577).  One
+// workgroup per isospin (diag_kernel) spends most of its time in the matrix
+// products of these steps; here every product is a launch with one
+// workgroup per 16 x 16 output tile, the four wavefronts of a workgroup
+// taking a quarter of K each and adding their partial tiles through LDS.
+// wd_prep_kernel forms E = S - I and chooses the series degree from |E|_F
+// as diag_kernel does; a block that needs the Newton-Schulz iteration or
+// the eigen-decomposition goes to diag_kernel, launched next with ctl
+// telling it which blocks are its own.  Work matrices (dense n x n in the
+// block's work area W): 0 E, then lambda+; 1 and 2 the Horner terms, then
+// H_old U and G_old U, then (1) the lambda partial sums; 3 H; 4 G.
+// ---------------------------------------------------------------------------
+constexpr int WD_NT = 256;
+
+__device__ __forceinline__ cx<double>* wd_slot(const DiagIso& d, int s) { return d.W + (size_t)s * d.nst * d.nst; }
+
+__global__ void __launch_bounds__(1024) wd_prep_kernel(DiagArgs a) {
+  __shared__ double red[32];
+  const int iq = blockIdx.x;
+  const DiagIso d = a.iso[iq];
+  const int n = d.nst;
+  if (n <= 0) {
+    if (threadIdx.x == 0) a.ctl[iq] = 1u;
+    return;
+  }
+  cx<double>* const E = wd_slot(d, 0);
+  double fr = 0.0, dummy = 0.0;
+  for (int e = threadIdx.x; e < n * n; e += 1024) {
+    const int i = e / n, j = e - i * n;
+    cx<double> v;
+    if (i > j)
+      v = d.S[i * n + j];
+    else if (i < j)
+      v = cconj(d.S[j * n + i]);
+    else
+      v = {d.S[i * n + i].x - 1.0, 0.0};
+    E[e] = v;
+    fr += cabs2(v);
+    if (i == j) a.sp_norm[d.start + i] = d.S[i * n + i].x;
+  }
+  block_sum2_all<1024>(fr, dummy, red);
+  if (threadIdx.x == 0) {
+    const unsigned deg = fr < 1e-6 ? 4u : (fr < 1e-4 ? 7u : 0u);
+    a.ctl[iq] = deg;
+    if (a.stats && deg != 0u) {
+      a.stats[2 + iq] = 0;
+      a.stats[iq] = 0;
+    }
+  }
+}
+
+// This wavefront's quarter of K of the 16 x 16 tile (ti, tj) of op(A) B
+// (op(A) = A or A^H; leading dimensions lda, ldb; entries beyond n are
+// zeros), in the D layout of v_mfma_f64_16x16x4f64; operands of four
+// k-steps are loaded together ahead of their products.
+__device__ __forceinline__ void wd_tile(const cx<double>* __restrict__ A, int lda, bool conjA,
+                                        const cx<double>* __restrict__ B, int ldb, int n, int ti, int tj, mm_d4& cr,
+                                        mm_d4& ci) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nstep = ((n + 15) >> 4);  // k-steps of 4 per wavefront: K padded to 16, split four ways
+  const int k0 = w * 4 * nstep;
+  const int i = ti * 16 + li, j = tj * 16 + li;
+  cr = mm_d4{0.0, 0.0, 0.0, 0.0};
+  ci = mm_d4{0.0, 0.0, 0.0, 0.0};
+  for (int s = 0; s < nstep; s += 4) {
+    cx<double> av[4], bv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + 4 * (s + u) + lk;
+      const bool kin = s + u < nstep && k < n;
+      const bool ka = kin && i < n, kb = kin && j < n;
+      const int kc = kin ? k : 0, ic = ka ? i : 0, jc = kb ? j : 0;
+      const cx<double> x = conjA ? cconj(A[kc * lda + ic]) : A[ic * lda + kc];
+      const cx<double> y = B[kc * ldb + jc];
+      av[u] = ka ? x : cx<double>{0.0, 0.0};
+      bv[u] = kb ? y : cx<double>{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (s + u < nstep) {
+        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].x, bv[u].x, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].x, bv[u].y, ci, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[u].y, bv[u].y, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].y, bv[u].x, ci, 0, 0, 0);
+      }
+    }
+  }
+}
+
+// The four K quarters summed (in wavefront order) through LDS; wavefront w
+// returns the entry (row ti*16 + lane/16 + 4w, column tj*16 + lane%16).
+__device__ __forceinline__ cx<double> wd_sum(const mm_d4& cr, const mm_d4& ci, double (*red)[8][64]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    red[w][r][lane] = cr[r];
+    red[w][4 + r][lane] = ci[r];
+  }
+  __syncthreads();
+  cx<double> v = {red[0][w][lane], red[0][4 + w][lane]};
+#pragma unroll
+  for (int q = 1; q < 4; ++q) {
+    v.x += red[q][w][lane];
+    v.y += red[q][4 + w][lane];
+  }
+  return v;
+}
+
+enum { WD_HORNER = 0, WD_HU = 1, WD_UHB = 2, WD_M2 = 3 };
+
+// One product of the wide path per launch, blockIdx = (tile, isospin,
+// right-hand block: 0 H, 1 G).
+//   WD_HORNER, step j of the series of degree deg (static.py:463-482,
+//     (I + E)^{-1/2} = sum_k c_k E^k by Horner from k = deg down to 0):
+//     j = 0: T = c_deg E E + c_{deg-1} E + c_{deg-2} I, then
+//     T <- E T + c_k I with k = deg - 2 - j; the k = 0 term is U (d.U).
+//     Steps beyond the block's degree return at once.
+//   WD_HU:  H_old U (z = 0), G_old U (z = 1) into work matrices 1, 2.
+//   WD_UHB: H = U^H (H_old U), G likewise into work matrices 3, 4.
+//   WD_M2:  M2 = U lambda+ (lambda+ in work matrix 0, or the stored lambda
+//     when construct is off, static.py:336-350, 557-570); the extra last
+//     workgroup of isospin 0 combines the fluctuation measures.
+template <int OP>
+__global__ void __launch_bounds__(WD_NT) wd_prod_kernel(DiagArgs a, int j) {
+  __shared__ double red[4][8][64];
+  const int iq = blockIdx.y;
+  const DiagIso d = a.iso[iq];
+  const int n = d.nst;
+  if (OP == WD_M2 && blockIdx.x == gridDim.x - 1) {
+    if (iq == 0 && a.construct && threadIdx.x < 64) {
+      // efluct1q/2q of the wide blocks from wd_lambda_kernel's partial
+      // sums, then efluct1 = max, efluct2 = mean over the isospins
+      // (static.py:571-576)
+      const int lane = threadIdx.x;
+      for (int q = 0; q < 2; ++q) {
+        const DiagIso dq = a.iso[q];
+        const int nq = dq.nst;
+        if (nq <= 0 || a.ctl[q] < 4u) continue;
+        const int nwg = (nq * nq + WD_NT - 1) / WD_NT;
+        const double* part = (const double*)wd_slot(dq, 1);
+        double mx = 0.0, sm = 0.0;
+        for (int b = lane; b < nwg; b += 64) {
+          mx = fmax(mx, part[2 * b]);
+          sm += part[2 * b + 1];
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+          mx = fmax(mx, __shfl_xor(mx, o, 64));
+          sm += __shfl_xor(sm, o, 64);
+        }
+        if (lane == 0) {
+          a.efl1q[q] = mx;
+          a.efl2q[q] = sqrt(sm / ((double)nq * nq));
+        }
+      }
+      if (lane == 0) {
+        a.efl1[0] = fmax(a.efl1q[0], a.efl1q[1]);
+        a.efl2[0] = 0.5 * (a.efl2q[0] + a.efl2q[1]);
+      }
+    }
+    return;
+  }
+  const unsigned deg = a.ctl[iq];
+  if (n <= 0 || deg < 4u) return;
+  const int nt = (n + 15) >> 4;
+  if ((int)blockIdx.x >= nt * nt) return;
+  const int ti = blockIdx.x / nt, tj = blockIdx.x - ti * nt;
+  const cx<double>* A;
+  const cx<double>* B;
+  int lda = n, ldb = n;
+  bool conjA = false;
+  cx<double>* C;
+  double alpha = 1.0, beta = 0.0, gamma = 0.0;
+  const cx<double>* Eadd = nullptr;
+  if constexpr (OP == WD_HORNER) {
+    constexpr double ck[8] = {1.0, -0.5, 0.375, -0.3125, 0.2734375, -0.24609375, 0.2255859375, -0.20947265625};
+    const int k = (int)deg - 2 - j;
+    if (k < 0) return;
+    A = wd_slot(d, 0);
+    if (j == 0) {
+      B = A;
+      Eadd = A;
+      alpha = ck[deg];
+      beta = ck[deg - 1];
+    } else {
+      B = wd_slot(d, 1 + ((j - 1) & 1));
+    }
+    gamma = ck[k];
+    C = k == 0 ? d.U : wd_slot(d, 1 + (j & 1));
+  } else if constexpr (OP == WD_HU) {
+    if (blockIdx.z == 1 && !(a.ipair && d.Gold)) return;
+    A = blockIdx.z == 0 ? d.Hold : d.Gold;
+    B = d.U;
+    C = wd_slot(d, 1 + blockIdx.z);
+  } else if constexpr (OP == WD_UHB) {
+    if (blockIdx.z == 1 && !(a.ipair && d.Gold)) return;
+    A = d.U;
+    conjA = true;
+    B = wd_slot(d, 1 + blockIdx.z);
+    C = wd_slot(d, 3 + blockIdx.z);
+  } else {
+    A = d.U;
+    if (a.construct) {
+      B = wd_slot(d, 0);
+    } else {
+      B = d.Ls;
+      ldb = a.nmax;
+    }
+    C = d.M2;
+  }
+  mm_d4 cr, ci;
+  wd_tile(A, lda, conjA, B, ldb, n, ti, tj, cr, ci);
+  cx<double> v = wd_sum(cr, ci, red);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int row = ti * 16 + (lane >> 4) + 4 * w, col = tj * 16 + (lane & 15);
+  if (row < n && col < n) {
+    v = cscale(v, alpha);
+    if (Eadd) v = cadd(v, cscale(Eadd[row * n + col], beta));
+    if (row == col) v.x += gamma;
+    C[row * n + col] = v;
+  }
+}
+
+// H, G and lambda of construct_hfb_matrices elementwise (static.py:540-576):
+// the static hmatrix/gapmatrix rows, sp_energy, deltaf, the off-diagonal
+// gap entries zeroed with tbcs, lambda's anti-Hermitian part (symcond),
+// its Hermitian part (lambda_save, and lambda+ into work matrix 0) and per
+// workgroup the largest |anti-Hermitian entry| and the sum of their squares.
+__global__ void __launch_bounds__(WD_NT) wd_lambda_kernel(DiagArgs a) {
+  __shared__ double red[32];
+  const int iq = blockIdx.y;
+  const DiagIso d = a.iso[iq];
+  const int n = d.nst;
+  if (n <= 0 || a.ctl[iq] < 4u || (int)blockIdx.x * WD_NT >= n * n) return;
+  const int nm = a.nmax;
+  const bool gp = a.ipair && d.Gold;
+  const cx<double>* H = wd_slot(d, 3);
+  const cx<double>* G = wd_slot(d, 4);
+  const int e = blockIdx.x * WD_NT + threadIdx.x;
+  double mx = 0.0, sm = 0.0;
+  if (e < n * n) {
+    const int i = e / n, j = e - i * n;
+    const bool zg = !gp || (a.tbcs && i != j);
+    const cx<double> hij = H[i * n + j], hji = H[j * n + i];
+    const cx<double> gij = zg ? cx<double>{0.0, 0.0} : G[i * n + j];
+    const cx<double> gji = zg ? cx<double>{0.0, 0.0} : G[j * n + i];
+    d.Hm[i * nm + j] = hij;
+    d.Gm[i * nm + j] = gij;
+    if (i == j) {
+      a.sp_energy[d.start + i] = hij.x;
+      if (a.ipair) a.deltaf[d.start + i] = gij.x * a.pairwg[d.start + i];
+    }
+    const int si = d.start + i, sj = d.start + j;
+    const double wj = a.wocc[sj] * a.wstates[sj];
+    const double uj = a.wguv[sj] * a.pairwg[sj] * a.wstates[sj];
+    const double wi = a.wocc[si] * a.wstates[si];
+    const double ui = a.wguv[si] * a.pairwg[si] * a.wstates[si];
+    const cx<double> lij = csub(cscale(hij, wj), cscale(gij, uj));
+    const cx<double> lji = csub(cscale(hji, wi), cscale(gji, ui));
+    const cx<double> lh = cconj(lji);
+    const cx<double> asym = cscale(csub(lij, lh), 0.5);
+    const cx<double> sym = cscale(cadd(lij, lh), 0.5);
+    d.Sym[i * nm + j] = asym;
+    d.Ls[i * nm + j] = sym;
+    wd_slot(d, 0)[i * n + j] = sym;
+    const double aa = cabs2(asym);
+    mx = sqrt(aa);
+    sm = aa;
+  }
+  const double M = block_max_all<WD_NT>(mx, red);
+  double S = sm, dummy = 0.0;
+  block_sum2_all<WD_NT>(S, dummy, red);
+  if (threadIdx.x == 0) {
+    double* part = (double*)wd_slot(d, 1);
+    part[2 * blockIdx.x] = M;
+    part[2 * blockIdx.x + 1] = S;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1409,7 +1698,9 @@ __global__ void __launch_bounds__(NAT_NT) lastdiag_kernel(LastArgs a) {
 }  // namespace
 
 // One workgroup per isospin; the work matrices of the larger block decide
-// the instance (LDS up to NDIAG_LDS states, global memory above).
+// the instance (LDS up to NDIAG_LDS states, global memory above).  With
+// a.ctl the wide kernels surround it: wd_prep_kernel before (it decides
+// which blocks diag_kernel keeps), the products after.
 static int launch_diag(const DiagArgs& a, int nbig, hipStream_t st) {
 #define HFB_DIAG(NPV, NTV, WMV, LDSV)                                                        \
   {                                                                                         \
@@ -1421,6 +1712,11 @@ static int launch_diag(const DiagArgs& a, int nbig, hipStream_t st) {
   // unit: the eigensolver's workgroup waits for its Loewdin workgroup)
   DiagArgs b = a;
   b.split = nbig <= NDIAG_LDS && a.tdiag && a.mode == DIAG_FULL;
+  if (a.ctl) {
+    hipLaunchKernelGGL(wd_prep_kernel, dim3(2), dim3(1024), 0, st, a);
+    const int st0 = launch_status();
+    if (st0 != HFB_OK) return st0;
+  }
   const size_t cz = sizeof(cx<double>);
   const size_t tri = (size_t)nbig * (nbig + 1) / 2;
   if (nbig <= NDIAG_LDS) {
@@ -1444,6 +1740,18 @@ static int launch_diag(const DiagArgs& a, int nbig, hipStream_t st) {
     HFB_DIAG(256, 1024, WM_GL3, 0)
   }
 #undef HFB_DIAG
+  if (a.ctl) {
+    const int nt = (nbig + 15) / 16, tiles = nt * nt;
+    const int nz = a.ipair ? 2 : 1;
+    for (int j = 0; j < 6; ++j)
+      hipLaunchKernelGGL(wd_prod_kernel<WD_HORNER>, dim3(tiles, 2, 1), dim3(WD_NT), 0, st, a, j);
+    if (a.construct) {
+      hipLaunchKernelGGL(wd_prod_kernel<WD_HU>, dim3(tiles, 2, nz), dim3(WD_NT), 0, st, a, 0);
+      hipLaunchKernelGGL(wd_prod_kernel<WD_UHB>, dim3(tiles, 2, nz), dim3(WD_NT), 0, st, a, 0);
+      hipLaunchKernelGGL(wd_lambda_kernel, dim3((nbig * nbig + WD_NT - 1) / WD_NT, 2), dim3(WD_NT), 0, st, a);
+    }
+    hipLaunchKernelGGL(wd_prod_kernel<WD_M2>, dim3(tiles + 1, 2, 1), dim3(WD_NT), 0, st, a, 0);
+  }
   return launch_status();
 }
 
@@ -1505,7 +1813,9 @@ extern "C" int hfb_diag_small(hfb_ctx* c, int nst0, int nst1, int nmax, const do
   a.tbcs = tbcs;
   a.construct = construct;
   a.stats = stats;
-  return launch_diag(a, nst0 > nst1 ? nst0 : nst1, (hipStream_t)stream);
+  // blocks above NDIAG_LDS states without the diagonalisation: the wide path
+  if (nbig0 > NDIAG_LDS && !tdiag) a.ctl = c->sync + HFB_SYNC_DIAG_CTL;
+  return launch_diag(a, nbig0, (hipStream_t)stream);
 }
 
 // Loewdin S^{-1/2} of one block (static.py:463-482): U (nst x nst complex)

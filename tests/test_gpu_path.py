@@ -115,6 +115,64 @@ def test_diagstep_matches_oracle(pkg, O, dev, npsi):
         lv.psi, lv.lagrange = saved["psi"], saved["lagrange"]
 
 
+@pytest.mark.parametrize("npsi,eps,tbcs", [([56, 47], 2e-5, True), ([90, 60], 1e-3, False), ([126, 82], 2e-5, False),
+                                           ([126, 82], 1e-3, True), ([126, 20], 2e-5, True)])
+def test_diagstep_wide_matches_oracle(pkg, O, dev, npsi, eps, tbcs):
+    """Loewdin + HFB matrices without the diagonalisation for blocks above 36
+    states on nearly orthonormal states, the steady-iteration case of the
+    large nuclei: the binomial series of degree 4 (eps = 2e-5) or 7 (1e-3)
+    and the products of construct_hfb_matrices on the wide kernels
+    (static.py:295-350, 463-577), BCS pairing matrices with and without the
+    tbcs cut of the gap matrix, against the oracle's eigh-based path; a
+    second call repeats the first bit for bit."""
+    ref, run = small_case(O, pkg, ipair=5, tbcs=tbcs, npsi=npsi)
+    O.diagstep(ref.energies, ref.forces, ref.grids, ref.levels, ref.static, False, True)
+    rng = np.random.default_rng(13)
+    psi = ref.levels.psi.copy()
+    for iq in range(2):
+        sl = ref.levels.block(iq)
+        m = sl.stop - sl.start
+        a = rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))
+        a = a + a.conj().T
+        a /= np.linalg.norm(a)
+        psi[sl] = psi[sl] + eps * np.einsum("ij,j...->i...", a, ref.levels.psi[sl])
+    ref.levels.psi = psi
+    run.levels.psi.copy_(T(psi))
+    shp = psi.shape
+    for nm in ("hmfpsi", "hampsi", "delpsi", "lagrange"):
+        a = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+        setattr(ref.levels, nm, a * 0.1)
+        getattr(run.levels, nm).copy_(T(a * 0.1))
+    ref.levels.wocc = rng.uniform(0.1, 1.0, ref.levels.nstmax)
+    ref.levels.wguv = rng.uniform(0.0, 0.5, ref.levels.nstmax)
+    run.levels.wocc.copy_(T(ref.levels.wocc))
+    run.levels.wguv.copy_(T(ref.levels.wguv))
+    lv = run.levels
+    saved = {k: getattr(lv, k).clone() for k in ("psi", "lagrange")}
+    O.diagstep(ref.energies, ref.forces, ref.grids, ref.levels, ref.static, False, True)
+    first = None
+    for rep in range(2):
+        lv.psi.copy_(saved["psi"])
+        lv.lagrange.copy_(saved["lagrange"])
+        pkg.static.diagstep(run.energies, run.forces, run.grids, lv, run.static, False, True)
+        torch.cuda.synchronize()
+        e = run.energies.host()
+        assert (e["ns_iterations"] == 0).all(), e["ns_iterations"]
+        assert rel(lv.sp_norm.cpu().numpy(), ref.levels.sp_norm) < 1e-12
+        assert rel(lv.sp_energy.cpu().numpy(), ref.levels.sp_energy) < 1e-10
+        assert rel(lv.deltaf.cpu().numpy(), ref.levels.deltaf) < 1e-10
+        assert abs(e["efluct1"] - ref.energies.efluct1[0]) < 1e-10 * max(1.0, ref.energies.efluct1[0])
+        assert abs(e["efluct2"] - ref.energies.efluct2[0]) < 1e-10 * max(1.0, ref.energies.efluct2[0])
+        assert rel(lv.psi.cpu().numpy(), ref.levels.psi) < 1e-11
+        assert rel(lv.lagrange.cpu().numpy(), ref.levels.lagrange) < 1e-10
+        out = [lv.psi.cpu(), lv.lagrange.cpu(), lv.sp_energy.cpu(), lv.deltaf.cpu()]
+        if first is None:
+            first = out
+        else:
+            for x, y in zip(first, out):
+                assert torch.equal(x, y)
+
+
 @pytest.mark.parametrize("eps,path", [(2e-5, "series4"), (1e-3, "series7"), (2e-2, "newton")])
 def test_loewdin_near_orthonormal(pkg, O, dev, eps, path):
     """Loewdin of nearly orthonormal states, psi_i + eps sum_j A_ij psi_j
