@@ -279,10 +279,14 @@ int hfb_gram2_ex(int prec, const void* const* A, const void* const* B, int nm, c
  * gave S^{-1/2}; -1: fell back to the eigensolve).  jacobi_tol: the
  * relative off-diagonal Frobenius norm at which the lambda_lin eigensolve
  * stops (0: 1e-13, the accuracy of FP64 subspace matrices; the mixed path
- * passes 1e-10 for matrices of FP32 states, whose entries carry about 1e-7;
+ * passes 1e-8 for matrices of FP32 states, whose entries carry about 1e-7;
  * HFB_E_ARG outside [0, 1)).
  * efluct1/2 are combined over the isospins inside the same launch (an
- * arrival counter of the context). */
+ * arrival counter of the context).  Above 36 states without tdiag the
+ * series, the products and the efluct combination run as a sequence of
+ * launches over many compute units (per-isospin control words of the
+ * context choose the blocks that stay on the one-workgroup kernel), so
+ * calls on one context must not overlap in time. */
 int hfb_diag_small(hfb_ctx* c, int nst0, int nst1, int nmax, const double* const* mats, int tdiag, int ipair, int tbcs,
                    int construct, const double* wocc, const double* wstates, const double* wguv,
                    const double* pairwg, double* sp_norm, double* sp_energy, double* deltaf, double* hmatrix,
