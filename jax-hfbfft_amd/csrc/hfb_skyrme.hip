@@ -211,35 +211,56 @@ __device__ __forceinline__ void skyrme_body(const SkyArgs& a, int vb, int vg) {
   double ep[1] = {0.0};
   double vm[2] = {-INFINITY, -INFINITY};
   for (int64_t v = vb * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)vg * blockDim.x) {
-    const double rn = a.rho[v], rp = a.rho[n + v];
+    // every input of the point first: the stores below may alias the
+    // inputs as far as the compiler knows, so loads written after a store
+    // would wait for it, one memory latency per group
+    double rho[2], chi[2], tau[2], lap[2], divj[2], grad[6], cur[6], curls[6], curlj[6];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      rho[q] = a.rho[q * n + v];
+      chi[q] = a.chi[q * n + v];
+      tau[q] = a.tau[q * n + v];
+      lap[q] = a.lap[q * n + v];
+      divj[q] = a.divj[q * n + v];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      grad[k] = a.grad[k * n + v];
+      cur[k] = a.cur[k * n + v];
+      curls[k] = a.curls[k * n + v];
+      curlj[k] = a.curlj[k * n + v];
+    }
+    const double wc = f.tcoul ? a.wcoul[v] : 0.0;
+    const double rn = rho[0], rp = rho[1];
     const double rt = rn + rp;
     const double rtp = pow(rt, f.power);
     double rearr = 0.0;
     if (f.ipair == 6) {
-      const double cn = a.chi[v], cp = a.chi[n + v];
+      const double cn = chi[0], cp = chi[1];
       rearr = (f.v0neut / f.rho0pr) * cn * cn + (f.v0prot / f.rho0pr) * cp * cp;
       ep[0] += rt * rearr;
     }
+#pragma unroll
     for (int iq = 0; iq < 2; ++iq) {
       const int ic = 1 - iq;
-      const double rq = a.rho[iq * n + v], rc = a.rho[ic * n + v];
+      const double rq = rho[iq], rc = rho[ic];
       // meanfield.py:181-190: three-body term
       double u = rtp * ((f.b3 * (f.power + 2.0) / 3.0 - 2.0 * f.b3p / 3.0) * rq + f.b3 * (f.power + 2.0) / 3.0 * rc -
                         (f.b3p * f.power / 3.0) * (rn * rn + rp * rp) / (rt + 1.0e-25));
       // meanfield.py:202-206: spin-orbit divergence
-      u += -(f.b4 + f.b4p) * a.divj[iq * n + v] - f.b4 * a.divj[ic * n + v];
+      u += -(f.b4 + f.b4p) * divj[iq] - f.b4 * divj[ic];
       // meanfield.py:211-221: Coulomb and Slater exchange for protons
       if (iq == 1 && f.tcoul) {
-        u += a.wcoul[v];
+        u += wc;
         if (f.ex != 0) u += -f.slate * pow(rp, 1.0 / 3.0);
       }
       // meanfield.py:234-241
-      u += (f.b0 - f.b0p) * rq + f.b0 * rc + (f.b1 - f.b1p) * a.tau[iq * n + v] + f.b1 * a.tau[ic * n + v] -
-           (f.b2 - f.b2p) * a.lap[iq * n + v] - f.b2 * a.lap[ic * n + v];
+      u += (f.b0 - f.b0p) * rq + f.b0 * rc + (f.b1 - f.b1p) * tau[iq] + f.b1 * tau[ic] - (f.b2 - f.b2p) * lap[iq] -
+           f.b2 * lap[ic];
       if (f.ipair == 6) u += rearr;  // meanfield.py:352-360
       const double b = f.h2m[iq] + (f.b1 - f.b1p) * rq + f.b1 * rc;  // meanfield.py:253-256
       const double v0 = iq == 1 ? f.v0prot : f.v0neut;
-      const double vp = f.ipair == 6 ? v0 * a.chi[iq * n + v] * (1.0 - rt / f.rho0pr) : v0 * a.chi[iq * n + v];
+      const double vp = f.ipair == 6 ? v0 * chi[iq] * (1.0 - rt / f.rho0pr) : v0 * chi[iq];
       a.upot[iq * n + v] = u;
       a.bmass[iq * n + v] = b;
       a.vpair[iq * n + v] = vp;
@@ -249,12 +270,14 @@ __device__ __forceinline__ void skyrme_body(const SkyArgs& a, int vb, int vg) {
         a.bmass32[iq * n + v] = float(b);
         a.vpair32[iq * n + v] = float(vp);
       }
+#pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const int64_t iqc = (int64_t)(iq * 3 + c) * n + v, icc = (int64_t)(ic * 3 + c) * n + v;
-        const double w = (f.b4 + f.b4p) * a.grad[iqc] + f.b4 * a.grad[icc];                      // 270-274
-        const double aq = -2.0 * (f.b1 - f.b1p) * a.cur[iqc] - 2.0 * f.b1 * a.cur[icc] -
-                          (f.b4 + f.b4p) * a.curls[iqc] - f.b4 * a.curls[icc];                      // 294-299
-        const double sp = -(f.b4 + f.b4p) * a.curlj[iqc] - f.b4 * a.curlj[icc];                   // 320-324
+        const int kq = iq * 3 + c, kc = ic * 3 + c;
+        const int64_t iqc = (int64_t)kq * n + v;
+        const double w = (f.b4 + f.b4p) * grad[kq] + f.b4 * grad[kc];                      // 270-274
+        const double aq = -2.0 * (f.b1 - f.b1p) * cur[kq] - 2.0 * f.b1 * cur[kc] - (f.b4 + f.b4p) * curls[kq] -
+                          f.b4 * curls[kc];                                                      // 294-299
+        const double sp = -(f.b4 + f.b4p) * curlj[kq] - f.b4 * curlj[kc];                   // 320-324
         a.wlspot[iqc] = w;
         a.aq[iqc] = aq;
         a.spot[iqc] = sp;
